@@ -1,0 +1,173 @@
+/*
+ * nicp_amd — C ABI of the MI355X (gfx950) implementation of NICP's
+ * registration loop. Plain pointers and sizes only; every function returns
+ * an int status (NICP_OK = 0, negative on error) and never throws.
+ * nicp_last_error() gives a message for the last failure on the calling thread.
+ *
+ * Conventions (as in yorsh87/nicp):
+ *  - images are row-major rows x cols (cv::Mat_ layout, nicp/nicp/definitions.h:37-74);
+ *  - 4x4 transforms are float[16] column-major (Eigen::Isometry3f::matrix());
+ *  - points are float4 (x, y, z, 1), normals float4 (x, y, z, 0)
+ *    (nicp/nicp/homogeneousvector4f.h:73-88);
+ *  - information matrices are float[16] column-major 4x4 with row/col 3 zero
+ *    (nicp/nicp/informationmatrix.h:13-84);
+ *  - an index image holds a point index or -1.
+ *  - T brings the current cloud onto the reference (nicp/nicp/alignerprojective.cpp:51,61).
+ *
+ * Threading: a context owns one HIP stream and its scratch memory; it is not
+ * thread-safe. Separate contexts are independent. Host buffers passed in are
+ * borrowed for the duration of the call only.
+ */
+#ifndef NICP_AMD_H
+#define NICP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NICP_OK 0
+#define NICP_ERR_INVALID (-1)
+#define NICP_ERR_HIP (-2)
+#define NICP_ERR_OOM (-3)
+#define NICP_ERR_STATE (-4)
+
+typedef struct nicp_ctx nicp_ctx;
+typedef struct nicp_cloud nicp_cloud;
+typedef struct nicp_batch nicp_batch;
+
+/* PinholePointProjector state (nicp/nicp/pinholepointprojector.h:67-70,
+ * nicp/nicp/pointprojector.cpp:9-10). K is the (already scaled) camera matrix. */
+typedef struct {
+  float K[9];               /* column-major 3x3 */
+  float min_distance;       /* must be > 0 */
+  float max_distance;
+} nicp_camera;
+
+/* StatsCalculatorIntegralImage (nicp/nicp/statscalculatorintegralimage.cpp:12-18) and
+ * Point/NormalInformationMatrixCalculator (nicp/nicp/informationmatrixcalculator.h:115-155). */
+typedef struct {
+  float world_radius;
+  int   min_image_radius;
+  int   max_image_radius;
+  int   min_points;
+  float curvature_threshold;
+  float point_flat[3], point_nonflat[3];
+  float normal_flat[3], normal_nonflat[3];
+  float point_curvature_threshold;
+  float normal_curvature_threshold;
+} nicp_stats_params;
+
+/* CorrespondenceFinder (nicp/nicp/correspondencefinder.cpp:9-18,
+ * nicp/nicp/correspondencefinderprojective.cpp:9-14), Linearizer
+ * (nicp/nicp/linearizer.cpp:9-19) and Aligner (nicp/nicp/aligner.cpp:21-41). */
+typedef struct {
+  float inlier_distance_threshold;
+  float inlier_normal_angular_threshold;   /* a cosine, compared with n_cur . n_ref */
+  float flat_curvature_threshold;
+  float inlier_curvature_ratio_threshold;
+  int   row_search_region;                 /* 0,0 = single-pixel search */
+  int   col_search_region;
+  int   demoted_to_gicp;
+  float inlier_max_chi2;
+  int   robust_kernel;
+  float normal_omega_scale;                /* Linearizer::_scale */
+  int   outer_iterations;
+  int   inner_iterations;
+  float lambda;
+  float translational_min_eigen_ratio;
+  float rotational_min_eigen_ratio;
+} nicp_align_params;
+
+typedef struct {
+  float  error;                /* Aligner::error(): chi2 of the last linearisation */
+  int    inliers;              /* Aligner::inliers() */
+  int    num_correspondences;  /* CorrespondenceFinder::numCorrespondences() of the last iteration */
+  double H[36];                /* last linearisation (no lambda), column-major 6x6 */
+  double b[6];
+  double mean[6];              /* Aligner::_computeStatistics outputs */
+  double omega[36];
+  double translational_ratio;
+  double rotational_ratio;
+  int    solution_valid;       /* Aligner::solutionValid(); 0 also when H is zero */
+  float  total_time_ms;        /* Aligner::totalTime(): device time of the call */
+} nicp_align_result;
+
+const char *nicp_last_error(void);
+
+/* context: one device, one stream (Aligner / converter object state) */
+int nicp_ctx_create(int device, nicp_ctx **out);
+int nicp_ctx_destroy(nicp_ctx *ctx);
+int nicp_ctx_synchronize(nicp_ctx *ctx);
+/* the HIP stream the context launches on (hipStream_t) */
+void *nicp_ctx_stream(nicp_ctx *ctx);
+
+/* Replaces DepthImage_convert_16UC1_to_32FC1 (nicp/nicp/imageutils.cpp:85-99) +
+ * DepthImageConverterIntegralImage::compute (nicp/nicp/depthimageconverterintegralimage.cpp:18-64)
+ * with an identity sensor offset. keep_stats also keeps the per-point Stats
+ * (eigenvectors, eigenvalues, mean, n) for download. */
+int nicp_cloud_from_depth_u16(nicp_ctx *ctx, const uint16_t *depth, int rows, int cols, float depth_scale,
+                              const nicp_camera *cam, const nicp_stats_params *sp, int keep_stats,
+                              nicp_cloud **out);
+/* Replaces filling a Cloud's vectors (nicp/nicp/cloud.h:20-193): points, normals, the curvature
+ * of Stats, and the two information matrices; any of normals/curvature/omegas may be NULL (zero). */
+int nicp_cloud_from_host(nicp_ctx *ctx, int n, const float *points4, const float *normals4, const float *curvature,
+                         const float *omega_p16, const float *omega_n16, nicp_cloud **out);
+int nicp_cloud_destroy(nicp_cloud *cloud);
+int nicp_cloud_size(const nicp_cloud *cloud, int *n);
+/* Cloud::points/normals/stats().curvature()/pointInformationMatrix/normalInformationMatrix */
+int nicp_cloud_download(nicp_cloud *cloud, float *points4, float *normals4, float *curvature, float *omega_p16,
+                        float *omega_n16);
+/* Cloud::stats(): eigenvectors (9 floats, column-major), eigenvalues (3), mean (4), n */
+int nicp_cloud_download_stats(nicp_cloud *cloud, float *eigvecs9, float *eigvals3, float *means4, int *npts);
+/* the index image of DepthImageConverter::_indexImage (rows*cols ints) */
+int nicp_cloud_index_image(nicp_cloud *cloud, int *index_image);
+
+/* Replaces PinholePointProjector::project (nicp/nicp/pinholepointprojector.cpp:33-66):
+ * z-buffer projection of the cloud seen from `transform`; depth_image may be NULL. */
+int nicp_project(nicp_ctx *ctx, const nicp_cloud *cloud, const float transform[16], const nicp_camera *cam,
+                 int rows, int cols, int *index_image, float *depth_image);
+
+/* Replaces CorrespondenceFinderProjective::compute (nicp/nicp/correspondencefinderprojective.cpp:16-21)
+ * for given index images; T is the finder's transform (the aligner passes T^-1). pairs receives
+ * (referenceIndex, currentIndex) in row-major pixel order; capacity rows*cols pairs. */
+int nicp_correspondences(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const int *ref_index,
+                         const int *cur_index, int rows, int cols, const float T[16], const nicp_align_params *ap,
+                         int *pairs, int *num_pairs);
+
+/* Replaces Linearizer::update (nicp/nicp/linearizer.cpp:21-134) over an explicit pair list. */
+int nicp_linearize(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const int *pairs, int num_pairs,
+                   const float T[16], const nicp_align_params *ap, double H[36], double b[6], double *error,
+                   int *inliers);
+
+/* Replaces AlignerProjective::align (nicp/nicp/alignerprojective.cpp:21-136) with identity
+ * sensor offsets and no priors; the image size and camera are the projector's. */
+int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const nicp_camera *cam, int rows,
+               int cols, const float init_guess[16], const nicp_align_params *ap, float T_out[16],
+               nicp_align_result *result);
+/* CorrespondenceFinder::correspondences() of the last nicp_align on ctx (capacity in pairs) */
+int nicp_last_correspondences(nicp_ctx *ctx, int *pairs, int capacity, int *num_pairs);
+
+/* Batched form (no reference counterpart; it replaces running one converter/aligner per
+ * frame pair on host threads): frames of uneven size live in one device arena, depth
+ * is uploaded once, then converted and aligned pair-wise in single launches per stage. */
+int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows, const int *cols, const nicp_camera *cams,
+                      const nicp_stats_params *sp, nicp_batch **out);
+int nicp_batch_destroy(nicp_batch *batch);
+int nicp_batch_upload_depth(nicp_batch *batch, int frame, const uint16_t *depth, float depth_scale);
+/* DepthImageConverterIntegralImage::compute for every frame of the batch */
+int nicp_batch_convert(nicp_batch *batch);
+/* AlignerProjective::align for pairs (ref_frame[i], cur_frame[i]); T_out and results may be NULL.
+ * use_graph != 0 captures the iteration chain in a hipGraph and replays it. */
+int nicp_batch_align(nicp_batch *batch, int num_pairs, const int *ref_frame, const int *cur_frame,
+                     const float *init_guesses16, const nicp_align_params *ap, int use_graph, float *T_out16,
+                     nicp_align_result *results);
+/* number of points of a batch frame, and a cloud view of it (downloads only) */
+int nicp_batch_frame_cloud(nicp_batch *batch, int frame, nicp_cloud **out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NICP_AMD_H */

@@ -1,0 +1,329 @@
+"""ctypes bindings of the nicp_amd C ABI (``include/nicp_amd.h``).
+
+The library ``libnicp_amd.so`` is the gfx950 build of ``csrc/``. Every call
+goes to the device; there is no CPU fallback: a missing library or a failing
+HIP call raises :class:`NicpError`.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnicp_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "nicp_amd.h")
+
+NICP_OK = 0
+
+
+class NicpError(RuntimeError):
+    pass
+
+
+class Camera(C.Structure):
+    _fields_ = [("K", C.c_float * 9), ("min_distance", C.c_float), ("max_distance", C.c_float)]
+
+
+class StatsParams(C.Structure):
+    _fields_ = [("world_radius", C.c_float), ("min_image_radius", C.c_int), ("max_image_radius", C.c_int),
+                ("min_points", C.c_int), ("curvature_threshold", C.c_float),
+                ("point_flat", C.c_float * 3), ("point_nonflat", C.c_float * 3),
+                ("normal_flat", C.c_float * 3), ("normal_nonflat", C.c_float * 3),
+                ("point_curvature_threshold", C.c_float), ("normal_curvature_threshold", C.c_float)]
+
+
+class AlignParams(C.Structure):
+    _fields_ = [("inlier_distance_threshold", C.c_float), ("inlier_normal_angular_threshold", C.c_float),
+                ("flat_curvature_threshold", C.c_float), ("inlier_curvature_ratio_threshold", C.c_float),
+                ("row_search_region", C.c_int), ("col_search_region", C.c_int), ("demoted_to_gicp", C.c_int),
+                ("inlier_max_chi2", C.c_float), ("robust_kernel", C.c_int), ("normal_omega_scale", C.c_float),
+                ("outer_iterations", C.c_int), ("inner_iterations", C.c_int), ("lambda_", C.c_float),
+                ("translational_min_eigen_ratio", C.c_float), ("rotational_min_eigen_ratio", C.c_float)]
+
+
+class AlignResult(C.Structure):
+    _fields_ = [("error", C.c_float), ("inliers", C.c_int), ("num_correspondences", C.c_int),
+                ("H", C.c_double * 36), ("b", C.c_double * 6), ("mean", C.c_double * 6),
+                ("omega", C.c_double * 36), ("translational_ratio", C.c_double),
+                ("rotational_ratio", C.c_double), ("solution_valid", C.c_int), ("total_time_ms", C.c_float)]
+
+
+#: every entry point declared in include/nicp_amd.h
+EXPORTS = [
+    "nicp_last_error", "nicp_ctx_create", "nicp_ctx_destroy", "nicp_ctx_synchronize", "nicp_ctx_stream",
+    "nicp_cloud_from_depth_u16", "nicp_cloud_from_host", "nicp_cloud_destroy", "nicp_cloud_size",
+    "nicp_cloud_download", "nicp_cloud_download_stats", "nicp_cloud_index_image", "nicp_project",
+    "nicp_correspondences", "nicp_linearize", "nicp_align", "nicp_last_correspondences", "nicp_batch_create",
+    "nicp_batch_destroy", "nicp_batch_upload_depth", "nicp_batch_convert", "nicp_batch_align",
+    "nicp_batch_frame_cloud",
+]
+
+_lib = None
+vp = C.c_void_p
+
+
+def lib():
+    """Loads libnicp_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NicpError("libnicp_amd.so is not built: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.nicp_last_error.restype = C.c_char_p
+        L.nicp_ctx_stream.restype = vp
+        sig = {
+            "nicp_ctx_create": [C.c_int, vp], "nicp_ctx_destroy": [vp], "nicp_ctx_synchronize": [vp],
+            "nicp_ctx_stream": [vp],
+            "nicp_cloud_from_depth_u16": [vp, vp, C.c_int, C.c_int, C.c_float, vp, vp, C.c_int, vp],
+            "nicp_cloud_from_host": [vp, C.c_int, vp, vp, vp, vp, vp, vp],
+            "nicp_cloud_destroy": [vp], "nicp_cloud_size": [vp, vp],
+            "nicp_cloud_download": [vp, vp, vp, vp, vp, vp],
+            "nicp_cloud_download_stats": [vp, vp, vp, vp, vp],
+            "nicp_cloud_index_image": [vp, vp],
+            "nicp_project": [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp],
+            "nicp_correspondences": [vp, vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp],
+            "nicp_linearize": [vp, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp],
+            "nicp_align": [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp],
+            "nicp_last_correspondences": [vp, vp, C.c_int, vp],
+            "nicp_batch_create": [vp, C.c_int, vp, vp, vp, vp, vp],
+            "nicp_batch_destroy": [vp], "nicp_batch_upload_depth": [vp, C.c_int, vp, C.c_float],
+            "nicp_batch_convert": [vp],
+            "nicp_batch_align": [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp],
+            "nicp_batch_frame_cloud": [vp, C.c_int, vp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            if name != "nicp_ctx_stream":
+                f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != NICP_OK:
+        raise NicpError("nicp_amd error %d: %s" % (rc, lib().nicp_last_error().decode(errors="replace")))
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def camera(fx, fy, cx, cy, min_d=0.5, max_d=4.5):
+    c = Camera()
+    c.K[:] = [fx, 0.0, 0.0, 0.0, fy, 0.0, cx, cy, 1.0]
+    c.min_distance = min_d
+    c.max_distance = max_d
+    return c
+
+
+def stats_params(world_radius=0.1, min_image_radius=10, max_image_radius=30, min_points=50,
+                 curvature_threshold=0.2, infomat_curvature_threshold=0.02):
+    """Values of nicp/nicp_conf/nicp_aligner_1_1.conf:22-27 (information matrices: informationmatrixcalculator.h)."""
+    s = StatsParams()
+    s.world_radius = world_radius
+    s.min_image_radius = min_image_radius
+    s.max_image_radius = max_image_radius
+    s.min_points = min_points
+    s.curvature_threshold = curvature_threshold
+    s.point_flat[:] = [1000.0, 0.001, 0.001]
+    s.point_nonflat[:] = [1.0, 0.1, 0.1]
+    s.normal_flat[:] = [1000.0, 0.001, 0.001]
+    s.normal_nonflat[:] = [0.1, 1.0, 1.0]
+    s.point_curvature_threshold = infomat_curvature_threshold
+    s.normal_curvature_threshold = infomat_curvature_threshold
+    return s
+
+
+def align_params(outer_iterations=10, inner_iterations=1, **kw):
+    """Values of nicp/nicp_conf/nicp_aligner_1_1.conf:30-44 and the Aligner constructor (aligner.cpp:21-41)."""
+    a = AlignParams()
+    a.inlier_distance_threshold = 1.0
+    a.inlier_normal_angular_threshold = 0.95
+    a.flat_curvature_threshold = 0.02
+    a.inlier_curvature_ratio_threshold = 1.3
+    a.row_search_region = 0
+    a.col_search_region = 0
+    a.demoted_to_gicp = 0
+    a.inlier_max_chi2 = 9000.0
+    a.robust_kernel = 1
+    a.normal_omega_scale = 1.0
+    a.outer_iterations = outer_iterations
+    a.inner_iterations = inner_iterations
+    a.lambda_ = 1e3
+    a.translational_min_eigen_ratio = 50.0
+    a.rotational_min_eigen_ratio = 50.0
+    for k, v in kw.items():
+        setattr(a, k, v)
+    return a
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().nicp_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().nicp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(lib().nicp_ctx_synchronize(self.h))
+
+    # -- clouds --------------------------------------------------------------------------
+    def cloud_from_depth(self, depth_u16, cam, sp, depth_scale=0.001, keep_stats=False):
+        d = np.ascontiguousarray(depth_u16, dtype=np.uint16)
+        h = C.c_void_p()
+        _check(lib().nicp_cloud_from_depth_u16(self.h, _p(d), d.shape[0], d.shape[1], depth_scale, C.byref(cam),
+                                               C.byref(sp), int(keep_stats), C.byref(h)))
+        return Cloud(self, h, d.shape)
+
+    def cloud_from_host(self, points, normals=None, curvature=None, omega_p=None, omega_n=None):
+        pts = np.ascontiguousarray(points, np.float32)
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (normals, curvature, omega_p, omega_n)]
+        h = C.c_void_p()
+        _check(lib().nicp_cloud_from_host(self.h, len(pts), _p(pts), *[_p(a) for a in arrs], C.byref(h)))
+        return Cloud(self, h, None)
+
+    # -- operations ----------------------------------------------------------------------
+    def project(self, cloud, transform16, cam, rows, cols):
+        idx = np.empty((rows, cols), np.int32)
+        dep = np.empty((rows, cols), np.float32)
+        _check(lib().nicp_project(self.h, cloud.h, _p(np.ascontiguousarray(transform16, np.float32)), C.byref(cam),
+                                  rows, cols, _p(idx), _p(dep)))
+        return idx, dep
+
+    def correspondences(self, ref, cur, ref_index, cur_index, T16, ap):
+        rows, cols = ref_index.shape
+        pairs = np.empty((rows * cols, 2), np.int32)
+        n = C.c_int()
+        _check(lib().nicp_correspondences(self.h, ref.h, cur.h, _p(np.ascontiguousarray(ref_index, np.int32)),
+                                          _p(np.ascontiguousarray(cur_index, np.int32)), rows, cols,
+                                          _p(np.ascontiguousarray(T16, np.float32)), C.byref(ap), _p(pairs),
+                                          C.byref(n)))
+        return pairs[:n.value].copy()
+
+    def linearize(self, ref, cur, pairs, T16, ap):
+        pairs = np.ascontiguousarray(pairs, np.int32)
+        H = np.empty(36, np.float64)
+        b = np.empty(6, np.float64)
+        e = C.c_double()
+        i = C.c_int()
+        _check(lib().nicp_linearize(self.h, ref.h, cur.h, _p(pairs), len(pairs),
+                                    _p(np.ascontiguousarray(T16, np.float32)), C.byref(ap), _p(H), _p(b), C.byref(e),
+                                    C.byref(i)))
+        return H.reshape(6, 6, order="F"), b, e.value, i.value
+
+    def align(self, ref, cur, cam, rows, cols, init16, ap):
+        T = np.empty(16, np.float32)
+        res = AlignResult()
+        _check(lib().nicp_align(self.h, ref.h, cur.h, C.byref(cam), rows, cols,
+                                _p(np.ascontiguousarray(init16, np.float32)), C.byref(ap), _p(T), C.byref(res)))
+        return T, res
+
+    def last_correspondences(self):
+        n = C.c_int()
+        _check(lib().nicp_last_correspondences(self.h, None, 0, C.byref(n)))
+        pairs = np.empty((max(n.value, 1), 2), np.int32)
+        _check(lib().nicp_last_correspondences(self.h, _p(pairs), n.value, C.byref(n)))
+        return pairs[:n.value].copy()
+
+
+class Cloud:
+    def __init__(self, ctx, h, shape):
+        self.ctx, self.h, self.shape = ctx, h, shape
+        n = C.c_int()
+        _check(lib().nicp_cloud_size(h, C.byref(n)))
+        self.n = n.value
+
+    def close(self):
+        if self.h:
+            lib().nicp_cloud_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def download(self):
+        n = self.n
+        pts = np.empty((n, 4), np.float32)
+        nrm = np.empty((n, 4), np.float32)
+        curv = np.empty(n, np.float32)
+        op = np.empty((n, 16), np.float32)
+        on = np.empty((n, 16), np.float32)
+        _check(lib().nicp_cloud_download(self.h, _p(pts), _p(nrm), _p(curv), _p(op), _p(on)))
+        return pts, nrm, curv, op, on
+
+    def download_stats(self):
+        n = self.n
+        ev = np.empty((n, 9), np.float32)
+        el = np.empty((n, 3), np.float32)
+        mn = np.empty((n, 4), np.float32)
+        npt = np.empty(n, np.int32)
+        _check(lib().nicp_cloud_download_stats(self.h, _p(ev), _p(el), _p(mn), _p(npt)))
+        return ev, el, mn, npt
+
+    def index_image(self):
+        idx = np.empty(self.shape, np.int32)
+        _check(lib().nicp_cloud_index_image(self.h, _p(idx)))
+        return idx
+
+
+class Batch:
+    """Frames of uneven size converted and aligned pair-wise in single launches per stage."""
+
+    def __init__(self, ctx, shapes, cams, sp):
+        self.ctx = ctx
+        n = len(shapes)
+        rows = np.array([s[0] for s in shapes], np.int32)
+        cols = np.array([s[1] for s in shapes], np.int32)
+        cam_arr = (Camera * n)(*cams)
+        h = C.c_void_p()
+        _check(lib().nicp_batch_create(ctx.h, n, _p(rows), _p(cols), cam_arr, C.byref(sp), C.byref(h)))
+        self.h = h
+        self.n = n
+
+    def close(self):
+        if self.h:
+            lib().nicp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, frame, depth_u16, depth_scale=0.001):
+        d = np.ascontiguousarray(depth_u16, np.uint16)
+        _check(lib().nicp_batch_upload_depth(self.h, frame, _p(d), depth_scale))
+
+    def convert(self):
+        _check(lib().nicp_batch_convert(self.h))
+
+    def align(self, ref, cur, init16, ap, use_graph=True, want_results=True):
+        ref = np.ascontiguousarray(ref, np.int32)
+        cur = np.ascontiguousarray(cur, np.int32)
+        n = len(ref)
+        init = None if init16 is None else np.ascontiguousarray(init16, np.float32).reshape(n, 16)
+        T = np.empty((n, 16), np.float32)
+        res = (AlignResult * n)() if want_results else None
+        _check(lib().nicp_batch_align(self.h, n, _p(ref), _p(cur), _p(init), C.byref(ap), int(use_graph),
+                                      _p(T) if want_results else None, res))
+        return T, res
+
+    def frame_cloud(self, frame, shape):
+        h = C.c_void_p()
+        _check(lib().nicp_batch_frame_cloud(self.h, frame, C.byref(h)))
+        return Cloud(self.ctx, h, shape)

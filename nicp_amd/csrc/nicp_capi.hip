@@ -1,0 +1,1330 @@
+// Host side of the nicp_amd C ABI (include/nicp_amd.h): device memory of
+// clouds and batches, launch sequences of the conversion and registration
+// stages, hipGraph capture of the iteration chain, and the FP64 solution
+// statistics of Aligner::_computeStatistics (nicp/nicp/aligner.cpp:82-135).
+// Single translation unit: the kernels are compiled here too.
+#include "nicp_kernels.hip"
+
+#include <float.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "../../include/nicp_amd.h"
+
+using namespace nicp_dev;
+using namespace nicp_math;
+
+// ---------------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) return set_err(NICP_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(x)                  \
+  do {                          \
+    int rc_ = (x);              \
+    if (rc_ != NICP_OK) return rc_; \
+  } while (0)
+
+extern "C" const char *nicp_last_error(void) { return g_err; }
+
+// ---------------------------------------------------------------------------------------
+// device memory helpers
+// ---------------------------------------------------------------------------------------
+namespace {
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// a growable device buffer
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int reserve(size_t n) {
+    if (n <= bytes) return NICP_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t want = align_up(std::max<size_t>(n, 256), 1 << 16);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return set_err(NICP_ERR_OOM, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
+    }
+    bytes = want;
+    return NICP_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T *as() const { return (T *)p; }
+};
+
+// carves aligned sub-ranges out of one allocation
+struct Carver {
+  size_t off = 0;
+  template <typename T>
+  size_t take(size_t count) {
+    size_t o = align_up(off, 256);
+    off = o + count * sizeof(T);
+    return o;
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// objects
+// ---------------------------------------------------------------------------------------
+struct nicp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // workspace of single-pair calls
+  DevBuf frames_tbl, pairs_tbl, maps, zbuf, cur_index, pairs_buf, state, partials, result, depth_tmp, scratch_in;
+  // last single-pair correspondences (device pointer into pairs_buf) and its image size
+  int last_rows = 0, last_cols = 0;
+  bool last_valid = false;
+};
+
+struct nicp_cloud {
+  nicp_ctx *ctx = nullptr;
+  FrameDev fd{};          // host copy holding device pointers
+  int n = 0;
+  int has_index = 0;
+  int has_stats = 0;
+  void *alloc = nullptr;  // owned allocation (null for batch views)
+};
+
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  std::vector<int> key;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key.clear();
+  }
+};
+
+struct nicp_batch {
+  nicp_ctx *ctx = nullptr;
+  int num_frames = 0;
+  std::vector<FrameDev> frames;
+  std::vector<int> npoints_host;
+  StatsParamsDev sp{};
+  void *arena = nullptr;
+  FrameDev *d_frames = nullptr;
+  int *d_blk_frame = nullptr, *d_xblk_frame = nullptr, *d_yblk_frame = nullptr;
+  int total_tiles = 0, total_xblk = 0, total_yblk = 0;
+  // align workspace
+  DevBuf pairs_tbl, maps, zbuf, cur_index, pairs_buf, state, partials, result;
+  GraphCache graph;
+  GraphCache conv_graph;
+};
+
+// ---------------------------------------------------------------------------------------
+// parameter conversion
+// ---------------------------------------------------------------------------------------
+namespace {
+
+int check_camera(const nicp_camera *cam) {
+  if (!cam) return set_err(NICP_ERR_INVALID, "camera is null");
+  if (!(cam->min_distance > 0.0f) || !(cam->max_distance >= cam->min_distance))
+    return set_err(NICP_ERR_INVALID, "camera distances must satisfy 0 < min_distance <= max_distance");
+  return NICP_OK;
+}
+
+StatsParamsDev to_dev(const nicp_stats_params *s, int keep_stats) {
+  StatsParamsDev d{};
+  d.min_r = s->min_image_radius;
+  d.max_r = s->max_image_radius;
+  d.min_points = s->min_points;
+  d.curvature_threshold = s->curvature_threshold;
+  for (int i = 0; i < 3; ++i) {
+    d.pflat[i] = s->point_flat[i];
+    d.pnonflat[i] = s->point_nonflat[i];
+    d.nflat[i] = s->normal_flat[i];
+    d.nnonflat[i] = s->normal_nonflat[i];
+  }
+  d.pthr = s->point_curvature_threshold;
+  d.nthr = s->normal_curvature_threshold;
+  d.keep_stats = keep_stats;
+  return d;
+}
+
+AlignParamsDev to_dev(const nicp_align_params *a) {
+  AlignParamsDev d{};
+  // CorrespondenceFinder::setInlierDistanceThreshold (correspondencefinder.h:86-89)
+  d.sq_thr = a->inlier_distance_threshold * a->inlier_distance_threshold;
+  d.cos_thr = a->inlier_normal_angular_threshold;
+  d.flat = a->flat_curvature_threshold;
+  // correspondencefinderprojective.cpp:33-34
+  d.min_ratio = 1.0f / a->inlier_curvature_ratio_threshold;
+  d.max_ratio = a->inlier_curvature_ratio_threshold;
+  d.row_region = a->row_search_region;
+  d.col_region = a->col_search_region;
+  d.gicp = a->demoted_to_gicp;
+  d.max_chi2 = a->inlier_max_chi2;
+  d.robust = a->robust_kernel;
+  d.nscale = a->normal_omega_scale;
+  d.lambda = a->lambda;
+  return d;
+}
+
+int check_align_params(const nicp_align_params *a) {
+  if (!a) return set_err(NICP_ERR_INVALID, "align params are null");
+  if (a->outer_iterations < 0 || a->inner_iterations < 1 || a->row_search_region < 0 || a->col_search_region < 0)
+    return set_err(NICP_ERR_INVALID, "invalid iteration counts or search regions");
+  return NICP_OK;
+}
+
+// fills the geometric members of a frame descriptor
+void frame_geometry(FrameDev &f, int rows, int cols, const nicp_camera *cam, float world_radius, float depth_scale) {
+  f.rows = rows;
+  f.cols = cols;
+  f.n_tiles = (rows * cols + kTile - 1) / kTile;
+  f.depth_scale = depth_scale;
+  f.min_d = cam->min_distance;
+  f.max_d = cam->max_distance;
+  for (int k = 0; k < 9; ++k) f.K[k] = cam->K[k];
+  float I4[16], KRt[16];
+  set_identity4(I4);
+  projector_matrices(cam->K, I4, KRt, f.iKRt);
+  // _projectInterval (pinholepointprojector.h:293-302): K * (R, R, 0), 3-term redux
+  f.ip0 = red3(cam->K[0] * world_radius, cam->K[3] * world_radius, cam->K[6] * 0.0f);
+  f.ip1 = red3(cam->K[1] * world_radius, cam->K[4] * world_radius, cam->K[7] * 0.0f);
+}
+
+int xblocks(int rows) { return (rows * 10 + kThreads - 1) / kThreads; }
+int yblocks(int cols) { return (cols * 10 + kThreads - 1) / kThreads; }
+
+// Layout of the per-frame device arrays inside one allocation.
+struct FrameLayout {
+  size_t depth, index, ii, pts, nrm, omg, stats, npoints, tcount, toff, total;
+};
+FrameLayout frame_layout(int rows, int cols, bool with_stats, bool with_ii, size_t base) {
+  Carver cv;
+  cv.off = base;
+  const size_t P = (size_t)rows * cols;
+  const size_t T = (P + kTile - 1) / kTile;
+  FrameLayout L;
+  L.depth = cv.take<uint16_t>(P);
+  L.index = cv.take<int>(P);
+  L.ii = with_ii ? cv.take<float>(P * 10) : 0;
+  L.pts = cv.take<float4>(P);
+  L.nrm = cv.take<float4>(P);
+  L.omg = cv.take<float4>(P * 3);
+  L.stats = with_stats ? cv.take<float4>(P * 4) : 0;
+  L.npoints = cv.take<int>(1);
+  L.tcount = cv.take<int>(T);
+  L.toff = cv.take<int>(T);
+  L.total = cv.off;
+  return L;
+}
+
+void frame_pointers(FrameDev &f, char *base, const FrameLayout &L, bool with_stats, bool with_ii) {
+  f.depth = (const uint16_t *)(base + L.depth);
+  f.index = (int *)(base + L.index);
+  f.ii = with_ii ? (float *)(base + L.ii) : nullptr;
+  f.pts = (float4 *)(base + L.pts);
+  f.nrm = (float4 *)(base + L.nrm);
+  f.omg = (float4 *)(base + L.omg);
+  f.stats = with_stats ? (float4 *)(base + L.stats) : nullptr;
+  f.npoints = (int *)(base + L.npoints);
+  f.tile_count = (int *)(base + L.tcount);
+  f.tile_off = (int *)(base + L.toff);
+}
+
+// Launch sequence of the conversion for a frame table already on the device.
+int launch_convert(hipStream_t s, const FrameDev *d_frames, int num_frames, const int *d_blk, int total_tiles,
+                   const int *d_x, int total_x, const int *d_y, int total_y, const StatsParamsDev &sp) {
+  if (total_tiles == 0) return NICP_OK;
+  hipLaunchKernelGGL(k_count, dim3(total_tiles), dim3(kThreads), 0, s, d_frames, d_blk);
+  hipLaunchKernelGGL(k_scan, dim3(num_frames), dim3(1024), 0, s, d_frames);
+  hipLaunchKernelGGL(k_unproject, dim3(total_tiles), dim3(kThreads), 0, s, d_frames, d_blk);
+  hipLaunchKernelGGL(k_integral_x, dim3(total_x), dim3(kThreads), 0, s, d_frames, d_x);
+  hipLaunchKernelGGL(k_integral_y, dim3(total_y), dim3(kThreads), 0, s, d_frames, d_y);
+  hipLaunchKernelGGL(k_stats, dim3(total_tiles), dim3(kThreads), 0, s, d_frames, d_blk, sp);
+  HIPCHK(hipGetLastError());
+  return NICP_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// registration workspace shared by single-pair and batched calls
+// ---------------------------------------------------------------------------------------
+struct PairPlan {
+  std::vector<PairDev> pairs;
+  std::vector<int> blk_pair, blk_ref, blk_cur;  // block maps
+  int total_pix_tiles = 0, total_rtiles = 0, total_ctiles = 0;
+  size_t total_pix = 0;
+};
+
+// hosts frames[] (host copies) and pair list -> plan with block maps and buffer offsets
+void make_plan(const std::vector<FrameDev> &frames, int num_pairs, const int *ref, const int *cur,
+               const nicp_camera *const *cams, PairPlan &plan) {
+  plan.pairs.resize(num_pairs);
+  plan.blk_pair.clear();
+  plan.blk_ref.clear();
+  plan.blk_cur.clear();
+  size_t pix = 0;
+  for (int i = 0; i < num_pairs; ++i) {
+    PairDev &pd = plan.pairs[i];
+    const FrameDev &fr = frames[ref[i]], &fc = frames[cur[i]];
+    pd.ref = ref[i];
+    pd.cur = cur[i];
+    pd.rows = fc.rows;
+    pd.cols = fc.cols;
+    const int P = pd.rows * pd.cols;
+    pd.n_tiles = (P + kTile - 1) / kTile;
+    pd.tile_base = (int)plan.blk_pair.size();
+    for (int t = 0; t < pd.n_tiles; ++t) plan.blk_pair.push_back(i);
+    pd.n_rtiles = (fr.rows * fr.cols + kTile - 1) / kTile;
+    pd.rtile_base = (int)plan.blk_ref.size();
+    for (int t = 0; t < pd.n_rtiles; ++t) plan.blk_ref.push_back(i);
+    pd.n_ctiles = (fc.rows * fc.cols + kTile - 1) / kTile;
+    pd.ctile_base = (int)plan.blk_cur.size();
+    for (int t = 0; t < pd.n_ctiles; ++t) plan.blk_cur.push_back(i);
+    for (int k = 0; k < 9; ++k) pd.K[k] = cams[i]->K[k];
+    pd.min_d = cams[i]->min_distance;
+    pd.max_d = cams[i]->max_distance;
+    pd.zbuf = nullptr;
+    // pixel offset of this pair (aligned to 64 entries)
+    pd.zbuf = (unsigned long long *)(uintptr_t)pix;  // placeholder: offset, fixed up later
+    pix += align_up((size_t)P, 64);
+  }
+  plan.total_pix_tiles = (int)plan.blk_pair.size();
+  plan.total_rtiles = (int)plan.blk_ref.size();
+  plan.total_ctiles = (int)plan.blk_cur.size();
+  plan.total_pix = pix;
+}
+
+struct AlignBuffers {
+  DevBuf *pairs_tbl, *maps, *zbuf, *cur_index, *pairs_buf, *state, *partials, *result;
+};
+
+// allocates/assigns the per-pair buffers and uploads tables and the initial state
+int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *init16, int **d_blk_pair,
+                  int **d_blk_ref, int **d_blk_cur, PairDev **d_pairs) {
+  const int np = (int)plan.pairs.size();
+  CHK(B.zbuf->reserve(plan.total_pix * sizeof(unsigned long long)));
+  CHK(B.cur_index->reserve(plan.total_pix * sizeof(int)));
+  CHK(B.pairs_buf->reserve(plan.total_pix * sizeof(int2)));
+  CHK(B.state->reserve((size_t)np * kStateFloats * sizeof(float)));
+  CHK(B.partials->reserve((size_t)std::max(plan.total_pix_tiles, 1) * kPartial * sizeof(double)));
+  CHK(B.result->reserve((size_t)np * 2 * kPartial * sizeof(double)));
+  size_t tile_part = 0;
+  std::vector<float> st((size_t)np * kStateFloats, 0.0f);
+  for (int i = 0; i < np; ++i) {
+    PairDev &pd = plan.pairs[i];
+    size_t off = (size_t)(uintptr_t)pd.zbuf;
+    pd.zbuf = B.zbuf->as<unsigned long long>() + off;
+    pd.cur_index = B.cur_index->as<int>() + off;
+    pd.pairs = B.pairs_buf->as<int2>() + off;
+    pd.state = B.state->as<float>() + (size_t)i * kStateFloats;
+    pd.partials = B.partials->as<double>() + tile_part * kPartial;
+    tile_part += pd.n_tiles;
+    pd.result = B.result->as<double>() + (size_t)i * 2 * kPartial;
+    // initial state: AlignerProjective::align, alignerprojective.cpp:36-54
+    float *S = &st[(size_t)i * kStateFloats];
+    float T[16];
+    for (int k = 0; k < 16; ++k) T[k] = init16 ? init16[(size_t)i * 16 + k] : (k % 5 == 0 ? 1.0f : 0.0f);
+    set_row3(T);
+    float Ti[16], I4[16], TR[16], KRt[16], iKRt[16];
+    iso_inverse(T, Ti);
+    set_identity4(I4);
+    iso_mul(T, I4, TR);
+    projector_matrices(pd.K, TR, KRt, iKRt);
+    for (int k = 0; k < 16; ++k) {
+      S[kSlotT + k] = T[k];
+      S[kSlotTinv + k] = Ti[k];
+      S[kSlotKRt + k] = KRt[k];
+    }
+    projector_matrices(pd.K, I4, KRt, iKRt);
+    for (int k = 0; k < 16; ++k) S[kSlotKRtCur + k] = KRt[k];
+  }
+  const size_t nb = plan.blk_pair.size() + plan.blk_ref.size() + plan.blk_cur.size();
+  CHK(B.pairs_tbl->reserve((size_t)np * sizeof(PairDev)));
+  CHK(B.maps->reserve(std::max<size_t>(nb, 1) * sizeof(int)));
+  std::vector<int> maps;
+  maps.reserve(nb);
+  maps.insert(maps.end(), plan.blk_pair.begin(), plan.blk_pair.end());
+  maps.insert(maps.end(), plan.blk_ref.begin(), plan.blk_ref.end());
+  maps.insert(maps.end(), plan.blk_cur.begin(), plan.blk_cur.end());
+  *d_blk_pair = B.maps->as<int>();
+  *d_blk_ref = *d_blk_pair + plan.blk_pair.size();
+  *d_blk_cur = *d_blk_ref + plan.blk_ref.size();
+  *d_pairs = B.pairs_tbl->as<PairDev>();
+  HIPCHK(hipMemcpyAsync(*d_pairs, plan.pairs.data(), (size_t)np * sizeof(PairDev), hipMemcpyHostToDevice, s));
+  if (nb) HIPCHK(hipMemcpyAsync(*d_blk_pair, maps.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(B.state->p, st.data(), st.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  return NICP_OK;
+}
+
+// the launch sequence of AlignerProjective::align for all pairs of a plan
+int launch_align(hipStream_t s, const PairPlan &plan, const PairDev *d_pairs, const FrameDev *d_frames,
+                 const int *d_blk_pair, const int *d_blk_ref, const int *d_blk_cur, const nicp_align_params *ap) {
+  const int np = (int)plan.pairs.size();
+  const AlignParamsDev a = to_dev(ap);
+  const dim3 tb(kThreads);
+  if (plan.total_pix_tiles == 0) return NICP_OK;
+  hipLaunchKernelGGL(k_zfill, dim3(plan.total_pix_tiles), tb, 0, s, d_pairs, d_blk_pair);
+  if (plan.total_ctiles)
+    hipLaunchKernelGGL(k_zproject, dim3(plan.total_ctiles), tb, 0, s, d_pairs, d_frames, d_blk_cur, 1);
+  hipLaunchKernelGGL(k_zresolve, dim3(plan.total_pix_tiles), tb, 0, s, d_pairs, d_blk_pair, (float *)nullptr);
+  const int inner = ap->inner_iterations;
+  for (int it = 0; it < ap->outer_iterations; ++it) {
+    const bool last = it == ap->outer_iterations - 1;
+    if (plan.total_rtiles)
+      hipLaunchKernelGGL(k_zproject, dim3(plan.total_rtiles), tb, 0, s, d_pairs, d_frames, d_blk_ref, 0);
+    hipLaunchKernelGGL(k_corr_lin, dim3(plan.total_pix_tiles), tb, 0, s, d_pairs, d_frames, d_blk_pair, a,
+                       (int)(last || inner > 1));
+    hipLaunchKernelGGL(k_solve, dim3(np), dim3(64), 0, s, d_pairs, ap->lambda, inner > 1 ? 1 : 0, 0);
+    for (int k = 1; k < inner; ++k) {
+      hipLaunchKernelGGL(k_lin_pairs, dim3(plan.total_pix_tiles), tb, 0, s, d_pairs, d_frames, d_blk_pair, a,
+                         kSlotInvTInner, 0);
+      hipLaunchKernelGGL(k_solve, dim3(np), dim3(64), 0, s, d_pairs, ap->lambda, k == inner - 1 ? 0 : 1, 1);
+    }
+  }
+  // Aligner::_computeStatistics: one more linearisation at the final T over the last correspondences
+  if (ap->outer_iterations > 0) {
+    hipLaunchKernelGGL(k_lin_pairs, dim3(plan.total_pix_tiles), tb, 0, s, d_pairs, d_frames, d_blk_pair, a,
+                       kSlotTinv, 0);
+    hipLaunchKernelGGL(k_solve, dim3(np), dim3(64), 0, s, d_pairs, 0.0f, 2, 0);
+  }
+  HIPCHK(hipGetLastError());
+  return NICP_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// FP64 solution statistics (Aligner::_computeStatistics + unscented.h:11-65)
+// ---------------------------------------------------------------------------------------
+// symmetric eigen-decomposition by cyclic Jacobi rotations (ascending eigenvalues)
+void sym_eigen(int n, const double *Ain, double *w, double *V) {
+  double a[36];
+  for (int i = 0; i < n * n; ++i) a[i] = Ain[i];
+  for (int i = 0; i < n * n; ++i) V[i] = 0.0;
+  for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int c = 0; c < n; ++c)
+      for (int r = 0; r < n; ++r) (r == c ? diag : off) += a[c * n + r] * a[c * n + r];
+    if (off <= 1e-32 * diag || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a[q * n + p];
+        if (apq == 0.0) continue;
+        const double th = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+        const double t = (th >= 0.0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double x = a[p * n + k], y = a[q * n + k];
+          a[p * n + k] = c * x - sn * y;
+          a[q * n + k] = sn * x + c * y;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double x = a[k * n + p], y = a[k * n + q];
+          a[k * n + p] = c * x - sn * y;
+          a[k * n + q] = sn * x + c * y;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double x = V[p * n + k], y = V[q * n + k];
+          V[p * n + k] = c * x - sn * y;
+          V[q * n + k] = sn * x + c * y;
+        }
+      }
+  }
+  for (int i = 0; i < n; ++i) w[i] = a[i * n + i];
+  // selection sort, ascending
+  for (int i = 0; i < n; ++i) {
+    int m = i;
+    for (int j = i + 1; j < n; ++j)
+      if (w[j] < w[m]) m = j;
+    if (m != i) {
+      std::swap(w[i], w[m]);
+      for (int k = 0; k < n; ++k) std::swap(V[i * n + k], V[m * n + k]);
+    }
+  }
+}
+
+bool cholesky6(const double *A, double *L) {
+  for (int i = 0; i < 36; ++i) L[i] = 0.0;
+  for (int j = 0; j < 6; ++j) {
+    double d = A[j * 6 + j];
+    for (int k = 0; k < j; ++k) d -= L[k * 6 + j] * L[k * 6 + j];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    L[j * 6 + j] = d;
+    for (int i = j + 1; i < 6; ++i) {
+      double v = A[j * 6 + i];
+      for (int k = 0; k < j; ++k) v -= L[k * 6 + i] * L[k * 6 + j];
+      L[j * 6 + i] = v / d;
+    }
+  }
+  return true;
+}
+
+// inverse of a symmetric positive definite 6x6 through its eigen-decomposition
+bool spd_inverse6(const double *A, double *Ai) {
+  double w[6], V[36];
+  sym_eigen(6, A, w, V);
+  if (!(w[0] > 0.0)) return false;
+  for (int c = 0; c < 6; ++c)
+    for (int r = 0; r < 6; ++r) {
+      double s = 0.0;
+      for (int k = 0; k < 6; ++k) s += V[k * 6 + r] * V[k * 6 + c] / w[k];
+      Ai[c * 6 + r] = s;
+    }
+  return true;
+}
+
+double block_ratio(const double *O, int off) {
+  double a[9], w[3], V[9];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) a[c * 3 + r] = O[(off + c) * 6 + off + r];
+  sym_eigen(3, a, w, V);
+  double s[3] = {std::fabs(w[0]), std::fabs(w[1]), std::fabs(w[2])};
+  std::sort(s, s + 3);
+  return s[2] / s[0];
+}
+
+void solution_statistics(const double *H, const float *Tf, const nicp_align_params *ap, nicp_align_result *res) {
+  // localSigma = pinv(H) with JacobiSVD's default threshold for Matrix6f (6 * FLT_EPSILON * s_max)
+  double w[6], V[36], Sigma[36];
+  sym_eigen(6, H, w, V);
+  double smax = 0.0;
+  for (int i = 0; i < 6; ++i) smax = std::max(smax, std::fabs(w[i]));
+  const double thr = std::max(smax * 6.0 * (double)FLT_EPSILON, 2.2250738585072014e-308);
+  for (int i = 0; i < 36; ++i) Sigma[i] = 0.0;
+  bool zero = true;
+  for (int i = 0; i < 36; ++i) zero = zero && H[i] == 0.0;
+  for (int k = 0; k < 6; ++k) {
+    if (std::fabs(w[k]) < thr) continue;
+    for (int c = 0; c < 6; ++c)
+      for (int r = 0; r < 6; ++r) Sigma[c * 6 + r] += V[k * 6 + r] * V[k * 6 + c] / w[k];
+  }
+  // unscented transform: 2 dim + 1 sigma points (alpha 1e-3, beta 2)
+  const int dim = 6;
+  const double alpha = 1e-3, beta = 2.0, lam = alpha * alpha * dim;
+  double C[36], L[36];
+  for (int i = 0; i < 36; ++i) C[i] = Sigma[i] * (dim + lam);
+  const bool chol = cholesky6(C, L);
+  double smp[13][6], wi[13], wp[13];
+  for (int k = 0; k < 6; ++k) smp[0][k] = 0.0;
+  wi[0] = lam / (dim + lam);
+  wp[0] = lam / (dim + lam) + (1.0 - alpha * alpha + beta);
+  for (int i = 0; i < dim; ++i)
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      const int j = 1 + 2 * i + sgn;
+      for (int k = 0; k < 6; ++k) smp[j][k] = sgn ? -L[i * 6 + k] : L[i * 6 + k];
+      wi[j] = wp[j] = 1.0 / (2.0 * (dim + lam));
+    }
+  double T[16];
+  for (int i = 0; i < 16; ++i) T[i] = Tf[i];
+  for (int j = 0; j < 13; ++j) {
+    double X[16], Y[16];
+    v2t<double>(smp[j], X);
+    // Y = T * X^-1
+    double Ri[9], ti[3];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) Ri[c * 3 + r] = X[r * 4 + c];
+    for (int r = 0; r < 3; ++r) ti[r] = -(Ri[r] * X[12] + Ri[3 + r] * X[13] + Ri[6 + r] * X[14]);
+    for (int i = 0; i < 16; ++i) Y[i] = 0.0;
+    Y[15] = 1.0;
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) Y[c * 4 + r] = T[r] * Ri[c * 3] + T[4 + r] * Ri[c * 3 + 1] + T[8 + r] * Ri[c * 3 + 2];
+      Y[12 + r] = T[r] * ti[0] + T[4 + r] * ti[1] + T[8 + r] * ti[2] + T[12 + r];
+    }
+    t2v<double>(Y, smp[j]);
+  }
+  double mean[6] = {0, 0, 0, 0, 0, 0}, cov[36];
+  for (int j = 0; j < 13; ++j)
+    for (int k = 0; k < 6; ++k) mean[k] += wi[j] * smp[j][k];
+  for (int i = 0; i < 36; ++i) cov[i] = 0.0;
+  for (int j = 0; j < 13; ++j) {
+    double d[6];
+    for (int k = 0; k < 6; ++k) d[k] = smp[j][k] - mean[k];
+    for (int c = 0; c < 6; ++c)
+      for (int r = 0; r < 6; ++r) cov[c * 6 + r] += wp[j] * d[r] * d[c];
+  }
+  double Om[36];
+  const bool inv = spd_inverse6(cov, Om);
+  for (int c = 0; c < 6; ++c)
+    for (int r = 0; r < 6; ++r) res->omega[c * 6 + r] = inv ? 0.5 * (Om[c * 6 + r] + Om[r * 6 + c]) : 0.0;
+  for (int k = 0; k < 6; ++k) res->mean[k] = mean[k];
+  res->translational_ratio = inv ? block_ratio(res->omega, 0) : INFINITY;
+  res->rotational_ratio = inv ? block_ratio(res->omega, 3) : INFINITY;
+  const bool finite = std::isfinite(res->translational_ratio) && std::isfinite(res->rotational_ratio);
+  res->solution_valid = !zero && chol && inv && finite &&
+                        !(res->rotational_ratio > ap->rotational_min_eigen_ratio) &&
+                        !(res->translational_ratio > ap->translational_min_eigen_ratio);
+}
+
+void fill_result(const double *r, const float *T, const nicp_align_params *ap, nicp_align_result *out,
+                 float ms) {
+  // r[0..31]: last in-loop linearisation; r[32..63]: statistics linearisation
+  double H[36];
+  int e = 0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) {
+      H[j * 6 + i] = H[i * 6 + j] = r[e];
+      ++e;
+    }
+  memcpy(out->H, H, sizeof(H));
+  for (int i = 0; i < 6; ++i) out->b[i] = r[21 + i];
+  out->error = (float)r[27];
+  out->inliers = (int)llround(r[28]);
+  out->num_correspondences = (int)llround(r[29]);
+  double Hs[36];
+  e = 0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) {
+      Hs[j * 6 + i] = Hs[i * 6 + j] = r[kPartial + e];
+      ++e;
+    }
+  out->total_time_ms = ms;
+  if (ap->outer_iterations > 0) {
+    solution_statistics(Hs, T, ap, out);
+  } else {
+    memset(out->mean, 0, sizeof(out->mean));
+    memset(out->omega, 0, sizeof(out->omega));
+    out->translational_ratio = out->rotational_ratio = INFINITY;
+    out->solution_valid = 0;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------------------
+extern "C" int nicp_ctx_create(int device, nicp_ctx **out) {
+  if (!out) return set_err(NICP_ERR_INVALID, "out is null");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return set_err(NICP_ERR_INVALID, "device %d out of range (%d devices)", device, n);
+  HIPCHK(hipSetDevice(device));
+  nicp_ctx *c = new (std::nothrow) nicp_ctx();
+  if (!c) return set_err(NICP_ERR_OOM, "out of host memory");
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) {
+    delete c;
+    return set_err(NICP_ERR_HIP, "stream/event creation: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return NICP_OK;
+}
+
+extern "C" int nicp_ctx_destroy(nicp_ctx *c) {
+  if (!c) return NICP_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  DevBuf *bufs[] = {&c->frames_tbl, &c->pairs_tbl, &c->maps, &c->zbuf, &c->cur_index, &c->pairs_buf,
+                    &c->state, &c->partials, &c->result, &c->depth_tmp, &c->scratch_in};
+  for (DevBuf *b : bufs) b->release();
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return NICP_OK;
+}
+
+extern "C" int nicp_ctx_synchronize(nicp_ctx *c) {
+  if (!c) return set_err(NICP_ERR_INVALID, "ctx is null");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NICP_OK;
+}
+
+extern "C" void *nicp_ctx_stream(nicp_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+// ---------------------------------------------------------------------------------------
+// clouds
+// ---------------------------------------------------------------------------------------
+extern "C" int nicp_cloud_from_depth_u16(nicp_ctx *ctx, const uint16_t *depth, int rows, int cols, float depth_scale,
+                                         const nicp_camera *cam, const nicp_stats_params *sp, int keep_stats,
+                                         nicp_cloud **out) {
+  if (!ctx || !depth || !sp || !out) return set_err(NICP_ERR_INVALID, "null argument");
+  if (rows <= 0 || cols <= 0 || (long long)rows * cols > (1ll << 28))
+    return set_err(NICP_ERR_INVALID, "bad image size %dx%d", rows, cols);
+  CHK(check_camera(cam));
+  HIPCHK(hipSetDevice(ctx->device));
+  nicp_cloud *cl = new (std::nothrow) nicp_cloud();
+  if (!cl) return set_err(NICP_ERR_OOM, "out of host memory");
+  cl->ctx = ctx;
+  FrameLayout L = frame_layout(rows, cols, keep_stats != 0, true, 0);
+  void *mem = nullptr;
+  hipError_t e = hipMalloc(&mem, L.total);
+  if (e != hipSuccess) {
+    delete cl;
+    return set_err(NICP_ERR_OOM, "hipMalloc(%zu): %s", L.total, hipGetErrorString(e));
+  }
+  cl->alloc = mem;
+  FrameDev &f = cl->fd;
+  frame_geometry(f, rows, cols, cam, sp->world_radius, depth_scale);
+  frame_pointers(f, (char *)mem, L, keep_stats != 0, true);
+  f.tile_base = 0;
+  f.xblk_base = 0;
+  f.yblk_base = 0;
+  cl->has_index = 1;
+  cl->has_stats = keep_stats != 0;
+  hipStream_t s = ctx->stream;
+  int rc = NICP_OK;
+  do {
+    if ((rc = ctx->frames_tbl.reserve(sizeof(FrameDev))) != NICP_OK) break;
+    const int nt = f.n_tiles, nx = xblocks(rows), ny = yblocks(cols);
+    if ((rc = ctx->maps.reserve((size_t)(nt + nx + ny) * sizeof(int))) != NICP_OK) break;
+    std::vector<int> zeros((size_t)(nt + nx + ny), 0);
+    FrameDev *d_f = ctx->frames_tbl.as<FrameDev>();
+    int *d_maps = ctx->maps.as<int>();
+    e = hipMemcpyAsync((void *)f.depth, depth, (size_t)rows * cols * sizeof(uint16_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_f, &f, sizeof(FrameDev), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_maps, zeros.data(), zeros.size() * sizeof(int), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) { rc = set_err(NICP_ERR_HIP, "upload: %s", hipGetErrorString(e)); break; }
+    rc = launch_convert(s, d_f, 1, d_maps, nt, d_maps + nt, nx, d_maps + nt + nx, ny, to_dev(sp, keep_stats != 0));
+    if (rc != NICP_OK) break;
+    e = hipMemcpyAsync(&cl->n, f.npoints, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { rc = set_err(NICP_ERR_HIP, "conversion: %s", hipGetErrorString(e)); break; }
+  } while (0);
+  if (rc != NICP_OK) {
+    (void)hipFree(mem);
+    delete cl;
+    return rc;
+  }
+  *out = cl;
+  return NICP_OK;
+}
+
+extern "C" int nicp_cloud_from_host(nicp_ctx *ctx, int n, const float *points4, const float *normals4,
+                                    const float *curvature, const float *omega_p16, const float *omega_n16,
+                                    nicp_cloud **out) {
+  if (!ctx || !out || (n > 0 && !points4) || n < 0) return set_err(NICP_ERR_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(ctx->device));
+  nicp_cloud *cl = new (std::nothrow) nicp_cloud();
+  if (!cl) return set_err(NICP_ERR_OOM, "out of host memory");
+  cl->ctx = ctx;
+  Carver cv;
+  const size_t cap = std::max(n, 1);
+  size_t o_pts = cv.take<float4>(cap), o_nrm = cv.take<float4>(cap), o_omg = cv.take<float4>(cap * 3),
+         o_np = cv.take<int>(1);
+  void *mem = nullptr;
+  hipError_t e = hipMalloc(&mem, cv.off);
+  if (e != hipSuccess) {
+    delete cl;
+    return set_err(NICP_ERR_OOM, "hipMalloc: %s", hipGetErrorString(e));
+  }
+  cl->alloc = mem;
+  cl->n = n;
+  FrameDev &f = cl->fd;
+  memset(&f, 0, sizeof(f));
+  f.pts = (float4 *)((char *)mem + o_pts);
+  f.nrm = (float4 *)((char *)mem + o_nrm);
+  f.omg = (float4 *)((char *)mem + o_omg);
+  f.npoints = (int *)((char *)mem + o_np);
+  std::vector<float4> nrm(cap), omg(cap * 3);
+  for (int i = 0; i < n; ++i) {
+    const float *nv = normals4 ? normals4 + 4 * (size_t)i : nullptr;
+    nrm[i] = make_float4(nv ? nv[0] : 0.0f, nv ? nv[1] : 0.0f, nv ? nv[2] : 0.0f, curvature ? curvature[i] : 0.0f);
+    float op[6] = {0, 0, 0, 0, 0, 0}, on[6] = {0, 0, 0, 0, 0, 0};
+    const int up[6] = {0, 4, 8, 5, 9, 10};
+    for (int k = 0; k < 6; ++k) {
+      if (omega_p16) op[k] = omega_p16[16 * (size_t)i + up[k]];
+      if (omega_n16) on[k] = omega_n16[16 * (size_t)i + up[k]];
+    }
+    omg[3 * i] = make_float4(op[0], op[1], op[2], op[3]);
+    omg[3 * i + 1] = make_float4(op[4], op[5], on[0], on[1]);
+    omg[3 * i + 2] = make_float4(on[2], on[3], on[4], on[5]);
+  }
+  hipStream_t s = ctx->stream;
+  if (n > 0) {
+    e = hipMemcpyAsync(f.pts, points4, (size_t)n * sizeof(float4), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(f.nrm, nrm.data(), (size_t)n * sizeof(float4), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(f.omg, omg.data(), (size_t)n * 3 * sizeof(float4), hipMemcpyHostToDevice, s);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(f.npoints, &cl->n, sizeof(int), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(mem);
+    delete cl;
+    return set_err(NICP_ERR_HIP, "upload: %s", hipGetErrorString(e));
+  }
+  *out = cl;
+  return NICP_OK;
+}
+
+extern "C" int nicp_cloud_destroy(nicp_cloud *cl) {
+  if (!cl) return NICP_OK;
+  if (cl->alloc) {
+    (void)hipSetDevice(cl->ctx->device);
+    (void)hipStreamSynchronize(cl->ctx->stream);
+    (void)hipFree(cl->alloc);
+  }
+  delete cl;
+  return NICP_OK;
+}
+
+extern "C" int nicp_cloud_size(const nicp_cloud *cl, int *n) {
+  if (!cl || !n) return set_err(NICP_ERR_INVALID, "null argument");
+  *n = cl->n;
+  return NICP_OK;
+}
+
+extern "C" int nicp_cloud_download(nicp_cloud *cl, float *points4, float *normals4, float *curvature,
+                                   float *omega_p16, float *omega_n16) {
+  if (!cl) return set_err(NICP_ERR_INVALID, "cloud is null");
+  const int n = cl->n;
+  if (n == 0) return NICP_OK;
+  hipStream_t s = cl->ctx->stream;
+  HIPCHK(hipSetDevice(cl->ctx->device));
+  std::vector<float4> pts, nrm, omg;
+  if (points4) HIPCHK(hipMemcpyAsync(points4, cl->fd.pts, (size_t)n * sizeof(float4), hipMemcpyDeviceToHost, s));
+  if (normals4 || curvature) {
+    nrm.resize(n);
+    HIPCHK(hipMemcpyAsync(nrm.data(), cl->fd.nrm, (size_t)n * sizeof(float4), hipMemcpyDeviceToHost, s));
+  }
+  if (omega_p16 || omega_n16) {
+    omg.resize((size_t)n * 3);
+    HIPCHK(hipMemcpyAsync(omg.data(), cl->fd.omg, (size_t)n * 3 * sizeof(float4), hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  for (int i = 0; i < n; ++i) {
+    if (normals4) {
+      normals4[4 * (size_t)i] = nrm[i].x;
+      normals4[4 * (size_t)i + 1] = nrm[i].y;
+      normals4[4 * (size_t)i + 2] = nrm[i].z;
+      normals4[4 * (size_t)i + 3] = 0.0f;
+    }
+    if (curvature) curvature[i] = nrm[i].w;
+    const float *o = (const float *)&omg[3 * (size_t)i];
+    for (int which = 0; which < 2; ++which) {
+      float *O = which ? omega_n16 : omega_p16;
+      if (!O) continue;
+      O += 16 * (size_t)i;
+      const float *u = o + 6 * which;
+      for (int k = 0; k < 16; ++k) O[k] = 0.0f;
+      O[0] = u[0]; O[4] = O[1] = u[1]; O[8] = O[2] = u[2];
+      O[5] = u[3]; O[9] = O[6] = u[4]; O[10] = u[5];
+    }
+  }
+  return NICP_OK;
+}
+
+extern "C" int nicp_cloud_download_stats(nicp_cloud *cl, float *eigvecs9, float *eigvals3, float *means4,
+                                         int *npts) {
+  if (!cl) return set_err(NICP_ERR_INVALID, "cloud is null");
+  if (!cl->has_stats || !cl->fd.stats) return set_err(NICP_ERR_STATE, "the cloud was converted without keep_stats");
+  const int n = cl->n;
+  std::vector<float4> st((size_t)n * 4);
+  HIPCHK(hipSetDevice(cl->ctx->device));
+  if (n) HIPCHK(hipMemcpyAsync(st.data(), cl->fd.stats, st.size() * sizeof(float4), hipMemcpyDeviceToHost, cl->ctx->stream));
+  HIPCHK(hipStreamSynchronize(cl->ctx->stream));
+  for (int i = 0; i < n; ++i) {
+    const float *s = (const float *)&st[4 * (size_t)i];
+    if (eigvecs9)
+      for (int k = 0; k < 9; ++k) eigvecs9[9 * (size_t)i + k] = s[k];
+    if (eigvals3)
+      for (int k = 0; k < 3; ++k) eigvals3[3 * (size_t)i + k] = s[9 + k];
+    const int cnt = __builtin_bit_cast(int, s[15]);
+    if (means4) {
+      means4[4 * (size_t)i] = s[12];
+      means4[4 * (size_t)i + 1] = s[13];
+      means4[4 * (size_t)i + 2] = s[14];
+      means4[4 * (size_t)i + 3] = cnt > 0 ? (float)cnt * (1.0f / (float)cnt) : 1.0f;
+    }
+    if (npts) npts[i] = cnt;
+  }
+  return NICP_OK;
+}
+
+extern "C" int nicp_cloud_index_image(nicp_cloud *cl, int *index_image) {
+  if (!cl || !index_image) return set_err(NICP_ERR_INVALID, "null argument");
+  if (!cl->has_index) return set_err(NICP_ERR_STATE, "the cloud has no index image");
+  HIPCHK(hipSetDevice(cl->ctx->device));
+  HIPCHK(hipMemcpyAsync(index_image, cl->fd.index, (size_t)cl->fd.rows * cl->fd.cols * sizeof(int),
+                        hipMemcpyDeviceToHost, cl->ctx->stream));
+  HIPCHK(hipStreamSynchronize(cl->ctx->stream));
+  return NICP_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// single-pair operations
+// ---------------------------------------------------------------------------------------
+namespace {
+
+AlignBuffers ctx_buffers(nicp_ctx *c) {
+  return AlignBuffers{&c->pairs_tbl, &c->maps, &c->zbuf, &c->cur_index, &c->pairs_buf, &c->state, &c->partials,
+                      &c->result};
+}
+
+// uploads a frame table of the given clouds
+int upload_frames(nicp_ctx *c, std::vector<FrameDev> &frames, FrameDev **d) {
+  CHK(c->frames_tbl.reserve(frames.size() * sizeof(FrameDev)));
+  *d = c->frames_tbl.as<FrameDev>();
+  HIPCHK(hipMemcpyAsync(*d, frames.data(), frames.size() * sizeof(FrameDev), hipMemcpyHostToDevice, c->stream));
+  return NICP_OK;
+}
+
+}  // namespace
+
+extern "C" int nicp_project(nicp_ctx *ctx, const nicp_cloud *cloud, const float transform[16], const nicp_camera *cam,
+                            int rows, int cols, int *index_image, float *depth_image) {
+  if (!ctx || !cloud || !transform || !index_image) return set_err(NICP_ERR_INVALID, "null argument");
+  if (rows <= 0 || cols <= 0) return set_err(NICP_ERR_INVALID, "bad image size");
+  CHK(check_camera(cam));
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<FrameDev> frames(1, cloud->fd);
+  frames[0].rows = rows;  // projection capacity follows the cloud size
+  frames[0].cols = std::max(1, cloud->n);
+  FrameDev *d_frames;
+  CHK(upload_frames(ctx, frames, &d_frames));
+  int zero = 0;
+  const nicp_camera *cams[1] = {cam};
+  PairPlan plan;
+  // a pair whose "current" frame is the cloud itself and whose image is rows x cols
+  frames[0].rows = 1;
+  frames[0].cols = std::max(1, cloud->n);
+  make_plan(frames, 1, &zero, &zero, cams, plan);
+  PairDev &pd = plan.pairs[0];
+  pd.rows = rows;
+  pd.cols = cols;
+  const int P = rows * cols;
+  pd.n_tiles = (P + kTile - 1) / kTile;
+  plan.blk_pair.assign(pd.n_tiles, 0);
+  plan.total_pix_tiles = pd.n_tiles;
+  plan.total_pix = align_up((size_t)P, 64);
+  plan.blk_ref.clear();
+  plan.total_rtiles = 0;
+  pd.rtile_base = 0;
+  pd.n_rtiles = 0;
+  AlignBuffers B = ctx_buffers(ctx);
+  int *d_bp, *d_br, *d_bc;
+  PairDev *d_pairs;
+  CHK(prepare_align(ctx->stream, plan, B, transform, &d_bp, &d_br, &d_bc, &d_pairs));
+  // the current-projection slot must hold the transform's KRt (PinholePointProjector::setTransform)
+  {
+    float T[16], KRt[16], iKRt[16];
+    memcpy(T, transform, sizeof(T));
+    set_row3(T);
+    projector_matrices(cam->K, T, KRt, iKRt);
+    HIPCHK(hipMemcpyAsync(ctx->state.as<float>() + kSlotKRtCur, KRt, sizeof(KRt), hipMemcpyHostToDevice, ctx->stream));
+  }
+  CHK(ctx->depth_tmp.reserve((size_t)P * sizeof(float)));
+  const dim3 tb(kThreads);
+  hipLaunchKernelGGL(k_zfill, dim3(pd.n_tiles), tb, 0, ctx->stream, d_pairs, d_bp);
+  if (plan.total_ctiles)
+    hipLaunchKernelGGL(k_zproject, dim3(plan.total_ctiles), tb, 0, ctx->stream, d_pairs, d_frames, d_bc, 1);
+  hipLaunchKernelGGL(k_zresolve, dim3(pd.n_tiles), tb, 0, ctx->stream, d_pairs, d_bp, ctx->depth_tmp.as<float>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(index_image, ctx->cur_index.p, (size_t)P * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  if (depth_image)
+    HIPCHK(hipMemcpyAsync(depth_image, ctx->depth_tmp.p, (size_t)P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  (void)zero;
+  return NICP_OK;
+}
+
+extern "C" int nicp_correspondences(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const int *ref_index,
+                                    const int *cur_index, int rows, int cols, const float T[16],
+                                    const nicp_align_params *ap, int *pairs, int *num_pairs) {
+  if (!ctx || !ref || !cur || !ref_index || !cur_index || !T || !pairs || !num_pairs)
+    return set_err(NICP_ERR_INVALID, "null argument");
+  if (rows <= 0 || cols <= 0) return set_err(NICP_ERR_INVALID, "bad image size");
+  CHK(check_align_params(ap));
+  const int P = rows * cols;
+  for (int i = 0; i < P; ++i)
+    if (ref_index[i] >= ref->n || cur_index[i] >= cur->n)
+      return set_err(NICP_ERR_INVALID, "index image entry out of range at pixel %d", i);
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<FrameDev> frames = {ref->fd, cur->fd};
+  frames[0].rows = 1; frames[0].cols = std::max(1, ref->n);
+  frames[1].rows = rows; frames[1].cols = cols;
+  FrameDev *d_frames;
+  CHK(upload_frames(ctx, frames, &d_frames));
+  nicp_camera cam{};
+  cam.K[0] = cam.K[4] = cam.K[8] = 1.0f;
+  cam.min_distance = 1.0f;
+  cam.max_distance = 1.0f;
+  const nicp_camera *cams[1] = {&cam};
+  int r0 = 0, c1 = 1;
+  PairPlan plan;
+  make_plan(frames, 1, &r0, &c1, cams, plan);
+  AlignBuffers B = ctx_buffers(ctx);
+  int *d_bp, *d_br, *d_bc;
+  PairDev *d_pairs;
+  CHK(prepare_align(ctx->stream, plan, B, nullptr, &d_bp, &d_br, &d_bc, &d_pairs));
+  // the finder's transform goes into the Tinv slot; index images into z-buffer / current index
+  HIPCHK(hipMemcpyAsync(ctx->state.as<float>() + kSlotTinv, T, 16 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  std::vector<unsigned long long> keys(P);
+  for (int i = 0; i < P; ++i) keys[i] = ref_index[i] < 0 ? kZInit : (unsigned long long)(unsigned int)ref_index[i];
+  HIPCHK(hipMemcpyAsync(ctx->zbuf.p, keys.data(), (size_t)P * sizeof(unsigned long long), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->cur_index.p, cur_index, (size_t)P * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_corr_lin, dim3(plan.total_pix_tiles), dim3(kThreads), 0, ctx->stream, d_pairs, d_frames, d_bp,
+                     to_dev(ap), 1);
+  HIPCHK(hipGetLastError());
+  std::vector<int2> pp(P);
+  HIPCHK(hipMemcpyAsync(pp.data(), ctx->pairs_buf.p, (size_t)P * sizeof(int2), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int n = 0;
+  for (int i = 0; i < P; ++i)
+    if (pp[i].x >= 0) {
+      pairs[2 * n] = pp[i].x;
+      pairs[2 * n + 1] = pp[i].y;
+      ++n;
+    }
+  *num_pairs = n;
+  return NICP_OK;
+}
+
+extern "C" int nicp_linearize(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const int *pairs,
+                              int num_pairs, const float T[16], const nicp_align_params *ap, double H[36],
+                              double b[6], double *error, int *inliers) {
+  if (!ctx || !ref || !cur || (!pairs && num_pairs > 0) || !T || num_pairs < 0)
+    return set_err(NICP_ERR_INVALID, "bad arguments");
+  CHK(check_align_params(ap));
+  for (int i = 0; i < num_pairs; ++i)
+    if (pairs[2 * i] < 0 || pairs[2 * i] >= ref->n || pairs[2 * i + 1] < 0 || pairs[2 * i + 1] >= cur->n)
+      return set_err(NICP_ERR_INVALID, "pair %d out of range", i);
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<FrameDev> frames = {ref->fd, cur->fd};
+  const int npx = std::max(num_pairs, 1);
+  frames[0].rows = 1; frames[0].cols = std::max(1, ref->n);
+  frames[1].rows = 1; frames[1].cols = npx;
+  FrameDev *d_frames;
+  CHK(upload_frames(ctx, frames, &d_frames));
+  nicp_camera cam{};
+  cam.K[0] = cam.K[4] = cam.K[8] = 1.0f;
+  cam.min_distance = cam.max_distance = 1.0f;
+  const nicp_camera *cams[1] = {&cam};
+  int r0 = 0, c1 = 1;
+  PairPlan plan;
+  make_plan(frames, 1, &r0, &c1, cams, plan);
+  AlignBuffers B = ctx_buffers(ctx);
+  int *d_bp, *d_br, *d_bc;
+  PairDev *d_pairs;
+  CHK(prepare_align(ctx->stream, plan, B, nullptr, &d_bp, &d_br, &d_bc, &d_pairs));
+  HIPCHK(hipMemcpyAsync(ctx->state.as<float>() + kSlotTinv, T, 16 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  std::vector<int2> pp(npx, make_int2(-1, -1));
+  for (int i = 0; i < num_pairs; ++i) pp[i] = make_int2(pairs[2 * i], pairs[2 * i + 1]);
+  HIPCHK(hipMemcpyAsync(ctx->pairs_buf.p, pp.data(), (size_t)npx * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_lin_pairs, dim3(plan.total_pix_tiles), dim3(kThreads), 0, ctx->stream, d_pairs, d_frames, d_bp,
+                     to_dev(ap), kSlotTinv, npx);
+  hipLaunchKernelGGL(k_solve, dim3(1), dim3(64), 0, ctx->stream, d_pairs, 0.0f, 2, 0);
+  HIPCHK(hipGetLastError());
+  double r[2 * kPartial];
+  HIPCHK(hipMemcpyAsync(r, ctx->result.p, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const double *t = r + kPartial;
+  int e = 0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) {
+      if (H) H[j * 6 + i] = H[i * 6 + j] = t[e];
+      ++e;
+    }
+  if (b)
+    for (int i = 0; i < 6; ++i) b[i] = t[21 + i];
+  if (error) *error = t[27];
+  if (inliers) *inliers = (int)llround(t[28]);
+  return NICP_OK;
+}
+
+extern "C" int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const nicp_camera *cam,
+                          int rows, int cols, const float init_guess[16], const nicp_align_params *ap, float T_out[16],
+                          nicp_align_result *result) {
+  if (!ctx || !ref || !cur) return set_err(NICP_ERR_INVALID, "null argument");
+  if (rows <= 0 || cols <= 0) return set_err(NICP_ERR_INVALID, "bad image size");
+  CHK(check_camera(cam));
+  CHK(check_align_params(ap));
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<FrameDev> frames = {ref->fd, cur->fd};
+  // projections cover the clouds' capacities and use the projector image rows x cols
+  frames[0].rows = 1; frames[0].cols = std::max(1, ref->n);
+  frames[1].rows = 1; frames[1].cols = std::max(1, cur->n);
+  FrameDev *d_frames;
+  CHK(upload_frames(ctx, frames, &d_frames));
+  const nicp_camera *cams[1] = {cam};
+  int r0 = 0, c1 = 1;
+  PairPlan plan;
+  make_plan(frames, 1, &r0, &c1, cams, plan);
+  PairDev &pd = plan.pairs[0];
+  pd.rows = rows;
+  pd.cols = cols;
+  pd.n_tiles = (rows * cols + kTile - 1) / kTile;
+  plan.blk_pair.assign(pd.n_tiles, 0);
+  plan.total_pix_tiles = pd.n_tiles;
+  plan.total_pix = align_up((size_t)rows * cols, 64);
+  AlignBuffers B = ctx_buffers(ctx);
+  int *d_bp, *d_br, *d_bc;
+  PairDev *d_pairs;
+  float I4[16];
+  set_identity4(I4);
+  CHK(prepare_align(ctx->stream, plan, B, init_guess ? init_guess : I4, &d_bp, &d_br, &d_bc, &d_pairs));
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  CHK(launch_align(ctx->stream, plan, d_pairs, d_frames, d_bp, d_br, d_bc, ap));
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  float T[16];
+  double r[2 * kPartial];
+  HIPCHK(hipMemcpyAsync(T, ctx->state.p, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(r, ctx->result.p, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (ap->outer_iterations == 0) memset(r, 0, sizeof(r));
+  if (T_out) memcpy(T_out, T, sizeof(T));
+  if (result) fill_result(r, T, ap, result, ms);
+  ctx->last_rows = rows;
+  ctx->last_cols = cols;
+  ctx->last_valid = ap->outer_iterations > 0;
+  return NICP_OK;
+}
+
+extern "C" int nicp_last_correspondences(nicp_ctx *ctx, int *pairs, int capacity, int *num_pairs) {
+  if (!ctx || !num_pairs) return set_err(NICP_ERR_INVALID, "null argument");
+  if (!ctx->last_valid) return set_err(NICP_ERR_STATE, "no alignment has been run on this context");
+  const int P = ctx->last_rows * ctx->last_cols;
+  std::vector<int2> pp(P);
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipMemcpyAsync(pp.data(), ctx->pairs_buf.p, (size_t)P * sizeof(int2), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int n = 0;
+  for (int i = 0; i < P; ++i)
+    if (pp[i].x >= 0) {
+      if (pairs && n < capacity) {
+        pairs[2 * n] = pp[i].x;
+        pairs[2 * n + 1] = pp[i].y;
+      }
+      ++n;
+    }
+  *num_pairs = n;
+  if (pairs && n > capacity) return set_err(NICP_ERR_INVALID, "capacity %d < %d correspondences", capacity, n);
+  return NICP_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------------------
+extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows, const int *cols,
+                                 const nicp_camera *cams, const nicp_stats_params *sp, nicp_batch **out) {
+  if (!ctx || num_frames <= 0 || !rows || !cols || !cams || !sp || !out)
+    return set_err(NICP_ERR_INVALID, "bad arguments");
+  for (int i = 0; i < num_frames; ++i) {
+    if (rows[i] <= 0 || cols[i] <= 0 || (long long)rows[i] * cols[i] > (1ll << 26))
+      return set_err(NICP_ERR_INVALID, "frame %d: bad size %dx%d", i, rows[i], cols[i]);
+    CHK(check_camera(&cams[i]));
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  nicp_batch *b = new (std::nothrow) nicp_batch();
+  if (!b) return set_err(NICP_ERR_OOM, "out of host memory");
+  b->ctx = ctx;
+  b->num_frames = num_frames;
+  b->frames.resize(num_frames);
+  b->npoints_host.assign(num_frames, 0);
+  b->sp = to_dev(sp, 0);
+  std::vector<FrameLayout> lay(num_frames);
+  size_t off = 0;
+  int tiles = 0, xb = 0, yb = 0;
+  std::vector<int> blk, xblk, yblk;
+  for (int i = 0; i < num_frames; ++i) {
+    lay[i] = frame_layout(rows[i], cols[i], false, true, off);
+    off = align_up(lay[i].total, 256);
+    FrameDev &f = b->frames[i];
+    frame_geometry(f, rows[i], cols[i], &cams[i], sp->world_radius, 0.001f);
+    f.tile_base = tiles;
+    f.xblk_base = xb;
+    f.yblk_base = yb;
+    tiles += f.n_tiles;
+    xb += xblocks(rows[i]);
+    yb += yblocks(cols[i]);
+    blk.insert(blk.end(), f.n_tiles, i);
+    xblk.insert(xblk.end(), xblocks(rows[i]), i);
+    yblk.insert(yblk.end(), yblocks(cols[i]), i);
+  }
+  Carver cv;
+  cv.off = off;
+  const size_t o_tbl = cv.take<FrameDev>(num_frames);
+  const size_t o_blk = cv.take<int>(blk.size());
+  const size_t o_x = cv.take<int>(xblk.size());
+  const size_t o_y = cv.take<int>(yblk.size());
+  hipError_t e = hipMalloc(&b->arena, cv.off);
+  if (e != hipSuccess) {
+    delete b;
+    return set_err(NICP_ERR_OOM, "hipMalloc(%zu): %s", cv.off, hipGetErrorString(e));
+  }
+  char *base = (char *)b->arena;
+  for (int i = 0; i < num_frames; ++i) frame_pointers(b->frames[i], base, lay[i], false, true);
+  b->d_frames = (FrameDev *)(base + o_tbl);
+  b->d_blk_frame = (int *)(base + o_blk);
+  b->d_xblk_frame = (int *)(base + o_x);
+  b->d_yblk_frame = (int *)(base + o_y);
+  b->total_tiles = tiles;
+  b->total_xblk = xb;
+  b->total_yblk = yb;
+  hipStream_t s = ctx->stream;
+  e = hipMemcpyAsync(b->d_frames, b->frames.data(), num_frames * sizeof(FrameDev), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->d_blk_frame, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->d_xblk_frame, xblk.data(), xblk.size() * sizeof(int), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->d_yblk_frame, yblk.data(), yblk.size() * sizeof(int), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(b->arena);
+    delete b;
+    return set_err(NICP_ERR_HIP, "batch upload: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return NICP_OK;
+}
+
+extern "C" int nicp_batch_destroy(nicp_batch *b) {
+  if (!b) return NICP_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  b->graph.reset();
+  b->conv_graph.reset();
+  DevBuf *bufs[] = {&b->pairs_tbl, &b->maps, &b->zbuf, &b->cur_index, &b->pairs_buf, &b->state, &b->partials, &b->result};
+  for (DevBuf *x : bufs) x->release();
+  (void)hipFree(b->arena);
+  delete b;
+  return NICP_OK;
+}
+
+extern "C" int nicp_batch_upload_depth(nicp_batch *b, int frame, const uint16_t *depth, float depth_scale) {
+  if (!b || !depth || frame < 0 || frame >= b->num_frames) return set_err(NICP_ERR_INVALID, "bad arguments");
+  FrameDev &f = b->frames[frame];
+  HIPCHK(hipSetDevice(b->ctx->device));
+  if (f.depth_scale != depth_scale) {
+    f.depth_scale = depth_scale;
+    HIPCHK(hipMemcpyAsync(&b->d_frames[frame], &f, sizeof(FrameDev), hipMemcpyHostToDevice, b->ctx->stream));
+  }
+  HIPCHK(hipMemcpyAsync((void *)f.depth, depth, (size_t)f.rows * f.cols * sizeof(uint16_t), hipMemcpyHostToDevice,
+                        b->ctx->stream));
+  HIPCHK(hipStreamSynchronize(b->ctx->stream));
+  return NICP_OK;
+}
+
+extern "C" int nicp_batch_convert(nicp_batch *b) {
+  if (!b) return set_err(NICP_ERR_INVALID, "batch is null");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  CHK(launch_convert(b->ctx->stream, b->d_frames, b->num_frames, b->d_blk_frame, b->total_tiles, b->d_xblk_frame,
+                     b->total_xblk, b->d_yblk_frame, b->total_yblk, b->sp));
+  return NICP_OK;
+}
+
+extern "C" int nicp_batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *cur_frame,
+                                const float *init_guesses16, const nicp_align_params *ap, int use_graph, float *T_out16,
+                                nicp_align_result *results) {
+  if (!b || num_pairs <= 0 || !ref_frame || !cur_frame) return set_err(NICP_ERR_INVALID, "bad arguments");
+  CHK(check_align_params(ap));
+  for (int i = 0; i < num_pairs; ++i)
+    if (ref_frame[i] < 0 || ref_frame[i] >= b->num_frames || cur_frame[i] < 0 || cur_frame[i] >= b->num_frames)
+      return set_err(NICP_ERR_INVALID, "pair %d: frame out of range", i);
+  nicp_ctx *ctx = b->ctx;
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<nicp_camera> camv(num_pairs);
+  std::vector<const nicp_camera *> cams(num_pairs);
+  for (int i = 0; i < num_pairs; ++i) {
+    const FrameDev &fc = b->frames[cur_frame[i]];
+    // the projector of a pair is the current frame's camera
+    nicp_camera &c = camv[i];
+    for (int k = 0; k < 9; ++k) c.K[k] = fc.K[k];
+    c.min_distance = fc.min_d;
+    c.max_distance = fc.max_d;
+    cams[i] = &c;
+  }
+  PairPlan plan;
+  make_plan(b->frames, num_pairs, ref_frame, cur_frame, cams.data(), plan);
+  AlignBuffers B{&b->pairs_tbl, &b->maps, &b->zbuf, &b->cur_index, &b->pairs_buf, &b->state, &b->partials, &b->result};
+  int *d_bp, *d_br, *d_bc;
+  PairDev *d_pairs;
+  CHK(prepare_align(ctx->stream, plan, B, init_guesses16, &d_bp, &d_br, &d_bc, &d_pairs));
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  if (use_graph) {
+    // graph key: the launch geometry and the parameters baked into the kernel arguments
+    std::vector<int> key = {num_pairs, plan.total_pix_tiles, plan.total_rtiles, plan.total_ctiles};
+    const int *apw = (const int *)ap;
+    key.insert(key.end(), apw, apw + sizeof(nicp_align_params) / sizeof(int));
+    key.push_back((int)(((uintptr_t)d_pairs) & 0x7fffffff));
+    key.push_back((int)(((uintptr_t)b->partials.p) & 0x7fffffff));
+    key.push_back((int)(((uintptr_t)b->zbuf.p) & 0x7fffffff));
+    if (b->graph.exec == nullptr || b->graph.key != key) {
+      b->graph.reset();
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      int rc = launch_align(ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap);
+      hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+      if (rc != NICP_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return set_err(NICP_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&b->graph.exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        b->graph.exec = nullptr;
+        return set_err(NICP_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+      }
+      b->graph.key = key;
+    }
+    HIPCHK(hipGraphLaunch(b->graph.exec, ctx->stream));
+  } else {
+    CHK(launch_align(ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap));
+  }
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  if (!T_out16 && !results) return NICP_OK;
+  std::vector<float> st((size_t)num_pairs * kStateFloats);
+  std::vector<double> rr((size_t)num_pairs * 2 * kPartial);
+  HIPCHK(hipMemcpyAsync(st.data(), b->state.p, st.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  if (results)
+    HIPCHK(hipMemcpyAsync(rr.data(), b->result.p, rr.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  for (int i = 0; i < num_pairs; ++i) {
+    const float *T = &st[(size_t)i * kStateFloats];
+    if (T_out16) memcpy(T_out16 + 16 * (size_t)i, T, 16 * sizeof(float));
+    if (results) {
+      if (ap->outer_iterations == 0) memset(&rr[(size_t)i * 2 * kPartial], 0, 2 * kPartial * sizeof(double));
+      fill_result(&rr[(size_t)i * 2 * kPartial], T, ap, &results[i], ms);
+    }
+  }
+  return NICP_OK;
+}
+
+extern "C" int nicp_batch_frame_cloud(nicp_batch *b, int frame, nicp_cloud **out) {
+  if (!b || !out || frame < 0 || frame >= b->num_frames) return set_err(NICP_ERR_INVALID, "bad arguments");
+  nicp_cloud *cl = new (std::nothrow) nicp_cloud();
+  if (!cl) return set_err(NICP_ERR_OOM, "out of host memory");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  int n = 0;
+  HIPCHK(hipMemcpyAsync(&n, b->frames[frame].npoints, sizeof(int), hipMemcpyDeviceToHost, b->ctx->stream));
+  HIPCHK(hipStreamSynchronize(b->ctx->stream));
+  cl->ctx = b->ctx;
+  cl->fd = b->frames[frame];
+  cl->n = n;
+  cl->has_index = 1;
+  cl->has_stats = 0;
+  cl->alloc = nullptr;
+  *out = cl;
+  return NICP_OK;
+}

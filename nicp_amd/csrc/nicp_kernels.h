@@ -1,0 +1,77 @@
+// Device-side descriptors shared by the kernels and the host launcher code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nicp_dev {
+
+constexpr int kTile = 1024;          // pixels (or points) per per-pixel block
+constexpr int kThreads = 256;        // threads per block (4 wavefronts of 64)
+constexpr int kPartial = 32;         // doubles per block partial of the linearizer
+constexpr uint64_t kZInit = ~0ull;   // empty z-buffer cell
+
+// One frame (a depth image and the cloud made from it). Every pointer is a
+// device pointer; arrays indexed by pixel hold rows*cols entries, arrays
+// indexed by point hold up to rows*cols entries (the capacity).
+struct FrameDev {
+  int rows, cols;
+  int n_tiles;          // ceil(rows*cols / kTile)
+  int tile_base;        // first global block of this frame in per-pixel launches
+  int xblk_base;        // first block of the x-prefix pass (rows*10 threads, padded to kThreads)
+  int yblk_base;        // first block of the y-prefix pass (cols*10 threads, padded to kThreads)
+  float depth_scale;
+  float min_d, max_d;
+  float ip0, ip1;       // K * (worldRadius, worldRadius, 0): rows 0 and 1 (interval numerators)
+  float K[9];           // camera matrix (column-major)
+  float iKRt[16];       // unprojection matrix (identity sensor pose)
+  const uint16_t *depth;
+  int *index;           // index image
+  float *ii;            // integral image, 10 floats per pixel
+  float4 *pts;          // (x, y, z, 1)
+  float4 *nrm;          // (nx, ny, nz, curvature)
+  float4 *omg;          // 3 float4 per point: Omega_p (00 01 02 11 12 22), Omega_n (same)
+  float4 *stats;        // 4 float4 per point or null: eigvecs (9), eigvals (3), mean (3), n
+  int *npoints;         // device scalar
+  int *tile_count;      // per tile
+  int *tile_off;        // per tile
+};
+
+// One registration pair: reference and current frame ids, the projector
+// image and the per-pair loop state.
+struct PairDev {
+  int ref, cur;
+  int rows, cols;       // projector image size
+  int tile_base, n_tiles;          // per-pixel blocks over the projector image
+  int rtile_base, n_rtiles;        // blocks over the reference cloud capacity
+  int ctile_base, n_ctiles;        // blocks over the current cloud capacity
+  float K[9];
+  float min_d, max_d;
+  unsigned long long *zbuf;        // rows*cols z-buffer keys (depth bits << 32 | index)
+  int *cur_index;                  // rows*cols
+  int2 *pairs;                     // rows*cols stored correspondences (or (-1,-1))
+  float *state;                    // kStateFloats: T, Tinv, KRt (reference), KRt (current), invT (inner)
+  double *partials;                // n_tiles * kPartial
+  double *result;                  // 2 * kPartial: last linearisation, statistics linearisation
+};
+
+constexpr int kStateFloats = 80;
+constexpr int kSlotT = 0, kSlotTinv = 16, kSlotKRt = 32, kSlotKRtCur = 48, kSlotInvTInner = 64;
+
+struct StatsParamsDev {
+  int min_r, max_r, min_points;
+  float curvature_threshold;
+  float pflat[3], pnonflat[3], nflat[3], nnonflat[3];
+  float pthr, nthr;
+  int keep_stats;
+};
+
+struct AlignParamsDev {
+  float sq_thr, cos_thr, flat, min_ratio, max_ratio;
+  int row_region, col_region, gicp;
+  float max_chi2;
+  int robust;
+  float nscale;
+  float lambda;
+};
+
+}  // namespace nicp_dev

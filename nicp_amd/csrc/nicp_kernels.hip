@@ -1,0 +1,841 @@
+// CDNA4 (gfx950) kernels of the NICP registration loop.
+//
+// Conversion of a depth frame into a cloud (DepthImageConverterIntegralImage::compute,
+// nicp/nicp/depthimageconverterintegralimage.cpp:18-64):
+//   k_count      u16 depth -> valid-pixel count per 1024-pixel tile
+//   k_scan       per frame: exclusive scan of tile counts (row-major point numbering)
+//   k_unproject  points + index image (PinholePointProjector::unProject, pinholepointprojector.cpp:93-134)
+//   k_integral_x x-prefix of (n, p, p p^T) per image row    (pointintegralimage.cpp:16-35)
+//   k_integral_y y-prefix per image column                  (pointintegralimage.cpp:38-43)
+//   k_stats      box query, covariance, 3x3 eigensolve, normal, curvature and
+//                both information matrices per pixel      (statscalculatorintegralimage.cpp:38-81,
+//                                                            informationmatrixcalculator.cpp:9-57)
+// One registration iteration (AlignerProjective::align, alignerprojective.cpp:45-101):
+//   k_zproject   z-buffer projection with a 64-bit atomicMin on (depth bits << 32 | index)
+//                (PinholePointProjector::project, pinholepointprojector.cpp:33-66)
+//   k_corr_lin   projective correspondence gates fused with the Linearizer's H/b terms
+//                (correspondencefinderprojective.cpp:23-257, linearizer.cpp:21-134):
+//                compensated FP32 per thread, FP64 wave/block partials
+//   k_solve      FP64 reduction of the partials, H + lambda I, LDLT, SE(3) update
+//
+// All kernels read frame / pair descriptors from device tables and map each
+// block to its frame or pair through a block map, so frames and pairs of
+// uneven size share one launch per stage.
+#include <float.h>
+
+#include "nicp_kernels.h"
+#include "nicp_math.h"
+
+using namespace nicp_dev;
+using namespace nicp_math;
+
+namespace {
+
+__device__ __forceinline__ float depth_of(uint16_t raw, float scale) { return raw ? scale * (float)raw : 0.0f; }
+__device__ __forceinline__ bool depth_valid(float d, float mn, float mx) { return !(d == 0.0f || d < mn || d > mx); }
+
+// PinholePointProjector::_unProject (pinholepointprojector.h:275-280): x = column, y = row
+__device__ __forceinline__ float4 unproject_point(const float *iKRt, int c, int r, float d) {
+  float o[4];
+  mul4_packet(iKRt, (float)c * d, (float)r * d, d, 1.0f, o);
+  return make_float4(o[0], o[1], o[2], o[3]);
+}
+
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// inclusive scan within a wavefront
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Conversion
+// ---------------------------------------------------------------------------------------
+
+extern "C" __global__ __launch_bounds__(kThreads) void k_count(const FrameDev *__restrict__ F,
+                                                              const int *__restrict__ blk_frame) {
+  const int g = blockIdx.x;
+  const FrameDev &fd = F[blk_frame[g]];
+  const int P = fd.rows * fd.cols;
+  const int base = (g - fd.tile_base) * kTile;
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int p = base + k * kThreads + threadIdx.x;
+    if (p < P) cnt += depth_valid(depth_of(fd.depth[p], fd.depth_scale), fd.min_d, fd.max_d);
+  }
+  __shared__ int ws[kThreads / 64];
+  cnt = wave_sum_i(cnt);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) fd.tile_count[g - fd.tile_base] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// one block of 1024 threads per frame
+extern "C" __global__ __launch_bounds__(1024) void k_scan(const FrameDev *__restrict__ F) {
+  const FrameDev &fd = F[blockIdx.x];
+  const int n = fd.n_tiles;
+  const int chunk = (n + 1023) / 1024;
+  const int start = threadIdx.x * chunk;
+  int local = 0;
+  for (int i = 0; i < chunk; ++i)
+    if (start + i < n) local += fd.tile_count[start + i];
+  __shared__ int wsum[16];
+  int incl = wave_incl_scan(local);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < 16; ++w) { int t = wsum[w]; wsum[w] = s; s += t; }
+    *fd.npoints = s;
+  }
+  __syncthreads();
+  int run = wsum[wave] + incl - local;
+  for (int i = 0; i < chunk; ++i)
+    if (start + i < n) {
+      int c = fd.tile_count[start + i];
+      fd.tile_off[start + i] = run;
+      run += c;
+    }
+}
+
+// Each thread owns 4 consecutive pixels, so the block's numbering is row-major.
+extern "C" __global__ __launch_bounds__(kThreads) void k_unproject(const FrameDev *__restrict__ F,
+                                                                  const int *__restrict__ blk_frame) {
+  const int g = blockIdx.x;
+  const FrameDev &fd = F[blk_frame[g]];
+  const int P = fd.rows * fd.cols;
+  const int t = g - fd.tile_base;
+  const int p0 = t * kTile + threadIdx.x * 4;
+  float dv[4];
+  int valid[4], cnt = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int p = p0 + k;
+    dv[k] = p < P ? depth_of(fd.depth[p], fd.depth_scale) : 0.0f;
+    valid[k] = p < P && depth_valid(dv[k], fd.min_d, fd.max_d);
+    cnt += valid[k];
+  }
+  __shared__ int wsum[kThreads / 64];
+  int incl = wave_incl_scan(cnt);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += wsum[w];
+  int idx = fd.tile_off[t] + before + incl - cnt;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int p = p0 + k;
+    if (p >= P) break;
+    if (valid[k]) {
+      int r = p / fd.cols, c = p - r * fd.cols;
+      fd.pts[idx] = unproject_point(fd.iKRt, c, r, dv[k]);
+      fd.index[p] = idx;
+      ++idx;
+    } else {
+      fd.index[p] = -1;
+    }
+  }
+}
+
+// x-prefix: one thread per (row, channel), sequential along the row as the
+// reference's pass over image columns. Channel ch = p[i] * p[j] with p = (x, y, z, 1).
+__constant__ int c_ch_i[10] = {3, 0, 1, 2, 0, 0, 0, 1, 1, 2};
+__constant__ int c_ch_j[10] = {3, 3, 3, 3, 0, 1, 2, 1, 2, 2};
+
+extern "C" __global__ __launch_bounds__(kThreads) void k_integral_x(const FrameDev *__restrict__ F,
+                                                                   const int *__restrict__ xblk_frame) {
+  const int g = blockIdx.x;
+  const FrameDev &fd = F[xblk_frame[g]];
+  const int lt = (g - fd.xblk_base) * kThreads + threadIdx.x;
+  const int rows = fd.rows, cols = fd.cols;
+  if (lt >= rows * 10) return;
+  const int r = lt / 10, ch = lt - r * 10;
+  const int ci = c_ch_i[ch], cj = c_ch_j[ch];
+  const uint16_t *drow = fd.depth + (size_t)r * cols;
+  float *orow = fd.ii + (size_t)r * cols * 10 + ch;
+  float iK[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) iK[k] = fd.iKRt[k];
+  const float scale = fd.depth_scale, mn = fd.min_d, mx = fd.max_d;
+  float acc = 0.0f;
+  int c = 0;
+  for (; c + 8 <= cols; c += 8) {
+    uint16_t raw[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) raw[k] = drow[c + k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      float d = depth_of(raw[k], scale);
+      float v = 0.0f;
+      if (depth_valid(d, mn, mx)) {
+        float o[4];
+        mul4_packet(iK, (float)(c + k) * d, (float)r * d, d, 1.0f, o);
+        float a = ci == 0 ? o[0] : ci == 1 ? o[1] : ci == 2 ? o[2] : o[3];
+        float b = cj == 0 ? o[0] : cj == 1 ? o[1] : cj == 2 ? o[2] : o[3];
+        v = a * b;
+      }
+      acc = v + acc;
+      orow[(size_t)(c + k) * 10] = acc;
+    }
+  }
+  for (; c < cols; ++c) {
+    float d = depth_of(drow[c], scale);
+    float v = 0.0f;
+    if (depth_valid(d, mn, mx)) {
+      float o[4];
+      mul4_packet(iK, (float)c * d, (float)r * d, d, 1.0f, o);
+      float a = ci == 0 ? o[0] : ci == 1 ? o[1] : ci == 2 ? o[2] : o[3];
+      float b = cj == 0 ? o[0] : cj == 1 ? o[1] : cj == 2 ? o[2] : o[3];
+      v = a * b;
+    }
+    acc = v + acc;
+    orow[(size_t)c * 10] = acc;
+  }
+}
+
+// y-prefix: one thread per (column, channel) slot; consecutive threads touch
+// consecutive floats of a row, so every step is one coalesced row segment.
+extern "C" __global__ __launch_bounds__(kThreads) void k_integral_y(const FrameDev *__restrict__ F,
+                                                                   const int *__restrict__ yblk_frame) {
+  const int g = blockIdx.x;
+  const FrameDev &fd = F[yblk_frame[g]];
+  const int j = (g - fd.yblk_base) * kThreads + threadIdx.x;
+  const int stride = fd.cols * 10;
+  if (j >= stride) return;
+  float *col = fd.ii + j;
+  const int rows = fd.rows;
+  float acc = 0.0f;
+  int r = 0;
+  for (; r + 8 <= rows; r += 8) {
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = col[(size_t)(r + k) * stride];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      acc = v[k] + acc;
+      col[(size_t)(r + k) * stride] = acc;
+    }
+  }
+  for (; r < rows; ++r) {
+    acc = col[(size_t)r * stride] + acc;
+    col[(size_t)r * stride] = acc;
+  }
+}
+
+// ---- Eigen SelfAdjointEigenSolver<Matrix3f>::computeDirect, restated (3x3 FP32) ----
+namespace {
+
+__device__ __forceinline__ void cross3(const float *a, const float *b, float *o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ float sqn3(const float *a) { return red3(a[0] * a[0], a[1] * a[1], a[2] * a[2]); }
+
+__device__ void eig_extract_kernel(const float *mat, float *res, float *rep_out) {
+  int i0 = 0;
+  float best = fabsf(mat[0]);
+  if (fabsf(mat[4]) > best) { best = fabsf(mat[4]); i0 = 1; }
+  if (fabsf(mat[8]) > best) { best = fabsf(mat[8]); i0 = 2; }
+  float rep[3] = {mat[3 * i0], mat[3 * i0 + 1], mat[3 * i0 + 2]};
+  const int i1 = (i0 + 1) % 3, i2 = (i0 + 2) % 3;
+  float c0[3], c1[3];
+  cross3(rep, mat + 3 * i1, c0);
+  cross3(rep, mat + 3 * i2, c1);
+  float n0 = sqn3(c0), n1 = sqn3(c1);
+  if (n0 > n1) {
+    float s = sqrtf(n0);
+    res[0] = c0[0] / s; res[1] = c0[1] / s; res[2] = c0[2] / s;
+  } else {
+    float s = sqrtf(n1);
+    res[0] = c1[0] / s; res[1] = c1[1] / s; res[2] = c1[2] / s;
+  }
+  if (rep_out) { rep_out[0] = rep[0]; rep_out[1] = rep[1]; rep_out[2] = rep[2]; }
+}
+
+// cov: symmetric 3x3 column-major; evals ascending, evecs column-major (col 0 = smallest)
+__device__ void eig3_direct(const float *cov, float *ev, float *V) {
+  float m[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) m[i] = cov[i];
+  const float shift = red3(m[0], m[4], m[8]) / 3.0f;
+  m[0] -= shift; m[4] -= shift; m[8] -= shift;
+  float scale = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) scale = fmaxf(scale, fabsf(m[i]));
+  if (scale > 0.0f) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) m[i] = m[i] / scale;
+  }
+  // roots of the characteristic polynomial
+  {
+    const float s_inv3 = 1.0f / 3.0f;
+    const float s_sqrt3 = sqrtf(3.0f);
+    float m00 = m[0], m11 = m[4], m22 = m[8], m10 = m[1], m20 = m[2], m21 = m[5];
+    float c0 = m00 * m11 * m22 + 2.0f * m10 * m20 * m21 - m00 * m21 * m21 - m11 * m20 * m20 - m22 * m10 * m10;
+    float c1 = m00 * m11 - m10 * m10 + m00 * m22 - m20 * m20 + m11 * m22 - m21 * m21;
+    float c2 = m00 + m11 + m22;
+    float c2_over_3 = c2 * s_inv3;
+    float a_over_3 = (c2 * c2_over_3 - c1) * s_inv3;
+    a_over_3 = fmaxf(a_over_3, 0.0f);
+    float half_b = 0.5f * (c0 + c2_over_3 * (2.0f * c2_over_3 * c2_over_3 - c1));
+    float q = a_over_3 * a_over_3 * a_over_3 - half_b * half_b;
+    q = fmaxf(q, 0.0f);
+    float rho = sqrtf(a_over_3);
+    float theta = atan2f(sqrtf(q), half_b) * s_inv3;
+    float cos_theta = cosf(theta);
+    float sin_theta = sinf(theta);
+    ev[0] = c2_over_3 - rho * (cos_theta + s_sqrt3 * sin_theta);
+    ev[1] = c2_over_3 - rho * (cos_theta - s_sqrt3 * sin_theta);
+    ev[2] = c2_over_3 + 2.0f * rho * cos_theta;
+  }
+  const float eps = FLT_EPSILON;
+  if ((ev[2] - ev[0]) <= eps) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) V[i] = 0.0f;
+    V[0] = V[4] = V[8] = 1.0f;
+  } else {
+    float tmp[9];
+    float d0 = ev[2] - ev[1];
+    float d1 = ev[1] - ev[0];
+    int k = 0, l = 2;
+    if (d0 > d1) { k = 2; l = 0; d0 = d1; }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) tmp[i] = m[i];
+    const float ek = k == 0 ? ev[0] : ev[2];
+    tmp[0] -= ek; tmp[4] -= ek; tmp[8] -= ek;
+    eig_extract_kernel(tmp, V + 3 * k, V + 3 * l);
+    if (d0 <= 2.0f * eps * d1) {
+      float *vl = V + 3 * l, *vk = V + 3 * k;
+      float d = red3(vk[0] * vl[0], vk[1] * vl[1], vk[2] * vl[2]);
+      float t0 = d * vl[0], t1 = d * vl[1], t2 = d * vl[2];
+      vl[0] -= t0; vl[1] -= t1; vl[2] -= t2;
+      float z = sqn3(vl);
+      if (z > 0.0f) {
+        float s = sqrtf(z);
+        vl[0] = vl[0] / s; vl[1] = vl[1] / s; vl[2] = vl[2] / s;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) tmp[i] = m[i];
+      const float el = l == 0 ? ev[0] : ev[2];
+      tmp[0] -= el; tmp[4] -= el; tmp[8] -= el;
+      eig_extract_kernel(tmp, V + 3 * l, nullptr);
+    }
+    float c[3];
+    cross3(V + 6, V + 0, c);
+    float z = sqn3(c);
+    if (z > 0.0f) {
+      float s = sqrtf(z);
+      c[0] = c[0] / s; c[1] = c[1] / s; c[2] = c[2] / s;
+    }
+    V[3] = c[0]; V[4] = c[1]; V[5] = c[2];
+  }
+  ev[0] = ev[0] * scale; ev[1] = ev[1] * scale; ev[2] = ev[2] * scale;
+  ev[0] = ev[0] + shift; ev[1] = ev[1] + shift; ev[2] = ev[2] + shift;
+}
+
+// Omega = U diag(dg) U^T in the association order of Eigen's lazy products
+// (informationmatrixcalculator.cpp:26-28); returns the upper triangle 00 01 02 11 12 22
+__device__ __forceinline__ void omega6(const float *U, const float *dg, float *o) {
+  const int rr[6] = {0, 0, 0, 1, 1, 2}, cc[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    int r = rr[e], c = cc[e];
+    float a0 = (U[0 + r] * dg[0]) * U[0 + c];
+    float a1 = (U[3 + r] * dg[1]) * U[3 + c];
+    float a2 = (U[6 + r] * dg[2]) * U[6 + c];
+    o[e] = (a0 + a1) + a2;
+  }
+}
+
+}  // namespace
+
+extern "C" __global__ __launch_bounds__(kThreads) void k_stats(const FrameDev *__restrict__ F,
+                                                              const int *__restrict__ blk_frame, StatsParamsDev sp) {
+  const int g = blockIdx.x;
+  const FrameDev &fd = F[blk_frame[g]];
+  const int rows = fd.rows, cols = fd.cols, P = rows * cols;
+  const int base = (g - fd.tile_base) * kTile;
+  for (int k = 0; k < 4; ++k) {
+    const int p = base + k * kThreads + threadIdx.x;
+    if (p >= P) break;
+    const int idx = fd.index[p];
+    if (idx < 0) continue;
+    const int r = p / cols, c = p - r * cols;
+    const float d = depth_of(fd.depth[p], fd.depth_scale);
+    bool ok = true;
+    int R = -1;
+    if (d < fd.min_d || d > fd.max_d) {
+      ok = false;
+    } else {
+      float inv = 1.0f / d;
+      float a = fd.ip0 * inv, b = fd.ip1 * inv;
+      R = (int)fmaxf(a, b);
+      if (R < 0 || R < sp.min_r) ok = false;
+      if (R > sp.max_r) R = sp.max_r;
+    }
+    float acc[10];
+    int n = 0;
+    if (ok) {
+      int xmin = min(max(c - R - 1, 0), cols - 1), xmax = min(max(c + R - 1, 0), cols - 1);
+      int ymin = min(max(r - R - 1, 0), rows - 1), ymax = min(max(r + R - 1, 0), rows - 1);
+      const float2 *A = (const float2 *)(fd.ii + ((size_t)ymax * cols + xmax) * 10);
+      const float2 *D = (const float2 *)(fd.ii + ((size_t)ymin * cols + xmin) * 10);
+      const float2 *B = (const float2 *)(fd.ii + ((size_t)ymax * cols + xmin) * 10);
+      const float2 *C = (const float2 *)(fd.ii + ((size_t)ymin * cols + xmax) * 10);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        float2 a = A[q], dd = D[q], bb = B[q], cc = C[q];
+        acc[2 * q] = ((a.x + dd.x) - bb.x) - cc.x;
+        acc[2 * q + 1] = ((a.y + dd.y) - bb.y) - cc.y;
+      }
+      n = (int)acc[0];
+      if (n < sp.min_points) ok = false;
+    }
+    float4 nrm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float op[6] = {0, 0, 0, 0, 0, 0}, on[6] = {0, 0, 0, 0, 0, 0};
+    float V[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, ev[3] = {0, 0, 0}, mean[3] = {0, 0, 0};
+    if (ok) {
+      const float dn = 1.0f / acc[0];
+      const float m0 = acc[1] * dn, m1 = acc[2] * dn, m2 = acc[3] * dn;
+      float cov[9];
+      cov[0] = acc[4] * dn - m0 * m0;
+      cov[1] = acc[5] * dn - m1 * m0;
+      cov[2] = acc[6] * dn - m2 * m0;
+      cov[3] = acc[5] * dn - m0 * m1;
+      cov[4] = acc[7] * dn - m1 * m1;
+      cov[5] = acc[8] * dn - m2 * m1;
+      cov[6] = acc[6] * dn - m0 * m2;
+      cov[7] = acc[8] * dn - m1 * m2;
+      cov[8] = acc[9] * dn - m2 * m2;
+      eig3_direct(cov, ev, V);
+      if (ev[0] < 0.0f) ev[0] = 0.0f;
+      mean[0] = m0; mean[1] = m1; mean[2] = m2;
+      // Stats::curvature (stats.h:93-98): the 1e-9 literal makes the last add and the divide double
+      const float curv = (float)((double)ev[0] / ((double)((ev[0] + ev[1]) + ev[2]) + 1e-9));
+      float nx = V[0], ny = V[1], nz = V[2];
+      if (curv < sp.curvature_threshold) {
+        const float4 pt = unproject_point(fd.iKRt, c, r, d);
+        if (dot4(nx, ny, nz, 0.0f, pt.x, pt.y, pt.z, pt.w) > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
+        nrm = make_float4(nx, ny, nz, curv);
+        const float *dp = curv < sp.pthr ? sp.pflat : sp.pnonflat;
+        const float *dq = curv < sp.nthr ? sp.nflat : sp.nnonflat;
+        omega6(V, dp, op);
+        omega6(V, dq, on);
+      } else {
+        nrm = make_float4(0.0f, 0.0f, 0.0f, curv);
+      }
+    }
+    fd.nrm[idx] = nrm;
+    float4 *o = fd.omg + 3 * (size_t)idx;
+    o[0] = make_float4(op[0], op[1], op[2], op[3]);
+    o[1] = make_float4(op[4], op[5], on[0], on[1]);
+    o[2] = make_float4(on[2], on[3], on[4], on[5]);
+    if (sp.keep_stats) {
+      float4 *s = fd.stats + 4 * (size_t)idx;
+      const float mw = ok ? acc[0] * (1.0f / acc[0]) : 1.0f;
+      s[0] = make_float4(V[0], V[1], V[2], V[3]);
+      s[1] = make_float4(V[4], V[5], V[6], V[7]);
+      s[2] = make_float4(V[8], ev[0], ev[1], ev[2]);
+      s[3] = make_float4(mean[0], mean[1], mean[2], __int_as_float(ok ? n : 0));
+      (void)mw;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Registration
+// ---------------------------------------------------------------------------------------
+
+extern "C" __global__ __launch_bounds__(kThreads) void k_zfill(const PairDev *__restrict__ Pp,
+                                                              const int *__restrict__ blk_pair) {
+  const int g = blockIdx.x;
+  const PairDev &pd = Pp[blk_pair[g]];
+  const int P = pd.rows * pd.cols;
+  const int base = (g - pd.tile_base) * kTile;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int p = base + k * kThreads + threadIdx.x;
+    if (p < P) pd.zbuf[p] = kZInit;
+  }
+}
+
+// which = 0: the reference cloud with the pair's current KRt; 1: the current cloud
+extern "C" __global__ __launch_bounds__(kThreads) void k_zproject(const PairDev *__restrict__ Pp,
+                                                                 const FrameDev *__restrict__ F,
+                                                                 const int *__restrict__ blk_pair, int which) {
+  const int g = blockIdx.x;
+  const PairDev &pd = Pp[blk_pair[g]];
+  const FrameDev &fd = F[which ? pd.cur : pd.ref];
+  const int n = *fd.npoints;
+  const int base = (g - (which ? pd.ctile_base : pd.rtile_base)) * kTile;
+  const float *S = pd.state + (which ? kSlotKRtCur : kSlotKRt);
+  float KRt[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) KRt[k] = S[k];
+  const int cols = pd.cols, rows = pd.rows;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i = base + k * kThreads + threadIdx.x;
+    if (i >= n) break;
+    const float4 p = fd.pts[i];
+    float ip[4];
+    mul4_packet(KRt, p.x, p.y, p.z, p.w, ip);
+    const float d = ip[2];
+    if (d < pd.min_d || d > pd.max_d) continue;
+    const float inv = 1.0f / d;
+    const int x = (int)roundf(ip[0] * inv), y = (int)roundf(ip[1] * inv);
+    if (x < 0 || x >= cols || y < 0 || y >= rows) continue;
+    const unsigned long long key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
+    atomicMin(pd.zbuf + (size_t)y * cols + x, key);
+  }
+}
+
+// z-buffer -> index image (and depth image); resets the z-buffer
+extern "C" __global__ __launch_bounds__(kThreads) void k_zresolve(const PairDev *__restrict__ Pp,
+                                                                 const int *__restrict__ blk_pair,
+                                                                 float *__restrict__ depth_out) {
+  const int g = blockIdx.x;
+  const PairDev &pd = Pp[blk_pair[g]];
+  const int P = pd.rows * pd.cols;
+  const int base = (g - pd.tile_base) * kTile;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int p = base + k * kThreads + threadIdx.x;
+    if (p >= P) break;
+    unsigned long long key = pd.zbuf[p];
+    pd.cur_index[p] = key == kZInit ? -1 : (int)(unsigned int)(key & 0xffffffffull);
+    if (depth_out) depth_out[p] = key == kZInit ? FLT_MAX : __uint_as_float((unsigned int)(key >> 32));
+    pd.zbuf[p] = kZInit;
+  }
+}
+
+namespace {
+
+// compensated accumulation (TwoSum): s + c carries the running sum
+struct Acc2 {
+  float s, c;
+};
+__device__ __forceinline__ void acc_add(Acc2 &a, float x) {
+  float t = a.s + x;
+  float bp = t - a.s;
+  float e = (a.s - (t - bp)) + (x - bp);
+  a.s = t;
+  a.c += e;
+}
+
+// The per-correspondence terms of Linearizer::update (linearizer.cpp:55-112) for
+// J_p = [I | S(p)], J_n = [0 | S(n)], S(v) = -2[v]x (bm_se3.h:55-66).
+// Output order: H upper triangle row-major (21), b (6), chi2, inliers, count.
+__device__ __forceinline__ void lin_terms(const float *rp, const float *rn, float4 cp, float4 cn, const float4 *om,
+                                          const AlignParamsDev &ap, Acc2 *acc) {
+  const float4 o0 = om[0], o1 = om[1], o2 = om[2];
+  // Omega_p / Omega_n (symmetric)
+  const float P00 = o0.x, P01 = o0.y, P02 = o0.z, P11 = o0.w, P12 = o1.x, P22 = o1.y;
+  const float ns = ap.gicp ? 0.0f : ap.nscale;
+  const float N00 = ns * o1.z, N01 = ns * o1.w, N02 = ns * o2.x, N11 = ns * o2.y, N12 = ns * o2.z, N22 = ns * o2.w;
+  const float pe0 = rp[0] - cp.x, pe1 = rp[1] - cp.y, pe2 = rp[2] - cp.z;
+  const float ne0 = rn[0] - cn.x, ne1 = rn[1] - cn.y, ne2 = rn[2] - cn.z;
+  const float ep0 = P00 * pe0 + P01 * pe1 + P02 * pe2;
+  const float ep1 = P01 * pe0 + P11 * pe1 + P12 * pe2;
+  const float ep2 = P02 * pe0 + P12 * pe1 + P22 * pe2;
+  const float en0 = N00 * ne0 + N01 * ne1 + N02 * ne2;
+  const float en1 = N01 * ne0 + N11 * ne1 + N12 * ne2;
+  const float en2 = N02 * ne0 + N12 * ne1 + N22 * ne2;
+  const float chi = (pe0 * ep0 + pe1 * ep1 + pe2 * ep2) + (ne0 * en0 + ne1 * en1 + ne2 * en2);
+  float k = 1.0f;
+  acc_add(acc[29], 1.0f);
+  if (chi > ap.max_chi2) {
+    if (ap.robust) k = ap.max_chi2 / chi;
+    else return;
+  }
+  // S(v) columns: col0 = (0, -2vz, 2vy), col1 = (2vz, 0, -2vx), col2 = (-2vy, 2vx, 0)
+  const float px = 2.0f * rp[0], py = 2.0f * rp[1], pz = 2.0f * rp[2];
+  const float nx = 2.0f * rn[0], ny = 2.0f * rn[1], nz = 2.0f * rn[2];
+  // A = Omega_p S(p): column j = Omega_p * S(p)_j
+  const float A00 = -P01 * pz + P02 * py, A10 = -P11 * pz + P12 * py, A20 = -P12 * pz + P22 * py;
+  const float A01 = P00 * pz - P02 * px, A11 = P01 * pz - P12 * px, A21 = P02 * pz - P22 * px;
+  const float A02 = -P00 * py + P01 * px, A12 = -P01 * py + P11 * px, A22 = -P02 * py + P12 * px;
+  const float B00 = -N01 * nz + N02 * ny, B10 = -N11 * nz + N12 * ny, B20 = -N12 * nz + N22 * ny;
+  const float B01 = N00 * nz - N02 * nx, B11 = N01 * nz - N12 * nx, B21 = N02 * nz - N22 * nx;
+  const float B02 = -N00 * ny + N01 * nx, B12 = -N01 * ny + N11 * nx, B22 = -N02 * ny + N12 * nx;
+  // S(v)^T w = (w1*(-2vz)... ) : S^T row i = S column i
+  // (S_p^T A)_{ij} = S_p col i . A col j
+#define STA(sx, sy, sz, a0, a1, a2, i) \
+  ((i) == 0 ? (-(sz)) * (a1) + (sy) * (a2) : (i) == 1 ? (sz) * (a0) - (sx) * (a2) : -(sy) * (a0) + (sx) * (a1))
+  const float R00 = STA(px, py, pz, A00, A10, A20, 0) + STA(nx, ny, nz, B00, B10, B20, 0);
+  const float R01 = STA(px, py, pz, A01, A11, A21, 0) + STA(nx, ny, nz, B01, B11, B21, 0);
+  const float R02 = STA(px, py, pz, A02, A12, A22, 0) + STA(nx, ny, nz, B02, B12, B22, 0);
+  const float R11 = STA(px, py, pz, A01, A11, A21, 1) + STA(nx, ny, nz, B01, B11, B21, 1);
+  const float R12 = STA(px, py, pz, A02, A12, A22, 1) + STA(nx, ny, nz, B02, B12, B22, 1);
+  const float R22 = STA(px, py, pz, A02, A12, A22, 2) + STA(nx, ny, nz, B02, B12, B22, 2);
+  const float br0 = STA(px, py, pz, ep0, ep1, ep2, 0) + STA(nx, ny, nz, en0, en1, en2, 0);
+  const float br1 = STA(px, py, pz, ep0, ep1, ep2, 1) + STA(nx, ny, nz, en0, en1, en2, 1);
+  const float br2 = STA(px, py, pz, ep0, ep1, ep2, 2) + STA(nx, ny, nz, en0, en1, en2, 2);
+#undef STA
+  // H upper triangle, row-major over the 6x6: rows 0-2 = [Htt | Htr], rows 3-5 = [. | Hrr]
+  acc_add(acc[0], k * P00); acc_add(acc[1], k * P01); acc_add(acc[2], k * P02);
+  acc_add(acc[3], k * A00); acc_add(acc[4], k * A01); acc_add(acc[5], k * A02);
+  acc_add(acc[6], k * P11); acc_add(acc[7], k * P12);
+  acc_add(acc[8], k * A10); acc_add(acc[9], k * A11); acc_add(acc[10], k * A12);
+  acc_add(acc[11], k * P22);
+  acc_add(acc[12], k * A20); acc_add(acc[13], k * A21); acc_add(acc[14], k * A22);
+  acc_add(acc[15], k * R00); acc_add(acc[16], k * R01); acc_add(acc[17], k * R02);
+  acc_add(acc[18], k * R11); acc_add(acc[19], k * R12);
+  acc_add(acc[20], k * R22);
+  acc_add(acc[21], k * ep0); acc_add(acc[22], k * ep1); acc_add(acc[23], k * ep2);
+  acc_add(acc[24], k * br0); acc_add(acc[25], k * br1); acc_add(acc[26], k * br2);
+  acc_add(acc[27], k * chi);
+  acc_add(acc[28], 1.0f);
+}
+
+// block-wide FP64 reduction of the 30 compensated accumulators into one partial
+__device__ void block_partial(Acc2 *acc, double *out) {
+  __shared__ double red[kThreads / 64][kPartial];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int v = 0; v < 30; ++v) {
+    double x = (double)acc[v].s + (double)acc[v].c;
+    x = wave_sum_d(x);
+    if (lane == 0) red[wave][v] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < kPartial) {
+    double s = 0.0;
+    if (threadIdx.x < 30)
+      for (int w = 0; w < kThreads / 64; ++w) s += red[w][threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+// CorrespondenceFinderProjective gates for (ref index ri, cur index ci); T = finder transform
+__device__ __forceinline__ bool gates(const FrameDev &fr, const FrameDev &fc, int ri, int ci, const float *T,
+                                      const AlignParamsDev &ap, float *rp, float *rn, float4 &cp, float4 &cn) {
+  cn = fc.nrm[ci];
+  const float4 rn0 = fr.nrm[ri];
+  if (!ap.gicp && (dot4(cn.x, cn.y, cn.z, 0.0f, cn.x, cn.y, cn.z, 0.0f) == 0.0f ||
+                   dot4(rn0.x, rn0.y, rn0.z, 0.0f, rn0.x, rn0.y, rn0.z, 0.0f) == 0.0f))
+    return false;
+  cp = fc.pts[ci];
+  const float4 rp0 = fr.pts[ri];
+  iso_mul3(T, rp0.x, rp0.y, rp0.z, rp0.w, rp);
+  iso_mul3(T, rn0.x, rn0.y, rn0.z, 0.0f, rn);
+  const float dv0 = cp.x - rp[0], dv1 = cp.y - rp[1], dv2 = cp.z - rp[2], dv3 = cp.w - rp0.w;
+  if (dot4(dv0, dv1, dv2, dv3, dv0, dv1, dv2, dv3) > ap.sq_thr) return false;
+  if (!ap.gicp) {
+    if (dot4(cn.x, cn.y, cn.z, 0.0f, rn[0], rn[1], rn[2], 0.0f) < ap.cos_thr) return false;
+    float rc = rn0.w, cc = cn.w;
+    if (rc < ap.flat) rc = ap.flat;
+    if (cc < ap.flat) cc = ap.flat;
+    const float ratio = (float)(((double)rc + 1e-5) / ((double)cc + 1e-5));
+    if (ratio < ap.min_ratio || ratio > ap.max_ratio) return false;
+  }
+  return true;
+}
+
+// windowed search of _computeMulti (correspondencefinderprojective.cpp:144-236)
+__device__ int window_search(const FrameDev &fr, const FrameDev &fc, const int *cur_index, int rows, int cols, int r,
+                             int c, int ri, const float *T, const AlignParamsDev &ap, float *rp, float *rn) {
+  const float4 rn0 = fr.nrm[ri];
+  if (dot4(rn0.x, rn0.y, rn0.z, 0.0f, rn0.x, rn0.y, rn0.z, 0.0f) == 0.0f) return -1;
+  const float4 rp0 = fr.pts[ri];
+  iso_mul3(T, rp0.x, rp0.y, rp0.z, rp0.w, rp);
+  iso_mul3(T, rn0.x, rn0.y, rn0.z, 0.0f, rn);
+  float rc = rn0.w;
+  if (rc < ap.flat) rc = ap.flat;
+  const int r0 = max(r - ap.row_region, 0), r1 = min(r + ap.row_region + 1, rows);
+  const int c0 = max(c - ap.col_region, 0), c1 = min(c + ap.col_region + 1, cols);
+  int best = -1;
+  float best_nd = 0.0f, best_pd = 1e20f;
+  for (int r2 = r0; r2 < r1; ++r2)
+    for (int c2 = c0; c2 < c1; ++c2) {
+      const int ci = cur_index[r2 * cols + c2];
+      if (ci < 0) continue;
+      const float4 cn = fc.nrm[ci];
+      if (dot4(cn.x, cn.y, cn.z, 0.0f, cn.x, cn.y, cn.z, 0.0f) == 0.0f) continue;
+      const float4 cp = fc.pts[ci];
+      const float nd = dot4(cn.x, cn.y, cn.z, 0.0f, rn[0], rn[1], rn[2], 0.0f);
+      if (nd < ap.cos_thr) continue;
+      const float dv0 = cp.x - rp[0], dv1 = cp.y - rp[1], dv2 = cp.z - rp[2], dv3 = cp.w - rp0.w;
+      const float pd = dot4(dv0, dv1, dv2, dv3, dv0, dv1, dv2, dv3);
+      if (pd > ap.sq_thr) continue;
+      float cc = cn.w;
+      if (cc < ap.flat) cc = ap.flat;
+      const float ratio = (float)(((double)rc + 1e-5) / ((double)cc + 1e-5));
+      if (ratio < ap.min_ratio || ratio > ap.max_ratio) continue;
+      if (best == -1 || (pd < best_pd && nd > best_nd)) { best = ci; best_nd = nd; best_pd = pd; }
+    }
+  return best;
+}
+
+}  // namespace
+
+// Fused correspondence + linearisation over the projector image of each pair.
+// Reads and resets the z-buffer of the reference projection.
+extern "C" __global__ __launch_bounds__(kThreads) void k_corr_lin(const PairDev *__restrict__ Pp,
+                                                                 const FrameDev *__restrict__ F,
+                                                                 const int *__restrict__ blk_pair, AlignParamsDev ap,
+                                                                 int store_pairs) {
+  const int g = blockIdx.x;
+  const PairDev &pd = Pp[blk_pair[g]];
+  const FrameDev &fr = F[pd.ref];
+  const FrameDev &fc = F[pd.cur];
+  const int P = pd.rows * pd.cols;
+  const int t = g - pd.tile_base;
+  const int base = t * kTile;
+  float T[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) T[k] = pd.state[kSlotTinv + k];
+  const bool multi = ap.row_region != 0 || ap.col_region != 0;
+  Acc2 acc[30];
+#pragma unroll
+  for (int v = 0; v < 30; ++v) acc[v] = Acc2{0.0f, 0.0f};
+  for (int k = 0; k < 4; ++k) {
+    const int p = base + k * kThreads + threadIdx.x;
+    if (p >= P) break;
+    const unsigned long long key = pd.zbuf[p];
+    pd.zbuf[p] = kZInit;
+    const int ri = key == kZInit ? -1 : (int)(unsigned int)(key & 0xffffffffull);
+    int ci = -1;
+    float rp[3], rn[3];
+    float4 cp, cn;
+    bool ok = false;
+    if (!multi) {
+      ci = pd.cur_index[p];
+      if (ri >= 0 && ci >= 0) ok = gates(fr, fc, ri, ci, T, ap, rp, rn, cp, cn);
+    } else if (ri >= 0) {
+      const int r = p / pd.cols, c = p - r * pd.cols;
+      ci = window_search(fr, fc, pd.cur_index, pd.rows, pd.cols, r, c, ri, T, ap, rp, rn);
+      if (ci >= 0) {
+        ok = true;
+        cp = fc.pts[ci];
+        cn = fc.nrm[ci];
+      }
+    }
+    if (store_pairs) pd.pairs[p] = ok ? make_int2(ri, ci) : make_int2(-1, -1);
+    if (ok) lin_terms(rp, rn, cp, cn, fc.omg + 3 * (size_t)ci, ap, acc);
+  }
+  block_partial(acc, pd.partials + (size_t)t * kPartial);
+}
+
+// Linearizer::update over stored pairs at the transform in state slot `slot`.
+extern "C" __global__ __launch_bounds__(kThreads) void k_lin_pairs(const PairDev *__restrict__ Pp,
+                                                                  const FrameDev *__restrict__ F,
+                                                                  const int *__restrict__ blk_pair, AlignParamsDev ap,
+                                                                  int slot, int num_override) {
+  const int g = blockIdx.x;
+  const PairDev &pd = Pp[blk_pair[g]];
+  const FrameDev &fr = F[pd.ref];
+  const FrameDev &fc = F[pd.cur];
+  const int P = num_override > 0 ? num_override : pd.rows * pd.cols;
+  const int t = g - pd.tile_base;
+  const int base = t * kTile;
+  float T[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) T[k] = pd.state[slot + k];
+  Acc2 acc[30];
+#pragma unroll
+  for (int v = 0; v < 30; ++v) acc[v] = Acc2{0.0f, 0.0f};
+  for (int k = 0; k < 4; ++k) {
+    const int p = base + k * kThreads + threadIdx.x;
+    if (p >= P) break;
+    const int2 pr = pd.pairs[p];
+    if (pr.x < 0) continue;
+    const float4 rp0 = fr.pts[pr.x], rn0 = fr.nrm[pr.x];
+    float rp[3], rn[3];
+    iso_mul3(T, rp0.x, rp0.y, rp0.z, rp0.w, rp);
+    iso_mul3(T, rn0.x, rn0.y, rn0.z, 0.0f, rn);
+    lin_terms(rp, rn, fc.pts[pr.y], fc.nrm[pr.y], fc.omg + 3 * (size_t)pr.y, ap, acc);
+  }
+  block_partial(acc, pd.partials + (size_t)t * kPartial);
+}
+
+namespace {
+__device__ void reduce_partials(const PairDev &pd, double *tot /* shared, kPartial */) {
+  const int lane = threadIdx.x;
+  for (int v = 0; v < 30; ++v) {
+    double s = 0.0;
+    for (int i = lane; i < pd.n_tiles; i += 64) s += pd.partials[(size_t)i * kPartial + v];
+    s = wave_sum_d(s);
+    if (lane == 0) tot[v] = s;
+  }
+}
+}  // namespace
+
+// One wavefront per pair: FP64 totals, H + lambda I, LDLT, update of the SE(3) state.
+// mode 0: last inner iteration (updates T, Tinv, KRt); 1: inner iteration (updates invT_inner);
+// 2: statistics linearisation (stores the totals only).
+extern "C" __global__ __launch_bounds__(64) void k_solve(const PairDev *__restrict__ Pp, float lambda, int mode,
+                                                        int from_inner) {
+  const PairDev &pd = Pp[blockIdx.x];
+  __shared__ double tot[kPartial];
+  reduce_partials(pd, tot);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double *res = pd.result + (mode == 2 ? kPartial : 0);
+  for (int v = 0; v < 30; ++v) res[v] = tot[v];
+  if (mode == 2) return;
+  // H from the row-major upper triangle
+  double H[36], b[6];
+  int e = 0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) {
+      H[j * 6 + i] = tot[e];
+      H[i * 6 + j] = tot[e];
+      ++e;
+    }
+  for (int i = 0; i < 6; ++i) b[i] = -tot[21 + i];
+  for (int i = 0; i < 6; ++i) H[i * 6 + i] += (double)lambda;
+  double dx[6];
+  ldlt_solve6<double>(H, b, dx);
+  float dxf[6];
+  for (int i = 0; i < 6; ++i) dxf[i] = (float)dx[i];
+  float *S = pd.state;
+  float invT[16];
+  for (int k = 0; k < 16; ++k) invT[k] = S[(from_inner ? kSlotInvTInner : kSlotTinv) + k];
+  set_row3(invT);
+  float dT[16];
+  v2t<float>(dxf, dT);
+  iso_mul(dT, invT, invT);
+  if (mode == 1) {
+    set_row3(invT);
+    for (int k = 0; k < 16; ++k) S[kSlotInvTInner + k] = invT[k];
+    return;
+  }
+  float T[16];
+  iso_inverse(invT, T);
+  float x[6];
+  t2v<float>(T, x);
+  v2t<float>(x, T);
+  set_row3(T);
+  float Ti[16], I4[16], TR[16], KRt[16], iKRt[16];
+  iso_inverse(T, Ti);
+  set_identity4(I4);
+  iso_mul(T, I4, TR);
+  projector_matrices(pd.K, TR, KRt, iKRt);
+  for (int k = 0; k < 16; ++k) {
+    S[kSlotT + k] = T[k];
+    S[kSlotTinv + k] = Ti[k];
+    S[kSlotKRt + k] = KRt[k];
+  }
+}

@@ -1,0 +1,213 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars: index images, points, z-buffers and correspondence sets bit-exact on
+identical float inputs; normals within 1e-5 (the eigensolver's atan2/cos/sin
+come from different math libraries); H and b within 1e-5 relative of the FP64
+twin of the linearizer on identical inputs; transforms within 1e-5 rad and
+1e-5 m of the oracle's AlignerProjective on identical clouds.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from nicp_amd import capi, synth
+from tests.helpers import aparams, cams, identity16, params
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _frames(seed, rows, cols, **kw):
+    dr, dc, T = synth.pair(seed, rows, cols, **kw)
+    return dr, dc, T
+
+
+@pytest.mark.parametrize("seed,res", [(0, (240, 320)), (1, (480, 640)), (2, (120, 160)), (3, (960, 1280))])
+def test_frame_conversion(ctx, seed, res):
+    rows, cols = res
+    say("case frame_conversion seed=%d %dx%d" % (seed, rows, cols))
+    dr, _, _ = _frames(seed, rows, cols)
+    ocam, dcam = cams(rows, cols)
+    osp, dsp = params()
+    fo = O.Frame(dr, 0.001, ocam, osp)
+    cl = ctx.cloud_from_depth(dr, dcam, dsp, 0.001, keep_stats=True)
+    assert cl.n == fo.n
+    np.testing.assert_array_equal(cl.index_image(), fo.index_image)
+    pts, nrm, curv, op, on = cl.download()
+    assert pts.view(np.uint32).tolist() == fo.points.view(np.uint32).tolist() or np.array_equal(
+        pts.view(np.uint32), fo.points.view(np.uint32))
+    # which points carry a normal
+    has_o = np.abs(fo.normals[:, :3]).sum(1) > 0
+    has_d = np.abs(nrm[:, :3]).sum(1) > 0
+    flips = int((has_o != has_d).sum())
+    both = has_o & has_d
+    err = np.abs(nrm[both, :3] - fo.normals[both, :3]).max(axis=1)
+    bad = int((err > 1e-5).sum())
+    say("  points %d normals %d flips %d |dn|max %.3g >1e-5: %d" % (fo.n, int(both.sum()), flips, err.max(), bad))
+    # gate flips (curvature at the stats threshold) and tolerance misses must be rare
+    assert flips <= max(2, fo.n // 20000)
+    assert bad <= max(2, int(both.sum()) // 10000)
+    cerr = np.abs(curv - fo.curvature)
+    assert np.quantile(cerr[both], 0.999) < 1e-5
+    ev, el, mn, npt = cl.download_stats()
+    np.testing.assert_array_equal(npt, fo.npts)
+    np.testing.assert_array_equal(mn[:, :3].view(np.uint32), fo.means[:, :3].view(np.uint32))
+    # information matrices: upper triangle within a relative 1e-4 of the largest entry
+    up = [0, 4, 8, 5, 9, 10]
+    for dev, ref in ((op, fo.omega_p), (on, fo.omega_n)):
+        d = np.abs(dev[both][:, up] - ref[both][:, up]).max(axis=1)
+        scale = np.abs(ref[both][:, up]).max(axis=1) + 1e-30
+        assert np.quantile(d / scale, 0.999) < 1e-4
+
+
+def test_project_bitexact(ctx):
+    rows, cols = 240, 320
+    say("case project_bitexact")
+    dr, _, Tgt = _frames(5, rows, cols)
+    ocam, dcam = cams(rows, cols)
+    osp, _ = params()
+    fo = O.Frame(dr, 0.001, ocam, osp)
+    cl = ctx.cloud_from_host(fo.points, fo.normals, fo.curvature, fo.omega_p, fo.omega_n)
+    for T in (identity16(), Tgt):
+        KRt, _ = O.projector_matrices(np.array(ocam.K, np.float32), T)
+        oi, od = O.project(fo.points, KRt, rows, cols, 0.5, 4.5)
+        di, dd = ctx.project(cl, T, dcam, rows, cols)
+        np.testing.assert_array_equal(di, oi)
+        np.testing.assert_array_equal(dd.view(np.uint32), od.view(np.uint32))
+
+
+def _oracle_clouds(seed, rows, cols, **kw):
+    dr, dc, Tgt = _frames(seed, rows, cols, **kw)
+    ocam, dcam = cams(rows, cols)
+    osp, _ = params()
+    fr = O.Frame(dr, 0.001, ocam, osp)
+    fc = O.Frame(dc, 0.001, ocam, osp)
+    return fr, fc, Tgt, ocam, dcam
+
+
+def _upload(ctx, f):
+    return ctx.cloud_from_host(f.points, f.normals, f.curvature, f.omega_p, f.omega_n)
+
+
+@pytest.mark.parametrize("multi", [False, True])
+def test_correspondences_bitexact(ctx, multi):
+    rows, cols = 240, 320
+    say("case correspondences multi=%s" % multi)
+    fr, fc, Tgt, ocam, dcam = _oracle_clouds(6, rows, cols)
+    kw = dict(row_search_region=1, col_search_region=2) if multi else {}
+    oap, dap = aparams(**kw)
+    # reference projected with T, current with the identity, finder transform T^-1
+    T = Tgt
+    KRt, _ = O.projector_matrices(np.array(ocam.K, np.float32), T)
+    ri, _ = O.project(fr.points, KRt, rows, cols, 0.5, 4.5)
+    KRt0, _ = O.projector_matrices(np.array(ocam.K, np.float32), identity16())
+    ci, _ = O.project(fc.points, KRt0, rows, cols, 0.5, 4.5)
+    Tinv = np.linalg.inv(T.reshape(4, 4, order="F").astype(np.float64)).astype(np.float32).flatten(order="F")
+    op = O.correspondences(fr.cloud(), fc.cloud(), ri, ci, Tinv, oap, 1)
+    dp = ctx.correspondences(_upload(ctx, fr), _upload(ctx, fc), ri, ci, Tinv, dap)
+    say("  pairs oracle %d device %d" % (len(op), len(dp)))
+    assert len(op) > 1000
+    np.testing.assert_array_equal(dp, op)
+
+
+def test_linearize_vs_fp64(ctx):
+    rows, cols = 480, 640
+    say("case linearize_vs_fp64")
+    fr, fc, Tgt, ocam, dcam = _oracle_clouds(7, rows, cols)
+    oap, dap = aparams()
+    KRt, _ = O.projector_matrices(np.array(ocam.K, np.float32), Tgt)
+    ri, _ = O.project(fr.points, KRt, rows, cols, 0.5, 4.5)
+    KRt0, _ = O.projector_matrices(np.array(ocam.K, np.float32), identity16())
+    ci, _ = O.project(fc.points, KRt0, rows, cols, 0.5, 4.5)
+    Tinv = np.linalg.inv(Tgt.reshape(4, 4, order="F").astype(np.float64)).astype(np.float32).flatten(order="F")
+    pairs = O.correspondences(fr.cloud(), fc.cloud(), ri, ci, Tinv, oap, 1)
+    H64, b64, e64, i64 = O.linearize_f64(fr.cloud(), fc.cloud(), pairs, Tinv, oap)
+    H32, b32, e32, i32 = O.linearize(fr.cloud(), fc.cloud(), pairs, Tinv, oap, 1)
+    Hd, bd, ed, idd = ctx.linearize(_upload(ctx, fr), _upload(ctx, fc), pairs, Tinv, dap)
+    relH = np.abs(Hd - H64).max() / np.abs(H64).max()
+    relb = np.abs(bd - b64).max() / np.abs(b64).max()
+    relH32 = np.abs(H32 - H64).max() / np.abs(H64).max()
+    relb32 = np.abs(b32 - b64).max() / np.abs(b64).max()
+    say("  pairs %d  device vs fp64: H %.2e b %.2e | fp32 oracle vs fp64: H %.2e b %.2e" % (
+        len(pairs), relH, relb, relH32, relb32))
+    assert idd == i64
+    assert relH < 1e-5 and relb < 1e-5
+    assert abs(ed - e64) / max(e64, 1e-30) < 1e-5
+
+
+@pytest.mark.parametrize("seed", [1, 4])
+def test_align_vs_oracle(ctx, seed):
+    rows, cols = 480, 640
+    say("case align_vs_oracle seed=%d" % seed)
+    fr, fc, Tgt, ocam, dcam = _oracle_clouds(seed, rows, cols)
+    oap, dap = aparams()
+    To, ro = O.align(fr.cloud(), fc.cloud(), rows, cols, ocam, identity16(), oap, 1)
+    Td, rd = ctx.align(_upload(ctx, fr), _upload(ctx, fc), dcam, rows, cols, identity16(), dap)
+    dr, dt = synth.pose_error(Td, To)
+    gr, gt = synth.pose_error(Td, Tgt)
+    say("  device vs oracle: %.2e rad %.2e m | vs truth %.4f deg %.5f m | inliers %d/%d valid %d/%d" % (
+        np.radians(dr), dt, gr, gt, rd.inliers, ro.inliers, rd.solution_valid, ro.solution_valid))
+    assert np.radians(dr) < 1e-5 and dt < 1e-5
+    assert abs(rd.inliers - ro.inliers) <= max(5, ro.inliers // 5000)
+    assert rd.solution_valid == ro.solution_valid
+    # the device's last correspondences agree with the oracle's up to gate-boundary flips
+    dp = ctx.last_correspondences()
+    assert abs(len(dp) - ro.num_correspondences) <= max(5, ro.num_correspondences // 5000)
+
+
+def test_pipeline_accuracy(ctx):
+    """Depth -> clouds -> align entirely on the device, against the sampled ground truth."""
+    rows, cols = 480, 640
+    ocam, dcam = cams(rows, cols)
+    _, dsp = params()
+    _, dap = aparams()
+    ok = 0
+    for seed in range(2, 6):
+        say("case pipeline seed=%d" % seed)
+        dr, dc, Tgt = _frames(seed, rows, cols)
+        cr = ctx.cloud_from_depth(dr, dcam, dsp)
+        cc = ctx.cloud_from_depth(dc, dcam, dsp)
+        T, res = ctx.align(cr, cc, dcam, rows, cols, identity16(), dap)
+        rot, tr = synth.pose_error(T, Tgt)
+        say("  err %.4f deg %.5f m valid %d time %.2f ms" % (rot, tr, res.solution_valid, res.total_time_ms))
+        ok += rot < 0.1 and tr < 0.002
+    assert ok >= 3
+
+
+def test_batch_matches_single(ctx):
+    shapes = [(120, 160), (240, 320), (480, 640), (240, 320)]
+    say("case batch_matches_single")
+    _, dsp = params()
+    _, dap = aparams()
+    frames, cam_list, singles = [], [], []
+    for i, (rows, cols) in enumerate(shapes):
+        dr, dc, Tgt = _frames(20 + i, rows, cols)
+        _, dcam = cams(rows, cols)
+        frames += [dr, dc]
+        cam_list += [dcam, dcam]
+    b = capi.Batch(ctx, [f.shape for f in frames], cam_list, dsp)
+    for k, f in enumerate(frames):
+        b.upload(k, f)
+    b.convert()
+    ref = np.arange(0, len(frames), 2)
+    cur = ref + 1
+    Tb, rb = b.align(ref, cur, None, dap, use_graph=False)
+    Tg, rg = b.align(ref, cur, None, dap, use_graph=True)
+    Tg2, _ = b.align(ref, cur, None, dap, use_graph=True)
+    for i, (rows, cols) in enumerate(shapes):
+        _, dcam = cams(rows, cols)
+        cr = ctx.cloud_from_depth(frames[2 * i], dcam, dsp)
+        cc = ctx.cloud_from_depth(frames[2 * i + 1], dcam, dsp)
+        Ts, rs = ctx.align(cr, cc, dcam, rows, cols, identity16(), dap)
+        np.testing.assert_array_equal(Tb[i].view(np.uint32), Ts.view(np.uint32))
+        assert rb[i].inliers == rs.inliers
+        bc = b.frame_cloud(2 * i, (rows, cols))
+        assert bc.n == cr.n
+    np.testing.assert_array_equal(Tg.view(np.uint32), Tb.view(np.uint32))
+    np.testing.assert_array_equal(Tg2.view(np.uint32), Tb.view(np.uint32))
