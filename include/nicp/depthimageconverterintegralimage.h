@@ -1,0 +1,120 @@
+// nicp::StatsCalculatorIntegralImage (nicp/nicp/statscalculatorintegralimage.h:14-133),
+// nicp::Point/NormalInformationMatrixCalculator (nicp/nicp/informationmatrixcalculator.h:14-171)
+// and nicp::DepthImageConverterIntegralImage (nicp/nicp/depthimageconverterintegralimage.h:15-63):
+// parameter holders with the reference's setters and defaults; compute() runs
+// the whole conversion on the device.
+#pragma once
+
+#include "cloud.h"
+#include "definitions.h"
+#include "pinholepointprojector.h"
+
+namespace nicp {
+
+class StatsCalculatorIntegralImage {
+ public:
+  float worldRadius() const { return _worldRadius; }
+  void setWorldRadius(float v) { _worldRadius = v; }
+  int maxImageRadius() const { return _maxImageRadius; }
+  void setMaxImageRadius(int v) { _maxImageRadius = v; }
+  int minImageRadius() const { return _minImageRadius; }
+  void setMinImageRadius(int v) { _minImageRadius = v; }
+  int minPoints() const { return _minPoints; }
+  void setMinPoints(int v) { _minPoints = v; }
+  float curvatureThreshold() const { return _curvatureThreshold; }
+  void setCurvatureThreshold(float v) { _curvatureThreshold = v; }
+
+ private:
+  // nicp/nicp/statscalculatorintegralimage.cpp:12-18
+  float _worldRadius = 0.1f;
+  int _maxImageRadius = 30, _minImageRadius = 10, _minPoints = 50;
+  float _curvatureThreshold = 0.02f;
+};
+
+class InformationMatrixCalculator {
+ public:
+  float curvatureThreshold() const { return _curvatureThreshold; }
+  void setCurvatureThreshold(float v) { _curvatureThreshold = v; }
+  // diagonals of the flat / non-flat information matrices in the eigenbasis
+  const float *flatInformationMatrix() const { return _flat; }
+  void setFlatInformationMatrix(float a, float b, float c) { _flat[0] = a; _flat[1] = b; _flat[2] = c; }
+  const float *nonFlatInformationMatrix() const { return _nonFlat; }
+  void setNonFlatInformationMatrix(float a, float b, float c) { _nonFlat[0] = a; _nonFlat[1] = b; _nonFlat[2] = c; }
+
+ protected:
+  float _flat[3] = {1.0f, 1.0f, 1.0f}, _nonFlat[3] = {1.0f, 1.0f, 1.0f};
+  float _curvatureThreshold = 0.02f;
+};
+
+// nicp/nicp/informationmatrixcalculator.h:115-119
+class PointInformationMatrixCalculator : public InformationMatrixCalculator {
+ public:
+  PointInformationMatrixCalculator() {
+    setFlatInformationMatrix(1000.0f, 0.001f, 0.001f);
+    setNonFlatInformationMatrix(1.0f, 0.1f, 0.1f);
+  }
+};
+
+// nicp/nicp/informationmatrixcalculator.h:150-155
+class NormalInformationMatrixCalculator : public InformationMatrixCalculator {
+ public:
+  NormalInformationMatrixCalculator() {
+    setFlatInformationMatrix(1000.0f, 0.001f, 0.001f);
+    setNonFlatInformationMatrix(0.1f, 1.0f, 1.0f);
+  }
+};
+
+class DepthImageConverterIntegralImage {
+ public:
+  DepthImageConverterIntegralImage(PinholePointProjector *projector, StatsCalculatorIntegralImage *statsCalculator,
+                                   PointInformationMatrixCalculator *pointInformationMatrixCalculator,
+                                   NormalInformationMatrixCalculator *normalInformationMatrixCalculator)
+      : _projector(projector),
+        _statsCalculator(statsCalculator),
+        _pointCalc(pointInformationMatrixCalculator),
+        _normalCalc(normalInformationMatrixCalculator) {}
+
+  nicp_stats_params statsParams() const {
+    nicp_stats_params s;
+    s.world_radius = _statsCalculator->worldRadius();
+    s.min_image_radius = _statsCalculator->minImageRadius();
+    s.max_image_radius = _statsCalculator->maxImageRadius();
+    s.min_points = _statsCalculator->minPoints();
+    s.curvature_threshold = _statsCalculator->curvatureThreshold();
+    std::memcpy(s.point_flat, _pointCalc->flatInformationMatrix(), sizeof(s.point_flat));
+    std::memcpy(s.point_nonflat, _pointCalc->nonFlatInformationMatrix(), sizeof(s.point_nonflat));
+    std::memcpy(s.normal_flat, _normalCalc->flatInformationMatrix(), sizeof(s.normal_flat));
+    std::memcpy(s.normal_nonflat, _normalCalc->nonFlatInformationMatrix(), sizeof(s.normal_nonflat));
+    s.point_curvature_threshold = _pointCalc->curvatureThreshold();
+    s.normal_curvature_threshold = _normalCalc->curvatureThreshold();
+    return s;
+  }
+
+  // DepthImage_convert_16UC1_to_32FC1 + DepthImageConverterIntegralImage::compute
+  // (nicp/nicp/depthimageconverterintegralimage.cpp:18-64); only the identity sensor
+  // offset is supported, which is what the reference's apps pass.
+  void compute(Cloud &cloud, const RawDepthImage &depth, float depthScale,
+               const Isometry3f &sensorOffset = Isometry3f::Identity()) {
+    if (!sensorOffset.isIdentity())
+      throw std::runtime_error("nicp_amd: DepthImageConverter supports the identity sensor offset only");
+    _projector->setImageSize(depth.rows, depth.cols);
+    nicp_camera cam = _projector->camera();
+    nicp_stats_params sp = statsParams();
+    nicp_cloud *h = nullptr;
+    check(nicp_cloud_from_depth_u16(Context::get(), depth.data.data(), depth.rows, depth.cols, depthScale, &cam, &sp,
+                                    0, &h));
+    cloud.reset(h);
+    _indexImage.create(depth.rows, depth.cols);
+    check(nicp_cloud_index_image(h, _indexImage.data.data()));
+  }
+  const IntImage &indexImage() const { return _indexImage; }
+
+ private:
+  PinholePointProjector *_projector;
+  StatsCalculatorIntegralImage *_statsCalculator;
+  PointInformationMatrixCalculator *_pointCalc;
+  NormalInformationMatrixCalculator *_normalCalc;
+  IntImage _indexImage;
+};
+
+}  // namespace nicp

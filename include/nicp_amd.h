@@ -92,6 +92,7 @@ typedef struct {
   double rotational_ratio;
   int    solution_valid;       /* Aligner::solutionValid(); 0 also when H is zero */
   float  total_time_ms;        /* Aligner::totalTime(): device time of the call */
+  int    overlap_pixels;       /* pixels of the last iteration where both index images hold a point */
 } nicp_align_result;
 
 const char *nicp_last_error(void);
@@ -165,6 +166,15 @@ int nicp_batch_align(nicp_batch *batch, int num_pairs, const int *ref_frame, con
                      nicp_align_result *results);
 /* number of points of a batch frame, and a cloud view of it (downloads only) */
 int nicp_batch_frame_cloud(nicp_batch *batch, int frame, nicp_cloud **out);
+
+/* Per-kernel device time (no reference counterpart; the reference only has
+ * Aligner::totalTime(), nicp/nicp/aligner.h:348). When enabled, every launch of the
+ * context is bracketed by HIP events on its stream (event nodes inside captured graphs);
+ * enabling resets the totals. total_ms / launches receive nicp_kernel_count() entries. */
+int nicp_kernel_count(void);
+const char *nicp_kernel_name(int kernel_id);
+int nicp_ctx_profile(nicp_ctx *ctx, int enable);
+int nicp_ctx_profile_read(nicp_ctx *ctx, double *total_ms, long long *launches);
 
 #ifdef __cplusplus
 }

@@ -45,7 +45,8 @@ class AlignResult(C.Structure):
     _fields_ = [("error", C.c_float), ("inliers", C.c_int), ("num_correspondences", C.c_int),
                 ("H", C.c_double * 36), ("b", C.c_double * 6), ("mean", C.c_double * 6),
                 ("omega", C.c_double * 36), ("translational_ratio", C.c_double),
-                ("rotational_ratio", C.c_double), ("solution_valid", C.c_int), ("total_time_ms", C.c_float)]
+                ("rotational_ratio", C.c_double), ("solution_valid", C.c_int), ("total_time_ms", C.c_float),
+                ("overlap_pixels", C.c_int)]
 
 
 #: every entry point declared in include/nicp_amd.h
@@ -55,7 +56,8 @@ EXPORTS = [
     "nicp_cloud_download", "nicp_cloud_download_stats", "nicp_cloud_index_image", "nicp_project",
     "nicp_correspondences", "nicp_linearize", "nicp_align", "nicp_last_correspondences", "nicp_batch_create",
     "nicp_batch_destroy", "nicp_batch_upload_depth", "nicp_batch_convert", "nicp_batch_align",
-    "nicp_batch_frame_cloud",
+    "nicp_batch_frame_cloud", "nicp_kernel_count", "nicp_kernel_name", "nicp_ctx_profile",
+    "nicp_ctx_profile_read",
 ]
 
 _lib = None
@@ -71,6 +73,9 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.nicp_last_error.restype = C.c_char_p
         L.nicp_ctx_stream.restype = vp
+        L.nicp_kernel_name.restype = C.c_char_p
+        L.nicp_kernel_name.argtypes = [C.c_int]
+        L.nicp_kernel_count.restype = C.c_int
         sig = {
             "nicp_ctx_create": [C.c_int, vp], "nicp_ctx_destroy": [vp], "nicp_ctx_synchronize": [vp],
             "nicp_ctx_stream": [vp],
@@ -90,6 +95,7 @@ def lib():
             "nicp_batch_convert": [vp],
             "nicp_batch_align": [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp],
             "nicp_batch_frame_cloud": [vp, C.c_int, vp],
+            "nicp_ctx_profile": [vp, C.c_int], "nicp_ctx_profile_read": [vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -177,6 +183,18 @@ class Context:
 
     def synchronize(self):
         _check(lib().nicp_ctx_synchronize(self.h))
+
+    def profile(self, enable=True):
+        """Starts (and resets) or stops per-kernel HIP-event timing on this context."""
+        _check(lib().nicp_ctx_profile(self.h, int(enable)))
+
+    def profile_read(self):
+        """{kernel name: (total ms, launches)} accumulated since profile(True)."""
+        n = lib().nicp_kernel_count()
+        ms = np.zeros(n, np.float64)
+        cnt = np.zeros(n, np.int64)
+        _check(lib().nicp_ctx_profile_read(self.h, _p(ms), _p(cnt)))
+        return {lib().nicp_kernel_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
 
     # -- clouds --------------------------------------------------------------------------
     def cloud_from_depth(self, depth_u16, cam, sp, depth_scale=0.001, keep_stats=False):

@@ -94,6 +94,90 @@ struct Carver {
 
 }  // namespace
 
+
+// ---------------------------------------------------------------------------------------
+// per-kernel device timing with HIP events (recorded around each launch on the
+// launching stream; inside a captured graph the records become event nodes)
+// ---------------------------------------------------------------------------------------
+enum KernelId {
+  KID_COUNT, KID_SCAN, KID_UNPROJECT, KID_INTEGRAL_X, KID_INTEGRAL_Y, KID_STATS, KID_ZFILL, KID_ZPROJECT_CUR,
+  KID_ZRESOLVE, KID_ZPROJECT_REF, KID_CORR_LIN, KID_SOLVE, KID_LIN_PAIRS, KID_SOLVE_STATS, KID_N
+};
+static const char *kKernelNames[KID_N] = {"k_count", "k_scan", "k_unproject", "k_integral_x", "k_integral_y",
+                                          "k_stats", "k_zfill", "k_zproject(current)", "k_zresolve",
+                                          "k_zproject(reference)", "k_corr_lin", "k_solve", "k_lin_pairs",
+                                          "k_solve(statistics)"};
+
+namespace {
+struct ProfEntry {
+  int id;
+  hipEvent_t a, b;
+};
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<ProfEntry> *sink = nullptr;  // where new entries go (eager list or a graph's list)
+  std::vector<ProfEntry> eager;
+  double total_ms[KID_N] = {0};
+  long long launches[KID_N] = {0};
+  hipEvent_t take() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+    return e;
+  }
+  void begin(int id, hipStream_t s, ProfEntry &en) {
+    en.id = id;
+    en.a = take();
+    en.b = take();
+    if (en.a) (void)hipEventRecord(en.a, s);
+  }
+  void end(hipStream_t s, ProfEntry &en) {
+    if (en.b) (void)hipEventRecord(en.b, s);
+    if (en.a && en.b && sink) sink->push_back(en);
+  }
+  // reads finished entries; recycle = return their events to the pool
+  void read(std::vector<ProfEntry> &list, bool recycle) {
+    for (const ProfEntry &en : list) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, en.a, en.b) == hipSuccess) {
+        total_ms[en.id] += ms;
+        launches[en.id] += 1;
+      }
+      if (recycle) {
+        pool.push_back(en.a);
+        pool.push_back(en.b);
+      }
+    }
+    if (recycle) list.clear();
+  }
+  void destroy_list(std::vector<ProfEntry> &list) {
+    for (const ProfEntry &en : list) {
+      (void)hipEventDestroy(en.a);
+      (void)hipEventDestroy(en.b);
+    }
+    list.clear();
+  }
+  void release() {
+    destroy_list(eager);
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+  }
+};
+}  // namespace
+
+#define LAUNCH(prof, id, kern, grid, block, stream, ...)                        \
+  do {                                                                          \
+    ProfEntry en_{};                                                            \
+    const bool p_ = (prof) != nullptr && (prof)->on;                           \
+    if (p_) (prof)->begin((id), (stream), en_);                                 \
+    hipLaunchKernelGGL(kern, grid, block, 0, stream, __VA_ARGS__);              \
+    if (p_) (prof)->end((stream), en_);                                         \
+  } while (0)
 // ---------------------------------------------------------------------------------------
 // objects
 // ---------------------------------------------------------------------------------------
@@ -106,6 +190,8 @@ struct nicp_ctx {
   // last single-pair correspondences (device pointer into pairs_buf) and its image size
   int last_rows = 0, last_cols = 0;
   bool last_valid = false;
+  Prof prof;
+  std::vector<ProfEntry> *pending_graph = nullptr;  // graph events not yet read
 };
 
 struct nicp_cloud {
@@ -120,10 +206,16 @@ struct nicp_cloud {
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
   std::vector<int> key;
+  std::vector<ProfEntry> events;  // timing events recorded inside the graph
   void reset() {
     if (exec) (void)hipGraphExecDestroy(exec);
     exec = nullptr;
     key.clear();
+    for (const ProfEntry &en : events) {
+      (void)hipEventDestroy(en.a);
+      (void)hipEventDestroy(en.b);
+    }
+    events.clear();
   }
 };
 
@@ -257,15 +349,15 @@ void frame_pointers(FrameDev &f, char *base, const FrameLayout &L, bool with_sta
 }
 
 // Launch sequence of the conversion for a frame table already on the device.
-int launch_convert(hipStream_t s, const FrameDev *d_frames, int num_frames, const int *d_blk, int total_tiles,
+int launch_convert(Prof *pf, hipStream_t s, const FrameDev *d_frames, int num_frames, const int *d_blk, int total_tiles,
                    const int *d_x, int total_x, const int *d_y, int total_y, const StatsParamsDev &sp) {
   if (total_tiles == 0) return NICP_OK;
-  hipLaunchKernelGGL(k_count, dim3(total_tiles), dim3(kThreads), 0, s, d_frames, d_blk);
-  hipLaunchKernelGGL(k_scan, dim3(num_frames), dim3(1024), 0, s, d_frames);
-  hipLaunchKernelGGL(k_unproject, dim3(total_tiles), dim3(kThreads), 0, s, d_frames, d_blk);
-  hipLaunchKernelGGL(k_integral_x, dim3(total_x), dim3(kThreads), 0, s, d_frames, d_x);
-  hipLaunchKernelGGL(k_integral_y, dim3(total_y), dim3(kThreads), 0, s, d_frames, d_y);
-  hipLaunchKernelGGL(k_stats, dim3(total_tiles), dim3(kThreads), 0, s, d_frames, d_blk, sp);
+  LAUNCH(pf, KID_COUNT, k_count, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
+  LAUNCH(pf, KID_SCAN, k_scan, dim3(num_frames), dim3(1024), s, d_frames);
+  LAUNCH(pf, KID_UNPROJECT, k_unproject, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
+  LAUNCH(pf, KID_INTEGRAL_X, k_integral_x, dim3(total_x), dim3(kThreads), s, d_frames, d_x);
+  LAUNCH(pf, KID_INTEGRAL_Y, k_integral_y, dim3(total_y), dim3(kThreads), s, d_frames, d_y);
+  LAUNCH(pf, KID_STATS, k_stats, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk, sp);
   HIPCHK(hipGetLastError());
   return NICP_OK;
 }
@@ -382,35 +474,35 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
 }
 
 // the launch sequence of AlignerProjective::align for all pairs of a plan
-int launch_align(hipStream_t s, const PairPlan &plan, const PairDev *d_pairs, const FrameDev *d_frames,
+int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d_pairs, const FrameDev *d_frames,
                  const int *d_blk_pair, const int *d_blk_ref, const int *d_blk_cur, const nicp_align_params *ap) {
   const int np = (int)plan.pairs.size();
   const AlignParamsDev a = to_dev(ap);
   const dim3 tb(kThreads);
   if (plan.total_pix_tiles == 0) return NICP_OK;
-  hipLaunchKernelGGL(k_zfill, dim3(plan.total_pix_tiles), tb, 0, s, d_pairs, d_blk_pair);
+  LAUNCH(pf, KID_ZFILL, k_zfill, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair);
   if (plan.total_ctiles)
-    hipLaunchKernelGGL(k_zproject, dim3(plan.total_ctiles), tb, 0, s, d_pairs, d_frames, d_blk_cur, 1);
-  hipLaunchKernelGGL(k_zresolve, dim3(plan.total_pix_tiles), tb, 0, s, d_pairs, d_blk_pair, (float *)nullptr);
+    LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1);
+  LAUNCH(pf, KID_ZRESOLVE, k_zresolve, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair, (float *)nullptr);
   const int inner = ap->inner_iterations;
   for (int it = 0; it < ap->outer_iterations; ++it) {
     const bool last = it == ap->outer_iterations - 1;
     if (plan.total_rtiles)
-      hipLaunchKernelGGL(k_zproject, dim3(plan.total_rtiles), tb, 0, s, d_pairs, d_frames, d_blk_ref, 0);
-    hipLaunchKernelGGL(k_corr_lin, dim3(plan.total_pix_tiles), tb, 0, s, d_pairs, d_frames, d_blk_pair, a,
-                       (int)(last || inner > 1));
-    hipLaunchKernelGGL(k_solve, dim3(np), dim3(64), 0, s, d_pairs, ap->lambda, inner > 1 ? 1 : 0, 0);
+      LAUNCH(pf, KID_ZPROJECT_REF, k_zproject, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0);
+    LAUNCH(pf, KID_CORR_LIN, k_corr_lin, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
+           (int)(last || inner > 1));
+    LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(64), s, d_pairs, ap->lambda, inner > 1 ? 1 : 0, 0);
     for (int k = 1; k < inner; ++k) {
-      hipLaunchKernelGGL(k_lin_pairs, dim3(plan.total_pix_tiles), tb, 0, s, d_pairs, d_frames, d_blk_pair, a,
-                         kSlotInvTInner, 0);
-      hipLaunchKernelGGL(k_solve, dim3(np), dim3(64), 0, s, d_pairs, ap->lambda, k == inner - 1 ? 0 : 1, 1);
+      LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
+             kSlotInvTInner, 0);
+      LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(64), s, d_pairs, ap->lambda, k == inner - 1 ? 0 : 1, 1);
     }
   }
   // Aligner::_computeStatistics: one more linearisation at the final T over the last correspondences
   if (ap->outer_iterations > 0) {
-    hipLaunchKernelGGL(k_lin_pairs, dim3(plan.total_pix_tiles), tb, 0, s, d_pairs, d_frames, d_blk_pair, a,
-                       kSlotTinv, 0);
-    hipLaunchKernelGGL(k_solve, dim3(np), dim3(64), 0, s, d_pairs, 0.0f, 2, 0);
+    LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
+           kSlotTinv, 0);
+    LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(64), s, d_pairs, 0.0f, 2, 0);
   }
   HIPCHK(hipGetLastError());
   return NICP_OK;
@@ -595,6 +687,7 @@ void fill_result(const double *r, const float *T, const nicp_align_params *ap, n
   out->error = (float)r[27];
   out->inliers = (int)llround(r[28]);
   out->num_correspondences = (int)llround(r[29]);
+  out->overlap_pixels = (int)llround(r[30]);
   double Hs[36];
   e = 0;
   for (int i = 0; i < 6; ++i)
@@ -645,6 +738,7 @@ extern "C" int nicp_ctx_destroy(nicp_ctx *c) {
   DevBuf *bufs[] = {&c->frames_tbl, &c->pairs_tbl, &c->maps, &c->zbuf, &c->cur_index, &c->pairs_buf,
                     &c->state, &c->partials, &c->result, &c->depth_tmp, &c->scratch_in};
   for (DevBuf *b : bufs) b->release();
+  c->prof.release();
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
   (void)hipStreamDestroy(c->stream);
@@ -703,7 +797,8 @@ extern "C" int nicp_cloud_from_depth_u16(nicp_ctx *ctx, const uint16_t *depth, i
     if (e == hipSuccess) e = hipMemcpyAsync(d_f, &f, sizeof(FrameDev), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d_maps, zeros.data(), zeros.size() * sizeof(int), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) { rc = set_err(NICP_ERR_HIP, "upload: %s", hipGetErrorString(e)); break; }
-    rc = launch_convert(s, d_f, 1, d_maps, nt, d_maps + nt, nx, d_maps + nt + nx, ny, to_dev(sp, keep_stats != 0));
+    ctx->prof.sink = &ctx->prof.eager;
+    rc = launch_convert(&ctx->prof, s, d_f, 1, d_maps, nt, d_maps + nt, nx, d_maps + nt + nx, ny, to_dev(sp, keep_stats != 0));
     if (rc != NICP_OK) break;
     e = hipMemcpyAsync(&cl->n, f.npoints, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1082,7 +1177,8 @@ extern "C" int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud
   set_identity4(I4);
   CHK(prepare_align(ctx->stream, plan, B, init_guess ? init_guess : I4, &d_bp, &d_br, &d_bc, &d_pairs));
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  CHK(launch_align(ctx->stream, plan, d_pairs, d_frames, d_bp, d_br, d_bc, ap));
+  ctx->prof.sink = &ctx->prof.eager;
+  CHK(launch_align(&ctx->prof, ctx->stream, plan, d_pairs, d_frames, d_bp, d_br, d_bc, ap));
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   float T[16];
   double r[2 * kPartial];
@@ -1200,6 +1296,7 @@ extern "C" int nicp_batch_destroy(nicp_batch *b) {
   if (!b) return NICP_OK;
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->ctx->pending_graph == &b->graph.events) b->ctx->pending_graph = nullptr;
   b->graph.reset();
   b->conv_graph.reset();
   DevBuf *bufs[] = {&b->pairs_tbl, &b->maps, &b->zbuf, &b->cur_index, &b->pairs_buf, &b->state, &b->partials, &b->result};
@@ -1226,7 +1323,8 @@ extern "C" int nicp_batch_upload_depth(nicp_batch *b, int frame, const uint16_t 
 extern "C" int nicp_batch_convert(nicp_batch *b) {
   if (!b) return set_err(NICP_ERR_INVALID, "batch is null");
   HIPCHK(hipSetDevice(b->ctx->device));
-  CHK(launch_convert(b->ctx->stream, b->d_frames, b->num_frames, b->d_blk_frame, b->total_tiles, b->d_xblk_frame,
+  b->ctx->prof.sink = &b->ctx->prof.eager;
+  CHK(launch_convert(&b->ctx->prof, b->ctx->stream, b->d_frames, b->num_frames, b->d_blk_frame, b->total_tiles, b->d_xblk_frame,
                      b->total_xblk, b->d_yblk_frame, b->total_yblk, b->sp));
   return NICP_OK;
 }
@@ -1267,11 +1365,14 @@ extern "C" int nicp_batch_align(nicp_batch *b, int num_pairs, const int *ref_fra
     key.push_back((int)(((uintptr_t)d_pairs) & 0x7fffffff));
     key.push_back((int)(((uintptr_t)b->partials.p) & 0x7fffffff));
     key.push_back((int)(((uintptr_t)b->zbuf.p) & 0x7fffffff));
+    key.push_back(ctx->prof.on ? 1 : 0);
     if (b->graph.exec == nullptr || b->graph.key != key) {
       b->graph.reset();
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      int rc = launch_align(ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap);
+      ctx->prof.sink = &b->graph.events;
+      int rc = launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap);
+      ctx->prof.sink = &ctx->prof.eager;
       hipError_t e = hipStreamEndCapture(ctx->stream, &g);
       if (rc != NICP_OK) {
         if (g) (void)hipGraphDestroy(g);
@@ -1287,8 +1388,10 @@ extern "C" int nicp_batch_align(nicp_batch *b, int num_pairs, const int *ref_fra
       b->graph.key = key;
     }
     HIPCHK(hipGraphLaunch(b->graph.exec, ctx->stream));
+    ctx->pending_graph = ctx->prof.on ? &b->graph.events : nullptr;
   } else {
-    CHK(launch_align(ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap));
+    ctx->prof.sink = &ctx->prof.eager;
+    CHK(launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap));
   }
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   if (!T_out16 && !results) return NICP_OK;
@@ -1300,6 +1403,11 @@ extern "C" int nicp_batch_align(nicp_batch *b, int num_pairs, const int *ref_fra
   HIPCHK(hipStreamSynchronize(ctx->stream));
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (ctx->prof.on) {
+    ctx->prof.read(ctx->prof.eager, true);
+    if (ctx->pending_graph) ctx->prof.read(*ctx->pending_graph, false);
+    ctx->pending_graph = nullptr;
+  }
   for (int i = 0; i < num_pairs; ++i) {
     const float *T = &st[(size_t)i * kStateFloats];
     if (T_out16) memcpy(T_out16 + 16 * (size_t)i, T, 16 * sizeof(float));
@@ -1326,5 +1434,40 @@ extern "C" int nicp_batch_frame_cloud(nicp_batch *b, int frame, nicp_cloud **out
   cl->has_stats = 0;
   cl->alloc = nullptr;
   *out = cl;
+  return NICP_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// per-kernel timing
+// ---------------------------------------------------------------------------------------
+extern "C" int nicp_kernel_count(void) { return KID_N; }
+
+extern "C" const char *nicp_kernel_name(int id) { return (id >= 0 && id < KID_N) ? kKernelNames[id] : ""; }
+
+extern "C" int nicp_ctx_profile(nicp_ctx *ctx, int enable) {
+  if (!ctx) return set_err(NICP_ERR_INVALID, "ctx is null");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->prof.read(ctx->prof.eager, true);
+  ctx->prof.on = enable != 0;
+  for (int i = 0; i < KID_N; ++i) {
+    ctx->prof.total_ms[i] = 0.0;
+    ctx->prof.launches[i] = 0;
+  }
+  ctx->pending_graph = nullptr;
+  return NICP_OK;
+}
+
+extern "C" int nicp_ctx_profile_read(nicp_ctx *ctx, double *total_ms, long long *launches) {
+  if (!ctx) return set_err(NICP_ERR_INVALID, "ctx is null");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->prof.read(ctx->prof.eager, true);
+  if (ctx->pending_graph) ctx->prof.read(*ctx->pending_graph, false);
+  ctx->pending_graph = nullptr;
+  for (int i = 0; i < KID_N; ++i) {
+    if (total_ms) total_ms[i] = ctx->prof.total_ms[i];
+    if (launches) launches[i] = ctx->prof.launches[i];
+  }
   return NICP_OK;
 }

@@ -8,6 +8,7 @@ namespace nicp_dev {
 constexpr int kTile = 1024;          // pixels (or points) per per-pixel block
 constexpr int kThreads = 256;        // threads per block (4 wavefronts of 64)
 constexpr int kPartial = 32;         // doubles per block partial of the linearizer
+constexpr int kVals = 31;            // used partial slots: H (21), b (6), chi2, inliers, count, both-valid pixels
 constexpr uint64_t kZInit = ~0ull;   // empty z-buffer cell
 
 // One frame (a depth image and the cloud made from it). Every pointer is a

@@ -546,7 +546,8 @@ __device__ __forceinline__ void acc_add(Acc2 &a, float x) {
 
 // The per-correspondence terms of Linearizer::update (linearizer.cpp:55-112) for
 // J_p = [I | S(p)], J_n = [0 | S(n)], S(v) = -2[v]x (bm_se3.h:55-66).
-// Output order: H upper triangle row-major (21), b (6), chi2, inliers, count.
+// Output order: H upper triangle row-major (21), b (6), chi2, inliers, count; slot 30 counts
+// the pixels where both index images hold a point.
 __device__ __forceinline__ void lin_terms(const float *rp, const float *rn, float4 cp, float4 cn, const float4 *om,
                                           const AlignParamsDev &ap, Acc2 *acc) {
   const float4 o0 = om[0], o1 = om[1], o2 = om[2];
@@ -609,12 +610,12 @@ __device__ __forceinline__ void lin_terms(const float *rp, const float *rn, floa
   acc_add(acc[28], 1.0f);
 }
 
-// block-wide FP64 reduction of the 30 compensated accumulators into one partial
+// block-wide FP64 reduction of the kVals compensated accumulators into one partial
 __device__ void block_partial(Acc2 *acc, double *out) {
   __shared__ double red[kThreads / 64][kPartial];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-  for (int v = 0; v < 30; ++v) {
+  for (int v = 0; v < kVals; ++v) {
     double x = (double)acc[v].s + (double)acc[v].c;
     x = wave_sum_d(x);
     if (lane == 0) red[wave][v] = x;
@@ -622,7 +623,7 @@ __device__ void block_partial(Acc2 *acc, double *out) {
   __syncthreads();
   if (threadIdx.x < kPartial) {
     double s = 0.0;
-    if (threadIdx.x < 30)
+    if (threadIdx.x < kVals)
       for (int w = 0; w < kThreads / 64; ++w) s += red[w][threadIdx.x];
     out[threadIdx.x] = s;
   }
@@ -707,9 +708,9 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_corr_lin(const PairDev 
 #pragma unroll
   for (int k = 0; k < 16; ++k) T[k] = pd.state[kSlotTinv + k];
   const bool multi = ap.row_region != 0 || ap.col_region != 0;
-  Acc2 acc[30];
+  Acc2 acc[kVals];
 #pragma unroll
-  for (int v = 0; v < 30; ++v) acc[v] = Acc2{0.0f, 0.0f};
+  for (int v = 0; v < kVals; ++v) acc[v] = Acc2{0.0f, 0.0f};
   for (int k = 0; k < 4; ++k) {
     const int p = base + k * kThreads + threadIdx.x;
     if (p >= P) break;
@@ -722,7 +723,10 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_corr_lin(const PairDev 
     bool ok = false;
     if (!multi) {
       ci = pd.cur_index[p];
-      if (ri >= 0 && ci >= 0) ok = gates(fr, fc, ri, ci, T, ap, rp, rn, cp, cn);
+      if (ri >= 0 && ci >= 0) {
+        acc_add(acc[30], 1.0f);
+        ok = gates(fr, fc, ri, ci, T, ap, rp, rn, cp, cn);
+      }
     } else if (ri >= 0) {
       const int r = p / pd.cols, c = p - r * pd.cols;
       ci = window_search(fr, fc, pd.cur_index, pd.rows, pd.cols, r, c, ri, T, ap, rp, rn);
@@ -753,9 +757,9 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_lin_pairs(const PairDev
   float T[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) T[k] = pd.state[slot + k];
-  Acc2 acc[30];
+  Acc2 acc[kVals];
 #pragma unroll
-  for (int v = 0; v < 30; ++v) acc[v] = Acc2{0.0f, 0.0f};
+  for (int v = 0; v < kVals; ++v) acc[v] = Acc2{0.0f, 0.0f};
   for (int k = 0; k < 4; ++k) {
     const int p = base + k * kThreads + threadIdx.x;
     if (p >= P) break;
@@ -773,7 +777,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_lin_pairs(const PairDev
 namespace {
 __device__ void reduce_partials(const PairDev &pd, double *tot /* shared, kPartial */) {
   const int lane = threadIdx.x;
-  for (int v = 0; v < 30; ++v) {
+  for (int v = 0; v < kVals; ++v) {
     double s = 0.0;
     for (int i = lane; i < pd.n_tiles; i += 64) s += pd.partials[(size_t)i * kPartial + v];
     s = wave_sum_d(s);
@@ -793,7 +797,7 @@ extern "C" __global__ __launch_bounds__(64) void k_solve(const PairDev *__restri
   __syncthreads();
   if (threadIdx.x != 0) return;
   double *res = pd.result + (mode == 2 ? kPartial : 0);
-  for (int v = 0; v < 30; ++v) res[v] = tot[v];
+  for (int v = 0; v < kVals; ++v) res[v] = tot[v];
   if (mode == 2) return;
   // H from the row-major upper triangle
   double H[36], b[6];
