@@ -111,6 +111,14 @@ void *nicp_ctx_stream(nicp_ctx *ctx);
 int nicp_cloud_from_depth_u16(nicp_ctx *ctx, const uint16_t *depth, int rows, int cols, float depth_scale,
                               const nicp_camera *cam, const nicp_stats_params *sp, int keep_stats,
                               nicp_cloud **out);
+/* The same conversion from a float depth image in metres (DepthImage, 32FC1). */
+int nicp_cloud_from_depth_f32(nicp_ctx *ctx, const float *depth, int rows, int cols, const nicp_camera *cam,
+                              const nicp_stats_params *sp, int keep_stats, nicp_cloud **out);
+/* Replaces DepthImage_scale (nicp/nicp/imageutils.cpp:6-37): (rows/step) x (cols/step) block means
+ * of the non-zero depths, 0 where the block variance exceeds max_depth_cov (reference default 0.01).
+ * Input is uint16 times depth_scale, or float metres when depthf is given. */
+int nicp_depth_scale(nicp_ctx *ctx, const uint16_t *depth, const float *depthf, int rows, int cols, float depth_scale,
+                     int step, float max_depth_cov, float *out);
 /* Replaces filling a Cloud's vectors (nicp/nicp/cloud.h:20-193): points, normals, the curvature
  * of Stats, and the two information matrices; any of normals/curvature/omegas may be NULL (zero). */
 int nicp_cloud_from_host(nicp_ctx *ctx, int n, const float *points4, const float *normals4, const float *curvature,
@@ -149,6 +157,21 @@ int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, cons
                nicp_align_result *result);
 /* CorrespondenceFinder::correspondences() of the last nicp_align on ctx (capacity in pairs) */
 int nicp_last_correspondences(nicp_ctx *ctx, int *pairs, int capacity, int *num_pairs);
+
+/* Replaces CorrespondenceFinderNN::init + compute (nicp/nicp/correspondencefindernn.cpp:44-97):
+ * the nearest neighbour of each transformed reference point [T p; s R n] among the current
+ * points [p; s n] within the squared inlier distance (exact, on a uniform grid of cell_size
+ * metres, <= 0 for the default; FLANN's approximate kd-tree is not reproduced; ties go to the
+ * lower index), then the reference's gates. pairs (capacity: reference points) are in
+ * reference-index order; nn_out (optional, one int per reference point) receives the raw
+ * neighbour or -1. */
+int nicp_nn_correspondences(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const float T[16],
+                            const nicp_align_params *ap, float normal_scaling, float cell_size, int *pairs,
+                            int *num_pairs, int *nn_out);
+/* Replaces AlignerNN::align (nicp/nicp/alignernn.cpp:21-118); one inner iteration. */
+int nicp_align_nn(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const float init_guess[16],
+                  const nicp_align_params *ap, float normal_scaling, float cell_size, float T_out[16],
+                  nicp_align_result *result);
 
 /* Batched form (no reference counterpart; it replaces running one converter/aligner per
  * frame pair on host threads): frames of uneven size live in one device arena, depth

@@ -57,7 +57,8 @@ EXPORTS = [
     "nicp_correspondences", "nicp_linearize", "nicp_align", "nicp_last_correspondences", "nicp_batch_create",
     "nicp_batch_destroy", "nicp_batch_upload_depth", "nicp_batch_convert", "nicp_batch_align",
     "nicp_batch_frame_cloud", "nicp_kernel_count", "nicp_kernel_name", "nicp_ctx_profile",
-    "nicp_ctx_profile_read",
+    "nicp_ctx_profile_read", "nicp_nn_correspondences", "nicp_align_nn", "nicp_cloud_from_depth_f32",
+    "nicp_depth_scale",
 ]
 
 _lib = None
@@ -96,6 +97,10 @@ def lib():
             "nicp_batch_align": [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp],
             "nicp_batch_frame_cloud": [vp, C.c_int, vp],
             "nicp_ctx_profile": [vp, C.c_int], "nicp_ctx_profile_read": [vp, vp, vp],
+            "nicp_nn_correspondences": [vp, vp, vp, vp, vp, C.c_float, C.c_float, vp, vp, vp],
+            "nicp_align_nn": [vp, vp, vp, vp, vp, C.c_float, C.c_float, vp, vp],
+            "nicp_cloud_from_depth_f32": [vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp],
+            "nicp_depth_scale": [vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -204,6 +209,25 @@ class Context:
                                                C.byref(sp), int(keep_stats), C.byref(h)))
         return Cloud(self, h, d.shape)
 
+    def cloud_from_depth_f32(self, depth_m, cam, sp, keep_stats=False):
+        d = np.ascontiguousarray(depth_m, dtype=np.float32)
+        h = C.c_void_p()
+        _check(lib().nicp_cloud_from_depth_f32(self.h, _p(d), d.shape[0], d.shape[1], C.byref(cam), C.byref(sp),
+                                               int(keep_stats), C.byref(h)))
+        return Cloud(self, h, d.shape)
+
+    def depth_scale(self, depth, step, max_depth_cov=0.01, depth_scale=0.001):
+        """DepthImage_scale of a uint16 (times depth_scale) or float (metres) image."""
+        rows, cols = depth.shape
+        out = np.empty((rows // step, cols // step), np.float32)
+        if depth.dtype == np.uint16:
+            d = np.ascontiguousarray(depth)
+            _check(lib().nicp_depth_scale(self.h, _p(d), None, rows, cols, depth_scale, step, max_depth_cov, _p(out)))
+        else:
+            d = np.ascontiguousarray(depth, np.float32)
+            _check(lib().nicp_depth_scale(self.h, None, _p(d), rows, cols, 1.0, step, max_depth_cov, _p(out)))
+        return out
+
     def cloud_from_host(self, points, normals=None, curvature=None, omega_p=None, omega_n=None):
         pts = np.ascontiguousarray(points, np.float32)
         arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (normals, curvature, omega_p, omega_n)]
@@ -245,6 +269,22 @@ class Context:
         res = AlignResult()
         _check(lib().nicp_align(self.h, ref.h, cur.h, C.byref(cam), rows, cols,
                                 _p(np.ascontiguousarray(init16, np.float32)), C.byref(ap), _p(T), C.byref(res)))
+        return T, res
+
+    def nn_correspondences(self, ref, cur, T16, ap, normal_scaling=1.0, cell_size=0.0):
+        """Returns (pairs (k, 2), raw nearest neighbour per reference point)."""
+        pairs = np.empty((max(ref.n, 1), 2), np.int32)
+        nn = np.empty(max(ref.n, 1), np.int32)
+        k = C.c_int()
+        _check(lib().nicp_nn_correspondences(self.h, ref.h, cur.h, _p(np.ascontiguousarray(T16, np.float32)),
+                                             C.byref(ap), normal_scaling, cell_size, _p(pairs), C.byref(k), _p(nn)))
+        return pairs[:k.value].copy(), nn[:ref.n].copy()
+
+    def align_nn(self, ref, cur, init16, ap, normal_scaling=1.0, cell_size=0.0):
+        T = np.empty(16, np.float32)
+        res = AlignResult()
+        _check(lib().nicp_align_nn(self.h, ref.h, cur.h, _p(np.ascontiguousarray(init16, np.float32)), C.byref(ap),
+                                   normal_scaling, cell_size, _p(T), C.byref(res)))
         return T, res
 
     def last_correspondences(self):

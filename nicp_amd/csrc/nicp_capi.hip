@@ -101,12 +101,12 @@ struct Carver {
 // ---------------------------------------------------------------------------------------
 enum KernelId {
   KID_COUNT, KID_SCAN, KID_UNPROJECT, KID_INTEGRAL_X, KID_INTEGRAL_Y, KID_STATS, KID_ZFILL, KID_ZPROJECT_CUR,
-  KID_ZRESOLVE, KID_ZPROJECT_REF, KID_CORR_LIN, KID_SOLVE, KID_LIN_PAIRS, KID_SOLVE_STATS, KID_N
+  KID_ZRESOLVE, KID_ZPROJECT_REF, KID_CORR_LIN, KID_SOLVE, KID_LIN_PAIRS, KID_SOLVE_STATS, KID_NN_QUERY, KID_N
 };
 static const char *kKernelNames[KID_N] = {"k_count", "k_scan", "k_unproject", "k_integral_x", "k_integral_y",
                                           "k_stats", "k_zfill", "k_zproject(current)", "k_zresolve",
                                           "k_zproject(reference)", "k_corr_lin", "k_solve", "k_lin_pairs",
-                                          "k_solve(statistics)"};
+                                          "k_solve(statistics)", "k_nn_query"};
 
 namespace {
 struct ProfEntry {
@@ -192,6 +192,8 @@ struct nicp_ctx {
   bool last_valid = false;
   Prof prof;
   std::vector<ProfEntry> *pending_graph = nullptr;  // graph events not yet read
+  // kNN grid of the current cloud
+  DevBuf grid_cells, grid_cursor, grid_sorted, grid_aux, nn_out;
 };
 
 struct nicp_cloud {
@@ -313,15 +315,16 @@ int yblocks(int cols) { return (cols * 10 + kThreads - 1) / kThreads; }
 
 // Layout of the per-frame device arrays inside one allocation.
 struct FrameLayout {
-  size_t depth, index, ii, pts, nrm, omg, stats, npoints, tcount, toff, total;
+  size_t depth, depthf, index, ii, pts, nrm, omg, stats, npoints, tcount, toff, total;
 };
-FrameLayout frame_layout(int rows, int cols, bool with_stats, bool with_ii, size_t base) {
+FrameLayout frame_layout(int rows, int cols, bool with_stats, bool with_ii, size_t base, bool with_f32 = false) {
   Carver cv;
   cv.off = base;
   const size_t P = (size_t)rows * cols;
   const size_t T = (P + kTile - 1) / kTile;
   FrameLayout L;
   L.depth = cv.take<uint16_t>(P);
+  L.depthf = with_f32 ? cv.take<float>(P) : 0;
   L.index = cv.take<int>(P);
   L.ii = with_ii ? cv.take<float>(P * 10) : 0;
   L.pts = cv.take<float4>(P);
@@ -335,8 +338,10 @@ FrameLayout frame_layout(int rows, int cols, bool with_stats, bool with_ii, size
   return L;
 }
 
-void frame_pointers(FrameDev &f, char *base, const FrameLayout &L, bool with_stats, bool with_ii) {
+void frame_pointers(FrameDev &f, char *base, const FrameLayout &L, bool with_stats, bool with_ii,
+                    bool with_f32 = false) {
   f.depth = (const uint16_t *)(base + L.depth);
+  f.depthf = with_f32 ? (const float *)(base + L.depthf) : nullptr;
   f.index = (int *)(base + L.index);
   f.ii = with_ii ? (float *)(base + L.ii) : nullptr;
   f.pts = (float4 *)(base + L.pts);
@@ -736,7 +741,8 @@ extern "C" int nicp_ctx_destroy(nicp_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   DevBuf *bufs[] = {&c->frames_tbl, &c->pairs_tbl, &c->maps, &c->zbuf, &c->cur_index, &c->pairs_buf,
-                    &c->state, &c->partials, &c->result, &c->depth_tmp, &c->scratch_in};
+                    &c->state, &c->partials, &c->result, &c->depth_tmp, &c->scratch_in, &c->grid_cells,
+                    &c->grid_cursor, &c->grid_sorted, &c->grid_aux, &c->nn_out};
   for (DevBuf *b : bufs) b->release();
   c->prof.release();
   (void)hipEventDestroy(c->ev0);
@@ -757,10 +763,10 @@ extern "C" void *nicp_ctx_stream(nicp_ctx *c) { return c ? (void *)c->stream : n
 // ---------------------------------------------------------------------------------------
 // clouds
 // ---------------------------------------------------------------------------------------
-extern "C" int nicp_cloud_from_depth_u16(nicp_ctx *ctx, const uint16_t *depth, int rows, int cols, float depth_scale,
-                                         const nicp_camera *cam, const nicp_stats_params *sp, int keep_stats,
-                                         nicp_cloud **out) {
-  if (!ctx || !depth || !sp || !out) return set_err(NICP_ERR_INVALID, "null argument");
+namespace {
+int cloud_from_depth(nicp_ctx *ctx, const uint16_t *depth, const float *depthf, int rows, int cols, float depth_scale,
+                     const nicp_camera *cam, const nicp_stats_params *sp, int keep_stats, nicp_cloud **out) {
+  if (!ctx || (!depth && !depthf) || !sp || !out) return set_err(NICP_ERR_INVALID, "null argument");
   if (rows <= 0 || cols <= 0 || (long long)rows * cols > (1ll << 28))
     return set_err(NICP_ERR_INVALID, "bad image size %dx%d", rows, cols);
   CHK(check_camera(cam));
@@ -768,7 +774,7 @@ extern "C" int nicp_cloud_from_depth_u16(nicp_ctx *ctx, const uint16_t *depth, i
   nicp_cloud *cl = new (std::nothrow) nicp_cloud();
   if (!cl) return set_err(NICP_ERR_OOM, "out of host memory");
   cl->ctx = ctx;
-  FrameLayout L = frame_layout(rows, cols, keep_stats != 0, true, 0);
+  FrameLayout L = frame_layout(rows, cols, keep_stats != 0, true, 0, depthf != nullptr);
   void *mem = nullptr;
   hipError_t e = hipMalloc(&mem, L.total);
   if (e != hipSuccess) {
@@ -778,7 +784,7 @@ extern "C" int nicp_cloud_from_depth_u16(nicp_ctx *ctx, const uint16_t *depth, i
   cl->alloc = mem;
   FrameDev &f = cl->fd;
   frame_geometry(f, rows, cols, cam, sp->world_radius, depth_scale);
-  frame_pointers(f, (char *)mem, L, keep_stats != 0, true);
+  frame_pointers(f, (char *)mem, L, keep_stats != 0, true, depthf != nullptr);
   f.tile_base = 0;
   f.xblk_base = 0;
   f.yblk_base = 0;
@@ -793,7 +799,8 @@ extern "C" int nicp_cloud_from_depth_u16(nicp_ctx *ctx, const uint16_t *depth, i
     std::vector<int> zeros((size_t)(nt + nx + ny), 0);
     FrameDev *d_f = ctx->frames_tbl.as<FrameDev>();
     int *d_maps = ctx->maps.as<int>();
-    e = hipMemcpyAsync((void *)f.depth, depth, (size_t)rows * cols * sizeof(uint16_t), hipMemcpyHostToDevice, s);
+    e = depthf ? hipMemcpyAsync((void *)f.depthf, depthf, (size_t)rows * cols * sizeof(float), hipMemcpyHostToDevice, s)
+               : hipMemcpyAsync((void *)f.depth, depth, (size_t)rows * cols * sizeof(uint16_t), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d_f, &f, sizeof(FrameDev), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d_maps, zeros.data(), zeros.size() * sizeof(int), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) { rc = set_err(NICP_ERR_HIP, "upload: %s", hipGetErrorString(e)); break; }
@@ -810,6 +817,44 @@ extern "C" int nicp_cloud_from_depth_u16(nicp_ctx *ctx, const uint16_t *depth, i
     return rc;
   }
   *out = cl;
+  return NICP_OK;
+}
+}  // namespace
+
+extern "C" int nicp_cloud_from_depth_u16(nicp_ctx *ctx, const uint16_t *depth, int rows, int cols, float depth_scale,
+                                         const nicp_camera *cam, const nicp_stats_params *sp, int keep_stats,
+                                         nicp_cloud **out) {
+  if (!depth) return set_err(NICP_ERR_INVALID, "depth is null");
+  return cloud_from_depth(ctx, depth, nullptr, rows, cols, depth_scale, cam, sp, keep_stats, out);
+}
+
+extern "C" int nicp_cloud_from_depth_f32(nicp_ctx *ctx, const float *depth, int rows, int cols, const nicp_camera *cam,
+                                         const nicp_stats_params *sp, int keep_stats, nicp_cloud **out) {
+  if (!depth) return set_err(NICP_ERR_INVALID, "depth is null");
+  return cloud_from_depth(ctx, nullptr, depth, rows, cols, 1.0f, cam, sp, keep_stats, out);
+}
+
+extern "C" int nicp_depth_scale(nicp_ctx *ctx, const uint16_t *depth, const float *depthf, int rows, int cols,
+                                float depth_scale, int step, float max_depth_cov, float *out) {
+  if (!ctx || (!depth && !depthf) || !out || rows <= 0 || cols <= 0 || step <= 0)
+    return set_err(NICP_ERR_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(ctx->device));
+  const size_t P = (size_t)rows * cols;
+  const int orows = rows / step, ocols = cols / step;
+  const size_t O = (size_t)orows * ocols;
+  if (O == 0) return NICP_OK;
+  CHK(ctx->scratch_in.reserve(P * sizeof(float)));
+  CHK(ctx->depth_tmp.reserve(O * sizeof(float)));
+  hipStream_t s = ctx->stream;
+  if (depthf) HIPCHK(hipMemcpyAsync(ctx->scratch_in.p, depthf, P * sizeof(float), hipMemcpyHostToDevice, s));
+  else HIPCHK(hipMemcpyAsync(ctx->scratch_in.p, depth, P * sizeof(uint16_t), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_depth_scale, dim3((unsigned)((O + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                     depthf ? (const uint16_t *)nullptr : ctx->scratch_in.as<uint16_t>(),
+                     depthf ? ctx->scratch_in.as<float>() : (const float *)nullptr, rows, cols, depth_scale, step,
+                     max_depth_cov, ctx->depth_tmp.as<float>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, ctx->depth_tmp.p, O * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
   return NICP_OK;
 }
 
@@ -1469,5 +1514,182 @@ extern "C" int nicp_ctx_profile_read(nicp_ctx *ctx, double *total_ms, long long 
     if (total_ms) total_ms[i] = ctx->prof.total_ms[i];
     if (launches) launches[i] = ctx->prof.launches[i];
   }
+  return NICP_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// kNN correspondences and AlignerNN (correspondencefindernn.cpp, alignernn.cpp)
+// ---------------------------------------------------------------------------------------
+namespace {
+
+constexpr float kDefaultCell = 0.03f;
+constexpr long long kMaxCells = 1ll << 25;
+
+// uniform grid over the current cloud: cell starts (exclusive scan of counts) and the
+// cloud sorted by cell
+int build_grid(nicp_ctx *ctx, const nicp_cloud *cur, float ns, float cell, GridDev &g) {
+  hipStream_t s = ctx->stream;
+  const int n = std::max(cur->n, 1);
+  CHK(ctx->grid_aux.reserve(64 * sizeof(int) + 16384 * sizeof(int)));
+  unsigned int init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  unsigned int *d_bbox = ctx->grid_aux.as<unsigned int>();
+  HIPCHK(hipMemcpyAsync(d_bbox, init, sizeof(init), hipMemcpyHostToDevice, s));
+  const int nb = std::min(1024, (n + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_nn_bbox, dim3(nb), dim3(kThreads), 0, s, cur->fd.pts, cur->fd.npoints, d_bbox);
+  unsigned int bb[6];
+  HIPCHK(hipMemcpyAsync(bb, d_bbox, sizeof(bb), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  float lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    unsigned int a = bb[k], b = bb[3 + k];
+    a = (a & 0x80000000u) ? (a & 0x7fffffffu) : ~a;
+    b = (b & 0x80000000u) ? (b & 0x7fffffffu) : ~b;
+    lo[k] = __builtin_bit_cast(float, a);
+    hi[k] = __builtin_bit_cast(float, b);
+    if (cur->n == 0 || !std::isfinite(lo[k]) || !std::isfinite(hi[k])) { lo[k] = 0.0f; hi[k] = 0.0f; }
+  }
+  float h = cell > 0.0f ? cell : kDefaultCell;
+  long long cells = 0;
+  int dims[3];
+  for (;;) {
+    for (int k = 0; k < 3; ++k) dims[k] = (int)std::floor((hi[k] - lo[k]) / h) + 1;
+    cells = (long long)dims[0] * dims[1] * dims[2];
+    if (cells <= kMaxCells) break;
+    h *= 1.25f;
+  }
+  g.ox = lo[0]; g.oy = lo[1]; g.oz = lo[2];
+  g.h = h;
+  g.inv_h = 1.0f / h;
+  g.nx = dims[0]; g.ny = dims[1]; g.nz = dims[2];
+  const int m = (int)cells + 1;
+  CHK(ctx->grid_cells.reserve((size_t)m * sizeof(int)));
+  CHK(ctx->grid_cursor.reserve((size_t)m * sizeof(int)));
+  CHK(ctx->grid_sorted.reserve((size_t)n * 2 * sizeof(float4)));
+  int *cells_d = ctx->grid_cells.as<int>();
+  HIPCHK(hipMemsetAsync(cells_d, 0, (size_t)m * sizeof(int), s));
+  const int pb = (n + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(k_nn_count, dim3(pb), dim3(kThreads), 0, s, cur->fd.pts, cur->fd.npoints, g, cells_d);
+  const int sb = (m + 4095) / 4096;
+  if (sb > 16384) return set_err(NICP_ERR_INVALID, "grid too large");
+  int *sums = (int *)(ctx->grid_aux.as<char>() + 64 * sizeof(int));
+  int *total = ctx->grid_aux.as<int>() + 8;
+  hipLaunchKernelGGL(k_scan_local, dim3(sb), dim3(1024), 0, s, cells_d, m, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, sb, total);
+  hipLaunchKernelGGL(k_scan_add, dim3(sb), dim3(1024), 0, s, cells_d, m, (const int *)sums, ctx->grid_cursor.as<int>());
+  hipLaunchKernelGGL(k_nn_scatter, dim3(pb), dim3(kThreads), 0, s, cur->fd.pts, cur->fd.nrm, cur->fd.npoints, g, ns,
+                     ctx->grid_cursor.as<int>(), ctx->grid_sorted.as<float4>());
+  HIPCHK(hipGetLastError());
+  return NICP_OK;
+}
+
+// plan of one NN pair: the pair's "image" is 1 x N_ref
+int nn_plan(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const float *init16, PairPlan &plan,
+            FrameDev **d_frames, PairDev **d_pairs, int **d_bp) {
+  std::vector<FrameDev> frames = {ref->fd, cur->fd};
+  frames[0].rows = 1; frames[0].cols = std::max(1, ref->n);
+  frames[1].rows = 1; frames[1].cols = std::max(1, ref->n);
+  CHK(upload_frames(ctx, frames, d_frames));
+  nicp_camera cam{};
+  cam.K[0] = cam.K[4] = cam.K[8] = 1.0f;
+  cam.min_distance = cam.max_distance = 1.0f;
+  const nicp_camera *cams[1] = {&cam};
+  int r0 = 0, c1 = 1;
+  make_plan(frames, 1, &r0, &c1, cams, plan);
+  plan.blk_ref.clear();
+  plan.blk_cur.clear();
+  plan.total_rtiles = plan.total_ctiles = 0;
+  AlignBuffers B = ctx_buffers(ctx);
+  int *d_br, *d_bc;
+  return prepare_align(ctx->stream, plan, B, init16, d_bp, &d_br, &d_bc, d_pairs);
+}
+
+}  // namespace
+
+extern "C" int nicp_nn_correspondences(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const float T[16],
+                                       const nicp_align_params *ap, float normal_scaling, float cell_size, int *pairs,
+                                       int *num_pairs, int *nn_out) {
+  if (!ctx || !ref || !cur || !T || !num_pairs) return set_err(NICP_ERR_INVALID, "null argument");
+  CHK(check_align_params(ap));
+  HIPCHK(hipSetDevice(ctx->device));
+  PairPlan plan;
+  FrameDev *d_frames;
+  PairDev *d_pairs;
+  int *d_bp;
+  CHK(nn_plan(ctx, ref, cur, nullptr, plan, &d_frames, &d_pairs, &d_bp));
+  HIPCHK(hipMemcpyAsync(ctx->state.as<float>() + kSlotTinv, T, 16 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  GridDev g;
+  CHK(build_grid(ctx, cur, normal_scaling, cell_size, g));
+  const int nr = std::max(ref->n, 1);
+  CHK(ctx->nn_out.reserve((size_t)nr * sizeof(int)));
+  hipLaunchKernelGGL(k_nn_query, dim3(plan.total_pix_tiles), dim3(kThreads), 0, ctx->stream, d_pairs, d_frames, d_bp, g,
+                     (const int *)ctx->grid_cells.as<int>(), (const float4 *)ctx->grid_sorted.as<float4>(), to_dev(ap),
+                     normal_scaling, ctx->nn_out.as<int>());
+  HIPCHK(hipGetLastError());
+  std::vector<int2> pp(nr);
+  HIPCHK(hipMemcpyAsync(pp.data(), ctx->pairs_buf.p, (size_t)nr * sizeof(int2), hipMemcpyDeviceToHost, ctx->stream));
+  if (nn_out && ref->n)
+    HIPCHK(hipMemcpyAsync(nn_out, ctx->nn_out.p, (size_t)ref->n * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int n = 0;
+  for (int i = 0; i < ref->n; ++i)
+    if (pp[i].x >= 0) {
+      if (pairs) {
+        pairs[2 * n] = pp[i].x;
+        pairs[2 * n + 1] = pp[i].y;
+      }
+      ++n;
+    }
+  *num_pairs = n;
+  return NICP_OK;
+}
+
+extern "C" int nicp_align_nn(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const float init_guess[16],
+                             const nicp_align_params *ap, float normal_scaling, float cell_size, float T_out[16],
+                             nicp_align_result *result) {
+  if (!ctx || !ref || !cur) return set_err(NICP_ERR_INVALID, "null argument");
+  CHK(check_align_params(ap));
+  if (ap->inner_iterations != 1) return set_err(NICP_ERR_INVALID, "the kNN aligner supports one inner iteration");
+  HIPCHK(hipSetDevice(ctx->device));
+  PairPlan plan;
+  FrameDev *d_frames;
+  PairDev *d_pairs;
+  int *d_bp;
+  float I4[16];
+  set_identity4(I4);
+  CHK(nn_plan(ctx, ref, cur, init_guess ? init_guess : I4, plan, &d_frames, &d_pairs, &d_bp));
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  GridDev g;
+  CHK(build_grid(ctx, cur, normal_scaling, cell_size, g));
+  const AlignParamsDev a = to_dev(ap);
+  Prof *pf = &ctx->prof;
+  pf->sink = &pf->eager;
+  const dim3 tb(kThreads);
+  hipStream_t s = ctx->stream;
+  for (int it = 0; it < ap->outer_iterations; ++it) {
+    LAUNCH(pf, KID_NN_QUERY, k_nn_query, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_bp, g,
+           (const int *)ctx->grid_cells.as<int>(), (const float4 *)ctx->grid_sorted.as<float4>(), a, normal_scaling,
+           (int *)nullptr);
+    LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
+    LAUNCH(pf, KID_SOLVE, k_solve, dim3(1), dim3(64), s, d_pairs, ap->lambda, 0, 0);
+  }
+  if (ap->outer_iterations > 0) {
+    LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
+    LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(1), dim3(64), s, d_pairs, 0.0f, 2, 0);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev1, s));
+  float T[16];
+  double r[2 * kPartial];
+  HIPCHK(hipMemcpyAsync(T, ctx->state.p, sizeof(T), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(r, ctx->result.p, sizeof(r), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (ap->outer_iterations == 0) memset(r, 0, sizeof(r));
+  if (T_out) memcpy(T_out, T, sizeof(T));
+  if (result) fill_result(r, T, ap, result, ms);
+  ctx->last_rows = 1;
+  ctx->last_cols = std::max(1, ref->n);
+  ctx->last_valid = ap->outer_iterations > 0;
   return NICP_OK;
 }

@@ -26,6 +26,7 @@ struct FrameDev {
   float K[9];           // camera matrix (column-major)
   float iKRt[16];       // unprojection matrix (identity sensor pose)
   const uint16_t *depth;
+  const float *depthf;  // float depth in metres (used instead of depth when set)
   int *index;           // index image
   float *ii;            // integral image, 10 floats per pixel
   float4 *pts;          // (x, y, z, 1)

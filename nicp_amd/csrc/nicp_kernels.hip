@@ -33,6 +33,10 @@ namespace {
 
 __device__ __forceinline__ float depth_of(uint16_t raw, float scale) { return raw ? scale * (float)raw : 0.0f; }
 __device__ __forceinline__ bool depth_valid(float d, float mn, float mx) { return !(d == 0.0f || d < mn || d > mx); }
+// the depth of pixel p of a frame: float input, or uint16 times the scale (imageutils.cpp:85-99)
+__device__ __forceinline__ float frame_depth(const FrameDev &fd, int p) {
+  return fd.depthf ? fd.depthf[p] : depth_of(fd.depth[p], fd.depth_scale);
+}
 
 // PinholePointProjector::_unProject (pinholepointprojector.h:275-280): x = column, y = row
 __device__ __forceinline__ float4 unproject_point(const float *iKRt, int c, int r, float d) {
@@ -78,7 +82,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_count(const FrameDev *_
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     int p = base + k * kThreads + threadIdx.x;
-    if (p < P) cnt += depth_valid(depth_of(fd.depth[p], fd.depth_scale), fd.min_d, fd.max_d);
+    if (p < P) cnt += depth_valid(frame_depth(fd, p), fd.min_d, fd.max_d);
   }
   __shared__ int ws[kThreads / 64];
   cnt = wave_sum_i(cnt);
@@ -129,7 +133,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_unproject(const FrameDe
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     int p = p0 + k;
-    dv[k] = p < P ? depth_of(fd.depth[p], fd.depth_scale) : 0.0f;
+    dv[k] = p < P ? frame_depth(fd, p) : 0.0f;
     valid[k] = p < P && depth_valid(dv[k], fd.min_d, fd.max_d);
     cnt += valid[k];
   }
@@ -170,21 +174,28 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_integral_x(const FrameD
   if (lt >= rows * 10) return;
   const int r = lt / 10, ch = lt - r * 10;
   const int ci = c_ch_i[ch], cj = c_ch_j[ch];
-  const uint16_t *drow = fd.depth + (size_t)r * cols;
-  float *orow = fd.ii + (size_t)r * cols * 10 + ch;
+  const size_t rowoff = (size_t)r * cols;
+  float *orow = fd.ii + rowoff * 10 + ch;
   float iK[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) iK[k] = fd.iKRt[k];
   const float scale = fd.depth_scale, mn = fd.min_d, mx = fd.max_d;
+  const uint16_t *drow = fd.depth ? fd.depth + rowoff : nullptr;
+  const float *frow = fd.depthf ? fd.depthf + rowoff : nullptr;
   float acc = 0.0f;
   int c = 0;
   for (; c + 8 <= cols; c += 8) {
-    uint16_t raw[8];
+    float dd[8];
+    if (frow) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) raw[k] = drow[c + k];
+      for (int k = 0; k < 8; ++k) dd[k] = frow[c + k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) dd[k] = depth_of(drow[c + k], scale);
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      float d = depth_of(raw[k], scale);
+      const float d = dd[k];
       float v = 0.0f;
       if (depth_valid(d, mn, mx)) {
         float o[4];
@@ -198,7 +209,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_integral_x(const FrameD
     }
   }
   for (; c < cols; ++c) {
-    float d = depth_of(drow[c], scale);
+    const float d = frow ? frow[c] : depth_of(drow[c], scale);
     float v = 0.0f;
     if (depth_valid(d, mn, mx)) {
       float o[4];
@@ -382,7 +393,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_stats(const FrameDev *_
     const int idx = fd.index[p];
     if (idx < 0) continue;
     const int r = p / cols, c = p - r * cols;
-    const float d = depth_of(fd.depth[p], fd.depth_scale);
+    const float d = frame_depth(fd, p);
     bool ok = true;
     int R = -1;
     if (d < fd.min_d || d > fd.max_d) {
@@ -842,4 +853,276 @@ extern "C" __global__ __launch_bounds__(64) void k_solve(const PairDev *__restri
     S[kSlotTinv + k] = Ti[k];
     S[kSlotKRt + k] = KRt[k];
   }
+}
+
+// ---------------------------------------------------------------------------------------
+// kNN correspondences (CorrespondenceFinderNN, correspondencefindernn.cpp:13-97): an exact
+// 6D nearest neighbour [p; s n] within the squared inlier distance, found on a uniform 3D
+// grid over the current cloud, followed by the reference's gates.
+// ---------------------------------------------------------------------------------------
+
+// order-preserving float <-> uint for atomic min/max
+__device__ __forceinline__ unsigned int f2ord(float f) {
+  unsigned int u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// bbox[0..2] = min keys (init 0xffffffff), bbox[3..5] = max keys (init 0)
+extern "C" __global__ __launch_bounds__(kThreads) void k_nn_bbox(const float4 *__restrict__ pts, const int *__restrict__ npts,
+                                                                unsigned int *__restrict__ bbox) {
+  const int n = *npts;
+  float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const float4 p = pts[i];
+    mn[0] = fminf(mn[0], p.x); mn[1] = fminf(mn[1], p.y); mn[2] = fminf(mn[2], p.z);
+    mx[0] = fmaxf(mx[0], p.x); mx[1] = fmaxf(mx[1], p.y); mx[2] = fmaxf(mx[2], p.z);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    for (int o = 32; o > 0; o >>= 1) {
+      mn[k] = fminf(mn[k], __shfl_xor(mn[k], o, 64));
+      mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      atomicMin(bbox + k, f2ord(mn[k]));
+      atomicMax(bbox + 3 + k, f2ord(mx[k]));
+    }
+  }
+}
+
+struct GridDev {
+  float ox, oy, oz, h, inv_h;
+  int nx, ny, nz;
+};
+
+__device__ __forceinline__ int cell_coord(float v, float o, float inv_h) { return (int)floorf((v - o) * inv_h); }
+
+extern "C" __global__ __launch_bounds__(kThreads) void k_nn_count(const float4 *__restrict__ pts, const int *__restrict__ npts,
+                                                                 GridDev g, int *__restrict__ counts) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= *npts) return;
+  const float4 p = pts[i];
+  int cx = min(max(cell_coord(p.x, g.ox, g.inv_h), 0), g.nx - 1);
+  int cy = min(max(cell_coord(p.y, g.oy, g.inv_h), 0), g.ny - 1);
+  int cz = min(max(cell_coord(p.z, g.oz, g.inv_h), 0), g.nz - 1);
+  atomicAdd(counts + ((size_t)cz * g.ny + cy) * g.nx + cx, 1);
+}
+
+// exclusive scan of n ints, pass 1: each block of 1024 threads scans 4096 entries
+extern "C" __global__ __launch_bounds__(1024) void k_scan_local(int *__restrict__ data, int n, int *__restrict__ block_sums) {
+  const int base = blockIdx.x * 4096 + threadIdx.x * 4;
+  int v[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = base + k < n ? data[base + k] : 0;
+    s += v[k];
+  }
+  __shared__ int wsum[16];
+  int incl = wave_incl_scan(s);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += wsum[w];
+  int run = before + incl - s;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) data[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 1023) block_sums[blockIdx.x] = before + incl;
+}
+
+// pass 2: one block scans the block sums (up to 1024 * 16 of them); total to *total
+extern "C" __global__ __launch_bounds__(1024) void k_scan_sums(int *__restrict__ sums, int nb, int *__restrict__ total) {
+  const int chunk = (nb + 1023) / 1024, start = threadIdx.x * chunk;
+  int local = 0;
+  for (int i = 0; i < chunk; ++i)
+    if (start + i < nb) local += sums[start + i];
+  __shared__ int wsum[16];
+  int incl = wave_incl_scan(local);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += wsum[w];
+  int run = before + incl - local;
+  for (int i = 0; i < chunk; ++i)
+    if (start + i < nb) {
+      int c = sums[start + i];
+      sums[start + i] = run;
+      run += c;
+    }
+  if (threadIdx.x == 1023) *total = before + incl;
+}
+
+// pass 3: add the block offsets; also writes the cursor copy used by the scatter
+extern "C" __global__ __launch_bounds__(1024) void k_scan_add(int *__restrict__ data, int n, const int *__restrict__ sums,
+                                                             int *__restrict__ cursor) {
+  const int base = blockIdx.x * 4096 + threadIdx.x * 4;
+  const int off = sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k < n) {
+      int v = data[base + k] + off;
+      data[base + k] = v;
+      cursor[base + k] = v;
+    }
+}
+
+// sorted current cloud: (x, y, z, index bits) and (s nx, s ny, s nz, 0) per point, in cell order
+extern "C" __global__ __launch_bounds__(kThreads) void k_nn_scatter(const float4 *__restrict__ pts, const float4 *__restrict__ nrm,
+                                                                   const int *__restrict__ npts, GridDev g, float ns,
+                                                                   int *__restrict__ cursor, float4 *__restrict__ sorted) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= *npts) return;
+  const float4 p = pts[i];
+  const float4 n = nrm[i];
+  int cx = min(max(cell_coord(p.x, g.ox, g.inv_h), 0), g.nx - 1);
+  int cy = min(max(cell_coord(p.y, g.oy, g.inv_h), 0), g.ny - 1);
+  int cz = min(max(cell_coord(p.z, g.oz, g.inv_h), 0), g.nz - 1);
+  const int slot = atomicAdd(cursor + ((size_t)cz * g.ny + cy) * g.nx + cx, 1);
+  sorted[2 * (size_t)slot] = make_float4(p.x, p.y, p.z, __int_as_float(i));
+  sorted[2 * (size_t)slot + 1] = make_float4(ns * n.x, ns * n.y, ns * n.z, 0.0f);
+}
+
+// one thread per reference point of pair blockIdx-mapped pd: nearest neighbour + gates.
+// The pair's "image" is 1 x N_ref; pairs[i] = (i, j) or (-1, -1).
+extern "C" __global__ __launch_bounds__(kThreads) void k_nn_query(const PairDev *__restrict__ Pp,
+                                                                 const FrameDev *__restrict__ F,
+                                                                 const int *__restrict__ blk_pair, GridDev g,
+                                                                 const int *__restrict__ cell_start,
+                                                                 const float4 *__restrict__ sorted, AlignParamsDev ap,
+                                                                 float ns, int *__restrict__ nn_out) {
+  const int gb = blockIdx.x;
+  const PairDev &pd = Pp[blk_pair[gb]];
+  const FrameDev &fr = F[pd.ref];
+  const FrameDev &fc = F[pd.cur];
+  const int nref = *fr.npoints;
+  const int base = (gb - pd.tile_base) * kTile;
+  float T[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) T[k] = pd.state[kSlotTinv + k];
+  // sR = T.linear() * ns
+  float sR[9];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) sR[c * 3 + r] = mc(T, r, c) * ns;
+  const float r2 = ap.sq_thr;
+  for (int kk = 0; kk < 4; ++kk) {
+    const int i = base + kk * kThreads + threadIdx.x;
+    if (i >= pd.cols) break;
+    if (i >= nref) {
+      pd.pairs[i] = make_int2(-1, -1);
+      continue;
+    }
+    const float4 p = fr.pts[i];
+    const float4 n = fr.nrm[i];
+    float q[6], pt[3];
+    iso_mul3(T, p.x, p.y, p.z, p.w, pt);
+    q[0] = pt[0]; q[1] = pt[1]; q[2] = pt[2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) q[3 + r] = red3(sR[r] * n.x, sR[3 + r] * n.y, sR[6 + r] * n.z);
+    const int cx = cell_coord(q[0], g.ox, g.inv_h), cy = cell_coord(q[1], g.oy, g.inv_h),
+              cz = cell_coord(q[2], g.oz, g.inv_h);
+    float best = r2;
+    int bi = -1;
+    // Chebyshev distance from the query's cell to the farthest grid cell bounds the shells
+    const int kcap = max(max(max(abs(cx), abs(cx - (g.nx - 1))), max(abs(cy), abs(cy - (g.ny - 1)))),
+                         max(abs(cz), abs(cz - (g.nz - 1))));
+    for (int k = 0; k <= kcap; ++k) {
+      if (k >= 1) {
+        const float lb = (float)(k - 1) * g.h;
+        const float lb2 = lb * lb * (1.0f - 1e-5f);
+        if (lb2 >= r2 || best < lb2) break;
+      }
+      for (int dz = -k; dz <= k; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= g.nz) continue;
+        for (int dy = -k; dy <= k; ++dy) {
+          const int y = cy + dy;
+          if (y < 0 || y >= g.ny) continue;
+          const bool face = (dz == -k || dz == k || dy == -k || dy == k);
+          const int step = face ? 1 : 2 * k;
+          for (int dx = -k; dx <= k; dx += (step > 0 ? step : 1)) {
+            const int x = cx + dx;
+            if (x >= 0 && x < g.nx) {
+              const size_t cell = ((size_t)z * g.ny + y) * g.nx + x;
+              const int s = cell_start[cell], e = cell_start[cell + 1];
+              for (int j = s; j < e; ++j) {
+                const float4 a = sorted[2 * (size_t)j], b = sorted[2 * (size_t)j + 1];
+                const float d0 = q[0] - a.x, d1 = q[1] - a.y, d2 = q[2] - a.z, d3 = q[3] - b.x, d4 = q[4] - b.y,
+                            d5 = q[5] - b.z;
+                // FLANN's L2 functor order for 6 dimensions
+                float d = 0.0f + (d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3);
+                d += d4 * d4;
+                d += d5 * d5;
+                const int idx = __float_as_int(a.w);
+                if (d < best || (d == best && bi >= 0 && idx < bi)) {
+                  best = d;
+                  bi = idx;
+                }
+              }
+            }
+            if (k == 0) break;
+          }
+        }
+      }
+    }
+    if (nn_out) nn_out[i] = bi;
+    bool ok = bi >= 0;
+    if (ok) {
+      float rn[3], rp[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        rn[r] = red3(mc(T, r, 0) * n.x, mc(T, r, 1) * n.y, mc(T, r, 2) * n.z);
+        rp[r] = red3(mc(T, r, 0) * p.x, mc(T, r, 1) * p.y, mc(T, r, 2) * p.z) + mc(T, r, 3);
+      }
+      const float4 cn = fc.nrm[bi], cp = fc.pts[bi];
+      if (!ap.gicp && (red3(rn[0] * rn[0], rn[1] * rn[1], rn[2] * rn[2]) == 0.0f ||
+                       red3(cn.x * cn.x, cn.y * cn.y, cn.z * cn.z) == 0.0f))
+        ok = false;
+      const float e0 = rp[0] - cp.x, e1 = rp[1] - cp.y, e2 = rp[2] - cp.z;
+      if (ok && red3(e0 * e0, e1 * e1, e2 * e2) > r2) ok = false;
+      if (ok && (isnan(cn.x) || isnan(cn.y) || isnan(cn.z) || isnan(rn[0]) || isnan(rn[1]) || isnan(rn[2]))) ok = false;
+      if (ok && !ap.gicp && red3(rn[0] * cn.x, rn[1] * cn.y, rn[2] * cn.z) < ap.cos_thr) ok = false;
+    }
+    pd.pairs[i] = ok ? make_int2(i, bi) : make_int2(-1, -1);
+  }
+}
+
+// DepthImage_scale (imageutils.cpp:6-37): block mean of step x step pixels over the
+// non-zero ones, rejected (0) when the block's depth variance exceeds max_cov. The
+// input is uint16 times scale (the reference scales the converted float image; the
+// conversion is exact per pixel). One thread per output pixel.
+extern "C" __global__ __launch_bounds__(kThreads) void k_depth_scale(const uint16_t *__restrict__ src, const float *__restrict__ srcf,
+                                                                    int rows, int cols, float scale, int step,
+                                                                    float max_cov, float *__restrict__ dst) {
+  const int orows = rows / step, ocols = cols / step;
+  const int o = blockIdx.x * kThreads + threadIdx.x;
+  if (o >= orows * ocols) return;
+  const int r = o / ocols, c = o - r * ocols;
+  float acc = 0.0f, acc2 = 0.0f;
+  int np = 0;
+  const int sr = r * step, sc = c * step;
+  for (int i = 0; i < step; ++i)
+    for (int j = 0; j < step; ++j)
+      if (sr + i < rows && sc + j < cols) {
+        const size_t q = (size_t)(sr + i) * cols + sc + j;
+        const float f = srcf ? srcf[q] : depth_of(src[q], scale);
+        acc += f;
+        acc2 += f * f;
+        np += f > 0.0f;
+      }
+  float out = 0.0f;
+  if (np) {
+    const float mu = acc / (float)np;
+    const float sigma = acc2 / (float)np - mu * mu;
+    if (!(sigma > max_cov)) out = mu;
+  }
+  dst[o] = out;
 }

@@ -173,6 +173,58 @@ double cast(const std::vector<Prim> &prims, const double *o, const double *d) {
   return best;
 }
 
+// like cast() but also returns the unit normal of the hit surface
+double cast_normal(const std::vector<Prim> &prims, const double *o, const double *d, double *nrm) {
+  double best = -1.0;
+  for (const Prim &p : prims) {
+    double oc[3] = {o[0] - p.c[0], o[1] - p.c[1], o[2] - p.c[2]};
+    double ol[3], dl[3];
+    for (int k = 0; k < 3; ++k) {
+      ol[k] = p.R[k * 3] * oc[0] + p.R[k * 3 + 1] * oc[1] + p.R[k * 3 + 2] * oc[2];
+      dl[k] = p.R[k * 3] * d[0] + p.R[k * 3 + 1] * d[1] + p.R[k * 3 + 2] * d[2];
+    }
+    double t = -1.0;
+    int axis = 2;
+    if (!p.box) {
+      if (std::fabs(dl[2]) < 1e-12) continue;
+      double tt = -ol[2] / dl[2];
+      if (tt <= 0.0) continue;
+      double x = ol[0] + tt * dl[0], y = ol[1] + tt * dl[1];
+      if (std::fabs(x) > p.h[0] || std::fabs(y) > p.h[1]) continue;
+      t = tt;
+    } else {
+      double t0 = -1e300, t1 = 1e300;
+      bool miss = false;
+      for (int k = 0; k < 3; ++k) {
+        if (std::fabs(dl[k]) < 1e-15) {
+          if (std::fabs(ol[k]) > p.h[k]) { miss = true; break; }
+          continue;
+        }
+        double a = (-p.h[k] - ol[k]) / dl[k], b = (p.h[k] - ol[k]) / dl[k];
+        if (a > b) std::swap(a, b);
+        if (a > t0) { t0 = a; axis = k; }
+        t1 = std::min(t1, b);
+      }
+      if (miss || t0 > t1 || t0 <= 0.0) continue;
+      t = t0;
+    }
+    if (t > 0.0 && (best < 0.0 || t < best)) {
+      best = t;
+      double n[3] = {p.R[axis * 3], p.R[axis * 3 + 1], p.R[axis * 3 + 2]};
+      if (n[0] * d[0] + n[1] * d[1] + n[2] * d[2] > 0.0) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
+      nrm[0] = n[0]; nrm[1] = n[1]; nrm[2] = n[2];
+    }
+  }
+  return best;
+}
+
+// flat information matrix U diag(a, b, b) U^T = b I + (a - b) n n^T, 4x4 column-major
+void flat_omega(const float *n, float a, float b, float *O) {
+  for (int k = 0; k < 16; ++k) O[k] = 0.0f;
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) O[c * 4 + r] = (float)((r == c ? (double)b : 0.0) + ((double)a - b) * n[r] * n[c]);
+}
+
 void render(const std::vector<Prim> &prims, int rows, int cols, double fx, double fy, double cx, double cy,
             const double *R /*3x3 cm*/, const double *t, double noise, double holes, uint64_t seed,
             uint64_t stream, uint16_t *out) {
@@ -232,6 +284,74 @@ void nicp_synth_motion(uint64_t seed, float max_rot_deg, float max_trans_m, floa
     for (int rr = 0; rr < 3; ++rr) T_gt[c * 4 + rr] = (float)R[c * 3 + rr];
   for (int k = 0; k < 3; ++k) T_gt[12 + k] = (float)(dir[k] * rad);
   T_gt[15] = 1.0f;
+}
+
+// An unstructured cloud pair (the kNN configuration): n points sampled from the
+// visible surfaces of scene `seed` (rendered noise-free at rows x cols), with the
+// exact surface normals, curvature 0 and flat information matrices; the target is
+// the source moved by T_gt^-1 with gaussian position noise `noise` (m), so that
+// p_source = T_gt p_target. Arrays: points/normals n*4, curvature n, omegas n*16.
+int nicp_synth_cloud_pair(uint64_t seed, int n, int rows, int cols, float fx, float fy, float cx, float cy,
+                          float max_rot_deg, float max_trans_m, float noise, float *src_points, float *src_normals,
+                          float *tgt_points, float *tgt_normals, float *curvature, float *omega_p, float *omega_n,
+                          float *T_gt) {
+  if (n <= 0 || rows <= 0 || cols <= 0) return -1;
+  std::vector<Prim> prims;
+  make_scene(seed, fx, fy, cx, cy, cols, rows, prims);
+  const size_t P = (size_t)rows * cols;
+  std::vector<float> hit(P * 6);
+  std::vector<char> ok(P, 0);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < cols; ++c) {
+      double d[3] = {(c - cx) / fx, (r - cy) / fy, 1.0}, o[3] = {0, 0, 0}, nn[3] = {0, 0, 0};
+      double t = cast_normal(prims, o, d, nn);
+      size_t i = (size_t)r * cols + c;
+      if (t > 0.5 && t < 4.5) {
+        ok[i] = 1;
+        for (int k = 0; k < 3; ++k) { hit[6 * i + k] = (float)(t * d[k]); hit[6 * i + 3 + k] = (float)nn[k]; }
+      }
+    }
+  std::vector<size_t> valid;
+  for (size_t i = 0; i < P; ++i)
+    if (ok[i]) valid.push_back(i);
+  if ((int)valid.size() < n) return -2;
+  // partial Fisher-Yates with the counter-based hash
+  for (int i = 0; i < n; ++i) {
+    size_t j = i + (size_t)(uniform(seed, 0x3131, i) * (valid.size() - i));
+    if (j >= valid.size()) j = valid.size() - 1;
+    std::swap(valid[i], valid[j]);
+  }
+  nicp_synth_motion(seed, max_rot_deg, max_trans_m, T_gt);
+  // inverse motion for the target: R^T (p - t)
+  double R[9], t[3];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) R[c * 3 + r] = T_gt[c * 4 + r];
+  for (int r = 0; r < 3; ++r) t[r] = T_gt[12 + r];
+  for (int i = 0; i < n; ++i) {
+    const float *h = &hit[6 * valid[i]];
+    for (int k = 0; k < 3; ++k) {
+      src_points[4 * i + k] = h[k];
+      src_normals[4 * i + k] = h[3 + k];
+    }
+    src_points[4 * i + 3] = 1.0f;
+    src_normals[4 * i + 3] = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+      double pk = 0.0, nk = 0.0;
+      for (int m = 0; m < 3; ++m) {
+        pk += R[k * 3 + m] * ((double)h[m] - t[m]);
+        nk += R[k * 3 + m] * (double)h[3 + m];
+      }
+      tgt_points[4 * i + k] = (float)(pk + noise * gaussian(seed, 0x4141 + k, i));
+      tgt_normals[4 * i + k] = (float)nk;
+    }
+    tgt_points[4 * i + 3] = 1.0f;
+    tgt_normals[4 * i + 3] = 0.0f;
+    curvature[i] = 0.0f;
+    flat_omega(&tgt_normals[4 * i], 1000.0f, 0.001f, omega_p + 16 * (size_t)i);
+    flat_omega(&tgt_normals[4 * i], 1000.0f, 0.001f, omega_n + 16 * (size_t)i);
+  }
+  return 0;
 }
 
 // A registration pair: the reference frame at the identity, the current frame

@@ -34,6 +34,8 @@ def lib():
         _lib.nicp_synth_render.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_float,
                                            C.c_float, C.c_float, C.c_void_p, C.c_float, C.c_float, C.c_void_p]
         _lib.nicp_synth_motion.argtypes = [C.c_uint64, C.c_float, C.c_float, C.c_void_p]
+        _lib.nicp_synth_cloud_pair.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                               C.c_float, C.c_float, C.c_float, C.c_float] + [C.c_void_p] * 8
     return _lib
 
 
@@ -82,3 +84,23 @@ def pose_error(T16, Tgt16):
     c = (np.trace(D[:3, :3]) - 1.0) / 2.0
     ang = np.degrees(np.arccos(np.clip(c, -1.0, 1.0)))
     return float(ang), float(np.linalg.norm(D[:3, 3]))
+
+
+def cloud_pair(seed, n=200000, max_rot_deg=10.0, max_trans_m=0.2, noise=0.002, rows=480, cols=640):
+    """Unstructured source/target clouds of the kNN configuration.
+
+    Returns dict with src/tgt points and normals (n, 4), curvature (n,), the target's flat
+    information matrices omega_p/omega_n (n, 16) and T_gt (16,) with p_src = T_gt p_tgt."""
+    fx, fy, cx, cy = camera_for(rows, cols)
+    out = {k: np.empty((n, 4), np.float32) for k in ("src_points", "src_normals", "tgt_points", "tgt_normals")}
+    out["curvature"] = np.empty(n, np.float32)
+    out["omega_p"] = np.empty((n, 16), np.float32)
+    out["omega_n"] = np.empty((n, 16), np.float32)
+    out["T_gt"] = np.empty(16, np.float32)
+    rc = lib().nicp_synth_cloud_pair(seed, n, rows, cols, fx, fy, cx, cy, max_rot_deg, max_trans_m, noise,
+                                     *[out[k].ctypes.data for k in ("src_points", "src_normals", "tgt_points",
+                                                                     "tgt_normals", "curvature", "omega_p", "omega_n",
+                                                                     "T_gt")])
+    if rc != 0:
+        raise RuntimeError("nicp_synth_cloud_pair failed (%d)" % rc)
+    return out

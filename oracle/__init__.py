@@ -306,3 +306,30 @@ def compute_statistics(H, T16, ap):
     Hc = np.ascontiguousarray(np.asarray(H, np.float64).flatten(order="F"))
     lib().orc_compute_statistics(_p(Hc), _p(np.ascontiguousarray(T16, np.float32)), C.byref(ap), C.byref(res))
     return res
+
+
+def nn_correspondences(ref_cloud, cur_cloud, T16, ap, normal_scaling=1.0):
+    """Returns (pairs (k, 2), nearest neighbour per reference point)."""
+    pairs = np.empty(2 * max(ref_cloud.n, 1), np.int32)
+    nn = np.empty(max(ref_cloud.n, 1), np.int32)
+    f = lib().orc_nn_correspondences
+    f.restype = C.c_int
+    k = f(C.byref(ref_cloud), C.byref(cur_cloud), _p(np.ascontiguousarray(T16, np.float32)), C.byref(ap),
+          C.c_float(normal_scaling), _p(pairs), _p(nn))
+    return pairs[:2 * k].reshape(k, 2).copy(), nn[:ref_cloud.n].copy()
+
+
+def align_nn(ref_cloud, cur_cloud, init16, ap, normal_scaling=1.0):
+    T = np.empty(16, np.float32)
+    res = AlignResult()
+    lib().orc_align_nn(C.byref(ref_cloud), C.byref(cur_cloud), _p(np.ascontiguousarray(init16, np.float32)),
+                       C.byref(ap), C.c_float(normal_scaling), _p(T), C.byref(res))
+    return T, res
+
+
+def depth_scale(depth_m, step, max_depth_cov=0.01):
+    d = np.ascontiguousarray(depth_m, np.float32)
+    rows, cols = d.shape
+    out = np.empty((rows // step, cols // step), np.float32)
+    lib().orc_depth_scale(_p(d), rows, cols, step, C.c_float(max_depth_cov), _p(out))
+    return out

@@ -528,6 +528,32 @@ void orc_depth_u16_to_f32(const uint16_t *raw, int n, float scale, float *out) {
   for (int i = 0; i < n; ++i) out[i] = raw[i] ? scale * (float)raw[i] : 0.0f;
 }
 
+/* DepthImage_scale, N/imageutils.cpp:6-37 */
+void orc_depth_scale(const float *src, int rows, int cols, int step, float max_cov, float *dst) {
+  const int orows = rows / step, ocols = cols / step;
+  for (int r = 0; r < orows; ++r)
+    for (int c = 0; c < ocols; ++c) {
+      float acc = 0.0f, acc2 = 0.0f;
+      int np = 0;
+      const int sr = r * step, sc = c * step;
+      for (int i = 0; i < step; ++i)
+        for (int j = 0; j < step; ++j)
+          if (sr + i < rows && sc + j < cols) {
+            const float f = src[(size_t)(sr + i) * cols + sc + j];
+            acc += f;
+            acc2 += f * f;
+            np += f > 0;
+          }
+      float out = 0.0f;
+      if (np) {
+        float mu = acc / np;
+        float sigma = acc2 / np - mu * mu;
+        if (!(sigma > max_cov)) out = mu;
+      }
+      dst[(size_t)r * ocols + c] = out;
+    }
+}
+
 /* A2: Eigen's 3x3 cofactor inverse (compute_inverse<...,3>), called at
  * N/pinholepointprojector.cpp:19-20 */
 void orc_inverse3(const float *K, float *iK) {
@@ -1177,6 +1203,123 @@ void orc_compute_statistics(const double *H, const float *Tf, const orc_align_pa
   for (int i = 0; i < 36; ++i) if (H[i] != 0.0) zeroH = false;
   res->solution_valid = ok && inv_ok && finite && !zeroH && !(rr > ap->rotational_min_eigen_ratio) &&
                         !(tr > ap->translational_min_eigen_ratio);
+}
+
+/* A22: CorrespondenceFinderNN, N/correspondencefindernn.cpp:13-97. FLANN's randomized kd-tree
+ * search (SearchParams(16), approximate) is replaced by exact brute force; the distance is
+ * FLANN's L2 functor order for 6 dimensions: (((d0^2 + d1^2) + d2^2) + d3^2) then + d4^2, + d5^2;
+ * FLANN's radius search keeps distances strictly below the radius (the squared threshold). */
+int orc_nn_correspondences(const orc_cloud *ref, const orc_cloud *cur, const float *Tin, const orc_align_params *ap,
+                           float ns, int *pairs, int *nn_out) {
+  float T[16];
+  std::memcpy(T, Tin, sizeof(T));
+  const float sq_thr = ap->inlier_distance_threshold * ap->inlier_distance_threshold;
+  const int nr = ref->n, nc = cur->n;
+  /* _model2linear of the current cloud: (p, ns * n) */
+  std::vector<float> cl((size_t)nc * 6);
+  for (int i = 0; i < nc; ++i) {
+    const float *p = cur->points + 4 * (size_t)i, *n = cur->normals + 4 * (size_t)i;
+    cl[6 * i] = p[0]; cl[6 * i + 1] = p[1]; cl[6 * i + 2] = p[2];
+    cl[6 * i + 3] = ns * n[0]; cl[6 * i + 4] = ns * n[1]; cl[6 * i + 5] = ns * n[2];
+  }
+  /* sR = T.linear() * ns (elementwise) */
+  float sR[9];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) sR[c * 3 + r] = mc(T, r, c) * ns;
+  std::vector<int> nn(nr, -1);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < nr; ++i) {
+    const float *p = ref->points + 4 * (size_t)i, *n = ref->normals + 4 * (size_t)i;
+    float q[6], pt[4];
+    iso_mul4(T, p, pt);
+    q[0] = pt[0]; q[1] = pt[1]; q[2] = pt[2];
+    for (int r = 0; r < 3; ++r) q[3 + r] = red3(sR[r] * n[0], sR[3 + r] * n[1], sR[6 + r] * n[2]);
+    float best = sq_thr;
+    int bi = -1;
+    for (int j = 0; j < nc; ++j) {
+      const float *c = &cl[6 * (size_t)j];
+      float d0 = q[0] - c[0], d1 = q[1] - c[1], d2 = q[2] - c[2], d3 = q[3] - c[3], d4 = q[4] - c[4], d5 = q[5] - c[5];
+      float d = 0.0f + (d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3);
+      d += d4 * d4;
+      d += d5 * d5;
+      if (d < best) { best = d; bi = j; }
+    }
+    nn[i] = bi;
+  }
+  if (nn_out) std::memcpy(nn_out, nn.data(), (size_t)nr * sizeof(int));
+  int k = 0;
+  for (int i = 0; i < nr; ++i) {
+    int ci = nn[i];
+    if (ci < 0) continue;
+    const float *p = ref->points + 4 * (size_t)i, *n = ref->normals + 4 * (size_t)i;
+    float rn[3], rp[3];
+    for (int r = 0; r < 3; ++r) {
+      rn[r] = red3(mc(T, r, 0) * n[0], mc(T, r, 1) * n[1], mc(T, r, 2) * n[2]);
+      rp[r] = red3(mc(T, r, 0) * p[0], mc(T, r, 1) * p[1], mc(T, r, 2) * p[2]) + mc(T, r, 3);
+    }
+    const float *cn = cur->normals + 4 * (size_t)ci, *cp = cur->points + 4 * (size_t)ci;
+    if (!ap->demoted_to_gicp && (red3(rn[0] * rn[0], rn[1] * rn[1], rn[2] * rn[2]) == 0.0f ||
+                                 red3(cn[0] * cn[0], cn[1] * cn[1], cn[2] * cn[2]) == 0.0f))
+      continue;
+    float dv[3] = {rp[0] - cp[0], rp[1] - cp[1], rp[2] - cp[2]};
+    if (red3(dv[0] * dv[0], dv[1] * dv[1], dv[2] * dv[2]) > sq_thr) continue;
+    if (std::isnan(cn[0]) || std::isnan(cn[1]) || std::isnan(cn[2]) || std::isnan(rn[0]) || std::isnan(rn[1]) ||
+        std::isnan(rn[2]))
+      continue;
+    if (!ap->demoted_to_gicp && red3(rn[0] * cn[0], rn[1] * cn[1], rn[2] * cn[2]) < ap->inlier_normal_angular_threshold)
+      continue;
+    pairs[2 * k] = i;
+    pairs[2 * k + 1] = ci;
+    ++k;
+  }
+  return k;
+}
+
+/* A22: AlignerNN::align, N/alignernn.cpp:21-118 */
+void orc_align_nn(const orc_cloud *ref, const orc_cloud *cur, const float *init_guess, const orc_align_params *ap,
+                  float ns, float *T_out, orc_align_result *res) {
+  std::vector<int> pairs(2 * (size_t)std::max(ref->n, 1));
+  float T[16];
+  std::memcpy(T, init_guess, sizeof(T));
+  float H[36] = {0}, b[6] = {0}, err = 0.0f;
+  int inl = 0, nc = 0;
+  for (int it = 0; it < ap->outer_iterations; ++it) {
+    mr(T, 3, 0) = 0.0f; mr(T, 3, 1) = 0.0f; mr(T, 3, 2) = 0.0f; mr(T, 3, 3) = 1.0f;
+    float Ti[16];
+    iso_inverse(T, Ti);
+    nc = orc_nn_correspondences(ref, cur, Ti, ap, ns, pairs.data(), nullptr);
+    float invT[16];
+    iso_inverse(T, invT);
+    for (int k = 0; k < ap->inner_iterations; ++k) {
+      mr(invT, 3, 0) = 0.0f; mr(invT, 3, 1) = 0.0f; mr(invT, 3, 2) = 0.0f; mr(invT, 3, 3) = 1.0f;
+      orc_linearize(ref, cur, pairs.data(), nc, invT, ap, 1, H, b, &err, &inl);
+      for (int d = 0; d < 6; ++d) H[d * 6 + d] = H[d * 6 + d] + ap->lambda;
+      float mb[6], dx[6];
+      for (int d = 0; d < 6; ++d) mb[d] = -b[d];
+      ldlt_solve<float>(H, mb, dx);
+      float dT[16];
+      v2t_t<float>(dx, dT);
+      iso_mul(dT, invT, invT);
+    }
+    iso_inverse(invT, T);
+    float x[6];
+    t2v_t<float>(T, x);
+    v2t_t<float>(x, T);
+  }
+  std::memcpy(T_out, T, sizeof(T));
+  res->error = err;
+  res->inliers = inl;
+  res->num_correspondences = nc;
+  float invT[16];
+  iso_inverse(T, invT);
+  float Hs[36], bs[6], es;
+  int is;
+  orc_linearize(ref, cur, pairs.data(), nc, invT, ap, 1, Hs, bs, &es, &is);
+  std::memcpy(res->H, Hs, sizeof(Hs));
+  std::memcpy(res->b, bs, sizeof(bs));
+  double Hd[36];
+  for (int i = 0; i < 36; ++i) Hd[i] = Hs[i];
+  orc_compute_statistics(Hd, T, ap, res);
 }
 
 /* A13: AlignerProjective::align, N/alignerprojective.cpp:21-136 (sensor offsets identity, no priors) */

@@ -82,6 +82,8 @@ typedef struct {
 
 /* A1 */
 void orc_depth_u16_to_f32(const uint16_t *raw, int n, float scale, float *out);
+/* DepthImage_scale (N/imageutils.cpp:6-37): (rows/step) x (cols/step) output */
+void orc_depth_scale(const float *src, int rows, int cols, int step, float max_depth_cov, float *dst);
 /* A2 */
 void orc_inverse3(const float K[9], float iK[9]);
 void orc_projector_matrices(const float K[9], const float transform[16], float KRt[16], float iKRt[16]);
@@ -139,6 +141,15 @@ int  orc_ldlt_solve_f64(const double H[36], const double rhs[6], double x[6]);
 void orc_align(const orc_cloud *ref, const orc_cloud *cur, int rows, int cols,
                const orc_camera *cam, const float init_guess[16], const orc_align_params *ap,
                int num_threads, float T_out[16], orc_align_result *res);
+/* A22: CorrespondenceFinderNN::compute (exact brute-force 1-NN in 6D within the squared
+ * inlier distance; ties -> lower index) followed by the reference's gates. pairs: 2 per ref
+ * point capacity; returns the number of pairs, in reference-index order. nn_out (optional,
+ * n_ref ints) receives the raw nearest neighbour per reference point (-1 when none). */
+int orc_nn_correspondences(const orc_cloud *ref, const orc_cloud *cur, const float T[16], const orc_align_params *ap,
+                           float normal_scaling, int *pairs, int *nn_out);
+/* A22: AlignerNN::align (nicp/nicp/alignernn.cpp:21-118) with the brute-force finder */
+void orc_align_nn(const orc_cloud *ref, const orc_cloud *cur, const float init_guess[16], const orc_align_params *ap,
+                  float normal_scaling, float T_out[16], orc_align_result *res);
 /* A20 only: statistics of a given H at a given T (FP64). */
 void orc_compute_statistics(const double H[36], const float T[16], const orc_align_params *ap,
                             orc_align_result *res);

@@ -213,3 +213,24 @@ def test_batch_matches_single(ctx):
         assert bc.n == cr.n
     np.testing.assert_array_equal(Tg.view(np.uint32), Tb.view(np.uint32))
     np.testing.assert_array_equal(Tg2.view(np.uint32), Tb.view(np.uint32))
+
+
+def test_depth_pyramid_and_float_input(ctx):
+    """DepthImage_scale bit-exact; the float-depth conversion equals the uint16 one."""
+    say("case depth_pyramid_and_float_input")
+    rows, cols = 480, 640
+    dr, _, _ = _frames(9, rows, cols)
+    dm = np.where(dr > 0, np.float32(0.001) * dr.astype(np.float32), 0).astype(np.float32)
+    for step in (1, 2, 3):
+        o = O.depth_scale(dm, step)
+        d16 = ctx.depth_scale(dr, step)
+        d32 = ctx.depth_scale(dm, step)
+        np.testing.assert_array_equal(d16.view(np.uint32), o.view(np.uint32))
+        np.testing.assert_array_equal(d32.view(np.uint32), o.view(np.uint32))
+    _, dcam = cams(rows, cols)
+    _, dsp = params()
+    a = ctx.cloud_from_depth(dr, dcam, dsp)
+    b = ctx.cloud_from_depth_f32(dm, dcam, dsp)
+    assert a.n == b.n
+    for x, y in zip(a.download(), b.download()):
+        np.testing.assert_array_equal(x.view(np.uint32), y.view(np.uint32))

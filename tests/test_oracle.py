@@ -382,3 +382,22 @@ def test_align_recovers_motion_and_statistics():
     Hp[0, 0] = Hp[1, 1] = 1e-3
     rp = O.compute_statistics(Hp, identity16(), O.align_params())
     assert rp.solution_valid == 0 and max(rp.translational_ratio, rp.rotational_ratio) > 50
+
+
+# ---- DepthImage_scale (N/imageutils.cpp:6-37) --------------------------------------------
+def test_depth_scale_closed_form():
+    dr, _, _ = _frame(4, 120, 160)
+    d = np.where(dr > 0, np.float32(0.001) * dr.astype(np.float32), 0).astype(np.float32)
+    assert np.array_equal(O.depth_scale(d, 1), d)              # step 1 is the identity
+    for step in (2, 3, 4):
+        out = O.depth_scale(d, step)
+        rows, cols = d.shape[0] // step, d.shape[1] // step
+        blk = d[:rows * step, :cols * step].reshape(rows, step, cols, step).transpose(0, 2, 1, 3).reshape(rows, cols, -1)
+        blk = blk.astype(np.float64)
+        npos = (blk > 0).sum(-1)
+        mu = np.where(npos > 0, blk.sum(-1) / np.maximum(npos, 1), 0)
+        var = np.where(npos > 0, (blk ** 2).sum(-1) / np.maximum(npos, 1) - mu ** 2, 0)
+        keep = (npos > 0) & (var <= 0.01)
+        safe = np.abs(var - 0.01) > 1e-4
+        assert np.allclose(out[keep & safe], mu[keep & safe], rtol=1e-6)
+        assert np.all(out[~keep & safe] == 0)
