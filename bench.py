@@ -99,10 +99,26 @@ def main():
     ap_.add_argument("--no-cpu", action="store_true")
     ap_.add_argument("--no-profile", action="store_true")
     ap_.add_argument("--plain-pairs", type=int, default=8)
+    ap_.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                     help="only run a device-to-device copy of MIB MiB (to scale rocprof's FETCH_SIZE/WRITE_SIZE)")
     args = ap_.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 5
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n,
+                          "copy_GB/s": round(8 * n / dt / 1e9, 1)}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (pairs shard "
-              "independently across GPUs; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "independently across GPUs). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from nicp_amd import capi, synth

@@ -64,6 +64,20 @@ class NormalInformationMatrixCalculator : public InformationMatrixCalculator {
   }
 };
 
+// DepthImage_convert_16UC1_to_32FC1 (nicp/nicp/imageutils.cpp:85-99)
+inline void DepthImage_convert_16UC1_to_32FC1(DepthImage &dest, const RawDepthImage &src, float scale = 0.001f) {
+  dest.create(src.rows, src.cols);
+  for (size_t i = 0; i < src.data.size(); ++i) dest.data[i] = src.data[i] ? scale * (float)src.data[i] : 0.0f;
+}
+
+// DepthImage_scale (nicp/nicp/imageutils.cpp:6-37), on the device
+inline void DepthImage_scale(DepthImage &dest, const DepthImage &src, int step, float maxDepthCov = 0.01f) {
+  dest.create(src.rows / step, src.cols / step);
+  if (dest.data.empty()) return;
+  check(nicp_depth_scale(Context::get(), nullptr, src.data.data(), src.rows, src.cols, 1.0f, step, maxDepthCov,
+                         dest.data.data()));
+}
+
 class DepthImageConverterIntegralImage {
  public:
   DepthImageConverterIntegralImage(PinholePointProjector *projector, StatsCalculatorIntegralImage *statsCalculator,
@@ -103,6 +117,19 @@ class DepthImageConverterIntegralImage {
     nicp_cloud *h = nullptr;
     check(nicp_cloud_from_depth_u16(Context::get(), depth.data.data(), depth.rows, depth.cols, depthScale, &cam, &sp,
                                     0, &h));
+    cloud.reset(h);
+    _indexImage.create(depth.rows, depth.cols);
+    check(nicp_cloud_index_image(h, _indexImage.data.data()));
+  }
+  // DepthImageConverterIntegralImage::compute(cloud, depthImage, sensorOffset) for a float image in metres
+  void compute(Cloud &cloud, const DepthImage &depth, const Isometry3f &sensorOffset = Isometry3f::Identity()) {
+    if (!sensorOffset.isIdentity())
+      throw std::runtime_error("nicp_amd: DepthImageConverter supports the identity sensor offset only");
+    _projector->setImageSize(depth.rows, depth.cols);
+    nicp_camera cam = _projector->camera();
+    nicp_stats_params sp = statsParams();
+    nicp_cloud *h = nullptr;
+    check(nicp_cloud_from_depth_f32(Context::get(), depth.data.data(), depth.rows, depth.cols, &cam, &sp, 0, &h));
     cloud.reset(h);
     _indexImage.create(depth.rows, depth.cols);
     check(nicp_cloud_index_image(h, _indexImage.data.data()));

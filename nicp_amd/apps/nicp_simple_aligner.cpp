@@ -101,6 +101,7 @@ int main(int argc, char **argv) {
   }
   auto get = [&](const char *k, float d) { auto it = p.find(k); return it == p.end() ? d : it->second; };
   const float depthScale = get("depthScale", 0.001f);
+  const int imageScale = (int)get("imageScale", 1.0f);
 
   PinholePointProjector projector;
   StatsCalculatorIntegralImage statsCalculator;
@@ -163,9 +164,16 @@ int main(int argc, char **argv) {
       std::cerr << "cannot read 16-bit PGM " << file << std::endl;
       return 1;
     }
-    if (!previous) projector.setImageSize(raw.rows, raw.cols);
+    // nicp_simple_aligner.cpp:138-146: convert, pyramid-scale, scale the projector once
+    DepthImage depth, scaled;
+    DepthImage_convert_16UC1_to_32FC1(depth, raw, depthScale);
+    DepthImage_scale(scaled, depth, imageScale);
+    if (!previous) {
+      projector.setImageSize(raw.rows, raw.cols);
+      projector.scale(1.0f / imageScale);
+    }
     cloud.reset(new Cloud());
-    converter.compute(*cloud, raw, depthScale);
+    converter.compute(*cloud, scaled);
     if (previous) {
       aligner.setReferenceCloud(previous.get());
       aligner.setCurrentCloud(cloud.get());

@@ -115,11 +115,21 @@ struct ProfEntry {
 };
 struct Prof {
   bool on = false;
+  hipError_t err = hipSuccess;     // first failure of a timing call (profiling then stops)
+  const char *err_where = "";
   std::vector<hipEvent_t> pool;
   std::vector<ProfEntry> *sink = nullptr;  // where new entries go (eager list or a graph's list)
   std::vector<ProfEntry> eager;
   double total_ms[KID_N] = {0};
   long long launches[KID_N] = {0};
+  void fail(hipError_t e, const char *where) {
+    if (err == hipSuccess) {
+      err = e;
+      err_where = where;
+    }
+    on = false;
+    (void)hipGetLastError();  // keep the failure out of the kernel-launch checks
+  }
   hipEvent_t take() {
     if (!pool.empty()) {
       hipEvent_t e = pool.back();
@@ -127,26 +137,41 @@ struct Prof {
       return e;
     }
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+    hipError_t r = hipEventCreateWithFlags(&e, hipEventDefault);
+    if (r != hipSuccess) {
+      fail(r, "hipEventCreateWithFlags");
+      e = nullptr;
+    }
     return e;
   }
   void begin(int id, hipStream_t s, ProfEntry &en) {
     en.id = id;
     en.a = take();
     en.b = take();
-    if (en.a) (void)hipEventRecord(en.a, s);
+    if (en.a) {
+      hipError_t r = hipEventRecord(en.a, s);
+      if (r != hipSuccess) fail(r, "hipEventRecord(start)");
+    }
   }
   void end(hipStream_t s, ProfEntry &en) {
-    if (en.b) (void)hipEventRecord(en.b, s);
-    if (en.a && en.b && sink) sink->push_back(en);
+    if (!en.a || !en.b) return;
+    hipError_t r = hipEventRecord(en.b, s);
+    if (r != hipSuccess) {
+      fail(r, "hipEventRecord(end)");
+      return;
+    }
+    if (sink) sink->push_back(en);
   }
   // reads finished entries; recycle = return their events to the pool
   void read(std::vector<ProfEntry> &list, bool recycle) {
     for (const ProfEntry &en : list) {
       float ms = 0.0f;
-      if (hipEventElapsedTime(&ms, en.a, en.b) == hipSuccess) {
+      hipError_t r = hipEventElapsedTime(&ms, en.a, en.b);
+      if (r == hipSuccess) {
         total_ms[en.id] += ms;
         launches[en.id] += 1;
+      } else {
+        fail(r, "hipEventElapsedTime");
       }
       if (recycle) {
         pool.push_back(en.a);
@@ -170,13 +195,15 @@ struct Prof {
 };
 }  // namespace
 
-#define LAUNCH(prof, id, kern, grid, block, stream, ...)                        \
-  do {                                                                          \
-    ProfEntry en_{};                                                            \
-    const bool p_ = (prof) != nullptr && (prof)->on;                           \
-    if (p_) (prof)->begin((id), (stream), en_);                                 \
-    hipLaunchKernelGGL(kern, grid, block, 0, stream, __VA_ARGS__);              \
-    if (p_) (prof)->end((stream), en_);                                         \
+#define LAUNCH(prof, id, kern, grid, block, stream, ...)                                   \
+  do {                                                                                     \
+    ProfEntry en_{};                                                                       \
+    const bool p_ = (prof) != nullptr && (prof)->on;                                      \
+    if (p_) (prof)->begin((id), (stream), en_);                                            \
+    hipLaunchKernelGGL(kern, grid, block, 0, stream, __VA_ARGS__);                         \
+    hipError_t le_ = hipGetLastError();                                                    \
+    if (le_ != hipSuccess) return set_err(NICP_ERR_HIP, "launch %s: %s", #kern, hipGetErrorString(le_)); \
+    if (p_) (prof)->end((stream), en_);                                                    \
   } while (0)
 // ---------------------------------------------------------------------------------------
 // objects
@@ -357,6 +384,7 @@ void frame_pointers(FrameDev &f, char *base, const FrameLayout &L, bool with_sta
 int launch_convert(Prof *pf, hipStream_t s, const FrameDev *d_frames, int num_frames, const int *d_blk, int total_tiles,
                    const int *d_x, int total_x, const int *d_y, int total_y, const StatsParamsDev &sp) {
   if (total_tiles == 0) return NICP_OK;
+  (void)hipGetLastError();
   LAUNCH(pf, KID_COUNT, k_count, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
   LAUNCH(pf, KID_SCAN, k_scan, dim3(num_frames), dim3(1024), s, d_frames);
   LAUNCH(pf, KID_UNPROJECT, k_unproject, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
@@ -485,6 +513,7 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
   const AlignParamsDev a = to_dev(ap);
   const dim3 tb(kThreads);
   if (plan.total_pix_tiles == 0) return NICP_OK;
+  (void)hipGetLastError();
   LAUNCH(pf, KID_ZFILL, k_zfill, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair);
   if (plan.total_ctiles)
     LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1);
@@ -1425,7 +1454,8 @@ extern "C" int nicp_batch_align(nicp_batch *b, int num_pairs, const int *ref_fra
       }
       if (e != hipSuccess) return set_err(NICP_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
       e = hipGraphInstantiate(&b->graph.exec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
+      hipError_t de = hipGraphDestroy(g);
+      if (e == hipSuccess && de != hipSuccess) e = de;
       if (e != hipSuccess) {
         b->graph.exec = nullptr;
         return set_err(NICP_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
@@ -1495,6 +1525,8 @@ extern "C" int nicp_ctx_profile(nicp_ctx *ctx, int enable) {
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->prof.read(ctx->prof.eager, true);
   ctx->prof.on = enable != 0;
+  ctx->prof.err = hipSuccess;
+  ctx->prof.err_where = "";
   for (int i = 0; i < KID_N; ++i) {
     ctx->prof.total_ms[i] = 0.0;
     ctx->prof.launches[i] = 0;
@@ -1505,6 +1537,8 @@ extern "C" int nicp_ctx_profile(nicp_ctx *ctx, int enable) {
 
 extern "C" int nicp_ctx_profile_read(nicp_ctx *ctx, double *total_ms, long long *launches) {
   if (!ctx) return set_err(NICP_ERR_INVALID, "ctx is null");
+  if (ctx->prof.err != hipSuccess)
+    return set_err(NICP_ERR_HIP, "kernel timing failed in %s: %s", ctx->prof.err_where, hipGetErrorString(ctx->prof.err));
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->prof.read(ctx->prof.eager, true);

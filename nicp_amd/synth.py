@@ -81,8 +81,12 @@ def pose_error(T16, Tgt16):
     A = np.asarray(T16, np.float64).reshape(4, 4, order="F")
     B = np.asarray(Tgt16, np.float64).reshape(4, 4, order="F")
     D = np.linalg.inv(B) @ A
-    c = (np.trace(D[:3, :3]) - 1.0) / 2.0
-    ang = np.degrees(np.arccos(np.clip(c, -1.0, 1.0)))
+    R = D[:3, :3]
+    # atan2 of the skew part and the trace: accurate for small angles, unlike arccos of the
+    # trace, whose resolution near the identity is ~sqrt(machine epsilon) of the inputs
+    s = 0.5 * np.linalg.norm([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]])
+    c = 0.5 * (np.trace(R) - 1.0)
+    ang = np.degrees(np.arctan2(s, c))
     return float(ang), float(np.linalg.norm(D[:3, 3]))
 
 

@@ -216,11 +216,13 @@ def linearize_f64(ref_cloud, cur_cloud, pairs, T16, ap):
     return H.reshape(6, 6, order="F"), b, e.value, i.value
 
 
-def align(ref_cloud, cur_cloud, rows, cols, cam, init16, ap, num_threads=1):
+def align(ref_cloud, cur_cloud, rows, cols, cam, init16, ap, num_threads=1, fp64_solve=False):
+    """AlignerProjective::align; fp64_solve runs H, b and the LDLT in FP64 (the device's design)."""
     T = np.empty(16, np.float32)
     res = AlignResult()
-    lib().orc_align(C.byref(ref_cloud), C.byref(cur_cloud), rows, cols, C.byref(cam),
-                    _p(np.ascontiguousarray(init16, np.float32)), C.byref(ap), num_threads, _p(T), C.byref(res))
+    lib().orc_align_mode(C.byref(ref_cloud), C.byref(cur_cloud), rows, cols, C.byref(cam),
+                         _p(np.ascontiguousarray(init16, np.float32)), C.byref(ap), num_threads, int(fp64_solve),
+                         _p(T), C.byref(res))
     return T, res
 
 
@@ -319,11 +321,11 @@ def nn_correspondences(ref_cloud, cur_cloud, T16, ap, normal_scaling=1.0):
     return pairs[:2 * k].reshape(k, 2).copy(), nn[:ref_cloud.n].copy()
 
 
-def align_nn(ref_cloud, cur_cloud, init16, ap, normal_scaling=1.0):
+def align_nn(ref_cloud, cur_cloud, init16, ap, normal_scaling=1.0, fp64_solve=False):
     T = np.empty(16, np.float32)
     res = AlignResult()
-    lib().orc_align_nn(C.byref(ref_cloud), C.byref(cur_cloud), _p(np.ascontiguousarray(init16, np.float32)),
-                       C.byref(ap), C.c_float(normal_scaling), _p(T), C.byref(res))
+    lib().orc_align_nn_mode(C.byref(ref_cloud), C.byref(cur_cloud), _p(np.ascontiguousarray(init16, np.float32)),
+                            C.byref(ap), C.c_float(normal_scaling), int(fp64_solve), _p(T), C.byref(res))
     return T, res
 
 

@@ -1205,6 +1205,9 @@ void orc_compute_statistics(const double *H, const float *Tf, const orc_align_pa
                         !(tr > ap->translational_min_eigen_ratio);
 }
 
+static void gn_step(const orc_cloud *ref, const orc_cloud *cur, const int *pairs, int nc, float *invT,
+                    const orc_align_params *ap, int num_threads, int mode, float *H, float *b, float *err, int *inl);
+
 /* A22: CorrespondenceFinderNN, N/correspondencefindernn.cpp:13-97. FLANN's randomized kd-tree
  * search (SearchParams(16), approximate) is replaced by exact brute force; the distance is
  * FLANN's L2 functor order for 6 dimensions: (((d0^2 + d1^2) + d2^2) + d3^2) then + d4^2, + d5^2;
@@ -1278,6 +1281,11 @@ int orc_nn_correspondences(const orc_cloud *ref, const orc_cloud *cur, const flo
 /* A22: AlignerNN::align, N/alignernn.cpp:21-118 */
 void orc_align_nn(const orc_cloud *ref, const orc_cloud *cur, const float *init_guess, const orc_align_params *ap,
                   float ns, float *T_out, orc_align_result *res) {
+  orc_align_nn_mode(ref, cur, init_guess, ap, ns, 0, T_out, res);
+}
+
+void orc_align_nn_mode(const orc_cloud *ref, const orc_cloud *cur, const float *init_guess, const orc_align_params *ap,
+                       float ns, int mode, float *T_out, orc_align_result *res) {
   std::vector<int> pairs(2 * (size_t)std::max(ref->n, 1));
   float T[16];
   std::memcpy(T, init_guess, sizeof(T));
@@ -1292,14 +1300,7 @@ void orc_align_nn(const orc_cloud *ref, const orc_cloud *cur, const float *init_
     iso_inverse(T, invT);
     for (int k = 0; k < ap->inner_iterations; ++k) {
       mr(invT, 3, 0) = 0.0f; mr(invT, 3, 1) = 0.0f; mr(invT, 3, 2) = 0.0f; mr(invT, 3, 3) = 1.0f;
-      orc_linearize(ref, cur, pairs.data(), nc, invT, ap, 1, H, b, &err, &inl);
-      for (int d = 0; d < 6; ++d) H[d * 6 + d] = H[d * 6 + d] + ap->lambda;
-      float mb[6], dx[6];
-      for (int d = 0; d < 6; ++d) mb[d] = -b[d];
-      ldlt_solve<float>(H, mb, dx);
-      float dT[16];
-      v2t_t<float>(dx, dT);
-      iso_mul(dT, invT, invT);
+      gn_step(ref, cur, pairs.data(), nc, invT, ap, 1, mode, H, b, &err, &inl);
     }
     iso_inverse(invT, T);
     float x[6];
@@ -1323,9 +1324,43 @@ void orc_align_nn(const orc_cloud *ref, const orc_cloud *cur, const float *init_
 }
 
 /* A13: AlignerProjective::align, N/alignerprojective.cpp:21-136 (sensor offsets identity, no priors) */
+/* one Gauss-Newton step of the aligner: mode 0 = the reference's FP32 (Linearizer FP32,
+ * LDLT<Matrix6f>); mode 1 = H and b in FP64 and an FP64 LDLT, the update still in FP32 */
+static void gn_step(const orc_cloud *ref, const orc_cloud *cur, const int *pairs, int nc, float *invT,
+                    const orc_align_params *ap, int num_threads, int mode, float *H, float *b, float *err, int *inl) {
+  float dx[6];
+  if (mode == 0) {
+    orc_linearize(ref, cur, pairs, nc, invT, ap, num_threads, H, b, err, inl);
+    for (int d = 0; d < 6; ++d) H[d * 6 + d] = H[d * 6 + d] + ap->lambda;
+    float mb[6];
+    for (int d = 0; d < 6; ++d) mb[d] = -b[d];
+    ldlt_solve<float>(H, mb, dx);
+  } else {
+    double Hd[36], bd[6], ed;
+    orc_linearize_f64(ref, cur, pairs, nc, invT, ap, Hd, bd, &ed, inl);
+    for (int i = 0; i < 36; ++i) H[i] = (float)Hd[i];
+    for (int i = 0; i < 6; ++i) b[i] = (float)bd[i];
+    *err = (float)ed;
+    for (int d = 0; d < 6; ++d) Hd[d * 6 + d] += (double)ap->lambda;
+    double mb[6], x[6];
+    for (int d = 0; d < 6; ++d) mb[d] = -bd[d];
+    ldlt_solve<double>(Hd, mb, x);
+    for (int d = 0; d < 6; ++d) dx[d] = (float)x[d];
+  }
+  float dT[16];
+  v2t_t<float>(dx, dT);
+  iso_mul(dT, invT, invT);
+}
+
 void orc_align(const orc_cloud *ref, const orc_cloud *cur, int rows, int cols, const orc_camera *cam,
                const float *init_guess, const orc_align_params *ap, int num_threads, float *T_out,
                orc_align_result *res) {
+  orc_align_mode(ref, cur, rows, cols, cam, init_guess, ap, num_threads, 0, T_out, res);
+}
+
+void orc_align_mode(const orc_cloud *ref, const orc_cloud *cur, int rows, int cols, const orc_camera *cam,
+                    const float *init_guess, const orc_align_params *ap, int num_threads, int mode, float *T_out,
+                    orc_align_result *res) {
   const size_t P = (size_t)rows * cols;
   std::vector<int> cur_index(P), ref_index(P), pairs(2 * P);
   float I4[16], KRt[16], iKRt[16];
@@ -1349,14 +1384,7 @@ void orc_align(const orc_cloud *ref, const orc_cloud *cur, int rows, int cols, c
     iso_inverse(T, invT);
     for (int k = 0; k < ap->inner_iterations; ++k) {
       mr(invT, 3, 0) = 0.0f; mr(invT, 3, 1) = 0.0f; mr(invT, 3, 2) = 0.0f; mr(invT, 3, 3) = 1.0f;
-      orc_linearize(ref, cur, pairs.data(), nc, invT, ap, num_threads, H, b, &err, &inl);
-      for (int d = 0; d < 6; ++d) H[d * 6 + d] = H[d * 6 + d] + ap->lambda;
-      float mb[6], dx[6];
-      for (int d = 0; d < 6; ++d) mb[d] = -b[d];
-      ldlt_solve<float>(H, mb, dx);
-      float dT[16];
-      v2t_t<float>(dx, dT);
-      iso_mul(dT, invT, invT);
+      gn_step(ref, cur, pairs.data(), nc, invT, ap, num_threads, mode, H, b, &err, &inl);
     }
     iso_inverse(invT, T);
     float x[6];

@@ -141,6 +141,13 @@ int  orc_ldlt_solve_f64(const double H[36], const double rhs[6], double x[6]);
 void orc_align(const orc_cloud *ref, const orc_cloud *cur, int rows, int cols,
                const orc_camera *cam, const float init_guess[16], const orc_align_params *ap,
                int num_threads, float T_out[16], orc_align_result *res);
+/* mode 0 = orc_align; mode 1 = the same loop with H, b and the LDLT in FP64 (updates in FP32) */
+void orc_align_mode(const orc_cloud *ref, const orc_cloud *cur, int rows, int cols,
+                    const orc_camera *cam, const float init_guess[16], const orc_align_params *ap,
+                    int num_threads, int mode, float T_out[16], orc_align_result *res);
+void orc_align_nn_mode(const orc_cloud *ref, const orc_cloud *cur, const float init_guess[16],
+                       const orc_align_params *ap, float normal_scaling, int mode, float T_out[16],
+                       orc_align_result *res);
 /* A22: CorrespondenceFinderNN::compute (exact brute-force 1-NN in 6D within the squared
  * inlier distance; ties -> lower index) followed by the reference's gates. pairs: 2 per ref
  * point capacity; returns the number of pairs, in reference-index order. nn_out (optional,

@@ -49,13 +49,16 @@ def test_align_nn_vs_oracle(ctx):
     oap, dap = O.align_params(outer_iterations=20), capi.align_params(outer_iterations=20)
     I = np.eye(4, dtype=np.float32).flatten(order="F")
     To, ro = O.align_nn(oref, ocur, I, oap)
+    T64, r64 = O.align_nn(oref, ocur, I, oap, fp64_solve=True)
     Td, rd = ctx.align_nn(dref, dcur, I, dap)
-    r, t = synth.pose_error(Td, To)
+    r, t = synth.pose_error(Td, T64)
+    r32, t32 = synth.pose_error(Td, To)
     gr, gt = synth.pose_error(Td, d["T_gt"])
-    say("  device vs oracle %.2e rad %.2e m | vs truth %.4f deg %.5f m | inliers %d/%d" % (
-        np.radians(r), t, gr, gt, rd.inliers, ro.inliers))
+    say("  device vs fp64-solve oracle %.2e rad %.2e m | vs FP32 reference %.2e rad %.2e m | vs truth %.4f deg %.5f m"
+        " | inliers %d/%d" % (np.radians(r), t, np.radians(r32), t32, gr, gt, rd.inliers, ro.inliers))
     assert np.radians(r) < 1e-5 and t < 1e-5
-    assert rd.inliers == ro.inliers
+    assert np.radians(r32) < 1e-5 and t32 < 1e-5
+    assert rd.inliers == r64.inliers
 
 
 def test_align_nn_200k(ctx):
