@@ -141,22 +141,39 @@ def main():
     cur = ref + 1
     Tgt = np.stack([p[2] for p in pairs])
 
-    def step():
+    def step(graph):
         batch.convert()
-        return batch.align(ref, cur, None, ap, use_graph=True)
+        return batch.align(ref, cur, None, ap, use_graph=graph)
 
+    # hipGraph replay of the iteration chain (HIP cannot time event nodes inside a graph,
+    # so the timed region below launches eagerly with an event pair around every kernel)
     for _ in range(args.warmup):
-        step()
+        step(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    ctx.synchronize()
+    graph_ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+    for _ in range(args.warmup):
+        step(False)
     ctx.synchronize()
     if not args.no_profile:
         ctx.profile(True)
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        T, res = step()
+        T, res = step(False)
     ctx.synchronize()
     dt = time.perf_counter() - t0
-    prof = ctx.profile_read() if not args.no_profile else {}
+    prof = {}
+    prof_error = None
+    if not args.no_profile:
+        try:
+            prof = ctx.profile_read()
+        except capi.NicpError as e:
+            prof_error = str(e)
+            log("kernel timing unavailable: %s" % e)
     ctx.profile(False)
 
     errs = np.array([synth.pose_error(T[i], Tgt[i]) for i in range(B)])
@@ -238,10 +255,12 @@ def main():
                    "pairs_per_step": B, "resolution": "640x480", "iterations": args.iters, "init": "identity"},
         "aligned_fraction": round(n_ok / B, 4),
         "pairs_per_s": round(B * args.steps / dt, 2),
+        "graph_ms_per_step": round(graph_ms, 3),
         "plain_single_pair_pairs_per_s": round(plain, 2) if plain else None,
         "batched_vs_plain": round((B * args.steps / dt) / plain, 2) if plain else None,
         "roofline": roofline,
         "kernels": kernels,
+        "kernel_timing_error": prof_error,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

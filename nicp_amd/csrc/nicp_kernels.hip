@@ -262,29 +262,45 @@ __device__ __forceinline__ void cross3(const float *a, const float *b, float *o)
 }
 __device__ __forceinline__ float sqn3(const float *a) { return red3(a[0] * a[0], a[1] * a[1], a[2] * a[2]); }
 
-__device__ void eig_extract_kernel(const float *mat, float *res, float *rep_out) {
+__device__ __forceinline__ float sel3(int i, float a, float b, float c) { return i == 0 ? a : (i == 1 ? b : c); }
+
+// extract_kernel of Eigen's computeDirect on a shifted matrix given by its three columns
+// (named values, so that the column choice stays a register select): the representative
+// column i0 = argmax |diagonal| and the better-conditioned of the two cross products with
+// the other columns, normalised. a x b == -(b x a) bitwise, so the cross products of the
+// three column pairs serve every i0.
+__device__ __forceinline__ void eig_extract_kernel(const float *mat, float *res, float *rep_out) {
+  const float a0 = mat[0], a1 = mat[1], a2 = mat[2];
+  const float b0 = mat[3], b1 = mat[4], b2 = mat[5];
+  const float d0 = mat[6], d1 = mat[7], d2 = mat[8];
   int i0 = 0;
-  float best = fabsf(mat[0]);
-  if (fabsf(mat[4]) > best) { best = fabsf(mat[4]); i0 = 1; }
-  if (fabsf(mat[8]) > best) { best = fabsf(mat[8]); i0 = 2; }
-  float rep[3] = {mat[3 * i0], mat[3 * i0 + 1], mat[3 * i0 + 2]};
-  const int i1 = (i0 + 1) % 3, i2 = (i0 + 2) % 3;
-  float c0[3], c1[3];
-  cross3(rep, mat + 3 * i1, c0);
-  cross3(rep, mat + 3 * i2, c1);
-  float n0 = sqn3(c0), n1 = sqn3(c1);
+  float best = fabsf(a0);
+  if (fabsf(b1) > best) { best = fabsf(b1); i0 = 1; }
+  if (fabsf(d2) > best) { best = fabsf(d2); i0 = 2; }
+  // col0 x col1, col0 x col2, col1 x col2
+  const float x01 = a1 * b2 - a2 * b1, y01 = a2 * b0 - a0 * b2, z01 = a0 * b1 - a1 * b0;
+  const float x02 = a1 * d2 - a2 * d1, y02 = a2 * d0 - a0 * d2, z02 = a0 * d1 - a1 * d0;
+  const float x12 = b1 * d2 - b2 * d1, y12 = b2 * d0 - b0 * d2, z12 = b0 * d1 - b1 * d0;
+  // i0 = 0: (c0, c1) = (c01, c02); 1: (c12, -c01); 2: (-c02, -c12)
+  const float c0x = sel3(i0, x01, x12, -x02), c0y = sel3(i0, y01, y12, -y02), c0z = sel3(i0, z01, z12, -z02);
+  const float c1x = sel3(i0, x02, -x01, -x12), c1y = sel3(i0, y02, -y01, -y12), c1z = sel3(i0, z02, -z01, -z12);
+  const float n0 = red3(c0x * c0x, c0y * c0y, c0z * c0z), n1 = red3(c1x * c1x, c1y * c1y, c1z * c1z);
   if (n0 > n1) {
-    float s = sqrtf(n0);
-    res[0] = c0[0] / s; res[1] = c0[1] / s; res[2] = c0[2] / s;
+    const float sq = sqrtf(n0);
+    res[0] = c0x / sq; res[1] = c0y / sq; res[2] = c0z / sq;
   } else {
-    float s = sqrtf(n1);
-    res[0] = c1[0] / s; res[1] = c1[1] / s; res[2] = c1[2] / s;
+    const float sq = sqrtf(n1);
+    res[0] = c1x / sq; res[1] = c1y / sq; res[2] = c1z / sq;
   }
-  if (rep_out) { rep_out[0] = rep[0]; rep_out[1] = rep[1]; rep_out[2] = rep[2]; }
+  if (rep_out) {
+    rep_out[0] = sel3(i0, a0, b0, d0);
+    rep_out[1] = sel3(i0, a1, b1, d1);
+    rep_out[2] = sel3(i0, a2, b2, d2);
+  }
 }
 
 // cov: symmetric 3x3 column-major; evals ascending, evecs column-major (col 0 = smallest)
-__device__ void eig3_direct(const float *cov, float *ev, float *V) {
+__device__ __forceinline__ void eig3_direct(const float *cov, float *ev, float *V) {
   float m[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) m[i] = cov[i];
@@ -328,29 +344,34 @@ __device__ void eig3_direct(const float *cov, float *ev, float *V) {
     float tmp[9];
     float d0 = ev[2] - ev[1];
     float d1 = ev[1] - ev[0];
-    int k = 0, l = 2;
-    if (d0 > d1) { k = 2; l = 0; d0 = d1; }
+    const bool swapped = d0 > d1;  // k = 2, l = 0 when true (else k = 0, l = 2)
+    if (swapped) d0 = d1;
 #pragma unroll
     for (int i = 0; i < 9; ++i) tmp[i] = m[i];
-    const float ek = k == 0 ? ev[0] : ev[2];
+    const float ek = swapped ? ev[2] : ev[0];
     tmp[0] -= ek; tmp[4] -= ek; tmp[8] -= ek;
-    eig_extract_kernel(tmp, V + 3 * k, V + 3 * l);
+    float vk[3], vl[3];
+    eig_extract_kernel(tmp, vk, vl);
     if (d0 <= 2.0f * eps * d1) {
-      float *vl = V + 3 * l, *vk = V + 3 * k;
       float d = red3(vk[0] * vl[0], vk[1] * vl[1], vk[2] * vl[2]);
       float t0 = d * vl[0], t1 = d * vl[1], t2 = d * vl[2];
       vl[0] -= t0; vl[1] -= t1; vl[2] -= t2;
       float z = sqn3(vl);
       if (z > 0.0f) {
-        float s = sqrtf(z);
-        vl[0] = vl[0] / s; vl[1] = vl[1] / s; vl[2] = vl[2] / s;
+        float sz = sqrtf(z);
+        vl[0] = vl[0] / sz; vl[1] = vl[1] / sz; vl[2] = vl[2] / sz;
       }
     } else {
 #pragma unroll
       for (int i = 0; i < 9; ++i) tmp[i] = m[i];
-      const float el = l == 0 ? ev[0] : ev[2];
+      const float el = swapped ? ev[0] : ev[2];
       tmp[0] -= el; tmp[4] -= el; tmp[8] -= el;
-      eig_extract_kernel(tmp, V + 3 * l, nullptr);
+      eig_extract_kernel(tmp, vl, nullptr);
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      V[r] = swapped ? vl[r] : vk[r];
+      V[6 + r] = swapped ? vk[r] : vl[r];
     }
     float c[3];
     cross3(V + 6, V + 0, c);
@@ -449,8 +470,12 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_stats(const FrameDev *_
         const float4 pt = unproject_point(fd.iKRt, c, r, d);
         if (dot4(nx, ny, nz, 0.0f, pt.x, pt.y, pt.z, pt.w) > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
         nrm = make_float4(nx, ny, nz, curv);
-        const float *dp = curv < sp.pthr ? sp.pflat : sp.pnonflat;
-        const float *dq = curv < sp.nthr ? sp.nflat : sp.nnonflat;
+        float dp[3], dq[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          dp[i] = curv < sp.pthr ? sp.pflat[i] : sp.pnonflat[i];
+          dq[i] = curv < sp.nthr ? sp.nflat[i] : sp.nnonflat[i];
+        }
         omega6(V, dp, op);
         omega6(V, dq, on);
       } else {
@@ -543,17 +568,11 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_zresolve(const PairDev 
 
 namespace {
 
-// compensated accumulation (TwoSum): s + c carries the running sum
-struct Acc2 {
-  float s, c;
-};
-__device__ __forceinline__ void acc_add(Acc2 &a, float x) {
-  float t = a.s + x;
-  float bp = t - a.s;
-  float e = (a.s - (t - bp)) + (x - bp);
-  a.s = t;
-  a.c += e;
-}
+// Per-thread accumulation of the FP32 per-correspondence terms in FP64: each term is
+// exact in double, and on CDNA4 one DADD costs less than the six FP32 operations of a
+// TwoSum step while leaving a smaller error than compensated FP32.
+typedef double Acc2;
+__device__ __forceinline__ void acc_add(Acc2 &a, float x) { a += (double)x; }
 
 // The per-correspondence terms of Linearizer::update (linearizer.cpp:55-112) for
 // J_p = [I | S(p)], J_n = [0 | S(n)], S(v) = -2[v]x (bm_se3.h:55-66).
@@ -621,14 +640,13 @@ __device__ __forceinline__ void lin_terms(const float *rp, const float *rn, floa
   acc_add(acc[28], 1.0f);
 }
 
-// block-wide FP64 reduction of the kVals compensated accumulators into one partial
+// block-wide FP64 reduction of the kVals accumulators into one partial
 __device__ void block_partial(Acc2 *acc, double *out) {
   __shared__ double red[kThreads / 64][kPartial];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int v = 0; v < kVals; ++v) {
-    double x = (double)acc[v].s + (double)acc[v].c;
-    x = wave_sum_d(x);
+    double x = wave_sum_d(acc[v]);
     if (lane == 0) red[wave][v] = x;
   }
   __syncthreads();
@@ -638,6 +656,27 @@ __device__ void block_partial(Acc2 *acc, double *out) {
       for (int w = 0; w < kThreads / 64; ++w) s += red[w][threadIdx.x];
     out[threadIdx.x] = s;
   }
+}
+
+// CorrespondenceFinderProjective gates on already loaded values (correspondencefinderprojective.cpp:62-95)
+__device__ __forceinline__ bool gates_loaded(const float4 &cn, const float4 &rn0, const float4 &cp, const float4 &rp0,
+                                             const float *T, const AlignParamsDev &ap, float *rp, float *rn) {
+  if (!ap.gicp && (dot4(cn.x, cn.y, cn.z, 0.0f, cn.x, cn.y, cn.z, 0.0f) == 0.0f ||
+                   dot4(rn0.x, rn0.y, rn0.z, 0.0f, rn0.x, rn0.y, rn0.z, 0.0f) == 0.0f))
+    return false;
+  iso_mul3(T, rp0.x, rp0.y, rp0.z, rp0.w, rp);
+  iso_mul3(T, rn0.x, rn0.y, rn0.z, 0.0f, rn);
+  const float dv0 = cp.x - rp[0], dv1 = cp.y - rp[1], dv2 = cp.z - rp[2], dv3 = cp.w - rp0.w;
+  if (dot4(dv0, dv1, dv2, dv3, dv0, dv1, dv2, dv3) > ap.sq_thr) return false;
+  if (!ap.gicp) {
+    if (dot4(cn.x, cn.y, cn.z, 0.0f, rn[0], rn[1], rn[2], 0.0f) < ap.cos_thr) return false;
+    float rc = rn0.w, cc = cn.w;
+    if (rc < ap.flat) rc = ap.flat;
+    if (cc < ap.flat) cc = ap.flat;
+    const float ratio = (float)(((double)rc + 1e-5) / ((double)cc + 1e-5));
+    if (ratio < ap.min_ratio || ratio > ap.max_ratio) return false;
+  }
+  return true;
 }
 
 // CorrespondenceFinderProjective gates for (ref index ri, cur index ci); T = finder transform
@@ -721,34 +760,80 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_corr_lin(const PairDev 
   const bool multi = ap.row_region != 0 || ap.col_region != 0;
   Acc2 acc[kVals];
 #pragma unroll
-  for (int v = 0; v < kVals; ++v) acc[v] = Acc2{0.0f, 0.0f};
-  for (int k = 0; k < 4; ++k) {
-    const int p = base + k * kThreads + threadIdx.x;
-    if (p >= P) break;
-    const unsigned long long key = pd.zbuf[p];
-    pd.zbuf[p] = kZInit;
-    const int ri = key == kZInit ? -1 : (int)(unsigned int)(key & 0xffffffffull);
-    int ci = -1;
-    float rp[3], rn[3];
-    float4 cp, cn;
-    bool ok = false;
-    if (!multi) {
-      ci = pd.cur_index[p];
-      if (ri >= 0 && ci >= 0) {
-        acc_add(acc[30], 1.0f);
-        ok = gates(fr, fc, ri, ci, T, ap, rp, rn, cp, cn);
-      }
-    } else if (ri >= 0) {
-      const int r = p / pd.cols, c = p - r * pd.cols;
-      ci = window_search(fr, fc, pd.cur_index, pd.rows, pd.cols, r, c, ri, T, ap, rp, rn);
-      if (ci >= 0) {
-        ok = true;
-        cp = fc.pts[ci];
-        cn = fc.nrm[ci];
+  for (int v = 0; v < kVals; ++v) acc[v] = 0.0;
+  if (!multi) {
+    // all loads of the thread's four pixels are issued before any of them is used
+    int ri[4], ci[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int p = base + k * kThreads + threadIdx.x;
+      const unsigned long long key = p < P ? pd.zbuf[p] : kZInit;
+      ri[k] = key == kZInit ? -1 : (int)(unsigned int)(key & 0xffffffffull);
+      ci[k] = p < P ? pd.cur_index[p] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int p = base + k * kThreads + threadIdx.x;
+      if (p < P) pd.zbuf[p] = kZInit;
+    }
+    float4 cn[4], rn0[4], cp[4], rp0[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (ri[k] >= 0 && ci[k] >= 0) {
+        cn[k] = fc.nrm[ci[k]];
+        rn0[k] = fr.nrm[ri[k]];
+        cp[k] = fc.pts[ci[k]];
+        rp0[k] = fr.pts[ri[k]];
       }
     }
-    if (store_pairs) pd.pairs[p] = ok ? make_int2(ri, ci) : make_int2(-1, -1);
-    if (ok) lin_terms(rp, rn, cp, cn, fc.omg + 3 * (size_t)ci, ap, acc);
+    bool ok[4];
+    float rp[4][3], rn[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      ok[k] = false;
+      if (ri[k] >= 0 && ci[k] >= 0) {
+        acc_add(acc[30], 1.0f);
+        ok[k] = gates_loaded(cn[k], rn0[k], cp[k], rp0[k], T, ap, rp[k], rn[k]);
+      }
+    }
+    float4 om[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (ok[k]) {
+        const float4 *o = fc.omg + 3 * (size_t)ci[k];
+        om[k][0] = o[0];
+        om[k][1] = o[1];
+        om[k][2] = o[2];
+      }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int p = base + k * kThreads + threadIdx.x;
+      if (store_pairs && p < P) pd.pairs[p] = ok[k] ? make_int2(ri[k], ci[k]) : make_int2(-1, -1);
+      if (ok[k]) lin_terms(rp[k], rn[k], cp[k], cn[k], om[k], ap, acc);
+    }
+  } else {
+    for (int k = 0; k < 4; ++k) {
+      const int p = base + k * kThreads + threadIdx.x;
+      if (p >= P) break;
+      const unsigned long long key = pd.zbuf[p];
+      pd.zbuf[p] = kZInit;
+      const int ri = key == kZInit ? -1 : (int)(unsigned int)(key & 0xffffffffull);
+      int ci = -1;
+      float rp[3], rn[3];
+      float4 cp, cn;
+      bool ok = false;
+      if (ri >= 0) {
+        const int r = p / pd.cols, c = p - r * pd.cols;
+        ci = window_search(fr, fc, pd.cur_index, pd.rows, pd.cols, r, c, ri, T, ap, rp, rn);
+        if (ci >= 0) {
+          ok = true;
+          cp = fc.pts[ci];
+          cn = fc.nrm[ci];
+        }
+      }
+      if (store_pairs) pd.pairs[p] = ok ? make_int2(ri, ci) : make_int2(-1, -1);
+      if (ok) lin_terms(rp, rn, cp, cn, fc.omg + 3 * (size_t)ci, ap, acc);
+    }
   }
   block_partial(acc, pd.partials + (size_t)t * kPartial);
 }
@@ -770,18 +855,34 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_lin_pairs(const PairDev
   for (int k = 0; k < 16; ++k) T[k] = pd.state[slot + k];
   Acc2 acc[kVals];
 #pragma unroll
-  for (int v = 0; v < kVals; ++v) acc[v] = Acc2{0.0f, 0.0f};
+  for (int v = 0; v < kVals; ++v) acc[v] = 0.0;
+  int2 pr[4];
+#pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int p = base + k * kThreads + threadIdx.x;
-    if (p >= P) break;
-    const int2 pr = pd.pairs[p];
-    if (pr.x < 0) continue;
-    const float4 rp0 = fr.pts[pr.x], rn0 = fr.nrm[pr.x];
-    float rp[3], rn[3];
-    iso_mul3(T, rp0.x, rp0.y, rp0.z, rp0.w, rp);
-    iso_mul3(T, rn0.x, rn0.y, rn0.z, 0.0f, rn);
-    lin_terms(rp, rn, fc.pts[pr.y], fc.nrm[pr.y], fc.omg + 3 * (size_t)pr.y, ap, acc);
+    pr[k] = p < P ? pd.pairs[p] : make_int2(-1, -1);
   }
+  float4 rp0[4], rn0[4], cp[4], cn[4], om[4][3];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (pr[k].x >= 0) {
+      rp0[k] = fr.pts[pr[k].x];
+      rn0[k] = fr.nrm[pr[k].x];
+      cp[k] = fc.pts[pr[k].y];
+      cn[k] = fc.nrm[pr[k].y];
+      const float4 *o = fc.omg + 3 * (size_t)pr[k].y;
+      om[k][0] = o[0];
+      om[k][1] = o[1];
+      om[k][2] = o[2];
+    }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (pr[k].x >= 0) {
+      float rp[3], rn[3];
+      iso_mul3(T, rp0[k].x, rp0[k].y, rp0[k].z, rp0[k].w, rp);
+      iso_mul3(T, rn0[k].x, rn0[k].y, rn0[k].z, 0.0f, rn);
+      lin_terms(rp, rn, cp[k], cn[k], om[k], ap, acc);
+    }
   block_partial(acc, pd.partials + (size_t)t * kPartial);
 }
 
