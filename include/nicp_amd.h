@@ -21,6 +21,7 @@
 #ifndef NICP_AMD_H
 #define NICP_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -36,6 +37,7 @@ extern "C" {
 typedef struct nicp_ctx nicp_ctx;
 typedef struct nicp_cloud nicp_cloud;
 typedef struct nicp_batch nicp_batch;
+typedef struct nicp_initnet nicp_initnet;
 
 /* PinholePointProjector state (nicp/nicp/pinholepointprojector.h:67-70,
  * nicp/nicp/pointprojector.cpp:9-10). K is the (already scaled) camera matrix. */
@@ -103,6 +105,12 @@ int nicp_ctx_destroy(nicp_ctx *ctx);
 int nicp_ctx_synchronize(nicp_ctx *ctx);
 /* the HIP stream the context launches on (hipStream_t) */
 void *nicp_ctx_stream(nicp_ctx *ctx);
+/* plain device memory on the context's device (no reference counterpart; for callers that
+ * keep datasets or network inputs resident). kind: 0 host->device, 1 device->host,
+ * 2 device->device; copies complete before the call returns. */
+int nicp_device_alloc(nicp_ctx *ctx, size_t bytes, void **ptr);
+int nicp_device_free(nicp_ctx *ctx, void *ptr);
+int nicp_device_copy(nicp_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
 
 /* Replaces DepthImage_convert_16UC1_to_32FC1 (nicp/nicp/imageutils.cpp:85-99) +
  * DepthImageConverterIntegralImage::compute (nicp/nicp/depthimageconverterintegralimage.cpp:18-64)
@@ -189,6 +197,58 @@ int nicp_batch_align(nicp_batch *batch, int num_pairs, const int *ref_frame, con
                      nicp_align_result *results);
 /* number of points of a batch frame, and a cloud view of it (downloads only) */
 int nicp_batch_frame_cloud(nicp_batch *batch, int frame, nicp_cloud **out);
+
+/* ---- Synthetic scenes (no reference counterpart: the training-set and benchmark input
+ * generator). Renders the seeded 20-primitive scene scene_seeds[i] into the depth of batch
+ * frame frames[i] from the camera pose poses16[i] (column-major, camera in scene frame),
+ * with the frame's intrinsics, depth noise sigma = noise_coeff * z^2 m, quantised to mm
+ * (the frame's depth scale becomes 0.001) and hole_fraction random holes. streams[i] salts
+ * the noise/hole draws. Same images as the host generator. */
+int nicp_batch_render_synthetic(nicp_batch *batch, int num_frames, const int *frames, const uint64_t *scene_seeds,
+                                const uint64_t *streams, const float *poses16, float noise_coeff, float hole_fraction);
+
+/* ---- Learned initial guess (no reference counterpart: the reference's Aligner takes its
+ * initial guess from the caller, nicp/nicp/aligner.h:102-106). A CNN maps the 60x80
+ * DepthImage_scale reductions (nicp/nicp/imageutils.cpp:6-37) of a reference/current pair
+ * to the pose 6-vector (t, qx, qy, qz) of v2t (nicp/nicp/bm_se3.h:38-45). Weights are fp32,
+ * the layers run as bf16 matrix-core GEMMs with fp32 accumulation. Input tensors are
+ * NHWC float [batch][60][80][2] (channel 0 = reference depth in metres, 1 = current). */
+int nicp_initnet_num_params(void);
+int nicp_initnet_create(nicp_ctx *ctx, uint64_t seed, nicp_initnet **out);
+int nicp_initnet_destroy(nicp_initnet *net);
+int nicp_initnet_get_params(nicp_initnet *net, float *params_out);
+int nicp_initnet_set_params(nicp_initnet *net, const float *params); /* also resets the optimiser */
+int nicp_initnet_set_loss_weights(nicp_initnet *net, const float weights6[6]);
+/* input/out6 are device pointers when on_device != 0, host pointers otherwise */
+int nicp_initnet_forward(nicp_initnet *net, const float *input, int batch, int on_device, float *out6);
+/* one step of the weighted squared error against target6 ([batch][6]); lr > 0 applies Adam */
+int nicp_initnet_train_step(nicp_initnet *net, const float *input, const float *target6, int batch, int on_device,
+                            float lr, float *loss);
+/* the same step on the minibatch rows[0..batch) of device-resident inputs ([num_rows][60][80][2]),
+ * each transformed by its symmetry flags (NULL = none; bit 0 swaps the two frames, bit 1
+ * mirrors the columns, bit 2 the rows), against host targets6 [batch][6] (the transformed
+ * poses are the caller's). rows and flags are host arrays. */
+int nicp_initnet_train_step_rows(nicp_initnet *net, const float *inputs, int num_rows, const int *rows,
+                                 const int *flags, const float *targets6, int batch, float lr, float *loss);
+/* 0: the first layer reads depth in metres (default); 1: inverse depth (0 stays 0) */
+int nicp_initnet_set_input_mode(nicp_initnet *net, int mode);
+/* copies of the last pass: what 0 = activation index (0 = input .. 6 = output), 1 = its
+ * gradient, 2 = parameter gradient (index ignored); count floats */
+int nicp_initnet_read(nicp_initnet *net, int what, int index, float *out, size_t count);
+/* the network input of pairs (ref_frame[i], cur_frame[i]) of a batch into dev_out (device
+ * pointer, [num_pairs][60][80][2]); frame sizes must be multiples of 80x60 */
+int nicp_batch_net_input(nicp_batch *batch, nicp_initnet *net, int num_pairs, const int *ref_frame,
+                         const int *cur_frame, float max_depth_cov, float *dev_out);
+/* AlignerProjective::align (nicp/nicp/alignerprojective.cpp:21-136) from a learned start:
+ * the network predicts a start per pair; one linearisation at the prediction and at the
+ * fallback guess (fallback16, or the identity when NULL) judges both, and the prediction is
+ * kept unless the fallback has a lower chi2 per correspondence with at least half the
+ * prediction's correspondences (or the prediction has none). start16 (may be NULL) receives
+ * the chosen starts, used_network (may be NULL) 1 where the prediction was kept.
+ * T_out16/results as nicp_batch_align. */
+int nicp_batch_align_learned(nicp_batch *batch, nicp_initnet *net, int num_pairs, const int *ref_frame,
+                             const int *cur_frame, const float *fallback16, const nicp_align_params *ap, int use_graph,
+                             float *T_out16, nicp_align_result *results, float *start16, int *used_network);
 
 /* Per-kernel device time (no reference counterpart; the reference only has
  * Aligner::totalTime(), nicp/nicp/aligner.h:348). When enabled, every launch of the

@@ -58,7 +58,11 @@ EXPORTS = [
     "nicp_batch_destroy", "nicp_batch_upload_depth", "nicp_batch_convert", "nicp_batch_align",
     "nicp_batch_frame_cloud", "nicp_kernel_count", "nicp_kernel_name", "nicp_ctx_profile",
     "nicp_ctx_profile_read", "nicp_nn_correspondences", "nicp_align_nn", "nicp_cloud_from_depth_f32",
-    "nicp_depth_scale",
+    "nicp_depth_scale", "nicp_batch_render_synthetic", "nicp_initnet_num_params", "nicp_initnet_create",
+    "nicp_initnet_destroy", "nicp_initnet_get_params", "nicp_initnet_set_params", "nicp_initnet_set_loss_weights",
+    "nicp_initnet_forward", "nicp_initnet_train_step", "nicp_initnet_read", "nicp_batch_net_input",
+    "nicp_batch_align_learned", "nicp_device_alloc", "nicp_device_free", "nicp_device_copy",
+    "nicp_initnet_train_step_rows", "nicp_initnet_set_input_mode",
 ]
 
 _lib = None
@@ -101,6 +105,20 @@ def lib():
             "nicp_align_nn": [vp, vp, vp, vp, vp, C.c_float, C.c_float, vp, vp],
             "nicp_cloud_from_depth_f32": [vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp],
             "nicp_depth_scale": [vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, vp],
+            "nicp_batch_render_synthetic": [vp, C.c_int, vp, vp, vp, vp, C.c_float, C.c_float],
+            "nicp_initnet_num_params": [],
+            "nicp_initnet_create": [vp, C.c_uint64, vp], "nicp_initnet_destroy": [vp],
+            "nicp_initnet_get_params": [vp, vp], "nicp_initnet_set_params": [vp, vp],
+            "nicp_initnet_set_loss_weights": [vp, vp],
+            "nicp_initnet_forward": [vp, vp, C.c_int, C.c_int, vp],
+            "nicp_initnet_train_step": [vp, vp, vp, C.c_int, C.c_int, C.c_float, vp],
+            "nicp_initnet_read": [vp, C.c_int, C.c_int, vp, C.c_size_t],
+            "nicp_batch_net_input": [vp, vp, C.c_int, vp, vp, C.c_float, vp],
+            "nicp_batch_align_learned": [vp, vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp],
+            "nicp_device_alloc": [vp, C.c_size_t, vp], "nicp_device_free": [vp, vp],
+            "nicp_device_copy": [vp, vp, vp, C.c_size_t, C.c_int],
+            "nicp_initnet_train_step_rows": [vp, vp, C.c_int, vp, vp, vp, C.c_int, C.c_float, vp],
+            "nicp_initnet_set_input_mode": [vp, C.c_int],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -385,3 +403,159 @@ class Batch:
         h = C.c_void_p()
         _check(lib().nicp_batch_frame_cloud(self.h, frame, C.byref(h)))
         return Cloud(self.ctx, h, shape)
+
+    def render_synthetic(self, frames, scene_seeds, streams, poses16, noise=1.5e-3, holes=0.1):
+        """Renders seeded synthetic scenes into the depth of the given frames on the device."""
+        frames = np.ascontiguousarray(frames, np.int32)
+        n = len(frames)
+        seeds = np.ascontiguousarray(scene_seeds, np.uint64)
+        streams = np.ascontiguousarray(streams, np.uint64)
+        poses = np.ascontiguousarray(poses16, np.float32).reshape(n, 16)
+        _check(lib().nicp_batch_render_synthetic(self.h, n, _p(frames), _p(seeds), _p(streams), _p(poses), noise,
+                                                 holes))
+
+    def net_input(self, net, ref, cur, dev_ptr, max_depth_cov=0.01):
+        """Writes the network input of the pairs to the device pointer dev_ptr ([n][60][80][2] float)."""
+        ref = np.ascontiguousarray(ref, np.int32)
+        cur = np.ascontiguousarray(cur, np.int32)
+        _check(lib().nicp_batch_net_input(self.h, net.h, len(ref), _p(ref), _p(cur), max_depth_cov,
+                                          C.c_void_p(int(dev_ptr))))
+
+    def align_learned(self, net, ref, cur, ap, fallback16=None, use_graph=True, want_results=True):
+        """Returns (T, results, starts, used_network)."""
+        ref = np.ascontiguousarray(ref, np.int32)
+        cur = np.ascontiguousarray(cur, np.int32)
+        n = len(ref)
+        fb = None if fallback16 is None else np.ascontiguousarray(fallback16, np.float32).reshape(n, 16)
+        T = np.empty((n, 16), np.float32)
+        starts = np.empty((n, 16), np.float32)
+        used = np.empty(n, np.int32)
+        res = (AlignResult * n)() if want_results else None
+        _check(lib().nicp_batch_align_learned(self.h, net.h, n, _p(ref), _p(cur), _p(fb), C.byref(ap), int(use_graph),
+                                              _p(T), res, _p(starts), _p(used)))
+        return T, res, starts, used
+
+
+class DeviceBuffer:
+    """Device memory of a context holding a float32 (or other dtype) array of a fixed shape."""
+
+    def __init__(self, ctx, shape, dtype=np.float32):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(lib().nicp_device_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def row_ptr(self, row):
+        """Address of row `row` along the first axis."""
+        return self.ptr + row * (self.nbytes // self.shape[0])
+
+    def upload(self, a, row=0):
+        a = np.ascontiguousarray(a, self.dtype)
+        if row * (self.nbytes // self.shape[0]) + a.nbytes > self.nbytes:
+            raise NicpError("upload past the end of the buffer")
+        _check(lib().nicp_device_copy(self.ctx.h, C.c_void_p(self.row_ptr(row)), _p(a), a.nbytes, 0))
+
+    def download(self, rows=None, row=0):
+        shape = self.shape if rows is None else (rows,) + self.shape[1:]
+        out = np.empty(shape, self.dtype)
+        if row * (self.nbytes // self.shape[0]) + out.nbytes > self.nbytes:
+            raise NicpError("download past the end of the buffer")
+        _check(lib().nicp_device_copy(self.ctx.h, _p(out), C.c_void_p(self.row_ptr(row)), out.nbytes, 1))
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().nicp_device_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class InitNet:
+    """The initial-guess CNN (device weights; bf16 matrix-core layers, fp32 accumulation)."""
+
+    IN_SHAPE = (60, 80, 2)
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib().nicp_initnet_create(ctx.h, seed, C.byref(h)))
+        self.h = h
+        self.num_params = lib().nicp_initnet_num_params()
+
+    def close(self):
+        if self.h:
+            lib().nicp_initnet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        out = np.empty(self.num_params, np.float32)
+        _check(lib().nicp_initnet_get_params(self.h, _p(out)))
+        return out
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.num_params:
+            raise NicpError("expected %d parameters, got %d" % (self.num_params, p.size))
+        _check(lib().nicp_initnet_set_params(self.h, _p(p)))
+
+    def set_loss_weights(self, w6):
+        w = np.ascontiguousarray(w6, np.float32)
+        _check(lib().nicp_initnet_set_loss_weights(self.h, _p(w)))
+
+    def forward(self, x):
+        """x: host array (B, 60, 80, 2) -> (B, 6)."""
+        x = np.ascontiguousarray(x, np.float32)
+        B = x.shape[0]
+        out = np.empty((B, 6), np.float32)
+        _check(lib().nicp_initnet_forward(self.h, _p(x), B, 0, _p(out)))
+        return out
+
+    def forward_device(self, x_ptr, batch, out_ptr):
+        _check(lib().nicp_initnet_forward(self.h, C.c_void_p(int(x_ptr)), batch, 1, C.c_void_p(int(out_ptr))))
+
+    def train_step(self, x, target, lr=0.0):
+        """Host arrays; returns the loss (lr > 0 applies one Adam step)."""
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        loss = C.c_float()
+        _check(lib().nicp_initnet_train_step(self.h, _p(x), _p(t), x.shape[0], 0, lr, C.byref(loss)))
+        return loss.value
+
+    def train_step_device(self, x_ptr, t_ptr, batch, lr):
+        loss = C.c_float()
+        _check(lib().nicp_initnet_train_step(self.h, C.c_void_p(int(x_ptr)), C.c_void_p(int(t_ptr)), batch, 1, lr,
+                                             C.byref(loss)))
+        return loss.value
+
+    def train_step_rows(self, inputs, rows, targets, lr, flags=None):
+        """One step on the minibatch `rows` of the device-resident DeviceBuffer `inputs`, each
+        sample transformed by its symmetry flags, against host `targets` (batch, 6)."""
+        r = np.ascontiguousarray(rows, np.int32)
+        f = None if flags is None else np.ascontiguousarray(flags, np.int32)
+        t = np.ascontiguousarray(targets, np.float32)
+        loss = C.c_float()
+        _check(lib().nicp_initnet_train_step_rows(self.h, C.c_void_p(inputs.ptr), inputs.shape[0], _p(r), _p(f),
+                                                  _p(t), len(r), lr, C.byref(loss)))
+        return loss.value
+
+    def set_input_mode(self, mode):
+        _check(lib().nicp_initnet_set_input_mode(self.h, int(mode)))
+
+    def read(self, what, index, count):
+        out = np.empty(count, np.float32)
+        _check(lib().nicp_initnet_read(self.h, what, index, _p(out), count))
+        return out

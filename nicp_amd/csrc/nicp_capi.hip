@@ -262,6 +262,7 @@ struct nicp_batch {
   DevBuf pairs_tbl, maps, zbuf, cur_index, pairs_buf, state, partials, result;
   GraphCache graph;
   GraphCache conv_graph;
+  DevBuf synth_buf;  // render job tables of nicp_batch_render_synthetic
 };
 
 // ---------------------------------------------------------------------------------------
@@ -508,7 +509,8 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
 
 // the launch sequence of AlignerProjective::align for all pairs of a plan
 int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d_pairs, const FrameDev *d_frames,
-                 const int *d_blk_pair, const int *d_blk_ref, const int *d_blk_cur, const nicp_align_params *ap) {
+                 const int *d_blk_pair, const int *d_blk_ref, const int *d_blk_cur, const nicp_align_params *ap,
+                 bool with_stats = true) {
   const int np = (int)plan.pairs.size();
   const AlignParamsDev a = to_dev(ap);
   const dim3 tb(kThreads);
@@ -533,7 +535,7 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
     }
   }
   // Aligner::_computeStatistics: one more linearisation at the final T over the last correspondences
-  if (ap->outer_iterations > 0) {
+  if (ap->outer_iterations > 0 && with_stats) {
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
            kSlotTinv, 0);
     LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(64), s, d_pairs, 0.0f, 2, 0);
@@ -788,6 +790,38 @@ extern "C" int nicp_ctx_synchronize(nicp_ctx *c) {
 }
 
 extern "C" void *nicp_ctx_stream(nicp_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+// plain device buffers for callers that keep data resident (datasets, network inputs)
+extern "C" int nicp_device_alloc(nicp_ctx *c, size_t bytes, void **ptr) {
+  if (!c || !ptr) return set_err(NICP_ERR_INVALID, "bad arguments");
+  *ptr = nullptr;
+  HIPCHK(hipSetDevice(c->device));
+  hipError_t e = hipMalloc(ptr, std::max<size_t>(bytes, 256));
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return set_err(NICP_ERR_OOM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  return NICP_OK;
+}
+
+extern "C" int nicp_device_free(nicp_ctx *c, void *ptr) {
+  if (!c) return set_err(NICP_ERR_INVALID, "context is null");
+  if (!ptr) return NICP_OK;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipFree(ptr));
+  return NICP_OK;
+}
+
+extern "C" int nicp_device_copy(nicp_ctx *c, void *dst, const void *src, size_t bytes, int kind) {
+  if (!c || (bytes && (!dst || !src)) || kind < 0 || kind > 2) return set_err(NICP_ERR_INVALID, "bad arguments");
+  if (!bytes) return NICP_OK;
+  HIPCHK(hipSetDevice(c->device));
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, k, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NICP_OK;
+}
 
 // ---------------------------------------------------------------------------------------
 // clouds
@@ -1373,7 +1407,8 @@ extern "C" int nicp_batch_destroy(nicp_batch *b) {
   if (b->ctx->pending_graph == &b->graph.events) b->ctx->pending_graph = nullptr;
   b->graph.reset();
   b->conv_graph.reset();
-  DevBuf *bufs[] = {&b->pairs_tbl, &b->maps, &b->zbuf, &b->cur_index, &b->pairs_buf, &b->state, &b->partials, &b->result};
+  DevBuf *bufs[] = {&b->pairs_tbl, &b->maps,  &b->zbuf,   &b->cur_index, &b->pairs_buf,
+                    &b->state,     &b->partials, &b->result, &b->synth_buf};
   for (DevBuf *x : bufs) x->release();
   (void)hipFree(b->arena);
   delete b;
@@ -1403,9 +1438,13 @@ extern "C" int nicp_batch_convert(nicp_batch *b) {
   return NICP_OK;
 }
 
-extern "C" int nicp_batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *cur_frame,
-                                const float *init_guesses16, const nicp_align_params *ap, int use_graph, float *T_out16,
-                                nicp_align_result *results) {
+namespace {
+// AlignerProjective::align for a list of pairs of a batch. with_stats = false skips the
+// statistics linearisation (results then hold the in-loop linearisation only and no
+// solution statistics); raw, when not null, receives the 2 x kPartial result slots per pair.
+int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *cur_frame, const float *init_guesses16,
+                const nicp_align_params *ap, int use_graph, float *T_out16, nicp_align_result *results,
+                bool with_stats, std::vector<double> *raw) {
   if (!b || num_pairs <= 0 || !ref_frame || !cur_frame) return set_err(NICP_ERR_INVALID, "bad arguments");
   CHK(check_align_params(ap));
   for (int i = 0; i < num_pairs; ++i)
@@ -1440,12 +1479,13 @@ extern "C" int nicp_batch_align(nicp_batch *b, int num_pairs, const int *ref_fra
     key.push_back((int)(((uintptr_t)b->partials.p) & 0x7fffffff));
     key.push_back((int)(((uintptr_t)b->zbuf.p) & 0x7fffffff));
     key.push_back(ctx->prof.on ? 1 : 0);
+    key.push_back(with_stats ? 1 : 0);
     if (b->graph.exec == nullptr || b->graph.key != key) {
       b->graph.reset();
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       ctx->prof.sink = &b->graph.events;
-      int rc = launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap);
+      int rc = launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats);
       ctx->prof.sink = &ctx->prof.eager;
       hipError_t e = hipStreamEndCapture(ctx->stream, &g);
       if (rc != NICP_OK) {
@@ -1466,14 +1506,16 @@ extern "C" int nicp_batch_align(nicp_batch *b, int num_pairs, const int *ref_fra
     ctx->pending_graph = ctx->prof.on ? &b->graph.events : nullptr;
   } else {
     ctx->prof.sink = &ctx->prof.eager;
-    CHK(launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap));
+    CHK(launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats));
   }
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-  if (!T_out16 && !results) return NICP_OK;
+  if (!T_out16 && !results && !raw) return NICP_OK;
   std::vector<float> st((size_t)num_pairs * kStateFloats);
-  std::vector<double> rr((size_t)num_pairs * 2 * kPartial);
+  std::vector<double> local;
+  std::vector<double> &rr = raw ? *raw : local;
+  rr.assign((size_t)num_pairs * 2 * kPartial, 0.0);
   HIPCHK(hipMemcpyAsync(st.data(), b->state.p, st.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
-  if (results)
+  if ((results || raw) && ap->outer_iterations > 0)
     HIPCHK(hipMemcpyAsync(rr.data(), b->result.p, rr.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   float ms = 0.0f;
@@ -1483,15 +1525,22 @@ extern "C" int nicp_batch_align(nicp_batch *b, int num_pairs, const int *ref_fra
     if (ctx->pending_graph) ctx->prof.read(*ctx->pending_graph, false);
     ctx->pending_graph = nullptr;
   }
+  nicp_align_params ap_res = *ap;
+  if (!with_stats) ap_res.outer_iterations = 0;  // no statistics were computed
   for (int i = 0; i < num_pairs; ++i) {
     const float *T = &st[(size_t)i * kStateFloats];
     if (T_out16) memcpy(T_out16 + 16 * (size_t)i, T, 16 * sizeof(float));
-    if (results) {
-      if (ap->outer_iterations == 0) memset(&rr[(size_t)i * 2 * kPartial], 0, 2 * kPartial * sizeof(double));
-      fill_result(&rr[(size_t)i * 2 * kPartial], T, ap, &results[i], ms);
-    }
+    if (results) fill_result(&rr[(size_t)i * 2 * kPartial], T, &ap_res, &results[i], ms);
   }
   return NICP_OK;
+}
+}  // namespace
+
+extern "C" int nicp_batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *cur_frame,
+                                const float *init_guesses16, const nicp_align_params *ap, int use_graph, float *T_out16,
+                                nicp_align_result *results) {
+  return batch_align(b, num_pairs, ref_frame, cur_frame, init_guesses16, ap, use_graph, T_out16, results, true,
+                     nullptr);
 }
 
 extern "C" int nicp_batch_frame_cloud(nicp_batch *b, int frame, nicp_cloud **out) {
@@ -1727,3 +1776,9 @@ extern "C" int nicp_align_nn(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cl
   ctx->last_valid = ap->outer_iterations > 0;
   return NICP_OK;
 }
+
+// ---------------------------------------------------------------------------------------
+// learned initial guess and the synthetic-scene renderer
+// ---------------------------------------------------------------------------------------
+#include "initnet.hip"
+#include "synth_gpu.hip"
