@@ -1,17 +1,24 @@
-"""Benchmark: aligned 640x480 depth pairs per second on one MI355X.
+"""Benchmark: aligned 640x480 depth pairs per second on one MI355X, batched NICP
+from a learned initial guess against plain single-pair NICP.
 
-One step = the registration path over one batch of synthetic pairs already
-resident in HBM: both depth frames of every pair converted to clouds
-(unprojection, integral-image normals, information matrices) and every pair
-aligned by 10 Gauss-Newton iterations of projective NICP (one launch per stage
-for the whole batch, the iteration chain replayed from a hipGraph). A pair
-counts as aligned when its rotation error is below 0.1 deg and its
-translation error below 2 mm against the sampled ground truth.
+Workload: held-out seeded synthetic pairs (20-plane/box scenes, second camera moved
+by up to 30 deg / 0.5 m, depth noise sigma = 1.5e-3 z^2, 10 % holes), rendered on
+the device into a batch of frames before each step (input generation, not timed).
+One step = the registration path over one batch of pairs resident in HBM: both
+frames of every pair converted to clouds (unprojection, integral-image normals,
+information matrices), the network input and forward pass of the learned initial
+guess, one linearisation at the predicted and at the identity start to choose the
+start, and 10 Gauss-Newton iterations of projective NICP from the chosen start --
+one launch per stage for the whole batch. A pair counts as aligned when its
+rotation error is below 0.1 deg and its translation error below 2 mm against the
+sampled ground truth.
 
-Prints one JSON line. Extra keys: the plain single-pair GPU rate (one pair per
-launch, no graph), per-kernel device times from HIP events recorded on the
-library's stream inside the timed region, the roofline of the dominant kernel,
-and the CPU oracle timed on a bounded sample on the host cores.
+Prints one JSON line. Besides the contract keys: the plain baselines (single pair
+per call from the identity, and batched from the identity = the same pairs without
+the network), per-kernel device times from HIP events recorded on the library's
+stream inside the timed region, the HBM roofline of the dominant kernel, the
+matrix-core rate of the network's forward pass, and the CPU oracle on a bounded
+sample on the host cores.
 """
 import argparse
 import json
@@ -25,13 +32,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+ROWS, COLS = 480, 640
+MAX_ROT_DEG, MAX_TRANS_M = 30.0, 0.5
+EVAL_SEED_BASE = 1000          # training pairs use seeds from 10,000,000 on
+WARMUP_SEED_BASE = 900000
+# multiply-adds of the network's forward pass per pair (conv 1..4, fc 1, fc 2), x2 flops
+NET_FLOPS_PER_PAIR = 2 * (1200 * 19 * 32 + 300 * 289 * 32 + 80 * 289 * 64 + 20 * 577 * 64 + 1281 * 256 + 257 * 6)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name, pairs_info, frames_info, iters):
+def kernel_bytes(name, pairs_info, frames_info):
     """Algorithmic HBM bytes of one launch of a kernel (covering the whole batch)."""
     P = sum(f[0] for f in frames_info)
     N = sum(f[1] for f in frames_info)
@@ -58,116 +72,145 @@ def kernel_bytes(name, pairs_info, frames_info, iters):
     return None
 
 
-def cpu_baseline(depth_pairs, rows, cols, budget_s=20.0, max_pairs=16):
+def aligned(T, Tgt):
+    """Aligned mask and (rotation deg, translation m) errors; a diverged pair (non-finite T, as
+    the reference's quat2mat gives for an update beyond the unit quaternion) has infinite error."""
+    from nicp_amd import synth
+    e = np.array([synth.pose_error(T[i], Tgt[i]) if np.all(np.isfinite(T[i])) else (np.inf, np.inf)
+                  for i in range(len(T))])
+    return (e[:, 0] < 0.1) & (e[:, 1] < 0.002), e
+
+
+def cpu_baseline(seeds, max_rot, max_trans, budget_s=20.0):
     """The CPU oracle (the reference's FP32 arithmetic, OpenMP as in the reference) on a
-    bounded sample: both conversions and AlignerProjective::align per pair."""
+    bounded sample: both conversions and AlignerProjective::align from the identity."""
     import oracle as O
     from nicp_amd import synth
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-    fx, fy, cx, cy = synth.camera_for(rows, cols)
+    fx, fy, cx, cy = synth.camera_for(ROWS, COLS)
     cam = O.camera(fx, fy, cx, cy)
     sp = O.stats_params()
     ap = O.align_params()
     I = np.eye(4, dtype=np.float32).flatten(order="F")
-    # warm-up
-    dr, dc, _ = depth_pairs[0]
-    fr = O.Frame(dr, 0.001, cam, sp)
-    fc = O.Frame(dc, 0.001, cam, sp)
-    O.align(fr.cloud(), fc.cloud(), rows, cols, cam, I, ap, threads)
+    pairs = [synth.pair(int(s), ROWS, COLS, max_rot_deg=max_rot, max_trans_m=max_trans) for s in seeds]
+    dr, dc, _ = pairs[0]
+    fr, fc = O.Frame(dr, 0.001, cam, sp), O.Frame(dc, 0.001, cam, sp)  # the clouds point into these
+    O.align(fr.cloud(), fc.cloud(), ROWS, COLS, cam, I, ap, threads)
     t0 = time.perf_counter()
     n = 0
-    for dr, dc, _ in depth_pairs[:max_pairs]:
+    for dr, dc, _ in pairs:
         fr = O.Frame(dr, 0.001, cam, sp)
         fc = O.Frame(dc, 0.001, cam, sp)
-        O.align(fr.cloud(), fc.cloud(), rows, cols, cam, I, ap, threads)
+        O.align(fr.cloud(), fc.cloud(), ROWS, COLS, cam, I, ap, threads)
         n += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": "%d pairs %dx%d (2 conversions + 10-iteration align each), OpenMP threads %d" % (
-                n, cols, rows, threads)}
+    return {"value": round(n / dt, 3), "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": "%d of the benchmark's pairs at %dx%d (2 conversions + 10-iteration align from the identity "
+                      "each), OpenMP threads %d" % (n, COLS, ROWS, threads)}
+
+
+def calibrate_copy(mib):
+    """A device-to-device copy of known size, for scaling rocprof's FETCH_SIZE / WRITE_SIZE."""
+    from nicp_amd import capi
+    ctx = capi.Context(0)
+    n = mib * (1 << 20)
+    a = capi.DeviceBuffer(ctx, (n,), np.uint8)
+    b = capi.DeviceBuffer(ctx, (n,), np.uint8)
+    a.upload(np.ones(n, np.uint8))
+    t0 = time.perf_counter()
+    for _ in range(5):
+        capi.lib().nicp_device_copy(ctx.h, capi.C.c_void_p(b.ptr), capi.C.c_void_p(a.ptr), n, 2)
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "copy_GB/s": round(2 * n / dt / 1e9, 1)}))
 
 
 def main():
     ap_ = argparse.ArgumentParser()
     ap_.add_argument("--gpus", type=int, default=1)
-    ap_.add_argument("--steps", type=int, default=10)
-    ap_.add_argument("--warmup", type=int, default=2)
-    ap_.add_argument("--pairs", type=int, default=128)
+    ap_.add_argument("--steps", type=int, default=32, help="timed steps (each a new batch of pairs)")
+    ap_.add_argument("--warmup", type=int, default=1)
+    ap_.add_argument("--pairs", type=int, default=128, help="pairs per step")
     ap_.add_argument("--iters", type=int, default=10)
+    ap_.add_argument("--weights", default=None, help="network weights (.npy of bf16 bits)")
     ap_.add_argument("--no-cpu", action="store_true")
     ap_.add_argument("--no-profile", action="store_true")
-    ap_.add_argument("--plain-pairs", type=int, default=8)
+    ap_.add_argument("--plain-pairs", type=int, default=32)
+    ap_.add_argument("--max-rot", type=float, default=MAX_ROT_DEG, help="motion range of the pairs (deg)")
+    ap_.add_argument("--max-trans", type=float, default=MAX_TRANS_M, help="motion range of the pairs (m)")
     ap_.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                      help="only run a device-to-device copy of MIB MiB (to scale rocprof's FETCH_SIZE/WRITE_SIZE)")
     args = ap_.parse_args()
     if args.calibrate_copy:
-        import torch
-        n = args.calibrate_copy * (1 << 20) // 4
-        a = torch.ones(n, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / 5
-        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n,
-                          "copy_GB/s": round(8 * n / dt / 1e9, 1)}))
+        calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (pairs shard "
               "independently across GPUs). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    from nicp_amd import capi, synth
-    rows, cols = 480, 640
-    fx, fy, cx, cy = synth.camera_for(rows, cols)
-    B = args.pairs
-    t0 = time.time()
-    pairs = [synth.pair(1000 + i, rows, cols, max_rot_deg=5.0, max_trans_m=0.05) for i in range(B)]
-    log("generated %d pairs in %.1f s" % (B, time.time() - t0))
+    from nicp_amd import capi, initnet
+    max_rot, max_trans = args.max_rot, args.max_trans
+    weights_path = args.weights or initnet.DEFAULT_WEIGHTS
+    weights = initnet.load_weights(weights_path)
+    if weights is None:
+        print("no trained weights at %s: train them with python -m nicp_amd.initnet" % weights_path,
+              file=sys.stderr)
+        sys.exit(1)
+    record_path = os.path.splitext(weights_path)[0] + ".json"
+    input_mode = json.load(open(record_path)).get("input_mode", 0) if os.path.exists(record_path) else 0
 
+    B = args.pairs
     ctx = capi.Context(0)
-    cam = capi.camera(fx, fy, cx, cy)
-    sp = capi.stats_params()
+    net = capi.InitNet(ctx, seed=1)
+    net.set_params(weights)
+    net.set_input_mode(input_mode)
     ap = capi.align_params(outer_iterations=args.iters)
-    batch = capi.Batch(ctx, [(rows, cols)] * (2 * B), [cam] * (2 * B), sp)
-    for i, (dr, dc, _) in enumerate(pairs):
-        batch.upload(2 * i, dr)
-        batch.upload(2 * i + 1, dc)
+    batch = initnet.make_batch(ctx, B)
     ref = np.arange(0, 2 * B, 2, dtype=np.int32)
     cur = ref + 1
-    Tgt = np.stack([p[2] for p in pairs])
 
-    def step(graph):
+    def seeds_of(base, k):
+        return np.arange(base + k * B, base + (k + 1) * B, dtype=np.uint64)
+
+    def learned_step(use_graph):
         batch.convert()
-        return batch.align(ref, cur, None, ap, use_graph=graph)
+        T, _, _, used = batch.align_learned(net, ref, cur, ap, use_graph=use_graph, want_results=False)
+        return T, used
 
-    # hipGraph replay of the iteration chain (HIP cannot time event nodes inside a graph,
-    # so the timed region below launches eagerly with an event pair around every kernel)
-    for _ in range(args.warmup):
-        step(True)
+    def plain_step(use_graph):
+        batch.convert()
+        T, _ = batch.align(ref, cur, None, ap, use_graph=use_graph, want_results=False)
+        return T
+
+    # warm-up (separate seeds)
+    for k in range(args.warmup):
+        initnet.render_pairs(batch, seeds_of(WARMUP_SEED_BASE, k), max_rot, max_trans)
+        learned_step(False)
+        plain_step(True)
     ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(True)
-    ctx.synchronize()
-    graph_ms = 1000.0 * (time.perf_counter() - t0) / args.steps
-    for _ in range(args.warmup):
-        step(False)
-    ctx.synchronize()
+
+    # timed region: every step a new batch, rendered beforehand; HIP events around every launch
     if not args.no_profile:
         ctx.profile(True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        T, res = step(False)
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
-    prof = {}
-    prof_error = None
+    t_total = 0.0
+    n_ok = 0
+    used_total = 0
+    Ts, Tgts = [], []
+    for k in range(args.steps):
+        Tgt = initnet.render_pairs(batch, seeds_of(EVAL_SEED_BASE, k), max_rot, max_trans)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        T, used = learned_step(False)
+        ctx.synchronize()
+        t_total += time.perf_counter() - t0
+        ok, _ = aligned(T, Tgt)
+        n_ok += int(ok.sum())
+        used_total += int(used.sum())
+        Ts.append(T)
+        Tgts.append(Tgt)
+    prof, prof_error = {}, None
     if not args.no_profile:
         try:
             prof = ctx.profile_read()
@@ -175,65 +218,101 @@ def main():
             prof_error = str(e)
             log("kernel timing unavailable: %s" % e)
     ctx.profile(False)
+    n_pairs = B * args.steps
+    value = n_ok / t_total
+    ms_step = 1000.0 * t_total / args.steps
+    _, err_learned = aligned(np.concatenate(Ts), np.concatenate(Tgts))
+    log("batched+learned: %.2f ms/step, %d/%d aligned, %.1f pairs/s, aligned %.1f pairs/s, network start kept %d" % (
+        ms_step, n_ok, n_pairs, n_pairs / t_total, value, used_total))
 
-    errs = np.array([synth.pose_error(T[i], Tgt[i]) for i in range(B)])
-    ok = (errs[:, 0] < 0.1) & (errs[:, 1] < 0.002)
-    n_ok = int(ok.sum())
-    ms_step = 1000.0 * dt / args.steps
-    value = n_ok * args.steps / dt
-    log("batched: %.2f ms/step, %d/%d aligned, %.1f pairs/s, aligned %.1f pairs/s" % (
-        ms_step, n_ok, B, B * args.steps / dt, value))
+    # the same pairs without the network (batched, identity start, graph replay); bit-identical
+    # to the single-pair path, so its accuracy is the plain single-pair accuracy
+    t_plain_b = 0.0
+    n_ok_plain = 0
+    for k in range(args.steps):
+        Tgt = initnet.render_pairs(batch, seeds_of(EVAL_SEED_BASE, k), max_rot, max_trans)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        T = plain_step(True)
+        ctx.synchronize()
+        t_plain_b += time.perf_counter() - t0
+        n_ok_plain += int(aligned(T, Tgt)[0].sum())
+    frac_plain = n_ok_plain / n_pairs
+    log("batched from identity: %.2f ms/step, %d/%d aligned" % (1000 * t_plain_b / args.steps, n_ok_plain, n_pairs))
 
-    # per-frame / per-pair counts for the byte model
-    frames_info = []
+    # learned path with the refinement replayed from a hipGraph (kernel events off)
+    t_graph = 0.0
+    gsteps = min(4, args.steps)
+    for k in range(gsteps):
+        initnet.render_pairs(batch, seeds_of(EVAL_SEED_BASE, k), max_rot, max_trans)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        learned_step(True)
+        ctx.synchronize()
+        t_graph += time.perf_counter() - t0
+
+    # per-frame / per-pair counts of the last timed batch for the byte model
+    frames_info, pairs_info = [], []
+    initnet.render_pairs(batch, seeds_of(EVAL_SEED_BASE, args.steps - 1), max_rot, max_trans)
+    batch.convert()
+    _, res, _, _ = batch.align_learned(net, ref, cur, ap, use_graph=False, want_results=True)
     for f in range(2 * B):
-        c = batch.frame_cloud(f, (rows, cols))
-        frames_info.append((rows * cols, c.n))
+        c = batch.frame_cloud(f, (ROWS, COLS))
+        frames_info.append((ROWS * COLS, c.n))
         c.close()
-    pairs_info = [(rows * cols, frames_info[2 * i][1], res[i].overlap_pixels) for i in range(B)]
+    pairs_info = [(ROWS * COLS, frames_info[2 * i][1], res[i].overlap_pixels) for i in range(B)]
     kernels = {}
+    total_ms = sum(v[0] for v in prof.values()) if prof else 0.0
     for name, (ms, cnt) in prof.items():
         if cnt == 0:
             continue
         avg = ms / cnt
-        nb = kernel_bytes(name, pairs_info, frames_info, args.iters)
+        nb = kernel_bytes(name, pairs_info, frames_info)
         kernels[name] = {"avg_ms": round(avg, 4), "launches": cnt,
                          "GB/s": round(nb / (avg * 1e-3) / 1e9, 1) if nb else None,
-                         "share": round(ms / max(sum(v[0] for v in prof.values()), 1e-9), 3)}
-    dom = max(kernels, key=lambda k: prof[k][0]) if kernels else None
+                         "share": round(ms / max(total_ms, 1e-9), 3)}
+    hbm = [k for k in kernels if kernel_bytes(k, pairs_info, frames_info)]
+    dom = max(hbm, key=lambda k: prof[k][0]) if hbm else None
     roofline = None
     if dom:
-        nb = kernel_bytes(dom, pairs_info, frames_info, args.iters)
+        nb = kernel_bytes(dom, pairs_info, frames_info)
         avg = kernels[dom]["avg_ms"]
-        ach = nb / (avg * 1e-3) / 1e9 if nb else None
-        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1) if ach else None, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4) if ach else None, "traffic": None,
-                    "bytes_per_launch": nb, "avg_launch_ms": avg}
+        ach = nb / (avg * 1e-3) / 1e9
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": nb,
+                    "avg_launch_ms": avg}
+    net_roofline = None
+    if "initnet_forward" in kernels:
+        avg = kernels["initnet_forward"]["avg_ms"]
+        tf = NET_FLOPS_PER_PAIR * B / (avg * 1e-3) / 1e12
+        net_roofline = {"stage": "initnet_forward", "bound": "mfma", "achieved": round(tf, 2),
+                        "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 5),
+                        "flops_per_launch": NET_FLOPS_PER_PAIR * B, "avg_launch_ms": avg}
 
-    # plain single-pair GPU NICP: one pair per launch, no graph, inputs resident
+    # plain single-pair GPU NICP: one pair per call, no graph, identity start, inputs resident
     plain = None
     if args.plain_pairs > 0:
-        pb = capi.Batch(ctx, [(rows, cols)] * 2, [cam] * 2, sp)
-        one_r, one_c = np.array([0], np.int32), np.array([1], np.int32)
+        pb = initnet.make_batch(ctx, 1)
+        one = np.array([0], np.int32), np.array([1], np.int32)
         times = []
-        for i in range(min(args.plain_pairs, B) + 1):
-            dr, dc, _ = pairs[i % B]
-            pb.upload(0, dr)
-            pb.upload(1, dc)
+        for i in range(args.plain_pairs + 1):
+            initnet.render_pairs(pb, np.array([EVAL_SEED_BASE + i], np.uint64), max_rot, max_trans)
             ctx.synchronize()
             s0 = time.perf_counter()
             pb.convert()
-            pb.align(one_r, one_c, None, ap, use_graph=False)
+            pb.align(one[0], one[1], None, ap, use_graph=False, want_results=False)
             ctx.synchronize()
             if i > 0:
                 times.append(time.perf_counter() - s0)
         plain = 1.0 / float(np.median(times))
         pb.close()
-        log("plain single-pair: %.1f pairs/s" % plain)
+        log("plain single-pair: %.1f pairs/s, aligned fraction %.4f -> %.1f aligned pairs/s" % (
+            plain, frac_plain, plain * frac_plain))
+    plain_aligned = plain * frac_plain if plain else None
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(pairs, rows, cols)
+        cpu = cpu_baseline(np.arange(EVAL_SEED_BASE, EVAL_SEED_BASE + 16), max_rot, max_trans)
         log("cpu baseline: %s" % cpu)
 
     out = {
@@ -249,22 +328,31 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "%d pairs of 640x480 uint16 depth per step (seeded 20-plane/box scenes, "
-                               "rot<=5deg, trans<=5cm, sigma=1.5e-3 z^2, 10%% holes); both conversions + "
-                               "%d-iteration projective NICP per pair, identity initial guess" % (B, args.iters),
-                   "pairs_per_step": B, "resolution": "640x480", "iterations": args.iters, "init": "identity"},
-        "aligned_fraction": round(n_ok / B, 4),
-        "pairs_per_s": round(B * args.steps / dt, 2),
-        "graph_ms_per_step": round(graph_ms, 3),
+        "config": {"workload": "%d held-out pairs of 640x480 uint16 depth per step, a new batch each step "
+                               "(seeded 20-plane/box scenes, rot<=%gdeg, trans<=%gm, sigma=1.5e-3 z^2, 10%% holes); "
+                               "both conversions + learned initial guess (bf16 CNN, chi2 fallback to identity) + "
+                               "%d-iteration projective NICP per pair" % (B, max_rot, max_trans, args.iters),
+                   "pairs_per_step": B, "eval_pairs": n_pairs, "resolution": "640x480", "iterations": args.iters,
+                   "init": "learned", "weights": os.path.relpath(weights_path, ROOT)},
+        "aligned_fraction": round(n_ok / n_pairs, 4),
+        "pairs_per_s": round(n_pairs / t_total, 2),
+        "network_start_kept": round(used_total / n_pairs, 4),
+        "rot_err_median_deg": round(float(np.median(err_learned[:, 0])), 4),
+        "diverged_pairs": int(np.sum(~np.isfinite(err_learned[:, 0]))),
+        "graph_ms_per_step": round(1000.0 * t_graph / gsteps, 3),
         "plain_single_pair_pairs_per_s": round(plain, 2) if plain else None,
-        "batched_vs_plain": round((B * args.steps / dt) / plain, 2) if plain else None,
+        "plain_aligned_fraction": round(frac_plain, 4),
+        "plain_single_pair_aligned_pairs_per_s": round(plain_aligned, 2) if plain_aligned else None,
+        "vs_plain": round(value / plain_aligned, 2) if plain_aligned else None,
+        "batched_identity_ms_per_step": round(1000.0 * t_plain_b / args.steps, 3),
+        "batched_identity_aligned_pairs_per_s": round(n_ok_plain / t_plain_b, 2),
         "roofline": roofline,
+        "net_roofline": net_roofline,
         "kernels": kernels,
         "kernel_timing_error": prof_error,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    batch.close()
     ctx.close()
 
 

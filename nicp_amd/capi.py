@@ -6,6 +6,7 @@ HIP call raises :class:`NicpError`.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -192,9 +193,16 @@ class Context:
         h = C.c_void_p()
         _check(lib().nicp_ctx_create(device, C.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()  # objects whose device memory belongs to this context
+
+    def adopt(self, obj):
+        self._children.add(obj)
 
     def close(self):
+        """Releases the context and, first, every object still alive on it."""
         if self.h:
+            for c in list(self._children):
+                c.close()
             lib().nicp_ctx_destroy(self.h)
             self.h = None
 
@@ -316,6 +324,7 @@ class Context:
 class Cloud:
     def __init__(self, ctx, h, shape):
         self.ctx, self.h, self.shape = ctx, h, shape
+        ctx.adopt(self)
         n = C.c_int()
         _check(lib().nicp_cloud_size(h, C.byref(n)))
         self.n = n.value
@@ -369,6 +378,7 @@ class Batch:
         _check(lib().nicp_batch_create(ctx.h, n, _p(rows), _p(cols), cam_arr, C.byref(sp), C.byref(h)))
         self.h = h
         self.n = n
+        ctx.adopt(self)
 
     def close(self):
         if self.h:
@@ -395,8 +405,7 @@ class Batch:
         init = None if init16 is None else np.ascontiguousarray(init16, np.float32).reshape(n, 16)
         T = np.empty((n, 16), np.float32)
         res = (AlignResult * n)() if want_results else None
-        _check(lib().nicp_batch_align(self.h, n, _p(ref), _p(cur), _p(init), C.byref(ap), int(use_graph),
-                                      _p(T) if want_results else None, res))
+        _check(lib().nicp_batch_align(self.h, n, _p(ref), _p(cur), _p(init), C.byref(ap), int(use_graph), _p(T), res))
         return T, res
 
     def frame_cloud(self, frame, shape):
@@ -447,6 +456,7 @@ class DeviceBuffer:
         p = C.c_void_p()
         _check(lib().nicp_device_alloc(ctx.h, self.nbytes, C.byref(p)))
         self.ptr = p.value
+        ctx.adopt(self)
 
     def row_ptr(self, row):
         """Address of row `row` along the first axis."""
@@ -489,6 +499,7 @@ class InitNet:
         _check(lib().nicp_initnet_create(ctx.h, seed, C.byref(h)))
         self.h = h
         self.num_params = lib().nicp_initnet_num_params()
+        ctx.adopt(self)
 
     def close(self):
         if self.h:

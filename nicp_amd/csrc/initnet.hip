@@ -665,7 +665,7 @@ int launch_net_input(nicp_initnet *n, nicp_batch *b, int num_pairs, const int *r
   hipLaunchKernelGGL(k_net_input, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, (const InputJob *)n->jobs,
                      num_pairs, max_cov, dst);
   HIPCHK(hipGetLastError());
-  // the host job table goes out of scope and the device table is reused by the next call
+  // the host job table must outlive the copy, and the device table is rewritten by the next call
   HIPCHK(hipStreamSynchronize(s));
   return NICP_OK;
 }
@@ -695,9 +695,23 @@ extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   // 1. prediction
+  Prof &pf = ctx->prof;
+  pf.sink = &pf.eager;
   CHK(net_reserve(n, num_pairs));
-  CHK(launch_net_input(n, b, num_pairs, ref_frame, cur_frame, 0.01f, n->act[0]));
-  CHK(net_forward(n, num_pairs));
+  {
+    ProfEntry en{};
+    const bool p = pf.on;
+    if (p) pf.begin(KID_NET_INPUT, s, en);
+    CHK(launch_net_input(n, b, num_pairs, ref_frame, cur_frame, 0.01f, n->act[0]));
+    if (p) pf.end(s, en);
+  }
+  {
+    ProfEntry en{};
+    const bool p = pf.on;
+    if (p) pf.begin(KID_NET_FORWARD, s, en);
+    CHK(net_forward(n, num_pairs));
+    if (p) pf.end(s, en);
+  }
   std::vector<float> pred((size_t)num_pairs * 6);
   HIPCHK(hipMemcpyAsync(pred.data(), n->act[kLayers], pred.size() * sizeof(float), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -726,7 +740,21 @@ extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_
   probe.outer_iterations = 1;
   probe.inner_iterations = 1;
   std::vector<double> raw;
-  CHK(batch_align(b, 2 * num_pairs, ref2.data(), cur2.data(), starts.data(), &probe, 0, nullptr, nullptr, false, &raw));
+  {
+    // timed as one stage: its kernels are not profiled one by one
+    ProfEntry en{};
+    const bool p = pf.on;
+    if (p) pf.begin(KID_START_PROBE, s, en);
+    pf.on = false;
+    const int rc = batch_align(b, 2 * num_pairs, ref2.data(), cur2.data(), starts.data(), &probe, 0, nullptr, nullptr,
+                               false, &raw);
+    pf.on = p;
+    CHK(rc);
+    if (p) {
+      pf.sink = &pf.eager;
+      pf.end(s, en);
+    }
+  }
   // 3. judge the starts by chi2 per correspondence (result slots 27 = chi2, 28 = inliers of
   //    the linearisation). With the robust kernel on, the Linearizer counts every gated
   //    correspondence as an inlier (linearizer.cpp:82-90), and the 1 m distance gate passes

@@ -101,12 +101,16 @@ struct Carver {
 // ---------------------------------------------------------------------------------------
 enum KernelId {
   KID_COUNT, KID_SCAN, KID_UNPROJECT, KID_INTEGRAL_X, KID_INTEGRAL_Y, KID_STATS, KID_ZFILL, KID_ZPROJECT_CUR,
-  KID_ZRESOLVE, KID_ZPROJECT_REF, KID_CORR_LIN, KID_SOLVE, KID_LIN_PAIRS, KID_SOLVE_STATS, KID_NN_QUERY, KID_N
+  KID_ZRESOLVE, KID_ZPROJECT_REF, KID_CORR_LIN, KID_SOLVE, KID_LIN_PAIRS, KID_SOLVE_STATS, KID_NN_QUERY,
+  KID_NET_INPUT, KID_NET_FORWARD, KID_START_PROBE, KID_N
 };
+// the last three are stages: the network input kernel, the network's forward pass (its six
+// GEMM launches) and the one-iteration probe of both starts of the learned-start aligner
 static const char *kKernelNames[KID_N] = {"k_count", "k_scan", "k_unproject", "k_integral_x", "k_integral_y",
                                           "k_stats", "k_zfill", "k_zproject(current)", "k_zresolve",
                                           "k_zproject(reference)", "k_corr_lin", "k_solve", "k_lin_pairs",
-                                          "k_solve(statistics)", "k_nn_query"};
+                                          "k_solve(statistics)", "k_nn_query", "k_net_input", "initnet_forward",
+                                          "start_probe"};
 
 namespace {
 struct ProfEntry {
