@@ -29,6 +29,7 @@
 namespace initnet {
 
 constexpr int kLayers = 6;
+constexpr int kProbeSub = 2;  // pixel stride of the start probe's linearisation
 constexpr int kInRows = 60, kInCols = 80, kInCh = 2;
 
 struct LayerShape {
@@ -715,8 +716,9 @@ extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_
   std::vector<float> pred((size_t)num_pairs * 6);
   HIPCHK(hipMemcpyAsync(pred.data(), n->act[kLayers], pred.size() * sizeof(float), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  // 2. both starts judged by one linearisation: pairs [0, n) start at the prediction,
-  //    [n, 2n) at the fallback guess
+  // 2. both starts judged by one linearisation over every second row and column (the whole
+  //    reference cloud is projected, so the correspondences there are the exact ones):
+  //    pairs [0, n) start at the prediction, [n, 2n) at the fallback guess
   std::vector<float> starts((size_t)2 * num_pairs * 16);
   std::vector<int> ref2(2 * (size_t)num_pairs), cur2(2 * (size_t)num_pairs);
   for (int i = 0; i < num_pairs; ++i) {
@@ -747,7 +749,7 @@ extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_
     if (p) pf.begin(KID_START_PROBE, s, en);
     pf.on = false;
     const int rc = batch_align(b, 2 * num_pairs, ref2.data(), cur2.data(), starts.data(), &probe, 0, nullptr, nullptr,
-                               false, &raw);
+                               false, &raw, kProbeSub);
     pf.on = p;
     CHK(rc);
     if (p) {

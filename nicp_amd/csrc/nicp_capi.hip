@@ -426,7 +426,7 @@ void make_plan(const std::vector<FrameDev> &frames, int num_pairs, const int *re
     pd.rows = fc.rows;
     pd.cols = fc.cols;
     const int P = pd.rows * pd.cols;
-    pd.n_tiles = (P + kTile - 1) / kTile;
+    pd.n_tiles = pair_tiles(pd.rows, pd.cols);
     pd.tile_base = (int)plan.blk_pair.size();
     for (int t = 0; t < pd.n_tiles; ++t) plan.blk_pair.push_back(i);
     pd.n_rtiles = (fr.rows * fr.cols + kTile - 1) / kTile;
@@ -438,15 +438,32 @@ void make_plan(const std::vector<FrameDev> &frames, int num_pairs, const int *re
     for (int k = 0; k < 9; ++k) pd.K[k] = cams[i]->K[k];
     pd.min_d = cams[i]->min_distance;
     pd.max_d = cams[i]->max_distance;
-    pd.zbuf = nullptr;
-    // pixel offset of this pair (aligned to 64 entries)
-    pd.zbuf = (unsigned long long *)(uintptr_t)pix;  // placeholder: offset, fixed up later
+    pd.cur_own = 0;
+    pd.sub = 1;
+    // pixel offset of this pair (aligned to 64 entries), turned into pointers by prepare_align
+    pd.zbuf = (unsigned long long *)(uintptr_t)pix;
     pix += align_up((size_t)P, 64);
   }
   plan.total_pix_tiles = (int)plan.blk_pair.size();
   plan.total_rtiles = (int)plan.blk_ref.size();
   plan.total_ctiles = (int)plan.blk_cur.size();
   plan.total_pix = pix;
+}
+
+// Batch frames all carry their index images and every pair's projector is its current
+// frame's camera and image (nicp_batch_align). Then the current cloud projected at the
+// identity lands every point on its own pixel, so the current index image of
+// AlignerProjective::align (alignerprojective.cpp:44-47) is the frame's own index image.
+void plan_batch_frames(const std::vector<FrameDev> &frames, PairPlan &plan) {
+  plan.blk_cur.clear();
+  for (size_t i = 0; i < plan.pairs.size(); ++i) {
+    PairDev &pd = plan.pairs[i];
+    pd.cur_own = 1;
+    pd.cur_index = frames[pd.cur].index;
+    pd.ctile_base = 0;
+    pd.n_ctiles = 0;
+  }
+  plan.total_ctiles = 0;
 }
 
 struct AlignBuffers {
@@ -469,7 +486,7 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
     PairDev &pd = plan.pairs[i];
     size_t off = (size_t)(uintptr_t)pd.zbuf;
     pd.zbuf = B.zbuf->as<unsigned long long>() + off;
-    pd.cur_index = B.cur_index->as<int>() + off;
+    if (!pd.cur_own) pd.cur_index = B.cur_index->as<int>() + off;
     pd.pairs = B.pairs_buf->as<int2>() + off;
     pd.state = B.state->as<float>() + (size_t)i * kStateFloats;
     pd.partials = B.partials->as<double>() + tile_part * kPartial;
@@ -521,9 +538,10 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
   if (plan.total_pix_tiles == 0) return NICP_OK;
   (void)hipGetLastError();
   LAUNCH(pf, KID_ZFILL, k_zfill, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair);
-  if (plan.total_ctiles)
+  if (plan.total_ctiles) {
     LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1);
-  LAUNCH(pf, KID_ZRESOLVE, k_zresolve, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair, (float *)nullptr);
+    LAUNCH(pf, KID_ZRESOLVE, k_zresolve, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair, (float *)nullptr);
+  }
   const int inner = ap->inner_iterations;
   for (int it = 0; it < ap->outer_iterations; ++it) {
     const bool last = it == ap->outer_iterations - 1;
@@ -1118,7 +1136,7 @@ extern "C" int nicp_project(nicp_ctx *ctx, const nicp_cloud *cloud, const float 
   pd.rows = rows;
   pd.cols = cols;
   const int P = rows * cols;
-  pd.n_tiles = (P + kTile - 1) / kTile;
+  pd.n_tiles = pair_tiles(rows, cols);
   plan.blk_pair.assign(pd.n_tiles, 0);
   plan.total_pix_tiles = pd.n_tiles;
   plan.total_pix = align_up((size_t)P, 64);
@@ -1278,7 +1296,7 @@ extern "C" int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud
   PairDev &pd = plan.pairs[0];
   pd.rows = rows;
   pd.cols = cols;
-  pd.n_tiles = (rows * cols + kTile - 1) / kTile;
+  pd.n_tiles = pair_tiles(rows, cols);
   plan.blk_pair.assign(pd.n_tiles, 0);
   plan.total_pix_tiles = pd.n_tiles;
   plan.total_pix = align_up((size_t)rows * cols, 64);
@@ -1445,10 +1463,11 @@ extern "C" int nicp_batch_convert(nicp_batch *b) {
 namespace {
 // AlignerProjective::align for a list of pairs of a batch. with_stats = false skips the
 // statistics linearisation (results then hold the in-loop linearisation only and no
-// solution statistics); raw, when not null, receives the 2 x kPartial result slots per pair.
+// solution statistics); raw, when not null, receives the 2 x kPartial result slots per pair;
+// sub > 1 linearises over every sub-th row and column only (the start probe).
 int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *cur_frame, const float *init_guesses16,
                 const nicp_align_params *ap, int use_graph, float *T_out16, nicp_align_result *results,
-                bool with_stats, std::vector<double> *raw) {
+                bool with_stats, std::vector<double> *raw, int sub = 1) {
   if (!b || num_pairs <= 0 || !ref_frame || !cur_frame) return set_err(NICP_ERR_INVALID, "bad arguments");
   CHK(check_align_params(ap));
   for (int i = 0; i < num_pairs; ++i)
@@ -1469,6 +1488,8 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
   }
   PairPlan plan;
   make_plan(b->frames, num_pairs, ref_frame, cur_frame, cams.data(), plan);
+  plan_batch_frames(b->frames, plan);
+  for (PairDev &pd : plan.pairs) pd.sub = sub;
   AlignBuffers B{&b->pairs_tbl, &b->maps, &b->zbuf, &b->cur_index, &b->pairs_buf, &b->state, &b->partials, &b->result};
   int *d_bp, *d_br, *d_bc;
   PairDev *d_pairs;
@@ -1484,6 +1505,7 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
     key.push_back((int)(((uintptr_t)b->zbuf.p) & 0x7fffffff));
     key.push_back(ctx->prof.on ? 1 : 0);
     key.push_back(with_stats ? 1 : 0);
+    key.push_back(sub);
     if (b->graph.exec == nullptr || b->graph.key != key) {
       b->graph.reset();
       hipGraph_t g = nullptr;

@@ -11,6 +11,9 @@ constexpr int kPartial = 32;         // doubles per block partial of the lineari
 constexpr int kVals = 31;            // used partial slots: H (21), b (6), chi2, inliers, count, both-valid pixels
 constexpr uint64_t kZInit = ~0ull;   // empty z-buffer cell
 
+// per-pixel blocks of a pair: 1024-pixel tiles of its projector image
+__host__ __device__ inline int pair_tiles(int rows, int cols) { return (rows * cols + kTile - 1) / kTile; }
+
 // One frame (a depth image and the cloud made from it). Every pointer is a
 // device pointer; arrays indexed by pixel hold rows*cols entries, arrays
 // indexed by point hold up to rows*cols entries (the capacity).
@@ -46,6 +49,8 @@ struct PairDev {
   int tile_base, n_tiles;          // per-pixel blocks over the projector image
   int rtile_base, n_rtiles;        // blocks over the reference cloud capacity
   int ctile_base, n_ctiles;        // blocks over the current cloud capacity
+  int cur_own;                     // 1: cur_index is the current frame's own index image
+  int sub;                         // k_corr_lin evaluates every sub-th row and column (1: all)
   float K[9];
   float min_d, max_d;
   unsigned long long *zbuf;        // rows*cols z-buffer keys (depth bits << 32 | index)

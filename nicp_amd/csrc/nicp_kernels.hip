@@ -547,12 +547,14 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_zproject(const PairDev 
   }
 }
 
-// z-buffer -> index image (and depth image); resets the z-buffer
+// z-buffer -> index image (and depth image); resets the z-buffer. Pairs that use the
+// current frame's own index image are skipped.
 extern "C" __global__ __launch_bounds__(kThreads) void k_zresolve(const PairDev *__restrict__ Pp,
                                                                  const int *__restrict__ blk_pair,
                                                                  float *__restrict__ depth_out) {
   const int g = blockIdx.x;
   const PairDev &pd = Pp[blk_pair[g]];
+  if (pd.cur_own) return;
   const int P = pd.rows * pd.cols;
   const int base = (g - pd.tile_base) * kTile;
 #pragma unroll
@@ -751,7 +753,10 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_corr_lin(const PairDev 
   const PairDev &pd = Pp[blk_pair[g]];
   const FrameDev &fr = F[pd.ref];
   const FrameDev &fc = F[pd.cur];
-  const int P = pd.rows * pd.cols;
+  // pd.sub > 1 (the start probe of the learned-start aligner) evaluates only the pixels of
+  // every sub-th row and column: linear index p runs over that grid
+  const int sub = pd.sub, gcols = (pd.cols + sub - 1) / sub;
+  const int P = ((pd.rows + sub - 1) / sub) * gcols;
   const int t = g - pd.tile_base;
   const int base = t * kTile;
   float T[16];
@@ -763,19 +768,23 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_corr_lin(const PairDev 
   for (int v = 0; v < kVals; ++v) acc[v] = 0.0;
   if (!multi) {
     // all loads of the thread's four pixels are issued before any of them is used
-    int ri[4], ci[4];
+    int ri[4], ci[4], pix[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int p = base + k * kThreads + threadIdx.x;
-      const unsigned long long key = p < P ? pd.zbuf[p] : kZInit;
+      if (sub == 1) {
+        pix[k] = p < P ? p : -1;
+      } else {
+        const int gr = p / gcols;
+        pix[k] = p < P ? gr * sub * pd.cols + (p - gr * gcols) * sub : -1;
+      }
+      const unsigned long long key = pix[k] >= 0 ? pd.zbuf[pix[k]] : kZInit;
       ri[k] = key == kZInit ? -1 : (int)(unsigned int)(key & 0xffffffffull);
-      ci[k] = p < P ? pd.cur_index[p] : -1;
+      ci[k] = pix[k] >= 0 ? pd.cur_index[pix[k]] : -1;
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int p = base + k * kThreads + threadIdx.x;
-      if (p < P) pd.zbuf[p] = kZInit;
-    }
+    for (int k = 0; k < 4; ++k)
+      if (pix[k] >= 0) pd.zbuf[pix[k]] = kZInit;
     float4 cn[4], rn0[4], cp[4], rp0[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -807,14 +816,15 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_corr_lin(const PairDev 
       }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int p = base + k * kThreads + threadIdx.x;
-      if (store_pairs && p < P) pd.pairs[p] = ok[k] ? make_int2(ri[k], ci[k]) : make_int2(-1, -1);
+      if (store_pairs && pix[k] >= 0) pd.pairs[pix[k]] = ok[k] ? make_int2(ri[k], ci[k]) : make_int2(-1, -1);
       if (ok[k]) lin_terms(rp[k], rn[k], cp[k], cn[k], om[k], ap, acc);
     }
   } else {
     for (int k = 0; k < 4; ++k) {
-      const int p = base + k * kThreads + threadIdx.x;
-      if (p >= P) break;
+      const int q = base + k * kThreads + threadIdx.x;
+      if (q >= P) break;
+      const int r = (q / gcols) * sub, c = (q - (q / gcols) * gcols) * sub;
+      const int p = r * pd.cols + c;
       const unsigned long long key = pd.zbuf[p];
       pd.zbuf[p] = kZInit;
       const int ri = key == kZInit ? -1 : (int)(unsigned int)(key & 0xffffffffull);
@@ -823,7 +833,6 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_corr_lin(const PairDev 
       float4 cp, cn;
       bool ok = false;
       if (ri >= 0) {
-        const int r = p / pd.cols, c = p - r * pd.cols;
         ci = window_search(fr, fc, pd.cur_index, pd.rows, pd.cols, r, c, ri, T, ap, rp, rn);
         if (ci >= 0) {
           ok = true;
