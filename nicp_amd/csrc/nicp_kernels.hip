@@ -530,20 +530,44 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_zproject(const PairDev 
 #pragma unroll
   for (int k = 0; k < 16; ++k) KRt[k] = S[k];
   const int cols = pd.cols, rows = pd.rows;
+  const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int i = base + k * kThreads + threadIdx.x;
-    if (i >= n) break;
-    const float4 p = fd.pts[i];
-    float ip[4];
-    mul4_packet(KRt, p.x, p.y, p.z, p.w, ip);
-    const float d = ip[2];
-    if (d < pd.min_d || d > pd.max_d) continue;
-    const float inv = 1.0f / d;
-    const int x = (int)roundf(ip[0] * inv), y = (int)roundf(ip[1] * inv);
-    if (x < 0 || x >= cols || y < 0 || y >= rows) continue;
-    const unsigned long long key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
-    atomicMin(pd.zbuf + (size_t)y * cols + x, key);
+    int cell = -1;
+    unsigned long long key = kZInit;
+    if (i < n) {
+      const float4 p = fd.pts[i];
+      float ip[4];
+      mul4_packet(KRt, p.x, p.y, p.z, p.w, ip);
+      const float d = ip[2];
+      if (!(d < pd.min_d || d > pd.max_d)) {
+        const float inv = 1.0f / d;
+        const int x = (int)roundf(ip[0] * inv), y = (int)roundf(ip[1] * inv);
+        if (x >= 0 && x < cols && y >= 0 && y < rows) {
+          cell = y * cols + x;
+          key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
+        }
+      }
+    }
+    // The lanes that hit the cell of the wavefront's first active lane send one atomicMin
+    // with their smallest key. Device-scope atomics run one by one on the memory side, so
+    // this keeps a pair whose points pile onto a few pixels (a diverging transform) from
+    // stalling the whole launch; the z-buffer is the same.
+    const unsigned long long active = __ballot(cell >= 0);
+    if (active == 0) continue;
+    const int lead = __ffsll((long long)active) - 1;
+    const int lead_cell = __shfl(cell, lead, 64);
+    const bool same = cell >= 0 && cell == lead_cell;
+    unsigned long long m = same ? key : kZInit;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned int lo = __shfl_xor((unsigned int)m, o, 64), hi = __shfl_xor((unsigned int)(m >> 32), o, 64);
+      const unsigned long long other = ((unsigned long long)hi << 32) | lo;
+      m = other < m ? other : m;
+    }
+    if (cell >= 0 && !same) atomicMin(pd.zbuf + cell, key);
+    else if (lane == lead) atomicMin(pd.zbuf + cell, m);
   }
 }
 
