@@ -111,6 +111,49 @@ def cpu_baseline(seeds, max_rot, max_trans, budget_s=20.0):
                       "each), OpenMP threads %d" % (n, COLS, ROWS, threads)}
 
 
+def uneven_batch(ctx, n_pairs, iters, steps):
+    """Batched plain NICP over pairs of uneven resolution in one launch per stage (graph replay)."""
+    from nicp_amd import capi, synth
+    res = [(120, 160), (240, 320), (480, 640), (960, 1280)]
+    rng = np.random.default_rng(7)
+    pick = rng.integers(0, len(res), n_pairs)
+    shapes, cams = [], []
+    for k in pick:
+        r, c = res[k]
+        cam = capi.camera(*synth.camera_for(r, c))
+        shapes += [(r, c), (r, c)]
+        cams += [cam, cam]
+    batch = capi.Batch(ctx, shapes, cams, capi.stats_params())
+    ap = capi.align_params(outer_iterations=iters)
+    seeds = np.arange(2_000_000, 2_000_000 + n_pairs, dtype=np.uint64)
+    T = np.stack([synth.motion(int(s), 8.0, 0.1) for s in seeds])
+    I = np.eye(4, dtype=np.float32).flatten(order="F")
+    poses = np.empty((2 * n_pairs, 16), np.float32)
+    poses[0::2] = I
+    poses[1::2] = T
+    batch.render_synthetic(np.arange(2 * n_pairs, dtype=np.int32), np.repeat(seeds, 2),
+                           np.tile(np.array([10, 11], np.uint64), n_pairs), poses)
+    ref = np.arange(0, 2 * n_pairs, 2, dtype=np.int32)
+    cur = ref + 1
+    batch.convert()
+    batch.align(ref, cur, None, ap, use_graph=True, want_results=False)  # warm-up + graph capture
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.convert()
+        Tout, _ = batch.align(ref, cur, None, ap, use_graph=True, want_results=False)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ok, _ = aligned(Tout, T)
+    px = sum(r * c for r, c in shapes)
+    batch.close()
+    return {"pairs": n_pairs, "iterations": iters, "ms_per_step": round(1000 * dt, 3),
+            "pairs_per_s": round(n_pairs / dt, 1), "aligned_fraction": round(float(ok.mean()), 4),
+            "depth_pixels_per_step": px,
+            "pairs_per_resolution": {"%dx%d" % (c, r): int((pick == k).sum()) for k, (r, c) in enumerate(res)},
+            "motion": "rot<=8deg, trans<=10cm", "graph": True}
+
+
 def calibrate_copy(mib):
     """A device-to-device copy of known size, for scaling rocprof's FETCH_SIZE / WRITE_SIZE."""
     from nicp_amd import capi
@@ -137,6 +180,8 @@ def main():
     ap_.add_argument("--no-cpu", action="store_true")
     ap_.add_argument("--no-profile", action="store_true")
     ap_.add_argument("--plain-pairs", type=int, default=32)
+    ap_.add_argument("--uneven-pairs", type=int, default=512, help="pairs of the uneven-resolution batch (0: skip)")
+    ap_.add_argument("--uneven-steps", type=int, default=3)
     ap_.add_argument("--max-rot", type=float, default=MAX_ROT_DEG, help="motion range of the pairs (deg)")
     ap_.add_argument("--max-trans", type=float, default=MAX_TRANS_M, help="motion range of the pairs (m)")
     ap_.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -310,6 +355,13 @@ def main():
             plain, frac_plain, plain * frac_plain))
     plain_aligned = plain * frac_plain if plain else None
 
+    # batched configuration with uneven frames: pairs of 160x120 / 320x240 / 640x480 /
+    # 1280x960 (uniform, seeded), motions <= 8 deg / 10 cm, 15 iterations replayed from a graph
+    uneven = None
+    if args.uneven_pairs > 0:
+        uneven = uneven_batch(ctx, args.uneven_pairs, 15, args.uneven_steps)
+        log("uneven batch: %s" % uneven)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(np.arange(EVAL_SEED_BASE, EVAL_SEED_BASE + 16), max_rot, max_trans)
@@ -346,6 +398,7 @@ def main():
         "vs_plain": round(value / plain_aligned, 2) if plain_aligned else None,
         "batched_identity_ms_per_step": round(1000.0 * t_plain_b / args.steps, 3),
         "batched_identity_aligned_pairs_per_s": round(n_ok_plain / t_plain_b, 2),
+        "uneven_batch": uneven,
         "roofline": roofline,
         "net_roofline": net_roofline,
         "kernels": kernels,

@@ -8,6 +8,7 @@
 #include <fstream>
 #include <vector>
 
+#include "cloudfile.h"
 #include "definitions.h"
 
 namespace nicp {
@@ -58,53 +59,50 @@ class Cloud {
   }
   std::vector<std::array<float, 16>> pointInformationMatrix() const { return getOmega(0); }
 
-  // Cloud::save (nicp/nicp/cloud.cpp:99-150): "NICPCLOUD n binary", the t2v of T, then per
-  // point Point, Normal and Stats. Binary records are 16 + 16 + 96 bytes: the Stats layout of
-  // the reference's x86-64 build (Matrix4f eigenvectors|mean at 0, int n at 64, eigenvalues at
-  // 68, curvature-computed flag at 80, curvature at 84, zero padding to 96). Stats are only
-  // available for clouds converted with keep_stats; otherwise they are written as zeros, as
-  // the reference does when its stats vector is empty. T is given as its 6-vector (t, qx qy qz).
+  // Cloud::save (nicp/nicp/cloud.cpp:99-150) in the file format of CloudFile; T is given as
+  // its 6-vector (t, qx qy qz). Stats are written when the cloud was converted with
+  // keep_stats, otherwise the reference's zero Stats.
   bool save(const char *filename, const float t2v6[6], int step = 1, bool binary = true) const {
-    std::ofstream os(filename, std::ios::binary);
-    if (!os) return false;
+    CloudFile f;
     const size_t n = size();
-    std::vector<std::array<float, 4>> pts = points(), nrm = normals();
+    f.resize(n);
+    for (int k = 0; k < 6; ++k) f.t2v[k] = t2v6[k];
+    f.points = points();
+    f.normals = normals();
     std::vector<float> ev(9 * n), el(3 * n), mn(4 * n);
     std::vector<int> np(n);
-    const bool have_stats = n && nicp_cloud_download_stats(h_, ev.data(), el.data(), mn.data(), np.data()) == NICP_OK;
-    const std::vector<float> curv = curvatures();
-    os << "NICPCLOUD " << n / step << " " << (binary ? 1 : 0) << "\n";
-    os << t2v6[0] << " " << t2v6[1] << " " << t2v6[2] << " " << t2v6[3] << " " << t2v6[4] << " " << t2v6[5] << " \n";
-    for (size_t i = 0; i < n; i += step) {
-      float S[16] = {0};
-      if (have_stats) {
+    const bool have_stats =
+        n && nicp_cloud_download_stats(h_, ev.data(), el.data(), mn.data(), np.data()) == NICP_OK;
+    if (have_stats) {
+      const std::vector<float> curv = curvatures();
+      for (size_t i = 0; i < n; ++i) {
+        std::array<float, 16> &S = f.stats[i];
+        S.fill(0.0f);
         for (int c = 0; c < 3; ++c)
           for (int r = 0; r < 3; ++r) S[c * 4 + r] = ev[9 * i + c * 3 + r];
         for (int r = 0; r < 4; ++r) S[12 + r] = mn[4 * i + r];
-      }
-      if (!binary) {
-        os << "POINTWITHSTATS ";
-        for (int k = 0; k < 3; ++k) os << pts[i][k] << " ";
-        for (int k = 0; k < 3; ++k) os << nrm[i][k] << " ";
-        for (int r = 0; r < 4; ++r)
-          for (int c = 0; c < 4; ++c) os << S[c * 4 + r] << " ";
-        os << "\n";
-      } else {
-        unsigned char rec[128] = {0};
-        std::memcpy(rec, pts[i].data(), 16);
-        std::memcpy(rec + 16, nrm[i].data(), 16);
-        std::memcpy(rec + 32, S, 64);
-        if (have_stats) {
-          std::memcpy(rec + 96, &np[i], 4);
-          std::memcpy(rec + 100, &el[3 * i], 12);
-          rec[112] = 1;
-          std::memcpy(rec + 116, &curv[i], 4);
-        }
-        os.write((const char *)rec, sizeof(rec));
+        f.n[i] = np[i];
+        for (int k = 0; k < 3; ++k) f.eigenvalues[i][k] = el[3 * i + k];
+        f.curvature_computed[i] = 1;
+        f.curvature[i] = curv[i];
       }
     }
-    return (bool)os;
+    return f.write(filename, binary, step, have_stats);
   }
+
+  // Cloud::load (nicp/nicp/cloud.cpp:13-97): points, normals and the curvature of the Stats
+  // records into a device cloud (no information matrices: the file holds none); t2v6
+  // receives the stored transform. Returns false on a malformed or short file.
+  bool load(float t2v6[6], const char *filename) {
+    CloudFile f;
+    if (!f.read(filename)) return false;
+    std::vector<float> curv(f.size());
+    for (size_t i = 0; i < f.size(); ++i) curv[i] = f.curvature_of(i);
+    setFromHost(f.points, f.normals, curv, {}, {});
+    for (int k = 0; k < 6; ++k) t2v6[k] = f.t2v[k];
+    return true;
+  }
+
   std::vector<std::array<float, 16>> normalInformationMatrix() const { return getOmega(1); }
 
  private:

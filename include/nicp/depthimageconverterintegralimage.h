@@ -116,7 +116,7 @@ class DepthImageConverterIntegralImage {
     nicp_stats_params sp = statsParams();
     nicp_cloud *h = nullptr;
     check(nicp_cloud_from_depth_u16(Context::get(), depth.data.data(), depth.rows, depth.cols, depthScale, &cam, &sp,
-                                    0, &h));
+                                    _keepStats ? 1 : 0, &h));
     cloud.reset(h);
     _indexImage.create(depth.rows, depth.cols);
     check(nicp_cloud_index_image(h, _indexImage.data.data()));
@@ -129,14 +129,20 @@ class DepthImageConverterIntegralImage {
     nicp_camera cam = _projector->camera();
     nicp_stats_params sp = statsParams();
     nicp_cloud *h = nullptr;
-    check(nicp_cloud_from_depth_f32(Context::get(), depth.data.data(), depth.rows, depth.cols, &cam, &sp, 0, &h));
+    check(nicp_cloud_from_depth_f32(Context::get(), depth.data.data(), depth.rows, depth.cols, &cam, &sp,
+                                    _keepStats ? 1 : 0, &h));
     cloud.reset(h);
     _indexImage.create(depth.rows, depth.cols);
     check(nicp_cloud_index_image(h, _indexImage.data.data()));
   }
   const IntImage &indexImage() const { return _indexImage; }
+  // The reference's clouds always hold their Stats (nicp/nicp/cloud.h:190); here they are
+  // kept on request only (64 B per point), e.g. for Cloud::save with real Stats records.
+  bool keepStats() const { return _keepStats; }
+  void setKeepStats(bool k) { _keepStats = k; }
 
  private:
+  bool _keepStats = false;
   PinholePointProjector *_projector;
   StatsCalculatorIntegralImage *_statsCalculator;
   PointInformationMatrixCalculator *_pointCalc;

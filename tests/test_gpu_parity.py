@@ -194,14 +194,17 @@ def test_pipeline_accuracy(ctx):
     assert ok >= 3
 
 
-def test_batch_matches_single(ctx):
-    shapes = [(120, 160), (240, 320), (480, 640), (240, 320)]
-    say("case batch_matches_single")
+@pytest.mark.parametrize("iters", [10, 15])
+def test_batch_matches_single(ctx, iters):
+    """Uneven batches (the four resolutions of the batched configuration) equal single-pair runs
+    bit for bit, eagerly and replayed from the captured graph."""
+    shapes = [(120, 160), (240, 320), (480, 640), (960, 1280), (240, 320)]
+    say("case batch_matches_single iters=%d" % iters)
     _, dsp = params()
-    _, dap = aparams()
+    _, dap = aparams(outer_iterations=iters)
     frames, cam_list, singles = [], [], []
     for i, (rows, cols) in enumerate(shapes):
-        dr, dc, Tgt = _frames(20 + i, rows, cols)
+        dr, dc, Tgt = _frames(20 + i, rows, cols, max_rot_deg=8.0, max_trans_m=0.1)
         _, dcam = cams(rows, cols)
         frames += [dr, dc]
         cam_list += [dcam, dcam]
