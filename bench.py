@@ -23,6 +23,7 @@ sample on the host cores.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -43,6 +44,19 @@ NET_FLOPS_PER_PAIR = 2 * (1200 * 19 * 32 + 300 * 289 * 32 + 80 * 289 * 64 + 20 *
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from its committed rocprofv3 counter summary
+    (profiles/<kernel>_pmc.md: FETCH_SIZE and WRITE_SIZE scaled by a copy of known size),
+    measured on the default workload; None when there is no such summary."""
+    path = os.path.join(ROOT, "profiles", "%s_pmc.md" % kernel)
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^%s_traffic_bytes_per_launch = (\d+)$" % re.escape(kernel), text, re.M)
+    return int(m.group(1)) if m else None
 
 
 def kernel_bytes(name, pairs_info, frames_info):
@@ -323,8 +337,11 @@ def main():
         nb = kernel_bytes(dom, pairs_info, frames_info)
         avg = kernels[dom]["avg_ms"]
         ach = nb / (avg * 1e-3) / 1e9
+        # the counter summary holds the default workload only
+        default = (args.pairs, args.iters, args.max_rot, args.max_trans) == (128, 10, MAX_ROT_DEG, MAX_TRANS_M)
         roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": nb,
+                    "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(dom) if default else None, "bytes_per_launch": nb,
                     "avg_launch_ms": avg}
     net_roofline = None
     if "initnet_forward" in kernels:
