@@ -141,6 +141,53 @@ int nicp_cloud_download_stats(nicp_cloud *cloud, float *eigvecs9, float *eigvals
 /* the index image of DepthImageConverter::_indexImage (rows*cols ints) */
 int nicp_cloud_index_image(nicp_cloud *cloud, int *index_image);
 
+/* Gaussian3f (nicp/nicp/gaussian.h:8-133) with the state of its lazy conversions between
+ * the moments form (mean, covariance) and the information form; 112 bytes, the layout of
+ * the device records. A form whose flag is clear holds stale values. */
+typedef struct {
+  float mean[3];
+  int   flags;                   /* bit 0: mean/covariance current; bit 1: information form current */
+  float covariance[9];           /* column-major */
+  float information[9];          /* column-major */
+  float information_vector[3];
+  float pad[3];
+} nicp_gaussian;
+
+/* Merger (nicp/nicp/merger.cpp:5-14) */
+typedef struct {
+  float distance_threshold;      /* _distanceThreshold, default 0.1 */
+  float normal_threshold;        /* _normalThreshold, default cos(10 deg); unused by the reference's merge */
+  float max_point_depth;         /* _maxPointDepth, default 10 */
+} nicp_merger_params;
+
+/* Replaces the Gaussian half of PinholePointProjector::unProject (nicp/nicp/pinholepointprojector.cpp:93-134),
+ * which DepthImageConverter::compute (nicp/nicp/depthimageconverterintegralimage.cpp:42) runs on every
+ * conversion: per point, mean = the point and covariance J diag(3, 3, zVar) J^T with
+ * J = K^-1 [[z,0,c],[0,z,r],[0,0,1]], zVar = alpha z^2 / (baseline K(0,0) + z alpha). The cloud must come
+ * from a depth conversion (it needs the index and depth images). baseline / alpha are the projector's
+ * (pinholepointprojector.cpp:11-12 defaults 0.075, 0.1). */
+int nicp_cloud_gaussians(nicp_cloud *cloud, float baseline, float alpha);
+/* Cloud::gaussians() of a host-built cloud (n records), and their download */
+int nicp_cloud_set_gaussians(nicp_cloud *cloud, const nicp_gaussian *gaussians);
+int nicp_cloud_download_gaussians(nicp_cloud *cloud, nicp_gaussian *gaussians);
+/* Replaces Cloud::transformInPlace (nicp/nicp/cloud.cpp:193-205): points, normals, Stats, Gaussians
+ * and both information matrices; nothing changes when T (with its last row set to 0 0 0 1) is the
+ * identity. Information matrices are kept as their upper triangles. */
+int nicp_cloud_transform(nicp_cloud *cloud, const float T[16]);
+/* Replaces Cloud::add (nicp/nicp/cloud.cpp:163-190): transforms src in place by T, then appends it to
+ * dst. dst becomes a plain point cloud (no index image); it keeps Stats when both clouds have them
+ * and Gaussians when both have them (an error when only dst has them). */
+int nicp_cloud_add(nicp_cloud *dst, nicp_cloud *src, const float T[16]);
+/* Replaces Merger::merge (nicp/nicp/merger.cpp:15-109): projects the cloud from `transform`
+ * with cam into a rows x cols z-buffer, folds the information of every point whose depth lies
+ * within distance_threshold of its pixel's winner into the winner's Gaussian (in point order),
+ * moves every winner that kept its own pixel to its Gaussian's mean, and compacts the cloud in
+ * order. num_suppressed (optional) receives the number of removed points; collapsed (optional,
+ * the old size) receives per point -1 (kept as is), its own index (kept, moved to the mean) or
+ * the winner it was folded into. The cloud must have Gaussians. */
+int nicp_merge(nicp_cloud *cloud, const float transform[16], const nicp_camera *cam, int rows, int cols,
+               const nicp_merger_params *params, int *num_suppressed, int *collapsed);
+
 /* Replaces PinholePointProjector::project (nicp/nicp/pinholepointprojector.cpp:33-66):
  * z-buffer projection of the cloud seen from `transform`; depth_image may be NULL. */
 int nicp_project(nicp_ctx *ctx, const nicp_cloud *cloud, const float transform[16], const nicp_camera *cam,

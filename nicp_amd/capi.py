@@ -42,6 +42,17 @@ class AlignParams(C.Structure):
                 ("translational_min_eigen_ratio", C.c_float), ("rotational_min_eigen_ratio", C.c_float)]
 
 
+class MergerParams(C.Structure):
+    _fields_ = [("distance_threshold", C.c_float), ("normal_threshold", C.c_float), ("max_point_depth", C.c_float)]
+
+
+def merger_params(distance_threshold=0.1, normal_threshold=float(np.cos(np.radians(10.0))), max_point_depth=10.0):
+    """Merger defaults (nicp/nicp/merger.cpp:5-14)."""
+    m = MergerParams()
+    m.distance_threshold, m.normal_threshold, m.max_point_depth = distance_threshold, normal_threshold, max_point_depth
+    return m
+
+
 class AlignResult(C.Structure):
     _fields_ = [("error", C.c_float), ("inliers", C.c_int), ("num_correspondences", C.c_int),
                 ("H", C.c_double * 36), ("b", C.c_double * 6), ("mean", C.c_double * 6),
@@ -63,7 +74,8 @@ EXPORTS = [
     "nicp_initnet_destroy", "nicp_initnet_get_params", "nicp_initnet_set_params", "nicp_initnet_set_loss_weights",
     "nicp_initnet_forward", "nicp_initnet_train_step", "nicp_initnet_read", "nicp_batch_net_input",
     "nicp_batch_align_learned", "nicp_device_alloc", "nicp_device_free", "nicp_device_copy",
-    "nicp_initnet_train_step_rows", "nicp_initnet_set_input_mode",
+    "nicp_initnet_train_step_rows", "nicp_initnet_set_input_mode", "nicp_cloud_gaussians", "nicp_cloud_set_gaussians",
+    "nicp_cloud_download_gaussians", "nicp_cloud_transform", "nicp_cloud_add", "nicp_merge",
 ]
 
 _lib = None
@@ -120,6 +132,10 @@ def lib():
             "nicp_device_copy": [vp, vp, vp, C.c_size_t, C.c_int],
             "nicp_initnet_train_step_rows": [vp, vp, C.c_int, vp, vp, vp, C.c_int, C.c_float, vp],
             "nicp_initnet_set_input_mode": [vp, C.c_int],
+            "nicp_cloud_gaussians": [vp, C.c_float, C.c_float], "nicp_cloud_set_gaussians": [vp, vp],
+            "nicp_cloud_download_gaussians": [vp, vp], "nicp_cloud_transform": [vp, vp],
+            "nicp_cloud_add": [vp, vp, vp],
+            "nicp_merge": [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -363,6 +379,50 @@ class Cloud:
         idx = np.empty(self.shape, np.int32)
         _check(lib().nicp_cloud_index_image(self.h, _p(idx)))
         return idx
+
+    def _size(self):
+        n = C.c_int()
+        _check(lib().nicp_cloud_size(self.h, C.byref(n)))
+        self.n = n.value
+
+    def compute_gaussians(self, baseline=0.075, alpha=0.1):
+        """The Gaussians of PinholePointProjector::unProject (needs a cloud made from depth)."""
+        _check(lib().nicp_cloud_gaussians(self.h, baseline, alpha))
+
+    def set_gaussians(self, g):
+        """g: (n, 28) float32 records (nicp_gaussian)."""
+        g = np.ascontiguousarray(g, np.float32)
+        assert g.shape == (self.n, 28)
+        _check(lib().nicp_cloud_set_gaussians(self.h, _p(g)))
+
+    def download_gaussians(self):
+        """(n, 28) float32 records: mean 0:3, int32 flags at 3, covariance 4:13, information 13:22,
+        information vector 22:25 (3x3 matrices column-major)."""
+        g = np.empty((self.n, 28), np.float32)
+        _check(lib().nicp_cloud_download_gaussians(self.h, _p(g)))
+        return g
+
+    def transform(self, T16):
+        """Cloud::transformInPlace."""
+        _check(lib().nicp_cloud_transform(self.h, _p(np.ascontiguousarray(T16, np.float32))))
+
+    def add(self, other, T16):
+        """Cloud::add: transforms `other` in place by T16 and appends it."""
+        _check(lib().nicp_cloud_add(self.h, other.h, _p(np.ascontiguousarray(T16, np.float32))))
+        self._size()
+        self.shape = (1, self.n)
+
+    def merge(self, T16, cam, rows, cols, params=None, want_collapsed=False):
+        """Merger::merge seen from T16; returns the number of suppressed points (and the
+        per-point decisions)."""
+        params = params or merger_params()
+        ns = C.c_int()
+        col = np.empty(self.n, np.int32) if want_collapsed else None
+        _check(lib().nicp_merge(self.h, _p(np.ascontiguousarray(T16, np.float32)), C.byref(cam), rows, cols,
+                                C.byref(params), C.byref(ns), _p(col)))
+        self._size()
+        self.shape = (1, self.n)
+        return (ns.value, col) if want_collapsed else ns.value
 
 
 class Batch:

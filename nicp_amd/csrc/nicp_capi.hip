@@ -234,6 +234,9 @@ struct nicp_cloud {
   int has_index = 0;
   int has_stats = 0;
   void *alloc = nullptr;  // owned allocation (null for batch views)
+  int cap = 0;            // points the arrays hold
+  float4 *gauss = nullptr;        // nicp_gaussian records (7 float4 each) or null
+  void *gauss_alloc = nullptr;    // owned allocation of gauss when it is not inside alloc
 };
 
 struct GraphCache {
@@ -867,6 +870,7 @@ int cloud_from_depth(nicp_ctx *ctx, const uint16_t *depth, const float *depthf, 
     return set_err(NICP_ERR_OOM, "hipMalloc(%zu): %s", L.total, hipGetErrorString(e));
   }
   cl->alloc = mem;
+  cl->cap = rows * cols;
   FrameDev &f = cl->fd;
   frame_geometry(f, rows, cols, cam, sp->world_radius, depth_scale);
   frame_pointers(f, (char *)mem, L, keep_stats != 0, true, depthf != nullptr);
@@ -963,6 +967,7 @@ extern "C" int nicp_cloud_from_host(nicp_ctx *ctx, int n, const float *points4, 
   }
   cl->alloc = mem;
   cl->n = n;
+  cl->cap = (int)cap;
   FrameDev &f = cl->fd;
   memset(&f, 0, sizeof(f));
   f.pts = (float4 *)((char *)mem + o_pts);
@@ -1002,10 +1007,11 @@ extern "C" int nicp_cloud_from_host(nicp_ctx *ctx, int n, const float *points4, 
 
 extern "C" int nicp_cloud_destroy(nicp_cloud *cl) {
   if (!cl) return NICP_OK;
-  if (cl->alloc) {
+  if (cl->alloc || cl->gauss_alloc) {
     (void)hipSetDevice(cl->ctx->device);
     (void)hipStreamSynchronize(cl->ctx->stream);
-    (void)hipFree(cl->alloc);
+    if (cl->alloc) (void)hipFree(cl->alloc);
+    if (cl->gauss_alloc) (void)hipFree(cl->gauss_alloc);
   }
   delete cl;
   return NICP_OK;
@@ -1077,7 +1083,8 @@ extern "C" int nicp_cloud_download_stats(nicp_cloud *cl, float *eigvecs9, float 
       means4[4 * (size_t)i] = s[12];
       means4[4 * (size_t)i + 1] = s[13];
       means4[4 * (size_t)i + 2] = s[14];
-      means4[4 * (size_t)i + 3] = cnt > 0 ? (float)cnt * (1.0f / (float)cnt) : 1.0f;
+      // Stats::mean() is a Point: w = 1 (pointaccumulator.h:66-70, homogeneousvector4f.h:19)
+      means4[4 * (size_t)i + 3] = 1.0f;
     }
     if (npts) npts[i] = cnt;
   }
@@ -1114,12 +1121,12 @@ int upload_frames(nicp_ctx *c, std::vector<FrameDev> &frames, FrameDev **d) {
 
 }  // namespace
 
-extern "C" int nicp_project(nicp_ctx *ctx, const nicp_cloud *cloud, const float transform[16], const nicp_camera *cam,
-                            int rows, int cols, int *index_image, float *depth_image) {
-  if (!ctx || !cloud || !transform || !index_image) return set_err(NICP_ERR_INVALID, "null argument");
-  if (rows <= 0 || cols <= 0) return set_err(NICP_ERR_INVALID, "bad image size");
-  CHK(check_camera(cam));
-  HIPCHK(hipSetDevice(ctx->device));
+namespace {
+// PinholePointProjector::project (pinholepointprojector.cpp:33-66) of a cloud seen from
+// `transform`; leaves the index image in ctx->cur_index and the depth image (FLT_MAX where
+// empty) in ctx->depth_tmp, both rows x cols on the device, enqueued on ctx->stream.
+int project_device(nicp_ctx *ctx, const nicp_cloud *cloud, const float transform[16], const nicp_camera *cam,
+                   int rows, int cols) {
   std::vector<FrameDev> frames(1, cloud->fd);
   frames[0].rows = rows;  // projection capacity follows the cloud size
   frames[0].cols = std::max(1, cloud->n);
@@ -1163,11 +1170,22 @@ extern "C" int nicp_project(nicp_ctx *ctx, const nicp_cloud *cloud, const float 
     hipLaunchKernelGGL(k_zproject, dim3(plan.total_ctiles), tb, 0, ctx->stream, d_pairs, d_frames, d_bc, 1);
   hipLaunchKernelGGL(k_zresolve, dim3(pd.n_tiles), tb, 0, ctx->stream, d_pairs, d_bp, ctx->depth_tmp.as<float>());
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(index_image, ctx->cur_index.p, (size_t)P * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  return NICP_OK;
+}
+}  // namespace
+
+extern "C" int nicp_project(nicp_ctx *ctx, const nicp_cloud *cloud, const float transform[16], const nicp_camera *cam,
+                            int rows, int cols, int *index_image, float *depth_image) {
+  if (!ctx || !cloud || !transform || !index_image) return set_err(NICP_ERR_INVALID, "null argument");
+  if (rows <= 0 || cols <= 0) return set_err(NICP_ERR_INVALID, "bad image size");
+  CHK(check_camera(cam));
+  HIPCHK(hipSetDevice(ctx->device));
+  CHK(project_device(ctx, cloud, transform, cam, rows, cols));
+  const size_t P = (size_t)rows * cols;
+  HIPCHK(hipMemcpyAsync(index_image, ctx->cur_index.p, P * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   if (depth_image)
-    HIPCHK(hipMemcpyAsync(depth_image, ctx->depth_tmp.p, (size_t)P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(depth_image, ctx->depth_tmp.p, P * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  (void)zero;
   return NICP_OK;
 }
 
@@ -1583,6 +1601,7 @@ extern "C" int nicp_batch_frame_cloud(nicp_batch *b, int frame, nicp_cloud **out
   cl->has_index = 1;
   cl->has_stats = 0;
   cl->alloc = nullptr;
+  cl->cap = b->frames[frame].rows * b->frames[frame].cols;
   *out = cl;
   return NICP_OK;
 }
@@ -1808,3 +1827,7 @@ extern "C" int nicp_align_nn(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cl
 // ---------------------------------------------------------------------------------------
 #include "initnet.hip"
 #include "synth_gpu.hip"
+// ---------------------------------------------------------------------------------------
+// Gaussians, cloud transform / add and the Merger
+// ---------------------------------------------------------------------------------------
+#include "merge.hip"

@@ -489,12 +489,10 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_stats(const FrameDev *_
     o[2] = make_float4(on[2], on[3], on[4], on[5]);
     if (sp.keep_stats) {
       float4 *s = fd.stats + 4 * (size_t)idx;
-      const float mw = ok ? acc[0] * (1.0f / acc[0]) : 1.0f;
       s[0] = make_float4(V[0], V[1], V[2], V[3]);
       s[1] = make_float4(V[4], V[5], V[6], V[7]);
       s[2] = make_float4(V[8], ev[0], ev[1], ev[2]);
       s[3] = make_float4(mean[0], mean[1], mean[2], __int_as_float(ok ? n : 0));
-      (void)mw;
     }
   }
 }

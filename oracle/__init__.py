@@ -335,3 +335,58 @@ def depth_scale(depth_m, step, max_depth_cov=0.01):
     out = np.empty((rows // step, cols // step), np.float32)
     lib().orc_depth_scale(_p(d), rows, cols, step, C.c_float(max_depth_cov), _p(out))
     return out
+
+
+#: columns of a Gaussian record (orc_gaussian / nicp_gaussian) in a (n, 28) float32 array;
+#: column 3 holds the int32 flags (bit 0 moments current, bit 1 information form current)
+G_MEAN, G_FLAGS, G_COV, G_INFO, G_IVEC = slice(0, 3), 3, slice(4, 13), slice(13, 22), slice(22, 25)
+
+
+def gaussian_flags(g):
+    return np.ascontiguousarray(g[:, G_FLAGS]).view(np.int32)
+
+
+def unproject_gaussians(depth_m, K9, iKRt, min_d, max_d, baseline=0.075, alpha=0.1):
+    """PinholePointProjector::unProject with Gaussians: (points, gaussians (n, 28), index image)."""
+    depth_m = np.ascontiguousarray(depth_m, np.float32)
+    rows, cols = depth_m.shape
+    pts = np.zeros((rows * cols, 4), np.float32)
+    g = np.zeros((rows * cols, 28), np.float32)
+    idx = np.empty(rows * cols, np.int32)
+    f = lib().orc_unproject_gaussians
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float,
+                  C.c_void_p, C.c_void_p, C.c_void_p]
+    n = f(_p(depth_m), rows, cols, _p(np.ascontiguousarray(K9, np.float32)), _p(np.ascontiguousarray(iKRt, np.float32)),
+          min_d, max_d, baseline, alpha, _p(pts), _p(g), _p(idx))
+    return pts[:n].copy(), g[:n].copy(), idx.reshape(rows, cols)
+
+
+def _f32(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous, "arrays are changed in place: float32, contiguous"
+    return a
+
+
+def cloud_transform(T16, points=None, normals=None, stats16=None, omega_p=None, omega_n=None, gaussians=None):
+    """Cloud::transformInPlace on float32 arrays, in place (stats16 / omegas: (n, 16) column-major)."""
+    arrs = [points, normals, stats16, omega_p, omega_n, gaussians]
+    n = next(len(a) for a in arrs if a is not None)
+    f = lib().orc_cloud_transform
+    f.argtypes = [C.c_int] + [C.c_void_p] * 7
+    f(n, _p(np.ascontiguousarray(T16, np.float32)), *[_p(_f32(a)) for a in arrs])
+
+
+def merge(points, gaussians, T16, cam, rows, cols, normals=None, curvature=None, stats16=None, omega_p=None,
+          omega_n=None, distance_threshold=0.1, max_point_depth=10.0):
+    """Merger::merge in place; returns (new size, per-point decisions)."""
+    n = len(points)
+    collapsed = np.empty(n, np.int32)
+    f = lib().orc_merge
+    f.restype = C.c_int
+    f.argtypes = [C.c_int] + [C.c_void_p] * 9 + [C.c_int, C.c_int, C.c_float, C.c_float, C.c_void_p]
+    k = f(n, _p(_f32(points)), _p(_f32(normals)), _p(_f32(curvature)), _p(_f32(stats16)), _p(_f32(omega_p)),
+          _p(_f32(omega_n)), _p(_f32(gaussians)), _p(np.ascontiguousarray(T16, np.float32)), C.byref(cam), rows, cols,
+          distance_threshold, max_point_depth, _p(collapsed))
+    return k, collapsed

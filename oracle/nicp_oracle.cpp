@@ -736,7 +736,9 @@ int orc_frame_from_depth(const uint16_t *raw, int rows, int cols, float depth_sc
       std::memcpy(eo, e, sizeof(e));
       float invn = 1.0f / acc[0];
       float *mo = &mean[4 * (size_t)index];
-      mo[0] = acc[1] * invn; mo[1] = acc[2] * invn; mo[2] = acc[3] * invn; mo[3] = acc[0] * invn;
+      /* PointAccumulator::mean() returns a Point, whose constructor sets w = 1
+       * (N/pointaccumulator.h:66-70, N/homogeneousvector4f.h:19) */
+      mo[0] = acc[1] * invn; mo[1] = acc[2] * invn; mo[2] = acc[3] * invn; mo[3] = 1.0f;
       nn[index] = accn;
       float k = stats_curvature(e);
       curv[index] = k;
@@ -1406,6 +1408,205 @@ void orc_align_mode(const orc_cloud *ref, const orc_cloud *cur, int rows, int co
   double Hd[36];
   for (int i = 0; i < 36; ++i) Hd[i] = Hs[i];
   orc_compute_statistics(Hd, T, ap, res);
+}
+
+
+/* ---------------- Gaussian3f, Cloud::transformInPlace, Merger::merge ----------------
+ * Gaussian3f keeps a moments form (mean, covariance) and an information form (matrix,
+ * vector), each computed from the other only when read (N/gaussian.h:104-122); the flags
+ * of orc_gaussian record which form is current. Parity: the covariance of the
+ * unprojection is pinned by its closed form in FP64, the fusion of merge by the
+ * information-form product of Gaussians in FP64, the transforms by R S R^T in FP64, the
+ * merge decisions by a brute force in tests/test_merge.py. The association orders of the
+ * 3x3 products (R3) are the ones of the rest of this file. */
+
+namespace {
+
+inline void mat3_vec(const float *M, const float *v, float *o) {
+  float r[3];
+  for (int i = 0; i < 3; ++i) r[i] = red3(M[i] * v[0], M[3 + i] * v[1], M[6 + i] * v[2]);
+  std::memcpy(o, r, sizeof(r));
+}
+
+/* Gaussian::_updateInfo (N/gaussian.h:116-122) */
+void g_update_info(orc_gaussian *g) {
+  if (g->flags & 2) return;
+  orc_inverse3(g->covariance, g->information);
+  mat3_vec(g->information, g->mean, g->information_vector);
+  g->flags |= 2;
+}
+
+/* Gaussian::_updateMoments (N/gaussian.h:108-114) */
+void g_update_moments(orc_gaussian *g) {
+  if (g->flags & 1) return;
+  orc_inverse3(g->information, g->covariance);
+  mat3_vec(g->covariance, g->information_vector, g->mean);
+  g->flags |= 1;
+}
+
+/* Gaussian::addInformation (N/gaussian.h:50-56) */
+void g_add_information(orc_gaussian *t, orc_gaussian *s) {
+  g_update_info(t);
+  g_update_info(s);
+  for (int k = 0; k < 9; ++k) t->information[k] += s->information[k];
+  for (int k = 0; k < 3; ++k) t->information_vector[k] += s->information_vector[k];
+  t->flags = 2;
+}
+
+/* 4x4 column-major product with P1 per column */
+void mat4_mul(const float *A, const float *B, float *C) {
+  float o[16];
+  for (int j = 0; j < 4; ++j) mul4_packet(A, B + 4 * j, o + 4 * j);
+  std::memcpy(C, o, sizeof(o));
+}
+
+}  // namespace
+
+/* PinholePointProjector::unProject with Gaussians, N/pinholepointprojector.cpp:93-134 */
+int orc_unproject_gaussians(const float *depth, int rows, int cols, const float *K, const float *iKRt, float min_d,
+                            float max_d, float baseline, float alpha, float *points, orc_gaussian *gaussians,
+                            int *index_image) {
+  const int n = orc_unproject(depth, rows, cols, iKRt, min_d, max_d, points, index_image);
+  float iK[9];
+  orc_inverse3(K, iK);
+  const float fB = baseline * K[0];
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < cols; ++c) {
+      const int i = index_image[r * cols + c];
+      if (i < 0) continue;
+      const float z = depth[r * cols + c];
+      const float zVariation = (alpha * z * z) / (fB + z * alpha);
+      /* J << z, 0, c, 0, z, r, 0, 0, 1 (row by row), then J = iK * J */
+      float J0[3][3] = {{z, 0.0f, (float)c}, {0.0f, z, (float)r}, {0.0f, 0.0f, 1.0f}};
+      float J[3][3];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          J[a][b] = red3(iK[0 * 3 + a] * J0[0][b], iK[1 * 3 + a] * J0[1][b], iK[2 * 3 + a] * J0[2][b]);
+      const float D[3] = {3.0f, 3.0f, zVariation};
+      orc_gaussian &g = gaussians[i];
+      std::memset(&g, 0, sizeof(g));
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          g.covariance[b * 3 + a] =
+              red3((J[a][0] * D[0]) * J[b][0], (J[a][1] * D[1]) * J[b][1], (J[a][2] * D[2]) * J[b][2]);
+      for (int k = 0; k < 3; ++k) g.mean[k] = points[4 * (size_t)i + k];
+      g.flags = 1;
+    }
+  return n;
+}
+
+/* Cloud::transformInPlace, N/cloud.cpp:193-205, with the transformable vectors of
+ * N/transformable_vector.h:31-37 (points, normals), N/stats.h:121-126, N/gaussian3.h:59-67,
+ * N/informationmatrix.h:111-121. Any array may be NULL. */
+void orc_cloud_transform(int n, const float *T, float *points, float *normals, float *stats16, float *omega_p,
+                         float *omega_n, orc_gaussian *g) {
+  float m[16];
+  std::memcpy(m, T, sizeof(m));
+  mr(m, 3, 0) = 0.0f; mr(m, 3, 1) = 0.0f; mr(m, 3, 2) = 0.0f; mr(m, 3, 3) = 1.0f;
+  /* the vectors are left alone when m is the identity (N/cloud.cpp:196) */
+  float I[16];
+  set_identity4(I);
+  bool identity = true;
+  for (int k = 0; k < 16; ++k) identity = identity && m[k] == I[k];
+  if (identity) return;
+  float Tz[16], Tt[16];
+  std::memcpy(Tz, m, sizeof(Tz));
+  for (int k = 0; k < 4; ++k) { mr(Tz, 3, k) = 0.0f; mr(Tz, k, 3) = 0.0f; }
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) mr(Tt, r, c) = mc(Tz, c, r);
+  for (int i = 0; i < n; ++i) {
+    if (points) {
+      float *p = points + 4 * (size_t)i;
+      mul4_packet(m, p, p);
+      p[3] = 1.0f;
+    }
+    if (normals) {
+      float *q = normals + 4 * (size_t)i;
+      mul4_packet(m, q, q);
+      q[3] = 0.0f;
+    }
+    if (stats16) mat4_mul(m, stats16 + 16 * (size_t)i, stats16 + 16 * (size_t)i);
+    for (float *O : {omega_p, omega_n}) {
+      if (!O) continue;
+      float *W = O + 16 * (size_t)i, A[16];
+      mat4_mul(Tz, W, A);
+      mat4_mul(A, Tt, W);
+    }
+    if (g) {
+      orc_gaussian &G = g[i];
+      g_update_moments(&G);
+      float mean[3], RC[9], cov[9];
+      for (int a = 0; a < 3; ++a)
+        mean[a] = red3(mc(m, a, 0) * G.mean[0], mc(m, a, 1) * G.mean[1], mc(m, a, 2) * G.mean[2]) + mc(m, a, 3);
+      for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < 3; ++k)
+          RC[k * 3 + a] = red3(mc(m, a, 0) * G.covariance[k * 3], mc(m, a, 1) * G.covariance[k * 3 + 1],
+                               mc(m, a, 2) * G.covariance[k * 3 + 2]);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          cov[b * 3 + a] = red3(RC[a] * mc(m, b, 0), RC[3 + a] * mc(m, b, 1), RC[6 + a] * mc(m, b, 2));
+      std::memcpy(G.mean, mean, sizeof(mean));
+      std::memcpy(G.covariance, cov, sizeof(cov));
+      G.flags = 1;
+    }
+  }
+}
+
+/* Merger::merge, N/merger.cpp:15-109, in place on the given arrays (any of normals,
+ * curvature, stats16, omega_p, omega_n may be NULL). Returns the new size; collapsed
+ * (optional, n ints) receives the per-point decision. */
+int orc_merge(int n, float *points, float *normals, float *curvature, float *stats16, float *omega_p, float *omega_n,
+              orc_gaussian *g, const float *transform, const orc_camera *cam, int rows, int cols,
+              float distance_threshold, float max_point_depth, int *collapsed_out) {
+  float T[16], KRt[16], iKRt[16];
+  std::memcpy(T, transform, sizeof(T));
+  mr(T, 3, 0) = 0.0f; mr(T, 3, 1) = 0.0f; mr(T, 3, 2) = 0.0f; mr(T, 3, 3) = 1.0f;
+  orc_projector_matrices(cam->K, T, KRt, iKRt);
+  std::vector<int> idx((size_t)rows * cols);
+  std::vector<float> dep((size_t)rows * cols);
+  orc_project(points, n, KRt, rows, cols, cam->min_distance, cam->max_distance, idx.data(), dep.data());
+  std::vector<int> collapsed((size_t)n, -1);
+  for (int i = 0; i < n; ++i) {
+    float ip[4];
+    mul4_packet(KRt, points + 4 * (size_t)i, ip);
+    const float depth = ip[2];
+    int r = -1, c = -1;
+    if (!(depth < cam->min_distance || depth > cam->max_distance)) {
+      const float inv = 1.0f / depth;
+      c = (int)std::round(ip[0] * inv);
+      r = (int)std::round(ip[1] * inv);
+    }
+    if (depth < 0 || depth > max_point_depth || r < 0 || r >= rows || c < 0 || c >= cols) continue;
+    const float targetZ = dep[(size_t)r * cols + c];
+    const int target = idx[(size_t)r * cols + c];
+    if (target < 0) continue;
+    if (target == i) {
+      collapsed[i] = i;
+    } else if (std::fabs(depth - targetZ) < distance_threshold) {
+      g_add_information(&g[target], &g[i]);
+      collapsed[i] = target;
+    }
+  }
+  int k = 0;
+  for (int i = 0; i < n; ++i) {
+    const int ci = collapsed[i];
+    if (ci == i) {
+      g_update_moments(&g[i]);
+      for (int e = 0; e < 3; ++e) points[4 * (size_t)i + e] = g[i].mean[e];
+    }
+    if (ci < 0 || ci == i) {
+      std::memmove(points + 4 * (size_t)k, points + 4 * (size_t)i, 4 * sizeof(float));
+      if (normals) std::memmove(normals + 4 * (size_t)k, normals + 4 * (size_t)i, 4 * sizeof(float));
+      if (curvature) curvature[k] = curvature[i];
+      if (stats16) std::memmove(stats16 + 16 * (size_t)k, stats16 + 16 * (size_t)i, 16 * sizeof(float));
+      if (omega_p) std::memmove(omega_p + 16 * (size_t)k, omega_p + 16 * (size_t)i, 16 * sizeof(float));
+      if (omega_n) std::memmove(omega_n + 16 * (size_t)k, omega_n + 16 * (size_t)i, 16 * sizeof(float));
+      g[k] = g[i];
+      ++k;
+    }
+  }
+  if (collapsed_out) std::memcpy(collapsed_out, collapsed.data(), (size_t)n * sizeof(int));
+  return k;
 }
 
 }  // extern "C"

@@ -161,6 +161,29 @@ void orc_align_nn(const orc_cloud *ref, const orc_cloud *cur, const float init_g
 void orc_compute_statistics(const double H[36], const float T[16], const orc_align_params *ap,
                             orc_align_result *res);
 
+/* Gaussian3f (N/gaussian.h:8-133) with its lazy-evaluation state: flags bit 0 = mean and
+ * covariance current, bit 1 = information form current. Same layout as nicp_gaussian. */
+typedef struct {
+  float mean[3];
+  int   flags;
+  float covariance[9];
+  float information[9];
+  float information_vector[3];
+  float pad[3];
+} orc_gaussian;
+
+/* PinholePointProjector::unProject with Gaussians (N/pinholepointprojector.cpp:93-134) */
+int orc_unproject_gaussians(const float *depth, int rows, int cols, const float K[9], const float iKRt[16],
+                            float min_d, float max_d, float baseline, float alpha, float *points,
+                            orc_gaussian *gaussians, int *index_image);
+/* Cloud::transformInPlace (N/cloud.cpp:193-205); stats16 / omegas are full column-major 4x4 */
+void orc_cloud_transform(int n, const float T[16], float *points, float *normals, float *stats16, float *omega_p,
+                         float *omega_n, orc_gaussian *gaussians);
+/* Merger::merge (N/merger.cpp:15-109) in place; returns the new size */
+int orc_merge(int n, float *points, float *normals, float *curvature, float *stats16, float *omega_p, float *omega_n,
+              orc_gaussian *gaussians, const float transform[16], const orc_camera *cam, int rows, int cols,
+              float distance_threshold, float max_point_depth, int *collapsed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,8 +39,10 @@ def test_struct_layouts_match_header():
 #include <stddef.h>
 #include "nicp_amd.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(nicp_camera), sizeof(nicp_stats_params), sizeof(nicp_align_params),
-         sizeof(nicp_align_result), offsetof(nicp_align_result, total_time_ms), offsetof(nicp_align_result, overlap_pixels));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(nicp_camera), sizeof(nicp_stats_params),
+         sizeof(nicp_align_params), sizeof(nicp_align_result), offsetof(nicp_align_result, total_time_ms),
+         offsetof(nicp_align_result, overlap_pixels), sizeof(nicp_gaussian), offsetof(nicp_gaussian, information_vector),
+         sizeof(nicp_merger_params));
   return 0;
 }
 '''
@@ -53,7 +55,7 @@ int main(void) {
         sizes = list(map(int, subprocess.check_output([os.path.join(d, "s")]).split()))
     assert sizes == [C.sizeof(capi.Camera), C.sizeof(capi.StatsParams), C.sizeof(capi.AlignParams),
                      C.sizeof(capi.AlignResult), capi.AlignResult.total_time_ms.offset,
-                     capi.AlignResult.overlap_pixels.offset]
+                     capi.AlignResult.overlap_pixels.offset, 28 * 4, 22 * 4, C.sizeof(capi.MergerParams)]
 
 
 def test_no_cpu_fallback_without_device():
