@@ -105,6 +105,25 @@ class Cloud {
 
   std::vector<std::array<float, 16>> normalInformationMatrix() const { return getOmega(1); }
 
+  // Cloud::gaussians(): the Gaussian3f records with their lazy-evaluation state
+  std::vector<nicp_gaussian> gaussians() const {
+    std::vector<nicp_gaussian> g(size());
+    if (!g.empty()) check(nicp_cloud_download_gaussians(h_, g.data()));
+    return g;
+  }
+  // Cloud::transformInPlace (nicp/nicp/cloud.cpp:193-205)
+  void transformInPlace(const Isometry3f &T) { check(nicp_cloud_transform(h_, T.matrix())); }
+  // Cloud::add (nicp/nicp/cloud.cpp:163-190): transforms `other` in place by T and appends it.
+  // An empty Cloud takes over a copy of `other`'s points.
+  void add(Cloud &other, const Isometry3f &T = Isometry3f::Identity()) {
+    if (!h_) {
+      nicp_cloud *h = nullptr;
+      check(nicp_cloud_from_host(Context::get(), 0, nullptr, nullptr, nullptr, nullptr, nullptr, &h));
+      h_ = h;
+    }
+    check(nicp_cloud_add(h_, other.handle(), T.matrix()));
+  }
+
  private:
   std::vector<std::array<float, 4>> get(int which) const {
     std::vector<std::array<float, 4>> v(size());

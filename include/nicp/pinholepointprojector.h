@@ -36,6 +36,11 @@ class PinholePointProjector {
   }
   int imageRows() const { return _imageRows; }
   int imageCols() const { return _imageCols; }
+  // sensor noise model of the Gaussians (pinholepointprojector.h:97-118, defaults .cpp:11-12)
+  float baseline() const { return _baseline; }
+  void setBaseline(float b) { _baseline = b; }
+  float alpha() const { return _alpha; }
+  void setAlpha(float a) { _alpha = a; }
   const Isometry3f &transform() const { return _transform; }
   void setTransform(const Isometry3f &t) { _transform = t; }
   // PinholePointProjector::scale (pinholepointprojector.cpp:150-155 + pointprojector.cpp:106-112)
@@ -71,6 +76,7 @@ class PinholePointProjector {
   }
   Matrix3f _originalCameraMatrix, _cameraMatrix;
   float _minDistance = 0.01f, _maxDistance = 6.0f, _scalingFactor = 1.0f;
+  float _baseline = 0.075f, _alpha = 0.1f;
   int _originalImageRows = 0, _originalImageCols = 0, _imageRows = 0, _imageCols = 0;
   Isometry3f _transform;
 };

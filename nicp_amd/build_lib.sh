@@ -15,3 +15,6 @@ mv "$here/libnicp_synth.so.tmp" "$here/libnicp_synth.so"
 g++ -O2 -std=c++17 -Wall -I "$here/../include" -o "$here/nicp_simple_aligner.tmp" "$here/apps/nicp_simple_aligner.cpp" \
     -L "$here" -lnicp_amd -Wl,-rpath,'$ORIGIN'
 mv "$here/nicp_simple_aligner.tmp" "$here/nicp_simple_aligner"
+g++ -O2 -std=c++17 -Wall -I "$here/../include" -o "$here/nicp_aligner.tmp" "$here/apps/nicp_aligner.cpp" \
+    -L "$here" -lnicp_amd -Wl,-rpath,'$ORIGIN'
+mv "$here/nicp_aligner.tmp" "$here/nicp_aligner"

@@ -87,3 +87,55 @@ def test_simple_aligner_trajectory(tmp_path):
         dist = np.linalg.norm(np.array(t[:3]) - P[:3, 3])
         print("frame %d: %.4f deg %.5f m" % (k, ang, dist), flush=True)
         assert ang < 0.1 * max(k, 1) + 1e-6 and dist < 0.003 * max(k, 1) + 1e-9
+
+
+APP2 = os.path.join(ROOT, "nicp_amd", "nicp_aligner")
+
+
+def test_scene_app_builds_on_cpu():
+    assert os.path.exists(APP2)
+    out = subprocess.run([APP2], capture_output=True, text=True)
+    assert "USAGE" in out.stdout
+
+
+@pytest.mark.gpu
+def test_scene_aligner_trajectory(tmp_path):
+    """nicp_aligner (nicp/nicp/nicp_aligner.cpp): registration against the merged local scene,
+    scenes restarted every chunkStep frames and saved, per-frame clouds saved."""
+    print("case scene_aligner_trajectory", flush=True)
+    rows, cols = 480, 640
+    scene = 78
+    poses = [np.eye(4)]
+    for k in range(1, 6):
+        step = synth.motion(600 + k, 3.0, 0.03).reshape(4, 4, order="F").astype(np.float64)
+        poses.append(poses[-1] @ step)
+    lines = []
+    for k, P in enumerate(poses):
+        d = synth.render(scene, P.astype(np.float32).flatten(order="F"), rows, cols, stream=200 + k)
+        f = tmp_path / ("d%d.pgm" % k)
+        write_pgm16(str(f), d)
+        lines.append("%d.0 %s" % (k, f))
+    (tmp_path / "list.txt").write_text("\n".join(lines) + "\n")
+    (tmp_path / "c.conf").write_text(CONF + "chunkStep 3\ndistanceThreshold 0.1\n")
+    out = subprocess.run([APP2, str(tmp_path / "c.conf"), str(tmp_path / "list.txt"), str(tmp_path / "odom.txt")],
+                         capture_output=True, text=True, timeout=180, cwd=str(tmp_path))
+    print(out.stdout[-3000:], out.stderr[-2000:], flush=True)
+    assert out.returncode == 0
+    traj = [list(map(float, l.split()[1:])) for l in (tmp_path / "odom.txt").read_text().split("\n") if l.strip()]
+    assert len(traj) == len(poses)
+    from scipy.spatial.transform import Rotation
+    for k, (t, P) in enumerate(zip(traj, poses)):
+        R = Rotation.from_quat(t[3:7]).as_matrix()
+        ang = np.degrees(np.arccos(np.clip((np.trace(R.T @ P[:3, :3]) - 1) / 2, -1, 1)))
+        dist = np.linalg.norm(np.array(t[:3]) - P[:3, 3])
+        print("frame %d: %.4f deg %.5f m" % (k, ang, dist), flush=True)
+        assert ang < 0.1 * max(k, 1) + 1e-6 and dist < 0.003 * max(k, 1) + 1e-9
+    # merging keeps the scene near one frame's size: suppressed points are reported
+    sizes = [int(l.split()[-1]) for l in out.stdout.splitlines() if l.startswith("scene points")]
+    assert len(sizes) == len(poses)
+    assert max(sizes) < 2.2 * sizes[0]
+    assert "number of suppressed points" in out.stdout
+    scenes = sorted(p.name for p in tmp_path.glob("scene-*.nicp"))
+    assert scenes == ["scene-001.nicp", "scene-004.nicp", "scene-005.nicp"], scenes
+    for k in range(len(poses)):
+        assert (tmp_path / ("d%d.pgm.nicp" % k)).stat().st_size > 128 * 100000
