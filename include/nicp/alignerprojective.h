@@ -110,12 +110,12 @@ class Linearizer {
 class AlignerProjective {
  public:
   AlignerProjective() {}
-  AlignerProjective(PinholePointProjector *p, Linearizer *l, CorrespondenceFinderProjective *c)
+  AlignerProjective(PointProjector *p, Linearizer *l, CorrespondenceFinderProjective *c)
       : _projector(p), _linearizer(l), _correspondenceFinder(c) {
     if (l) l->setAligner(this);
   }
-  void setProjector(PinholePointProjector *p) { _projector = p; }
-  PinholePointProjector *projector() const { return _projector; }
+  void setProjector(PointProjector *p) { _projector = p; }
+  PointProjector *projector() const { return _projector; }
   // Aligner::setLinearizer writes the back-pointer (nicp/nicp/aligner.h:212-216)
   void setLinearizer(Linearizer *l) {
     _linearizer = l;
@@ -211,7 +211,7 @@ class AlignerProjective {
   }
 
  private:
-  PinholePointProjector *_projector = nullptr;
+  PointProjector *_projector = nullptr;
   Linearizer *_linearizer = nullptr;
   CorrespondenceFinderProjective *_correspondenceFinder = nullptr;
   const Cloud *_referenceCloud = nullptr, *_currentCloud = nullptr;

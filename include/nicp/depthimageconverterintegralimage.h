@@ -80,7 +80,7 @@ inline void DepthImage_scale(DepthImage &dest, const DepthImage &src, int step, 
 
 class DepthImageConverterIntegralImage {
  public:
-  DepthImageConverterIntegralImage(PinholePointProjector *projector, StatsCalculatorIntegralImage *statsCalculator,
+  DepthImageConverterIntegralImage(PointProjector *projector, StatsCalculatorIntegralImage *statsCalculator,
                                    PointInformationMatrixCalculator *pointInformationMatrixCalculator,
                                    NormalInformationMatrixCalculator *normalInformationMatrixCalculator)
       : _projector(projector),
@@ -118,7 +118,7 @@ class DepthImageConverterIntegralImage {
     check(nicp_cloud_from_depth_u16(Context::get(), depth.data.data(), depth.rows, depth.cols, depthScale, &cam, &sp,
                                     _keepStats ? 1 : 0, &h));
     cloud.reset(h);
-    // PinholePointProjector::unProject fills Cloud::gaussians() on every conversion
+    // the projector's unProject fills Cloud::gaussians() on every conversion
     check(nicp_cloud_gaussians(h, _projector->baseline(), _projector->alpha()));
     _indexImage.create(depth.rows, depth.cols);
     check(nicp_cloud_index_image(h, _indexImage.data.data()));
@@ -134,14 +134,14 @@ class DepthImageConverterIntegralImage {
     check(nicp_cloud_from_depth_f32(Context::get(), depth.data.data(), depth.rows, depth.cols, &cam, &sp,
                                     _keepStats ? 1 : 0, &h));
     cloud.reset(h);
-    // PinholePointProjector::unProject fills Cloud::gaussians() on every conversion
+    // the projector's unProject fills Cloud::gaussians() on every conversion
     check(nicp_cloud_gaussians(h, _projector->baseline(), _projector->alpha()));
     _indexImage.create(depth.rows, depth.cols);
     check(nicp_cloud_index_image(h, _indexImage.data.data()));
   }
   const IntImage &indexImage() const { return _indexImage; }
-  PinholePointProjector *projector() const { return _projector; }
-  void setProjector(PinholePointProjector *p) { _projector = p; }
+  PointProjector *projector() const { return _projector; }
+  void setProjector(PointProjector *p) { _projector = p; }
   // The reference's clouds always hold their Stats (nicp/nicp/cloud.h:190); here they are
   // kept on request only (64 B per point), e.g. for Cloud::save with real Stats records.
   bool keepStats() const { return _keepStats; }
@@ -149,7 +149,7 @@ class DepthImageConverterIntegralImage {
 
  private:
   bool _keepStats = false;
-  PinholePointProjector *_projector;
+  PointProjector *_projector;
   StatsCalculatorIntegralImage *_statsCalculator;
   PointInformationMatrixCalculator *_pointCalc;
   NormalInformationMatrixCalculator *_normalCalc;

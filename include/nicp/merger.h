@@ -36,7 +36,7 @@ class Merger {
   void merge(Cloud *cloud, const Isometry3f &transform = Isometry3f::Identity()) {
     if (_rows <= 0 || _cols <= 0) throw std::runtime_error("Merger: image size is zero");
     if (!_converter || !_converter->projector()) throw std::runtime_error("Merger: missing depth image converter");
-    PinholePointProjector *p = _converter->projector();
+    PointProjector *p = _converter->projector();
     if (p->imageRows() <= 0 || p->imageCols() <= 0) throw std::runtime_error("Merger: the projector has no image size");
     nicp_camera cam = p->camera();
     nicp_merger_params mp;

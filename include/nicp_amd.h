@@ -39,12 +39,24 @@ typedef struct nicp_cloud nicp_cloud;
 typedef struct nicp_batch nicp_batch;
 typedef struct nicp_initnet nicp_initnet;
 
-/* PinholePointProjector state (nicp/nicp/pinholepointprojector.h:67-70,
- * nicp/nicp/pointprojector.cpp:9-10). K is the (already scaled) camera matrix. */
+#define NICP_PINHOLE 0
+#define NICP_SPHERICAL 1
+
+/* Projector state. model NICP_PINHOLE: PinholePointProjector (nicp/nicp/pinholepointprojector.h:67-70,
+ * nicp/nicp/pointprojector.cpp:9-10), K the (already scaled) camera matrix. model NICP_SPHERICAL:
+ * SphericalPointProjector (nicp/nicp/sphericalpointprojector.h:252-313, .cpp:9-27): half fields of
+ * view in radians and the image centre in pixels; the angular resolutions follow from the image size
+ * of each call (cols / (2 horizontal_fov), rows / (2 vertical_fov)). A zero-initialised tail is the
+ * pinhole model. */
 typedef struct {
-  float K[9];               /* column-major 3x3 */
+  float K[9];               /* column-major 3x3 (pinhole) */
   float min_distance;       /* must be > 0 */
   float max_distance;
+  int   model;              /* NICP_PINHOLE or NICP_SPHERICAL */
+  float horizontal_fov;     /* spherical: reference defaults pi, pi/4 */
+  float vertical_fov;
+  float horizontal_center;  /* spherical: reference defaults cols/2, rows/2 of its 640x480 default */
+  float vertical_center;
 } nicp_camera;
 
 /* StatsCalculatorIntegralImage (nicp/nicp/statscalculatorintegralimage.cpp:12-18) and

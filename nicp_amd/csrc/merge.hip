@@ -57,9 +57,10 @@ __device__ __forceinline__ void ensure_moments(float *r) {
 }
 
 struct GaussCam {
-  float iK[9];  // PinholePointProjector::_iK (column-major)
-  float fB;     // _baseline * _cameraMatrix(0, 0)
+  float iK[9];    // PinholePointProjector::_iK (column-major)
+  float fB;       // _baseline * _cameraMatrix(0, 0)
   float alpha;
+  int spherical;  // SphericalPointProjector: identity covariance (sphericalpointprojector.cpp:112-113)
 };
 
 struct Xform {
@@ -99,6 +100,19 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_gauss_unproject(const F
   if (p >= P) return;
   const int idx = fd.index[p];
   if (idx < 0) return;
+  float rec[kGF4 * 4];
+#pragma unroll
+  for (int k = 0; k < kGF4 * 4; ++k) rec[k] = 0.0f;
+  const float4 pt = fd.pts[idx];
+  rec[gMean] = pt.x;
+  rec[gMean + 1] = pt.y;
+  rec[gMean + 2] = pt.z;
+  g_set_flags(rec, kMoments);
+  if (gc.spherical) {
+    rec[gCov] = rec[gCov + 4] = rec[gCov + 8] = 1.0f;
+    g_store(gauss + (size_t)kGF4 * idx, rec);
+    return;
+  }
   const float z = frame_depth(fd, p);
   const int r = p / fd.cols, c = p - r * fd.cols;
   const float zvar = (gc.alpha * z * z) / (gc.fB + z * gc.alpha);
@@ -111,20 +125,12 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_gauss_unproject(const F
     for (int i = 0; i < 3; ++i)
       J[j * 3 + i] = red3(gc.iK[i] * J0[j * 3], gc.iK[3 + i] * J0[j * 3 + 1], gc.iK[6 + i] * J0[j * 3 + 2]);
   const float D[3] = {3.0f, 3.0f, zvar};
-  float rec[kGF4 * 4];
-#pragma unroll
-  for (int k = 0; k < kGF4 * 4; ++k) rec[k] = 0.0f;
   // covariance = (J diag(D)) J^T
 #pragma unroll
   for (int j = 0; j < 3; ++j)
 #pragma unroll
     for (int i = 0; i < 3; ++i)
       rec[gCov + j * 3 + i] = red3((J[i] * D[0]) * J[j], (J[3 + i] * D[1]) * J[3 + j], (J[6 + i] * D[2]) * J[6 + j]);
-  const float4 pt = fd.pts[idx];
-  rec[gMean] = pt.x;
-  rec[gMean + 1] = pt.y;
-  rec[gMean + 2] = pt.z;
-  g_set_flags(rec, kMoments);
   g_store(gauss + (size_t)kGF4 * idx, rec);
 }
 
@@ -225,7 +231,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_merge_init(int n, int *
 // pixel), or the pixel's owner whose Gaussian takes its information; slot[i] numbers the points
 // folded into the same owner (in arbitrary order; the owner sorts them).
 extern "C" __global__ __launch_bounds__(kThreads) void k_merge_classify(
-    int n, Xform X, float min_d, float max_d, int rows, int cols, float dist_thr, float max_depth,
+    int n, Xform X, int model, SphDev sph, float min_d, float max_d, int rows, int cols, float dist_thr, float max_depth,
     const float4 *__restrict__ pts, const int *__restrict__ index_image, const float *__restrict__ depth_image,
     int *__restrict__ collapsed, int *__restrict__ count, int *__restrict__ slot) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
@@ -233,14 +239,23 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_merge_classify(
   // PinholePointProjector::_project (pinholepointprojector.h:253-262) with the KRt in X.m:
   // depth is set first, the pixel only when the depth passes the projector's range
   const float4 p = pts[i];
-  float ip[4];
-  mul4_packet(X.m, p.x, p.y, p.z, p.w, ip);
-  const float d = ip[2];
+  float d;
   int r = -1, c = -1;
-  if (!(d < min_d || d > max_d)) {
-    const float inv = 1.0f / d;
-    c = (int)roundf(ip[0] * inv);
-    r = (int)roundf(ip[1] * inv);
+  if (model == kSpherical) {
+    int x, y;
+    if (sph_project(sph, X.m, p, min_d, max_d, x, y, d)) {
+      c = x;
+      r = y;
+    }
+  } else {
+    float ip[4];
+    mul4_packet(X.m, p.x, p.y, p.z, p.w, ip);
+    d = ip[2];
+    if (!(d < min_d || d > max_d)) {
+      const float inv = 1.0f / d;
+      c = (int)roundf(ip[0] * inv);
+      r = (int)roundf(ip[1] * inv);
+    }
   }
   int out = -1;
   if (!(d < 0.0f || d > max_depth || r < 0 || r >= rows || c < 0 || c >= cols)) {
@@ -501,6 +516,7 @@ extern "C" int nicp_cloud_gaussians(nicp_cloud *cl, float baseline, float alpha)
   inverse3(cl->fd.K, gc.iK);
   gc.fB = baseline * cl->fd.K[0];
   gc.alpha = alpha;
+  gc.spherical = cl->fd.model == kSpherical;
   const long long P = (long long)cl->fd.rows * cl->fd.cols;
   (void)hipGetLastError();
   hipLaunchKernelGGL(k_gauss_unproject, dim3(grid_of(P)), dim3(kThreads), 0, ctx->stream, d_f, gc, cl->gauss);
@@ -604,18 +620,19 @@ extern "C" int nicp_merge(nicp_cloud *cl, const float transform[16], const nicp_
   int *kept_dev = total + 16;
   Xform X;
   {
-    float T[16], iKRt[16];
+    float T[16];
     memcpy(T, transform, sizeof(T));
     set_row3(T);
-    projector_matrices(cam->K, T, X.m, iKRt);
+    projector_state(cam->model, cam->K, T, X.m);
     memset(X.mz, 0, sizeof(X.mz));
   }
+  const SphDev sph = sph_params(cam, rows, cols);
   const int zero = 0;
   HIPCHK(hipMemcpyAsync(total, &zero, sizeof(int), hipMemcpyHostToDevice, s));
   const dim3 tb(kThreads), g(grid_of(n));
   (void)hipGetLastError();
   hipLaunchKernelGGL(k_merge_init, g, tb, 0, s, n, count);
-  hipLaunchKernelGGL(k_merge_classify, g, tb, 0, s, n, X, cam->min_distance, cam->max_distance, rows, cols,
+  hipLaunchKernelGGL(k_merge_classify, g, tb, 0, s, n, X, cam->model, sph, cam->min_distance, cam->max_distance, rows, cols,
                      mp->distance_threshold, mp->max_point_depth, (const float4 *)cl->fd.pts,
                      (const int *)ctx->cur_index.as<int>(), (const float *)ctx->depth_tmp.as<float>(), collapsed,
                      count, slot);

@@ -281,7 +281,26 @@ int check_camera(const nicp_camera *cam) {
   if (!cam) return set_err(NICP_ERR_INVALID, "camera is null");
   if (!(cam->min_distance > 0.0f) || !(cam->max_distance >= cam->min_distance))
     return set_err(NICP_ERR_INVALID, "camera distances must satisfy 0 < min_distance <= max_distance");
+  if (cam->model != NICP_PINHOLE && cam->model != NICP_SPHERICAL)
+    return set_err(NICP_ERR_INVALID, "unknown projector model %d", cam->model);
+  if (cam->model == NICP_SPHERICAL && !(cam->horizontal_fov > 0.0f && cam->vertical_fov > 0.0f))
+    return set_err(NICP_ERR_INVALID, "spherical fields of view must be > 0");
   return NICP_OK;
+}
+
+// SphericalPointProjector::_updateParameters (sphericalpointprojector.cpp:19-27) at an image size
+SphDev sph_params(const nicp_camera *cam, int rows, int cols) {
+  SphDev s{};
+  if (cam->model != NICP_SPHERICAL) return s;
+  s.hfov = cam->horizontal_fov;
+  s.vfov = cam->vertical_fov;
+  s.hc = cam->horizontal_center;
+  s.vc = cam->vertical_center;
+  s.hres = (float)cols / (2.0f * s.hfov);
+  s.ihres = 1.0f / s.hres;
+  s.vres = (float)rows / (2.0f * s.vfov);
+  s.ivres = 1.0f / s.vres;
+  return s;
 }
 
 StatsParamsDev to_dev(const nicp_stats_params *s, int keep_stats) {
@@ -337,8 +356,17 @@ void frame_geometry(FrameDev &f, int rows, int cols, const nicp_camera *cam, flo
   f.min_d = cam->min_distance;
   f.max_d = cam->max_distance;
   for (int k = 0; k < 9; ++k) f.K[k] = cam->K[k];
+  f.model = cam->model;
+  f.sph = sph_params(cam, rows, cols);
   float I4[16], KRt[16];
   set_identity4(I4);
+  if (cam->model == NICP_SPHERICAL) {
+    // the converter's projector transform is the identity (depthimageconverterintegralimage.cpp:41);
+    // the interval takes the world radius itself
+    set_identity4(f.iKRt);
+    f.ip0 = f.ip1 = world_radius;
+    return;
+  }
   projector_matrices(cam->K, I4, KRt, f.iKRt);
   // _projectInterval (pinholepointprojector.h:293-302): K * (R, R, 0), 3-term redux
   f.ip0 = red3(cam->K[0] * world_radius, cam->K[3] * world_radius, cam->K[6] * 0.0f);
@@ -389,16 +417,22 @@ void frame_pointers(FrameDev &f, char *base, const FrameLayout &L, bool with_sta
 }
 
 // Launch sequence of the conversion for a frame table already on the device.
+// All frames of one launch share the projector model.
 int launch_convert(Prof *pf, hipStream_t s, const FrameDev *d_frames, int num_frames, const int *d_blk, int total_tiles,
-                   const int *d_x, int total_x, const int *d_y, int total_y, const StatsParamsDev &sp) {
+                   const int *d_x, int total_x, const int *d_y, int total_y, const StatsParamsDev &sp, int model) {
   if (total_tiles == 0) return NICP_OK;
   (void)hipGetLastError();
-  LAUNCH(pf, KID_COUNT, k_count, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
+  const bool sph = model == NICP_SPHERICAL;
+  if (sph) LAUNCH(pf, KID_COUNT, k_count_sph, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
+  else LAUNCH(pf, KID_COUNT, k_count, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
   LAUNCH(pf, KID_SCAN, k_scan, dim3(num_frames), dim3(1024), s, d_frames);
-  LAUNCH(pf, KID_UNPROJECT, k_unproject, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
-  LAUNCH(pf, KID_INTEGRAL_X, k_integral_x, dim3(total_x), dim3(kThreads), s, d_frames, d_x);
+  if (sph) LAUNCH(pf, KID_UNPROJECT, k_unproject_sph, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
+  else LAUNCH(pf, KID_UNPROJECT, k_unproject, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
+  if (sph) LAUNCH(pf, KID_INTEGRAL_X, k_integral_x_sph, dim3(total_x), dim3(kThreads), s, d_frames, d_x);
+  else LAUNCH(pf, KID_INTEGRAL_X, k_integral_x, dim3(total_x), dim3(kThreads), s, d_frames, d_x);
   LAUNCH(pf, KID_INTEGRAL_Y, k_integral_y, dim3(total_y), dim3(kThreads), s, d_frames, d_y);
-  LAUNCH(pf, KID_STATS, k_stats, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk, sp);
+  if (sph) LAUNCH(pf, KID_STATS, k_stats_sph, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk, sp);
+  else LAUNCH(pf, KID_STATS, k_stats, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk, sp);
   HIPCHK(hipGetLastError());
   return NICP_OK;
 }
@@ -441,6 +475,8 @@ void make_plan(const std::vector<FrameDev> &frames, int num_pairs, const int *re
     for (int k = 0; k < 9; ++k) pd.K[k] = cams[i]->K[k];
     pd.min_d = cams[i]->min_distance;
     pd.max_d = cams[i]->max_distance;
+    pd.model = cams[i]->model;
+    pd.sph = sph_params(cams[i], pd.rows, pd.cols);
     pd.cur_own = 0;
     pd.sub = 1;
     // pixel offset of this pair (aligned to 64 entries), turned into pointers by prepare_align
@@ -457,16 +493,23 @@ void make_plan(const std::vector<FrameDev> &frames, int num_pairs, const int *re
 // frame's camera and image (nicp_batch_align). Then the current cloud projected at the
 // identity lands every point on its own pixel, so the current index image of
 // AlignerProjective::align (alignerprojective.cpp:44-47) is the frame's own index image.
+// A spherical projection does not return every point to its own pixel (azimuths are truncated),
+// so spherical pairs keep projecting their current cloud.
 void plan_batch_frames(const std::vector<FrameDev> &frames, PairPlan &plan) {
   plan.blk_cur.clear();
   for (size_t i = 0; i < plan.pairs.size(); ++i) {
     PairDev &pd = plan.pairs[i];
+    if (pd.model == NICP_SPHERICAL) {
+      pd.ctile_base = (int)plan.blk_cur.size();
+      for (int t = 0; t < pd.n_ctiles; ++t) plan.blk_cur.push_back((int)i);
+      continue;
+    }
     pd.cur_own = 1;
     pd.cur_index = frames[pd.cur].index;
     pd.ctile_base = 0;
     pd.n_ctiles = 0;
   }
-  plan.total_ctiles = 0;
+  plan.total_ctiles = (int)plan.blk_cur.size();
 }
 
 struct AlignBuffers {
@@ -504,13 +547,14 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
     iso_inverse(T, Ti);
     set_identity4(I4);
     iso_mul(T, I4, TR);
-    projector_matrices(pd.K, TR, KRt, iKRt);
+    projector_state(pd.model, pd.K, TR, KRt);
     for (int k = 0; k < 16; ++k) {
       S[kSlotT + k] = T[k];
       S[kSlotTinv + k] = Ti[k];
       S[kSlotKRt + k] = KRt[k];
     }
-    projector_matrices(pd.K, I4, KRt, iKRt);
+    projector_state(pd.model, pd.K, I4, KRt);
+    (void)iKRt;
     for (int k = 0; k < 16; ++k) S[kSlotKRtCur + k] = KRt[k];
   }
   const size_t nb = plan.blk_pair.size() + plan.blk_ref.size() + plan.blk_cur.size();
@@ -539,17 +583,22 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
   const AlignParamsDev a = to_dev(ap);
   const dim3 tb(kThreads);
   if (plan.total_pix_tiles == 0) return NICP_OK;
+  // one projector model per plan (a batch's frames share it)
+  const bool sph = plan.pairs[0].model == NICP_SPHERICAL;
   (void)hipGetLastError();
   LAUNCH(pf, KID_ZFILL, k_zfill, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair);
   if (plan.total_ctiles) {
-    LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1);
+    if (sph) LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject_sph, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1);
+    else LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1);
     LAUNCH(pf, KID_ZRESOLVE, k_zresolve, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair, (float *)nullptr);
   }
   const int inner = ap->inner_iterations;
   for (int it = 0; it < ap->outer_iterations; ++it) {
     const bool last = it == ap->outer_iterations - 1;
-    if (plan.total_rtiles)
-      LAUNCH(pf, KID_ZPROJECT_REF, k_zproject, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0);
+    if (plan.total_rtiles) {
+      if (sph) LAUNCH(pf, KID_ZPROJECT_REF, k_zproject_sph, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0);
+      else LAUNCH(pf, KID_ZPROJECT_REF, k_zproject, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0);
+    }
     LAUNCH(pf, KID_CORR_LIN, k_corr_lin, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
            (int)(last || inner > 1));
     LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(64), s, d_pairs, ap->lambda, inner > 1 ? 1 : 0, 0);
@@ -894,7 +943,8 @@ int cloud_from_depth(nicp_ctx *ctx, const uint16_t *depth, const float *depthf, 
     if (e == hipSuccess) e = hipMemcpyAsync(d_maps, zeros.data(), zeros.size() * sizeof(int), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) { rc = set_err(NICP_ERR_HIP, "upload: %s", hipGetErrorString(e)); break; }
     ctx->prof.sink = &ctx->prof.eager;
-    rc = launch_convert(&ctx->prof, s, d_f, 1, d_maps, nt, d_maps + nt, nx, d_maps + nt + nx, ny, to_dev(sp, keep_stats != 0));
+    rc = launch_convert(&ctx->prof, s, d_f, 1, d_maps, nt, d_maps + nt, nx, d_maps + nt + nx, ny, to_dev(sp, keep_stats != 0),
+                        cam->model);
     if (rc != NICP_OK) break;
     e = hipMemcpyAsync(&cl->n, f.npoints, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1142,6 +1192,7 @@ int project_device(nicp_ctx *ctx, const nicp_cloud *cloud, const float transform
   PairDev &pd = plan.pairs[0];
   pd.rows = rows;
   pd.cols = cols;
+  pd.sph = sph_params(cam, rows, cols);
   const int P = rows * cols;
   pd.n_tiles = pair_tiles(rows, cols);
   plan.blk_pair.assign(pd.n_tiles, 0);
@@ -1157,17 +1208,21 @@ int project_device(nicp_ctx *ctx, const nicp_cloud *cloud, const float transform
   CHK(prepare_align(ctx->stream, plan, B, transform, &d_bp, &d_br, &d_bc, &d_pairs));
   // the current-projection slot must hold the transform's KRt (PinholePointProjector::setTransform)
   {
-    float T[16], KRt[16], iKRt[16];
+    float T[16], KRt[16];
     memcpy(T, transform, sizeof(T));
     set_row3(T);
-    projector_matrices(cam->K, T, KRt, iKRt);
+    projector_state(cam->model, cam->K, T, KRt);
     HIPCHK(hipMemcpyAsync(ctx->state.as<float>() + kSlotKRtCur, KRt, sizeof(KRt), hipMemcpyHostToDevice, ctx->stream));
   }
   CHK(ctx->depth_tmp.reserve((size_t)P * sizeof(float)));
   const dim3 tb(kThreads);
   hipLaunchKernelGGL(k_zfill, dim3(pd.n_tiles), tb, 0, ctx->stream, d_pairs, d_bp);
-  if (plan.total_ctiles)
-    hipLaunchKernelGGL(k_zproject, dim3(plan.total_ctiles), tb, 0, ctx->stream, d_pairs, d_frames, d_bc, 1);
+  if (plan.total_ctiles) {
+    if (cam->model == NICP_SPHERICAL)
+      hipLaunchKernelGGL(k_zproject_sph, dim3(plan.total_ctiles), tb, 0, ctx->stream, d_pairs, d_frames, d_bc, 1);
+    else
+      hipLaunchKernelGGL(k_zproject, dim3(plan.total_ctiles), tb, 0, ctx->stream, d_pairs, d_frames, d_bc, 1);
+  }
   hipLaunchKernelGGL(k_zresolve, dim3(pd.n_tiles), tb, 0, ctx->stream, d_pairs, d_bp, ctx->depth_tmp.as<float>());
   HIPCHK(hipGetLastError());
   return NICP_OK;
@@ -1314,6 +1369,7 @@ extern "C" int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud
   PairDev &pd = plan.pairs[0];
   pd.rows = rows;
   pd.cols = cols;
+  pd.sph = sph_params(cam, rows, cols);
   pd.n_tiles = pair_tiles(rows, cols);
   plan.blk_pair.assign(pd.n_tiles, 0);
   plan.total_pix_tiles = pd.n_tiles;
@@ -1377,6 +1433,7 @@ extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows,
     if (rows[i] <= 0 || cols[i] <= 0 || (long long)rows[i] * cols[i] > (1ll << 26))
       return set_err(NICP_ERR_INVALID, "frame %d: bad size %dx%d", i, rows[i], cols[i]);
     CHK(check_camera(&cams[i]));
+    if (cams[i].model != cams[0].model) return set_err(NICP_ERR_INVALID, "the frames of a batch share one projector model");
   }
   HIPCHK(hipSetDevice(ctx->device));
   nicp_batch *b = new (std::nothrow) nicp_batch();
@@ -1474,7 +1531,7 @@ extern "C" int nicp_batch_convert(nicp_batch *b) {
   HIPCHK(hipSetDevice(b->ctx->device));
   b->ctx->prof.sink = &b->ctx->prof.eager;
   CHK(launch_convert(&b->ctx->prof, b->ctx->stream, b->d_frames, b->num_frames, b->d_blk_frame, b->total_tiles, b->d_xblk_frame,
-                     b->total_xblk, b->d_yblk_frame, b->total_yblk, b->sp));
+                     b->total_xblk, b->d_yblk_frame, b->total_yblk, b->sp, b->frames[0].model));
   return NICP_OK;
 }
 
@@ -1502,12 +1559,30 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
     for (int k = 0; k < 9; ++k) c.K[k] = fc.K[k];
     c.min_distance = fc.min_d;
     c.max_distance = fc.max_d;
+    c.model = fc.model;
+    c.horizontal_fov = fc.sph.hfov;
+    c.vertical_fov = fc.sph.vfov;
+    c.horizontal_center = fc.sph.hc;
+    c.vertical_center = fc.sph.vc;
     cams[i] = &c;
   }
+  if (sub < 1 || (sub > 1 && (ap->inner_iterations > 1 || with_stats)))
+    return set_err(NICP_ERR_INVALID, "a subsampled linearisation runs one inner iteration and no statistics");
   PairPlan plan;
   make_plan(b->frames, num_pairs, ref_frame, cur_frame, cams.data(), plan);
   plan_batch_frames(b->frames, plan);
-  for (PairDev &pd : plan.pairs) pd.sub = sub;
+  if (sub > 1) {
+    // per-pixel blocks cover the subsampled grid only; k_zfill still clears whole z-buffers
+    plan.blk_pair.clear();
+    for (int i = 0; i < num_pairs; ++i) {
+      PairDev &pd = plan.pairs[i];
+      pd.sub = sub;
+      pd.n_tiles = pair_tiles((pd.rows + sub - 1) / sub, (pd.cols + sub - 1) / sub);
+      pd.tile_base = (int)plan.blk_pair.size();
+      for (int t = 0; t < pd.n_tiles; ++t) plan.blk_pair.push_back(i);
+    }
+    plan.total_pix_tiles = (int)plan.blk_pair.size();
+  }
   AlignBuffers B{&b->pairs_tbl, &b->maps, &b->zbuf, &b->cur_index, &b->pairs_buf, &b->state, &b->partials, &b->result};
   int *d_bp, *d_br, *d_bc;
   PairDev *d_pairs;

@@ -14,6 +14,13 @@ constexpr uint64_t kZInit = ~0ull;   // empty z-buffer cell
 // per-pixel blocks of a pair: 1024-pixel tiles of its projector image
 __host__ __device__ inline int pair_tiles(int rows, int cols) { return (rows * cols + kTile - 1) / kTile; }
 
+// SphericalPointProjector parameters (sphericalpointprojector.h:304-313): half fields of view,
+// image centre, angular resolutions in pixels per radian and their inverses.
+struct SphDev {
+  float hfov, vfov, hc, vc, hres, vres, ihres, ivres;
+};
+constexpr int kPinhole = 0, kSpherical = 1;
+
 // One frame (a depth image and the cloud made from it). Every pointer is a
 // device pointer; arrays indexed by pixel hold rows*cols entries, arrays
 // indexed by point hold up to rows*cols entries (the capacity).
@@ -27,7 +34,9 @@ struct FrameDev {
   float min_d, max_d;
   float ip0, ip1;       // K * (worldRadius, worldRadius, 0): rows 0 and 1 (interval numerators)
   float K[9];           // camera matrix (column-major)
-  float iKRt[16];       // unprojection matrix (identity sensor pose)
+  float iKRt[16];       // unprojection matrix (pinhole) or the projector transform (spherical), identity pose
+  int model;            // kPinhole or kSpherical
+  SphDev sph;
   const uint16_t *depth;
   const float *depthf;  // float depth in metres (used instead of depth when set)
   int *index;           // index image
@@ -53,6 +62,8 @@ struct PairDev {
   int sub;                         // k_corr_lin evaluates every sub-th row and column (1: all)
   float K[9];
   float min_d, max_d;
+  int model;                       // kPinhole or kSpherical
+  SphDev sph;                      // spherical parameters at the pair's image size
   unsigned long long *zbuf;        // rows*cols z-buffer keys (depth bits << 32 | index)
   int *cur_index;                  // rows*cols
   int2 *pairs;                     // rows*cols stored correspondences (or (-1,-1))

@@ -45,6 +45,46 @@ __device__ __forceinline__ float4 unproject_point(const float *iKRt, int c, int 
   return make_float4(o[0], o[1], o[2], o[3]);
 }
 
+// SphericalPointProjector::_unProject (sphericalpointprojector.h:275-289): the depth range and
+// the field of view decide validity; x = sin(theta) d, z = cos(theta) d, y = d tan(phi), then
+// the projector transform T (Isometry3f * Vector3f: translation + linear * v). The trigonometric
+// functions are evaluated in double and rounded to float: the correctly rounded value of the
+// float functions the reference calls.
+__device__ __forceinline__ bool sph_valid(const SphDev &s, int c, int r, float d, float mn, float mx) {
+  if (d < mn || d > mx) return false;
+  const float theta = s.ihres * ((float)c - s.hc);
+  const float phi = s.ivres * ((float)r - s.vc);
+  return !(fabsf(theta) > s.hfov) && !(fabsf(phi) > s.vfov);
+}
+__device__ __forceinline__ float4 sph_point(const SphDev &s, const float *T, int c, int r, float d) {
+  const float theta = s.ihres * ((float)c - s.hc);
+  const float phi = s.ivres * ((float)r - s.vc);
+  const float x = (float)sin((double)theta) * d;
+  const float z = (float)cos((double)theta) * d;
+  const float y = d * (float)tan((double)phi);
+  float o[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) o[i] = mc(T, i, 3) + red3(mc(T, i, 0) * x, mc(T, i, 1) * y, mc(T, i, 2) * z);
+  return make_float4(o[0], o[1], o[2], 1.0f);
+}
+// SphericalPointProjector::_project (sphericalpointprojector.h:252-265) with S = _iT: P3 product,
+// azimuth and elevation, the cylinder distance as depth; the reference tests fabs(theta > fov),
+// so angles below -fov pass; pixels are truncated, not rounded.
+__device__ __forceinline__ bool sph_project(const SphDev &s, const float *S, float4 p, float mn, float mx, int &x,
+                                            int &y, float &d) {
+  float cp[3];
+  iso_mul3(S, p.x, p.y, p.z, p.w, cp);
+  const float theta = (float)atan2((double)cp[0], (double)cp[2]);
+  d = sqrtf(cp[0] * cp[0] + cp[2] * cp[2]);
+  const float phi = (float)atan2((double)cp[1], (double)d);
+  if (d > mx || d < mn) return false;
+  if (theta > s.hfov) return false;
+  if (phi > s.vfov) return false;
+  x = (int)(s.hres * theta + s.hc);
+  y = (int)(s.vres * phi + s.vc);
+  return true;
+}
+
 __device__ __forceinline__ int wave_sum_i(int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -72,8 +112,17 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 // Conversion
 // ---------------------------------------------------------------------------------------
 
-extern "C" __global__ __launch_bounds__(kThreads) void k_count(const FrameDev *__restrict__ F,
-                                                              const int *__restrict__ blk_frame) {
+template <int M>
+__device__ __forceinline__ bool pixel_valid(const FrameDev &fd, int p, float d) {
+  if (M == kSpherical) {
+    const int r = p / fd.cols;
+    return sph_valid(fd.sph, p - r * fd.cols, r, d, fd.min_d, fd.max_d);
+  }
+  return depth_valid(d, fd.min_d, fd.max_d);
+}
+
+template <int M>
+__device__ __forceinline__ void count_body(const FrameDev *__restrict__ F, const int *__restrict__ blk_frame) {
   const int g = blockIdx.x;
   const FrameDev &fd = F[blk_frame[g]];
   const int P = fd.rows * fd.cols;
@@ -82,13 +131,21 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_count(const FrameDev *_
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     int p = base + k * kThreads + threadIdx.x;
-    if (p < P) cnt += depth_valid(frame_depth(fd, p), fd.min_d, fd.max_d);
+    if (p < P) cnt += pixel_valid<M>(fd, p, frame_depth(fd, p));
   }
   __shared__ int ws[kThreads / 64];
   cnt = wave_sum_i(cnt);
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = cnt;
   __syncthreads();
   if (threadIdx.x == 0) fd.tile_count[g - fd.tile_base] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+extern "C" __global__ __launch_bounds__(kThreads) void k_count(const FrameDev *__restrict__ F,
+                                                              const int *__restrict__ blk_frame) {
+  count_body<kPinhole>(F, blk_frame);
+}
+extern "C" __global__ __launch_bounds__(kThreads) void k_count_sph(const FrameDev *__restrict__ F,
+                                                                  const int *__restrict__ blk_frame) {
+  count_body<kSpherical>(F, blk_frame);
 }
 
 // one block of 1024 threads per frame
@@ -121,8 +178,8 @@ extern "C" __global__ __launch_bounds__(1024) void k_scan(const FrameDev *__rest
 }
 
 // Each thread owns 4 consecutive pixels, so the block's numbering is row-major.
-extern "C" __global__ __launch_bounds__(kThreads) void k_unproject(const FrameDev *__restrict__ F,
-                                                                  const int *__restrict__ blk_frame) {
+template <int M>
+__device__ __forceinline__ void unproject_body(const FrameDev *__restrict__ F, const int *__restrict__ blk_frame) {
   const int g = blockIdx.x;
   const FrameDev &fd = F[blk_frame[g]];
   const int P = fd.rows * fd.cols;
@@ -134,7 +191,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_unproject(const FrameDe
   for (int k = 0; k < 4; ++k) {
     int p = p0 + k;
     dv[k] = p < P ? frame_depth(fd, p) : 0.0f;
-    valid[k] = p < P && depth_valid(dv[k], fd.min_d, fd.max_d);
+    valid[k] = p < P && pixel_valid<M>(fd, p, dv[k]);
     cnt += valid[k];
   }
   __shared__ int wsum[kThreads / 64];
@@ -151,13 +208,21 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_unproject(const FrameDe
     if (p >= P) break;
     if (valid[k]) {
       int r = p / fd.cols, c = p - r * fd.cols;
-      fd.pts[idx] = unproject_point(fd.iKRt, c, r, dv[k]);
+      fd.pts[idx] = M == kSpherical ? sph_point(fd.sph, fd.iKRt, c, r, dv[k]) : unproject_point(fd.iKRt, c, r, dv[k]);
       fd.index[p] = idx;
       ++idx;
     } else {
       fd.index[p] = -1;
     }
   }
+}
+extern "C" __global__ __launch_bounds__(kThreads) void k_unproject(const FrameDev *__restrict__ F,
+                                                                  const int *__restrict__ blk_frame) {
+  unproject_body<kPinhole>(F, blk_frame);
+}
+extern "C" __global__ __launch_bounds__(kThreads) void k_unproject_sph(const FrameDev *__restrict__ F,
+                                                                      const int *__restrict__ blk_frame) {
+  unproject_body<kSpherical>(F, blk_frame);
 }
 
 // x-prefix: one thread per (row, channel), sequential along the row as the
@@ -216,6 +281,36 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_integral_x(const FrameD
       mul4_packet(iK, (float)c * d, (float)r * d, d, 1.0f, o);
       float a = ci == 0 ? o[0] : ci == 1 ? o[1] : ci == 2 ? o[2] : o[3];
       float b = cj == 0 ? o[0] : cj == 1 ? o[1] : cj == 2 ? o[2] : o[3];
+      v = a * b;
+    }
+    acc = v + acc;
+    orow[(size_t)c * 10] = acc;
+  }
+}
+
+// x-prefix of a spherical frame: the points are read from the cloud through the index image,
+// as PointIntegralImage::compute does (pointintegralimage.cpp:7-44), since recomputing them
+// from depth would take three trigonometric functions per channel.
+extern "C" __global__ __launch_bounds__(kThreads) void k_integral_x_sph(const FrameDev *__restrict__ F,
+                                                                       const int *__restrict__ xblk_frame) {
+  const int g = blockIdx.x;
+  const FrameDev &fd = F[xblk_frame[g]];
+  const int lt = (g - fd.xblk_base) * kThreads + threadIdx.x;
+  const int rows = fd.rows, cols = fd.cols;
+  if (lt >= rows * 10) return;
+  const int r = lt / 10, ch = lt - r * 10;
+  const int ci = c_ch_i[ch], cj = c_ch_j[ch];
+  const size_t rowoff = (size_t)r * cols;
+  float *orow = fd.ii + rowoff * 10 + ch;
+  const int *irow = fd.index + rowoff;
+  float acc = 0.0f;
+  for (int c = 0; c < cols; ++c) {
+    const int idx = irow[c];
+    float v = 0.0f;
+    if (idx >= 0) {
+      const float4 q = fd.pts[idx];
+      const float a = ci == 0 ? q.x : ci == 1 ? q.y : ci == 2 ? q.z : q.w;
+      const float b = cj == 0 ? q.x : cj == 1 ? q.y : cj == 2 ? q.z : q.w;
       v = a * b;
     }
     acc = v + acc;
@@ -402,8 +497,9 @@ __device__ __forceinline__ void omega6(const float *U, const float *dg, float *o
 
 }  // namespace
 
-extern "C" __global__ __launch_bounds__(kThreads) void k_stats(const FrameDev *__restrict__ F,
-                                                              const int *__restrict__ blk_frame, StatsParamsDev sp) {
+template <int M>
+__device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const int *__restrict__ blk_frame,
+                                           const StatsParamsDev &sp) {
   const int g = blockIdx.x;
   const FrameDev &fd = F[blk_frame[g]];
   const int rows = fd.rows, cols = fd.cols, P = rows * cols;
@@ -416,21 +512,30 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_stats(const FrameDev *_
     const int r = p / cols, c = p - r * cols;
     const float d = frame_depth(fd, p);
     bool ok = true;
-    int R = -1;
+    int R0 = -1, R1 = -1;
     if (d < fd.min_d || d > fd.max_d) {
       ok = false;
+    } else if (M == kSpherical) {
+      // SphericalPointProjector::_projectInterval (sphericalpointprojector.h:291-303); the
+      // converter suppresses the image-radius limits for this projector
+      // (depthimageconverterintegralimage.cpp:48-52, statscalculatorintegralimage.cpp:45-51)
+      const float a = (float)atan2((double)fd.ip0, (double)d);
+      R0 = (int)(fd.sph.hres * a);
+      R1 = (int)(fd.sph.vres * a);
+      if (R0 < 0 || R1 < 0) ok = false;
     } else {
       float inv = 1.0f / d;
       float a = fd.ip0 * inv, b = fd.ip1 * inv;
-      R = (int)fmaxf(a, b);
+      int R = (int)fmaxf(a, b);
       if (R < 0 || R < sp.min_r) ok = false;
       if (R > sp.max_r) R = sp.max_r;
+      R0 = R1 = R;
     }
     float acc[10];
     int n = 0;
     if (ok) {
-      int xmin = min(max(c - R - 1, 0), cols - 1), xmax = min(max(c + R - 1, 0), cols - 1);
-      int ymin = min(max(r - R - 1, 0), rows - 1), ymax = min(max(r + R - 1, 0), rows - 1);
+      int xmin = min(max(c - R0 - 1, 0), cols - 1), xmax = min(max(c + R0 - 1, 0), cols - 1);
+      int ymin = min(max(r - R1 - 1, 0), rows - 1), ymax = min(max(r + R1 - 1, 0), rows - 1);
       const float2 *A = (const float2 *)(fd.ii + ((size_t)ymax * cols + xmax) * 10);
       const float2 *D = (const float2 *)(fd.ii + ((size_t)ymin * cols + xmin) * 10);
       const float2 *B = (const float2 *)(fd.ii + ((size_t)ymax * cols + xmin) * 10);
@@ -467,7 +572,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_stats(const FrameDev *_
       const float curv = (float)((double)ev[0] / ((double)((ev[0] + ev[1]) + ev[2]) + 1e-9));
       float nx = V[0], ny = V[1], nz = V[2];
       if (curv < sp.curvature_threshold) {
-        const float4 pt = unproject_point(fd.iKRt, c, r, d);
+        const float4 pt = M == kSpherical ? fd.pts[idx] : unproject_point(fd.iKRt, c, r, d);
         if (dot4(nx, ny, nz, 0.0f, pt.x, pt.y, pt.z, pt.w) > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
         nrm = make_float4(nx, ny, nz, curv);
         float dp[3], dq[3];
@@ -496,6 +601,14 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_stats(const FrameDev *_
     }
   }
 }
+extern "C" __global__ __launch_bounds__(kThreads) void k_stats(const FrameDev *__restrict__ F,
+                                                              const int *__restrict__ blk_frame, StatsParamsDev sp) {
+  stats_body<kPinhole>(F, blk_frame, sp);
+}
+extern "C" __global__ __launch_bounds__(kThreads) void k_stats_sph(const FrameDev *__restrict__ F,
+                                                                  const int *__restrict__ blk_frame, StatsParamsDev sp) {
+  stats_body<kSpherical>(F, blk_frame, sp);
+}
 
 // ---------------------------------------------------------------------------------------
 // Registration
@@ -506,18 +619,21 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_zfill(const PairDev *__
   const int g = blockIdx.x;
   const PairDev &pd = Pp[blk_pair[g]];
   const int P = pd.rows * pd.cols;
-  const int base = (g - pd.tile_base) * kTile;
+  // a subsampled pair has fewer blocks than its z-buffer has tiles: stride over them
+  const int stride = pd.n_tiles * kTile;
+  for (int base = (g - pd.tile_base) * kTile; base < P; base += stride) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int p = base + k * kThreads + threadIdx.x;
-    if (p < P) pd.zbuf[p] = kZInit;
+    for (int k = 0; k < 4; ++k) {
+      int p = base + k * kThreads + threadIdx.x;
+      if (p < P) pd.zbuf[p] = kZInit;
+    }
   }
 }
 
 // which = 0: the reference cloud with the pair's current KRt; 1: the current cloud
-extern "C" __global__ __launch_bounds__(kThreads) void k_zproject(const PairDev *__restrict__ Pp,
-                                                                 const FrameDev *__restrict__ F,
-                                                                 const int *__restrict__ blk_pair, int which) {
+template <int M>
+__device__ __forceinline__ void zproject_body(const PairDev *__restrict__ Pp, const FrameDev *__restrict__ F,
+                                              const int *__restrict__ blk_pair, int which) {
   const int g = blockIdx.x;
   const PairDev &pd = Pp[blk_pair[g]];
   const FrameDev &fd = F[which ? pd.cur : pd.ref];
@@ -536,15 +652,24 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_zproject(const PairDev 
     unsigned long long key = kZInit;
     if (i < n) {
       const float4 p = fd.pts[i];
-      float ip[4];
-      mul4_packet(KRt, p.x, p.y, p.z, p.w, ip);
-      const float d = ip[2];
-      if (!(d < pd.min_d || d > pd.max_d)) {
-        const float inv = 1.0f / d;
-        const int x = (int)roundf(ip[0] * inv), y = (int)roundf(ip[1] * inv);
-        if (x >= 0 && x < cols && y >= 0 && y < rows) {
+      if (M == kSpherical) {
+        int x, y;
+        float d;
+        if (sph_project(pd.sph, KRt, p, pd.min_d, pd.max_d, x, y, d) && x >= 0 && x < cols && y >= 0 && y < rows) {
           cell = y * cols + x;
           key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
+        }
+      } else {
+        float ip[4];
+        mul4_packet(KRt, p.x, p.y, p.z, p.w, ip);
+        const float d = ip[2];
+        if (!(d < pd.min_d || d > pd.max_d)) {
+          const float inv = 1.0f / d;
+          const int x = (int)roundf(ip[0] * inv), y = (int)roundf(ip[1] * inv);
+          if (x >= 0 && x < cols && y >= 0 && y < rows) {
+            cell = y * cols + x;
+            key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
+          }
         }
       }
     }
@@ -567,6 +692,16 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_zproject(const PairDev 
     if (cell >= 0 && !same) atomicMin(pd.zbuf + cell, key);
     else if (lane == lead) atomicMin(pd.zbuf + cell, m);
   }
+}
+extern "C" __global__ __launch_bounds__(kThreads) void k_zproject(const PairDev *__restrict__ Pp,
+                                                                 const FrameDev *__restrict__ F,
+                                                                 const int *__restrict__ blk_pair, int which) {
+  zproject_body<kPinhole>(Pp, F, blk_pair, which);
+}
+extern "C" __global__ __launch_bounds__(kThreads) void k_zproject_sph(const PairDev *__restrict__ Pp,
+                                                                     const FrameDev *__restrict__ F,
+                                                                     const int *__restrict__ blk_pair, int which) {
+  zproject_body<kSpherical>(Pp, F, blk_pair, which);
 }
 
 // z-buffer -> index image (and depth image); resets the z-buffer. Pairs that use the
@@ -979,7 +1114,8 @@ extern "C" __global__ __launch_bounds__(64) void k_solve(const PairDev *__restri
   iso_inverse(T, Ti);
   set_identity4(I4);
   iso_mul(T, I4, TR);
-  projector_matrices(pd.K, TR, KRt, iKRt);
+  projector_state(pd.model, pd.K, TR, KRt);
+  (void)iKRt;
   for (int k = 0; k < 16; ++k) {
     S[kSlotT + k] = T[k];
     S[kSlotTinv + k] = Ti[k];

@@ -118,6 +118,20 @@ NICP_HD void projector_matrices(const float *K, const float *transform, float *K
   }
 }
 
+// The projection state of a projector seen from `transform`: PinholePointProjector's KRt, or
+// SphericalPointProjector's _iT = transform^-1 (sphericalpointprojector.cpp:19-21).
+NICP_HD void projector_state(int model, const float *K, const float *transform, float *out) {
+  if (model == 1) {
+    float t[16];
+    for (int i = 0; i < 16; ++i) t[i] = transform[i];
+    set_row3(t);
+    iso_inverse(t, out);
+  } else {
+    float iKRt[16];
+    projector_matrices(K, transform, out, iKRt);
+  }
+}
+
 // quat2mat / mat2quat / v2t / t2v (nicp/nicp/bm_se3.h:10-51); mat2quat restates
 // Eigen's Quaternion(const Matrix3&) (Shepperd's method), normalize, w >= 0.
 template <typename S>

@@ -33,7 +33,9 @@ def lib():
 
 
 class Camera(C.Structure):
-    _fields_ = [("K", C.c_float * 9), ("min_distance", C.c_float), ("max_distance", C.c_float)]
+    _fields_ = [("K", C.c_float * 9), ("min_distance", C.c_float), ("max_distance", C.c_float),
+                ("model", C.c_int), ("horizontal_fov", C.c_float), ("vertical_fov", C.c_float),
+                ("horizontal_center", C.c_float), ("vertical_center", C.c_float)]
 
 
 class StatsParams(C.Structure):
@@ -75,6 +77,20 @@ def camera(fx, fy, cx, cy, min_d=0.5, max_d=4.5):
     c.K[:] = K.flatten(order="F").tolist()
     c.min_distance = min_d
     c.max_distance = max_d
+    return c
+
+
+def spherical_camera(rows, cols, horizontal_fov=float(np.pi), vertical_fov=float(np.pi / 4), min_d=0.01, max_d=6.0,
+                     horizontal_center=None, vertical_center=None):
+    """SphericalPointProjector: half fields of view (defaults pi, pi/4, sphericalpointprojector.cpp:9-16)
+    and the image centre (default the middle of the image)."""
+    c = Camera()
+    c.K[:] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    c.min_distance, c.max_distance = min_d, max_d
+    c.model = 1
+    c.horizontal_fov, c.vertical_fov = horizontal_fov, vertical_fov
+    c.horizontal_center = cols / 2.0 if horizontal_center is None else horizontal_center
+    c.vertical_center = rows / 2.0 if vertical_center is None else vertical_center
     return c
 
 

@@ -681,6 +681,117 @@ void orc_eigen3_direct(const float *cov, float *evals, float *evecs) { eig3_dire
 /* A5: DepthImageConverterIntegralImage::compute, N/depthimageconverterintegralimage.cpp:18-64,
  * with A8 (N/pointaccumulator.h:66-110), A9 (N/statscalculatorintegralimage.cpp:20-82),
  * A10 (N/stats.h:93-98), A11 (N/informationmatrixcalculator.cpp:9-57). Sensor offset identity. */
+/* ---------------- SphericalPointProjector ----------------
+ * N/sphericalpointprojector.cpp:19-27 (_updateParameters) at the image size of the call,
+ * N/sphericalpointprojector.h:252-303 (_project, _unProject, _projectInterval). The float
+ * overloads of sin, cos, tanf, atan2 and sqrt are the reference's calls. Quirks kept: the field
+ * of view test of _project is fabs(theta > fov), so angles below -fov pass; pixels are
+ * truncated toward zero. */
+struct SphRes {
+  float hres, ihres, vres, ivres;
+};
+static SphRes sph_res(const orc_camera *cam, int rows, int cols) {
+  SphRes s;
+  s.hres = (float)cols / (2.0f * cam->horizontal_fov);
+  s.ihres = 1.0f / s.hres;
+  s.vres = (float)rows / (2.0f * cam->vertical_fov);
+  s.ivres = 1.0f / s.vres;
+  return s;
+}
+
+/* _project: p seen through iT (P3), returns false where the reference does */
+static bool sph_project1(const orc_camera *cam, const SphRes &s, const float *iT, const float *p, int &x, int &y,
+                         float &d) {
+  float cp[4];
+  iso_mul4(iT, p, cp);
+  const float theta = std::atan2(cp[0], cp[2]);
+  d = std::sqrt(cp[0] * cp[0] + cp[2] * cp[2]);
+  const float phi = std::atan2(cp[1], d);
+  if (d > cam->max_distance || d < cam->min_distance) return false;
+  if (std::fabs((float)(theta > cam->horizontal_fov))) return false;
+  if (std::fabs((float)(phi > cam->vertical_fov))) return false;
+  x = (int)(s.hres * theta + cam->horizontal_center);
+  y = (int)(s.vres * phi + cam->vertical_center);
+  return true;
+}
+
+int orc_spherical_unproject(const float *depth, int rows, int cols, const orc_camera *cam, const float *T,
+                            float *points, int *index_image) {
+  const SphRes s = sph_res(cam, rows, cols);
+  int count = 0;
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < cols; ++c) {
+      const float d = depth[r * cols + c];
+      index_image[r * cols + c] = -1;
+      if (d < cam->min_distance || d > cam->max_distance) continue;
+      const float theta = s.ihres * (c - cam->horizontal_center);
+      const float phi = s.ivres * (r - cam->vertical_center);
+      if (std::fabs(theta) > cam->horizontal_fov) continue;
+      if (std::fabs(phi) > cam->vertical_fov) continue;
+      const float x = std::sin(theta) * d;
+      const float z = std::cos(theta) * d;
+      const float y = d * tanf(phi);
+      float *o = points + 4 * (size_t)count;
+      for (int i = 0; i < 3; ++i) o[i] = mc(T, i, 3) + red3(mc(T, i, 0) * x, mc(T, i, 1) * y, mc(T, i, 2) * z);
+      o[3] = 1.0f;
+      index_image[r * cols + c] = count++;
+    }
+  return count;
+}
+
+void orc_spherical_intervals(const float *depth, int rows, int cols, const orc_camera *cam, float world_radius,
+                             int *intervals) {
+  const SphRes s = sph_res(cam, rows, cols);
+  for (int i = 0; i < rows * cols; ++i) {
+    const float d = depth[i];
+    if (d > cam->max_distance || d < cam->min_distance) {
+      intervals[2 * i] = intervals[2 * i + 1] = -1;
+      continue;
+    }
+    const float theta = std::atan2(world_radius, d);
+    const float phi = std::atan2(world_radius, d);
+    intervals[2 * i] = (int)(s.hres * theta);
+    intervals[2 * i + 1] = (int)(s.vres * phi);
+  }
+}
+
+void orc_spherical_project(const float *points, int n, const float *transform, const orc_camera *cam, int rows,
+                           int cols, int *index_image, float *depth_image) {
+  const SphRes s = sph_res(cam, rows, cols);
+  float t[16], iT[16];
+  std::memcpy(t, transform, sizeof(t));
+  mr(t, 3, 0) = 0.0f; mr(t, 3, 1) = 0.0f; mr(t, 3, 2) = 0.0f; mr(t, 3, 3) = 1.0f;
+  iso_inverse(t, iT);
+  const size_t P = (size_t)rows * cols;
+  std::vector<float> dimg(P, std::numeric_limits<float>::max());
+  for (size_t i = 0; i < P; ++i) index_image[i] = -1;
+  for (int i = 0; i < n; ++i) {
+    int x, y;
+    float d;
+    if (!sph_project1(cam, s, iT, points + 4 * (size_t)i, x, y, d) || d < cam->min_distance ||
+        d > cam->max_distance || x < 0 || x >= cols || y < 0 || y >= rows)
+      continue;
+    float &other = dimg[(size_t)y * cols + x];
+    if (!other || other > d) {
+      other = d;
+      index_image[(size_t)y * cols + x] = i;
+    }
+  }
+  if (depth_image) std::memcpy(depth_image, dimg.data(), P * sizeof(float));
+}
+
+/* the projection of a cloud from `transform` by either projector */
+static void project_cam(const orc_camera *cam, const float *transform, const float *points, int n, int rows, int cols,
+                        int *index_image, float *depth_image) {
+  if (cam->model == 1) {
+    orc_spherical_project(points, n, transform, cam, rows, cols, index_image, depth_image);
+    return;
+  }
+  float KRt[16], iKRt[16];
+  orc_projector_matrices(cam->K, transform, KRt, iKRt);
+  orc_project(points, n, KRt, rows, cols, cam->min_distance, cam->max_distance, index_image, depth_image);
+}
+
 int orc_frame_from_depth(const uint16_t *raw, int rows, int cols, float depth_scale, const orc_camera *cam,
                          const orc_stats_params *sp, int *index_image, float *points, float *normals,
                          float *curvature, float *eigvecs, float *eigvals, float *means, int *npts,
@@ -691,10 +802,19 @@ int orc_frame_from_depth(const uint16_t *raw, int rows, int cols, float depth_sc
   orc_depth_u16_to_f32(raw, (int)P, depth_scale, depth.data());
   float I4[16], KRt[16], iKRt[16];
   set_identity4(I4);
-  orc_projector_matrices(cam->K, I4, KRt, iKRt);
-  int n = orc_unproject(depth.data(), rows, cols, iKRt, cam->min_distance, cam->max_distance, pts.data(), idx.data());
-  orc_project_intervals(depth.data(), rows, cols, cam->K, cam->min_distance, cam->max_distance, sp->world_radius,
-                        iv.data());
+  int n;
+  /* the spherical projector suppresses the image-radius limits of the stats
+   * (N/depthimageconverterintegralimage.cpp:48-52) */
+  const bool suppress = cam->model == 1;
+  if (suppress) {
+    n = orc_spherical_unproject(depth.data(), rows, cols, cam, I4, pts.data(), idx.data());
+    orc_spherical_intervals(depth.data(), rows, cols, cam, sp->world_radius, iv.data());
+  } else {
+    orc_projector_matrices(cam->K, I4, KRt, iKRt);
+    n = orc_unproject(depth.data(), rows, cols, iKRt, cam->min_distance, cam->max_distance, pts.data(), idx.data());
+    orc_project_intervals(depth.data(), rows, cols, cam->K, cam->min_distance, cam->max_distance, sp->world_radius,
+                          iv.data());
+  }
   std::vector<float> ii(P * 10);
   orc_integral_image(idx.data(), rows, cols, pts.data(), ii.data());
 
@@ -713,9 +833,11 @@ int orc_frame_from_depth(const uint16_t *raw, int rows, int cols, float depth_sc
       int index = idx[pix];
       int R0 = iv[2 * pix], R1 = iv[2 * pix + 1];
       if (index < 0 || R0 < 0 || R1 < 0) continue;
-      if (R0 < sp->min_image_radius || R1 < sp->min_image_radius) continue;
-      if (R0 > sp->max_image_radius) R0 = sp->max_image_radius;
-      if (R1 > sp->max_image_radius) R1 = sp->max_image_radius;
+      if (!suppress) {
+        if (R0 < sp->min_image_radius || R1 < sp->min_image_radius) continue;
+        if (R0 > sp->max_image_radius) R0 = sp->max_image_radius;
+        if (R1 > sp->max_image_radius) R1 = sp->max_image_radius;
+      }
       float acc[10];
       orc_get_region(ii.data(), rows, cols, c - R0, c + R0, r - R1, r + R1, acc);
       int accn = (int)acc[0];
@@ -809,8 +931,12 @@ void orc_stats_f64_bruteforce(const uint16_t *raw, int rows, int cols, float dep
   std::vector<float> depth(P);
   std::vector<int> iv(2 * P);
   orc_depth_u16_to_f32(raw, (int)P, depth_scale, depth.data());
-  orc_project_intervals(depth.data(), rows, cols, cam->K, cam->min_distance, cam->max_distance, sp->world_radius,
-                        iv.data());
+  const bool suppress = cam->model == 1;
+  if (suppress)
+    orc_spherical_intervals(depth.data(), rows, cols, cam, sp->world_radius, iv.data());
+  else
+    orc_project_intervals(depth.data(), rows, cols, cam->K, cam->min_distance, cam->max_distance, sp->world_radius,
+                          iv.data());
   for (int i = 0; i < n; ++i) {
     valid[i] = 0;
     curvature64[i] = 0.0;
@@ -824,9 +950,11 @@ void orc_stats_f64_bruteforce(const uint16_t *raw, int rows, int cols, float dep
       int index = index_image[pix];
       int R0 = iv[2 * pix], R1 = iv[2 * pix + 1];
       if (index < 0 || R0 < 0 || R1 < 0) continue;
-      if (R0 < sp->min_image_radius || R1 < sp->min_image_radius) continue;
-      if (R0 > sp->max_image_radius) R0 = sp->max_image_radius;
-      if (R1 > sp->max_image_radius) R1 = sp->max_image_radius;
+      if (!suppress) {
+        if (R0 < sp->min_image_radius || R1 < sp->min_image_radius) continue;
+        if (R0 > sp->max_image_radius) R0 = sp->max_image_radius;
+        if (R1 > sp->max_image_radius) R1 = sp->max_image_radius;
+      }
       int xmin = clampi(c - R0 - 1, 0, cols - 1), xmax = clampi(c + R0 - 1, 0, cols - 1);
       int ymin = clampi(r - R1 - 1, 0, rows - 1), ymax = clampi(r + R1 - 1, 0, rows - 1);
       double acc[10];
@@ -1365,10 +1493,9 @@ void orc_align_mode(const orc_cloud *ref, const orc_cloud *cur, int rows, int co
                     orc_align_result *res) {
   const size_t P = (size_t)rows * cols;
   std::vector<int> cur_index(P), ref_index(P), pairs(2 * P);
-  float I4[16], KRt[16], iKRt[16];
+  float I4[16];
   set_identity4(I4);
-  orc_projector_matrices(cam->K, I4, KRt, iKRt);
-  orc_project(cur->points, cur->n, KRt, rows, cols, cam->min_distance, cam->max_distance, cur_index.data(), nullptr);
+  project_cam(cam, I4, cur->points, cur->n, rows, cols, cur_index.data(), nullptr);
   float T[16];
   std::memcpy(T, init_guess, sizeof(T));
   float H[36] = {0}, b[6] = {0}, err = 0.0f;
@@ -1377,8 +1504,7 @@ void orc_align_mode(const orc_cloud *ref, const orc_cloud *cur, int rows, int co
     mr(T, 3, 0) = 0.0f; mr(T, 3, 1) = 0.0f; mr(T, 3, 2) = 0.0f; mr(T, 3, 3) = 1.0f;
     float TR[16];
     iso_mul(T, I4, TR);
-    orc_projector_matrices(cam->K, TR, KRt, iKRt);
-    orc_project(ref->points, ref->n, KRt, rows, cols, cam->min_distance, cam->max_distance, ref_index.data(), nullptr);
+    project_cam(cam, TR, ref->points, ref->n, rows, cols, ref_index.data(), nullptr);
     float Ti[16];
     iso_inverse(T, Ti);
     nc = orc_correspondences(ref, cur, ref_index.data(), cur_index.data(), rows, cols, Ti, ap, num_threads, pairs.data());
@@ -1564,17 +1690,29 @@ int orc_merge(int n, float *points, float *normals, float *curvature, float *sta
   orc_projector_matrices(cam->K, T, KRt, iKRt);
   std::vector<int> idx((size_t)rows * cols);
   std::vector<float> dep((size_t)rows * cols);
-  orc_project(points, n, KRt, rows, cols, cam->min_distance, cam->max_distance, idx.data(), dep.data());
+  project_cam(cam, T, points, n, rows, cols, idx.data(), dep.data());
+  const SphRes sr = sph_res(cam, rows, cols);
+  float iT[16];
+  iso_inverse(T, iT);
   std::vector<int> collapsed((size_t)n, -1);
   for (int i = 0; i < n; ++i) {
-    float ip[4];
-    mul4_packet(KRt, points + 4 * (size_t)i, ip);
-    const float depth = ip[2];
+    float depth;
     int r = -1, c = -1;
-    if (!(depth < cam->min_distance || depth > cam->max_distance)) {
-      const float inv = 1.0f / depth;
-      c = (int)std::round(ip[0] * inv);
-      r = (int)std::round(ip[1] * inv);
+    if (cam->model == 1) {
+      int x, y;
+      if (sph_project1(cam, sr, iT, points + 4 * (size_t)i, x, y, depth)) {
+        c = x;
+        r = y;
+      }
+    } else {
+      float ip[4];
+      mul4_packet(KRt, points + 4 * (size_t)i, ip);
+      depth = ip[2];
+      if (!(depth < cam->min_distance || depth > cam->max_distance)) {
+        const float inv = 1.0f / depth;
+        c = (int)std::round(ip[0] * inv);
+        r = (int)std::round(ip[1] * inv);
+      }
     }
     if (depth < 0 || depth > max_point_depth || r < 0 || r >= rows || c < 0 || c >= cols) continue;
     const float targetZ = dep[(size_t)r * cols + c];

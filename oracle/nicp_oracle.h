@@ -24,6 +24,11 @@ typedef struct {
   float K[9];              /* camera matrix, column-major */
   float min_distance;      /* N/pointprojector.cpp:9-10 */
   float max_distance;
+  int   model;             /* 0 PinholePointProjector, 1 SphericalPointProjector */
+  float horizontal_fov;    /* spherical: half fields of view, image centre (N/sphericalpointprojector.h:304-313) */
+  float vertical_fov;
+  float horizontal_center;
+  float vertical_center;
 } orc_camera;
 
 typedef struct {
@@ -160,6 +165,16 @@ void orc_align_nn(const orc_cloud *ref, const orc_cloud *cur, const float init_g
 /* A20 only: statistics of a given H at a given T (FP64). */
 void orc_compute_statistics(const double H[36], const float T[16], const orc_align_params *ap,
                             orc_align_result *res);
+
+/* SphericalPointProjector (N/sphericalpointprojector.h:252-313, .cpp:19-141) at an image size:
+ * unProject with the projector transform T (the converter passes the identity), the interval
+ * image, and the z-buffer projection seen from `transform` (its _iT = transform^-1). */
+int orc_spherical_unproject(const float *depth, int rows, int cols, const orc_camera *cam, const float T[16],
+                            float *points, int *index_image);
+void orc_spherical_intervals(const float *depth, int rows, int cols, const orc_camera *cam, float world_radius,
+                             int *intervals);
+void orc_spherical_project(const float *points, int n, const float transform[16], const orc_camera *cam, int rows,
+                           int cols, int *index_image, float *depth_image);
 
 /* Gaussian3f (N/gaussian.h:8-133) with its lazy-evaluation state: flags bit 0 = mean and
  * covariance current, bit 1 = information form current. Same layout as nicp_gaussian. */

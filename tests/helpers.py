@@ -26,3 +26,43 @@ def identity16():
 
 def angle_dist(T16, U16):
     return synth.pose_error(T16, U16)
+
+
+def spherical_room(rows, cols, cam, pose=None, seed=0, noise=0.002, holes=0.05):
+    """A panoramic depth image (uint16 mm) of a box room with a few box obstacles, seen by the
+    spherical camera `cam` from `pose` (4x4 column-major camera-to-room transform, default the
+    identity). A pixel (r, c) looks along (sin t, tan f, cos t) with t = (c - hc) / hres and
+    f = (r - vc) / vres, so the depth is the distance from the camera's vertical axis."""
+    hres = cols / (2.0 * cam.horizontal_fov)
+    vres = rows / (2.0 * cam.vertical_fov)
+    cc, rr = np.meshgrid(np.arange(cols, dtype=np.float64), np.arange(rows, dtype=np.float64))
+    t = (cc - cam.horizontal_center) / hres
+    f = (rr - cam.vertical_center) / vres
+    v = np.stack([np.sin(t), np.tan(f), np.cos(t)], -1)
+    P = np.eye(4) if pose is None else np.asarray(pose, np.float64).reshape(4, 4, order="F")
+    R, o = P[:3, :3], P[:3, 3]
+    w = v @ R.T  # ray directions in the room; depth d scales v
+    lo, hi = np.array([-4.0, -1.5, -5.0]), np.array([5.0, 1.2, 4.5])
+    boxes = [(np.array([1.0, -1.5, 2.0]), np.array([2.0, 0.2, 3.0])),
+             (np.array([-3.0, -1.5, -2.5]), np.array([-2.2, -0.4, -1.0])),
+             (np.array([-1.0, 0.6, -4.0]), np.array([0.5, 1.2, -3.2])),
+             (np.array([2.5, -1.5, -3.5]), np.array([3.5, -0.9, -2.0]))]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tr = np.where(w > 0, (hi - o) / w, np.where(w < 0, (lo - o) / w, np.inf))
+        d = tr.min(-1)
+        for bl, bh in boxes:
+            t1, t2 = (bl - o) / w, (bh - o) / w
+            tn = np.nanmax(np.minimum(t1, t2), -1)
+            tf = np.nanmin(np.maximum(t1, t2), -1)
+            hit = (tn <= tf) & (tn > 0)
+            d = np.where(hit & (tn < d), tn, d)
+    rng = np.random.default_rng(seed)
+    d = d * (1.0 + noise * rng.standard_normal(d.shape))
+    d[rng.uniform(size=d.shape) < holes] = 0
+    d[~np.isfinite(d)] = 0
+    return np.clip(np.round(d * 1000.0), 0, 65535).astype(np.uint16)
+
+
+def spherical_cams(rows, cols, hfov=np.pi, vfov=np.pi / 4, min_d=0.5, max_d=10.0):
+    return (O.spherical_camera(rows, cols, hfov, vfov, min_d, max_d),
+            capi.spherical_camera(rows, cols, hfov, vfov, min_d, max_d))
