@@ -150,7 +150,9 @@ def uneven_batch(ctx, n_pairs, iters, steps):
     ref = np.arange(0, 2 * n_pairs, 2, dtype=np.int32)
     cur = ref + 1
     batch.convert()
-    batch.align(ref, cur, None, ap, use_graph=True, want_results=False)  # warm-up + graph capture
+    batch.set_autotune(True)
+    batch.align(ref, cur, None, ap, use_graph=True, want_results=False)  # tuning call (eager)
+    batch.align(ref, cur, None, ap, use_graph=True, want_results=False)  # graph capture
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -160,8 +162,10 @@ def uneven_batch(ctx, n_pairs, iters, steps):
     dt = (time.perf_counter() - t0) / steps
     ok, _ = aligned(Tout, T)
     px = sum(r * c for r, c in shapes)
+    tuned = batch.autotune_state()
     batch.close()
     return {"pairs": n_pairs, "iterations": iters, "ms_per_step": round(1000 * dt, 3),
+            "zproject_strategy": [c for c, _, _ in tuned],
             "pairs_per_s": round(n_pairs / dt, 1), "aligned_fraction": round(float(ok.mean()), 4),
             "depth_pixels_per_step": px,
             "pairs_per_resolution": {"%dx%d" % (c, r): int((pick == k).sum()) for k, (r, c) in enumerate(res)},
@@ -227,6 +231,7 @@ def main():
     net.set_input_mode(input_mode)
     ap = capi.align_params(outer_iterations=args.iters)
     batch = initnet.make_batch(ctx, B)
+    batch.set_autotune(True)  # the projection strategy is picked during the warm-up
     ref = np.arange(0, 2 * B, 2, dtype=np.int32)
     cur = ref + 1
 
@@ -269,6 +274,7 @@ def main():
         used_total += int(used.sum())
         Ts.append(T)
         Tgts.append(Tgt)
+    tuned_main = batch.autotune_state()
     prof, prof_error = {}, None
     if not args.no_profile:
         try:
@@ -419,6 +425,8 @@ def main():
         "roofline": roofline,
         "net_roofline": net_roofline,
         "kernels": kernels,
+        "autotune": [{"zproject_strategy": c, "ms_strategy0": round(m0, 4), "ms_strategy1": round(m1, 4)}
+                     for c, m0, m1 in tuned_main],
         "kernel_timing_error": prof_error,
         "cpu_baseline": cpu,
     }

@@ -256,6 +256,16 @@ int nicp_batch_align(nicp_batch *batch, int num_pairs, const int *ref_frame, con
                      nicp_align_result *results);
 /* number of points of a batch frame, and a cloud view of it (downloads only) */
 int nicp_batch_frame_cloud(nicp_batch *batch, int frame, nicp_cloud **out);
+/* Autotuner (no reference counterpart): with enable, the first nicp_batch_align call of each plan
+ * shape (pairs, image tiles, cloud tiles, subsampling, statistics) runs eagerly and projects the
+ * reference clouds with two strategies on alternate iterations -- every wave-merged atomicMin
+ * issued (0), or the z-buffer cell read first and the atomic skipped when it cannot lower the
+ * cell (1); both give the same z-buffer -- and later calls of the shape use the faster. Enabling
+ * or disabling forgets earlier choices. */
+int nicp_batch_set_autotune(nicp_batch *batch, int enable);
+/* the tuned shapes: choice per shape, ms = 2 per shape (average ms per launch of strategies 0 and 1,
+ * -1 where unmeasured) */
+int nicp_batch_autotune_read(nicp_batch *batch, int capacity, int *num_shapes, int *choice, float *ms);
 
 /* ---- Synthetic scenes (no reference counterpart: the training-set and benchmark input
  * generator). Renders the seeded 20-primitive scene scene_seeds[i] into the depth of batch

@@ -78,6 +78,7 @@ EXPORTS = [
     "nicp_batch_align_learned", "nicp_device_alloc", "nicp_device_free", "nicp_device_copy",
     "nicp_initnet_train_step_rows", "nicp_initnet_set_input_mode", "nicp_cloud_gaussians", "nicp_cloud_set_gaussians",
     "nicp_cloud_download_gaussians", "nicp_cloud_transform", "nicp_cloud_add", "nicp_merge",
+    "nicp_batch_set_autotune", "nicp_batch_autotune_read",
 ]
 
 _lib = None
@@ -138,6 +139,7 @@ def lib():
             "nicp_cloud_download_gaussians": [vp, vp], "nicp_cloud_transform": [vp, vp],
             "nicp_cloud_add": [vp, vp, vp],
             "nicp_merge": [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp],
+            "nicp_batch_set_autotune": [vp, C.c_int], "nicp_batch_autotune_read": [vp, C.c_int, vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -483,6 +485,20 @@ class Batch:
         res = (AlignResult * n)() if want_results else None
         _check(lib().nicp_batch_align(self.h, n, _p(ref), _p(cur), _p(init), C.byref(ap), int(use_graph), _p(T), res))
         return T, res
+
+    def set_autotune(self, enable=True):
+        """Times the projection strategies on each new plan shape's first align call (eagerly) and
+        uses the faster one afterwards."""
+        _check(lib().nicp_batch_set_autotune(self.h, int(enable)))
+
+    def autotune_state(self):
+        """[(choice, ms strategy 0, ms strategy 1)] per tuned shape."""
+        n = C.c_int()
+        _check(lib().nicp_batch_autotune_read(self.h, 0, C.byref(n), None, None))
+        ch = np.empty(max(n.value, 1), np.int32)
+        ms = np.empty(2 * max(n.value, 1), np.float32)
+        _check(lib().nicp_batch_autotune_read(self.h, n.value, C.byref(n), _p(ch), _p(ms)))
+        return [(int(ch[i]), float(ms[2 * i]), float(ms[2 * i + 1])) for i in range(n.value)]
 
     def frame_cloud(self, frame, shape):
         h = C.c_void_p()

@@ -270,6 +270,12 @@ struct nicp_batch {
   GraphCache graph;
   GraphCache conv_graph;
   DevBuf synth_buf;  // render job tables of nicp_batch_render_synthetic
+  // autotuner of the reference projection (nicp_batch_set_autotune): per plan shape, the
+  // strategy of k_zproject chosen from the timed iterations of the shape's first call
+  int autotune = 0;
+  std::vector<std::vector<int>> tuned_keys;
+  std::vector<int> tuned_choice;
+  std::vector<float> tuned_ms;  // 2 per shape: average ms of strategies 0 and 1 (-1 unmeasured)
 };
 
 // ---------------------------------------------------------------------------------------
@@ -576,9 +582,22 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
 }
 
 // the launch sequence of AlignerProjective::align for all pairs of a plan
+// Timing of the reference projections of a tuning call: iteration i uses strategy i % 2 and
+// records its launch between two events.
+struct ZTune {
+  std::vector<hipEvent_t> ev;  // 2 per outer iteration
+  std::vector<int> strategy;
+  void release() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+    ev.clear();
+    strategy.clear();
+  }
+};
+
 int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d_pairs, const FrameDev *d_frames,
                  const int *d_blk_pair, const int *d_blk_ref, const int *d_blk_cur, const nicp_align_params *ap,
-                 bool with_stats = true) {
+                 bool with_stats = true, int zstrategy = 0, ZTune *tune = nullptr) {
   const int np = (int)plan.pairs.size();
   const AlignParamsDev a = to_dev(ap);
   const dim3 tb(kThreads);
@@ -588,16 +607,27 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
   (void)hipGetLastError();
   LAUNCH(pf, KID_ZFILL, k_zfill, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair);
   if (plan.total_ctiles) {
-    if (sph) LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject_sph, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1);
-    else LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1);
+    if (sph) LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject_sph, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1, zstrategy);
+    else LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1, zstrategy);
     LAUNCH(pf, KID_ZRESOLVE, k_zresolve, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair, (float *)nullptr);
   }
   const int inner = ap->inner_iterations;
   for (int it = 0; it < ap->outer_iterations; ++it) {
     const bool last = it == ap->outer_iterations - 1;
     if (plan.total_rtiles) {
-      if (sph) LAUNCH(pf, KID_ZPROJECT_REF, k_zproject_sph, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0);
-      else LAUNCH(pf, KID_ZPROJECT_REF, k_zproject, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0);
+      const int zs = tune ? (it & 1) : zstrategy;
+      if (tune) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPCHK(hipEventCreate(&e0));
+        tune->ev.push_back(e0);
+        HIPCHK(hipEventCreate(&e1));
+        tune->ev.push_back(e1);
+        tune->strategy.push_back(zs);
+        HIPCHK(hipEventRecord(e0, s));
+      }
+      if (sph) LAUNCH(pf, KID_ZPROJECT_REF, k_zproject_sph, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0, zs);
+      else LAUNCH(pf, KID_ZPROJECT_REF, k_zproject, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0, zs);
+      if (tune) HIPCHK(hipEventRecord(tune->ev.back(), s));
     }
     LAUNCH(pf, KID_CORR_LIN, k_corr_lin, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
            (int)(last || inner > 1));
@@ -1219,9 +1249,9 @@ int project_device(nicp_ctx *ctx, const nicp_cloud *cloud, const float transform
   hipLaunchKernelGGL(k_zfill, dim3(pd.n_tiles), tb, 0, ctx->stream, d_pairs, d_bp);
   if (plan.total_ctiles) {
     if (cam->model == NICP_SPHERICAL)
-      hipLaunchKernelGGL(k_zproject_sph, dim3(plan.total_ctiles), tb, 0, ctx->stream, d_pairs, d_frames, d_bc, 1);
+      hipLaunchKernelGGL(k_zproject_sph, dim3(plan.total_ctiles), tb, 0, ctx->stream, d_pairs, d_frames, d_bc, 1, 0);
     else
-      hipLaunchKernelGGL(k_zproject, dim3(plan.total_ctiles), tb, 0, ctx->stream, d_pairs, d_frames, d_bc, 1);
+      hipLaunchKernelGGL(k_zproject, dim3(plan.total_ctiles), tb, 0, ctx->stream, d_pairs, d_frames, d_bc, 1, 0);
   }
   hipLaunchKernelGGL(k_zresolve, dim3(pd.n_tiles), tb, 0, ctx->stream, d_pairs, d_bp, ctx->depth_tmp.as<float>());
   HIPCHK(hipGetLastError());
@@ -1587,8 +1617,55 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
   int *d_bp, *d_br, *d_bc;
   PairDev *d_pairs;
   CHK(prepare_align(ctx->stream, plan, B, init_guesses16, &d_bp, &d_br, &d_bc, &d_pairs));
+  // autotuner: the first call of a shape times both projection strategies on alternate
+  // iterations (eagerly), later calls use the faster one
+  int zstrategy = 0;
+  bool tuning = false;
+  size_t shape = 0;
+  if (b->autotune) {
+    const std::vector<int> skey = {num_pairs, plan.total_pix_tiles, plan.total_rtiles, plan.total_ctiles, sub,
+                                   with_stats ? 1 : 0};
+    while (shape < b->tuned_keys.size() && b->tuned_keys[shape] != skey) ++shape;
+    if (shape == b->tuned_keys.size()) {
+      tuning = ap->outer_iterations >= 2;
+      if (!tuning) {  // nothing to compare: keep the default
+        b->tuned_keys.push_back(skey);
+        b->tuned_choice.push_back(0);
+        b->tuned_ms.push_back(-1.0f);
+        b->tuned_ms.push_back(-1.0f);
+      }
+    } else {
+      zstrategy = b->tuned_choice[shape];
+    }
+  }
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  if (use_graph) {
+  if (tuning) {
+    ZTune tune;
+    ctx->prof.sink = &ctx->prof.eager;
+    int rc = launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats, 0,
+                          &tune);
+    hipError_t e = rc == NICP_OK ? hipStreamSynchronize(ctx->stream) : hipSuccess;
+    double sum[2] = {0.0, 0.0};
+    int cnt[2] = {0, 0};
+    for (size_t i = 0; rc == NICP_OK && e == hipSuccess && i < tune.strategy.size(); ++i) {
+      float ms = 0.0f;
+      e = hipEventElapsedTime(&ms, tune.ev[2 * i], tune.ev[2 * i + 1]);
+      // the first iteration also warms caches and code: it does not vote
+      if (i > 0) {
+        sum[tune.strategy[i]] += ms;
+        ++cnt[tune.strategy[i]];
+      }
+    }
+    tune.release();
+    if (rc != NICP_OK) return rc;
+    if (e != hipSuccess) return set_err(NICP_ERR_HIP, "autotune timing: %s", hipGetErrorString(e));
+    const float m0 = cnt[0] ? (float)(sum[0] / cnt[0]) : -1.0f, m1 = cnt[1] ? (float)(sum[1] / cnt[1]) : -1.0f;
+    b->tuned_keys.push_back({num_pairs, plan.total_pix_tiles, plan.total_rtiles, plan.total_ctiles, sub,
+                             with_stats ? 1 : 0});
+    b->tuned_choice.push_back(m0 >= 0.0f && m1 >= 0.0f && m1 < m0 ? 1 : 0);
+    b->tuned_ms.push_back(m0);
+    b->tuned_ms.push_back(m1);
+  } else if (use_graph) {
     // graph key: the launch geometry and the parameters baked into the kernel arguments
     std::vector<int> key = {num_pairs, plan.total_pix_tiles, plan.total_rtiles, plan.total_ctiles};
     const int *apw = (const int *)ap;
@@ -1599,12 +1676,14 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
     key.push_back(ctx->prof.on ? 1 : 0);
     key.push_back(with_stats ? 1 : 0);
     key.push_back(sub);
+    key.push_back(zstrategy);
     if (b->graph.exec == nullptr || b->graph.key != key) {
       b->graph.reset();
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       ctx->prof.sink = &b->graph.events;
-      int rc = launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats);
+      int rc = launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats,
+                            zstrategy);
       ctx->prof.sink = &ctx->prof.eager;
       hipError_t e = hipStreamEndCapture(ctx->stream, &g);
       if (rc != NICP_OK) {
@@ -1625,7 +1704,7 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
     ctx->pending_graph = ctx->prof.on ? &b->graph.events : nullptr;
   } else {
     ctx->prof.sink = &ctx->prof.eager;
-    CHK(launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats));
+    CHK(launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats, zstrategy));
   }
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   if (!T_out16 && !results && !raw) return NICP_OK;
@@ -1660,6 +1739,29 @@ extern "C" int nicp_batch_align(nicp_batch *b, int num_pairs, const int *ref_fra
                                 nicp_align_result *results) {
   return batch_align(b, num_pairs, ref_frame, cur_frame, init_guesses16, ap, use_graph, T_out16, results, true,
                      nullptr);
+}
+
+extern "C" int nicp_batch_set_autotune(nicp_batch *b, int enable) {
+  if (!b) return set_err(NICP_ERR_INVALID, "batch is null");
+  b->autotune = enable != 0;
+  b->tuned_keys.clear();
+  b->tuned_choice.clear();
+  b->tuned_ms.clear();
+  return NICP_OK;
+}
+
+extern "C" int nicp_batch_autotune_read(nicp_batch *b, int capacity, int *num_shapes, int *choice, float *ms) {
+  if (!b || !num_shapes) return set_err(NICP_ERR_INVALID, "null argument");
+  const int n = (int)b->tuned_choice.size();
+  *num_shapes = n;
+  for (int i = 0; i < n && i < capacity; ++i) {
+    if (choice) choice[i] = b->tuned_choice[i];
+    if (ms) {
+      ms[2 * i] = b->tuned_ms[2 * i];
+      ms[2 * i + 1] = b->tuned_ms[2 * i + 1];
+    }
+  }
+  return NICP_OK;
 }
 
 extern "C" int nicp_batch_frame_cloud(nicp_batch *b, int frame, nicp_cloud **out) {

@@ -631,9 +631,12 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_zfill(const PairDev *__
 }
 
 // which = 0: the reference cloud with the pair's current KRt; 1: the current cloud
+// strategy 0: every merged atomic is issued; 1: the cell is read first and the atomic skipped
+// when the stored key is already no larger (the read may be stale, but only ever larger than
+// the cell's current value, so a skip never loses a smaller key). The z-buffer is the same.
 template <int M>
 __device__ __forceinline__ void zproject_body(const PairDev *__restrict__ Pp, const FrameDev *__restrict__ F,
-                                              const int *__restrict__ blk_pair, int which) {
+                                              const int *__restrict__ blk_pair, int which, int strategy) {
   const int g = blockIdx.x;
   const PairDev &pd = Pp[blk_pair[g]];
   const FrameDev &fd = F[which ? pd.cur : pd.ref];
@@ -689,19 +692,24 @@ __device__ __forceinline__ void zproject_body(const PairDev *__restrict__ Pp, co
       const unsigned long long other = ((unsigned long long)hi << 32) | lo;
       m = other < m ? other : m;
     }
-    if (cell >= 0 && !same) atomicMin(pd.zbuf + cell, key);
-    else if (lane == lead) atomicMin(pd.zbuf + cell, m);
+    if (cell >= 0 && !same) {
+      if (strategy == 0 || pd.zbuf[cell] > key) atomicMin(pd.zbuf + cell, key);
+    } else if (lane == lead) {
+      if (strategy == 0 || pd.zbuf[cell] > m) atomicMin(pd.zbuf + cell, m);
+    }
   }
 }
 extern "C" __global__ __launch_bounds__(kThreads) void k_zproject(const PairDev *__restrict__ Pp,
                                                                  const FrameDev *__restrict__ F,
-                                                                 const int *__restrict__ blk_pair, int which) {
-  zproject_body<kPinhole>(Pp, F, blk_pair, which);
+                                                                 const int *__restrict__ blk_pair, int which,
+                                                                 int strategy) {
+  zproject_body<kPinhole>(Pp, F, blk_pair, which, strategy);
 }
 extern "C" __global__ __launch_bounds__(kThreads) void k_zproject_sph(const PairDev *__restrict__ Pp,
                                                                      const FrameDev *__restrict__ F,
-                                                                     const int *__restrict__ blk_pair, int which) {
-  zproject_body<kSpherical>(Pp, F, blk_pair, which);
+                                                                     const int *__restrict__ blk_pair, int which,
+                                                                     int strategy) {
+  zproject_body<kSpherical>(Pp, F, blk_pair, which, strategy);
 }
 
 // z-buffer -> index image (and depth image); resets the z-buffer. Pairs that use the
@@ -817,6 +825,15 @@ __device__ void block_partial(Acc2 *acc, double *out) {
   }
 }
 
+// Workgroups are dispatched to the eight XCDs in turn. Block b takes tile
+// start(b % 8) + b / 8, where XCD x owns the contiguous range of q + (x < r) tiles starting at
+// x q + min(x, r) (q = grid / 8, r = grid % 8): neighbouring tiles of a pair, whose reference
+// gathers overlap, then run on the same XCD and share its L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 // CorrespondenceFinderProjective gates on already loaded values (correspondencefinderprojective.cpp:62-95)
 __device__ __forceinline__ bool gates_loaded(const float4 &cn, const float4 &rn0, const float4 &cp, const float4 &rp0,
                                              const float *T, const AlignParamsDev &ap, float *rp, float *rn) {
@@ -906,7 +923,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_corr_lin(const PairDev 
                                                                  const FrameDev *__restrict__ F,
                                                                  const int *__restrict__ blk_pair, AlignParamsDev ap,
                                                                  int store_pairs) {
-  const int g = blockIdx.x;
+  const int g = xcd_tile(blockIdx.x, gridDim.x);
   const PairDev &pd = Pp[blk_pair[g]];
   const FrameDev &fr = F[pd.ref];
   const FrameDev &fc = F[pd.cur];
