@@ -45,11 +45,15 @@ class HostCloud:
     def append(self, other):
         for k in ("points", "normals", "curvature", "omega_p", "omega_n", "gauss"):
             setattr(self, k, np.ascontiguousarray(np.concatenate([getattr(self, k), getattr(other, k)])))
-        self.stats16 = None
+        if self.stats16 is not None and other.stats16 is not None:
+            self.stats16 = np.ascontiguousarray(np.concatenate([self.stats16, other.stats16]))
+        else:
+            self.stats16 = None
 
     def truncate(self, k):
-        for k_ in ("points", "normals", "curvature", "omega_p", "omega_n", "gauss"):
-            setattr(self, k_, np.ascontiguousarray(getattr(self, k_)[:k]))
+        for k_ in ("points", "normals", "curvature", "omega_p", "omega_n", "gauss", "stats16"):
+            if getattr(self, k_) is not None:
+                setattr(self, k_, np.ascontiguousarray(getattr(self, k_)[:k]))
 
 
 def _check_equal(cl, hc, stats=False):
@@ -118,20 +122,21 @@ def _frames(scene, poses, rows, cols):
     return [synth.render(scene, P, rows, cols, stream=10 + i) for i, P in enumerate(poses)]
 
 
-@pytest.mark.parametrize("rows,cols,thr", [(120, 160, 0.1), (480, 640, 0.1), (240, 320, 0.02)])
-def test_add_and_merge_match_oracle(ctx, rows, cols, thr):
+@pytest.mark.parametrize("rows,cols,thr,stats", [(120, 160, 0.1, False), (480, 640, 0.1, False),
+                                                (240, 320, 0.02, False), (240, 320, 0.1, True)])
+def test_add_and_merge_match_oracle(ctx, rows, cols, thr, stats):
     """nicp_aligner's map loop (nicp/nicp_aligner.cpp:200-210): three views of one scene added
     at their poses, merged after each add from the newest camera. The second merge meets
     Gaussians whose information form is current from the first."""
-    print("case add+merge", rows, cols, thr, flush=True)
+    print("case add+merge", rows, cols, thr, stats, flush=True)
     ocam, dcam = cams(rows, cols)
     _, dsp = params(min_image_radius=3)
     poses = [identity16(), synth.motion(31, 6.0, 0.08), synth.motion(32, 10.0, 0.15)]
     depths = _frames(7, poses, rows, cols)
-    clouds = [ctx.cloud_from_depth(d, dcam, dsp) for d in depths]
+    clouds = [ctx.cloud_from_depth(d, dcam, dsp, keep_stats=stats) for d in depths]
     for c in clouds:
         c.compute_gaussians()
-    hosts = [HostCloud(c) for c in clouds]
+    hosts = [HostCloud(c, stats) for c in clouds]
     mp = capi.merger_params(distance_threshold=thr)
     world = clouds[0]
     hw = hosts[0]
@@ -140,18 +145,19 @@ def test_add_and_merge_match_oracle(ctx, rows, cols, thr):
         world.add(clouds[k], poses[k])
         O.cloud_transform(poses[k], **hosts[k].arrays())
         hw.append(hosts[k])
-        _check_equal(world, hw)
-        _check_equal(clouds[k], hosts[k])  # Cloud::add transforms its argument in place
+        _check_equal(world, hw, stats)
+        _check_equal(clouds[k], hosts[k], stats)  # Cloud::add transforms its argument in place
         ns, col = world.merge(poses[k], dcam, rows, cols, mp, want_collapsed=True)
         kk, col_o = O.merge(hw.points, hw.gauss, poses[k], ocam, rows, cols, normals=hw.normals,
-                            curvature=hw.curvature, omega_p=hw.omega_p, omega_n=hw.omega_n, distance_threshold=thr)
+                            curvature=hw.curvature, stats16=hw.stats16, omega_p=hw.omega_p, omega_n=hw.omega_n,
+                            distance_threshold=thr)
         hw.truncate(kk)
         assert np.array_equal(col, col_o)
         assert world.n == kk and ns == len(col) - kk
         n = len(col)
         folded = (col >= 0) & (col != np.arange(n))
         total_folded += int(folded.sum())
-        _check_equal(world, hw)
+        _check_equal(world, hw, stats)
     # the views overlap: the merges fold many points, some owners take several
     assert total_folded > 0.2 * clouds[1].n
     flags = O.gaussian_flags(world.download_gaussians())
