@@ -172,6 +172,47 @@ def uneven_batch(ctx, n_pairs, iters, steps):
             "motion": "rot<=8deg, trans<=10cm", "graph": True}
 
 
+def nn_batch(ctx, n_pairs, n_points, iters, steps):
+    """kNN configuration: unstructured clouds of n_points (seeded plane/box scenes, rot <= 10 deg,
+    trans <= 0.2 m, sigma 2 mm) aligned by AlignerNN from the identity, all pairs in one
+    nicp_align_nn_batch call per step (clouds resident), next to the same pairs one call each."""
+    from nicp_amd import capi, synth
+    refs, curs, Tgt = [], [], []
+    for s in range(n_pairs):
+        d = synth.cloud_pair(3_000_000 + s, n=n_points)
+        refs.append(ctx.cloud_from_host(d["src_points"], d["src_normals"], d["curvature"], d["omega_p"], d["omega_n"]))
+        curs.append(ctx.cloud_from_host(d["tgt_points"], d["tgt_normals"], d["curvature"], d["omega_p"], d["omega_n"]))
+        Tgt.append(d["T_gt"])
+    Tgt = np.stack(Tgt)
+    ap = capi.align_params(outer_iterations=iters)
+    ctx.align_nn_batch(refs, curs, None, ap)  # warm-up: buffers grow to size
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        T, _ = ctx.align_nn_batch(refs, curs, None, ap)
+    dt = (time.perf_counter() - t0) / steps
+    ctx.profile(True)
+    T1, _ = ctx.align_nn_batch(refs, curs, None, ap)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    I = np.eye(4, dtype=np.float32).flatten(order="F")
+    t0 = time.perf_counter()
+    Ts = np.stack([ctx.align_nn(refs[i], curs[i], I, ap)[0] for i in range(n_pairs)])
+    dt_single = time.perf_counter() - t0
+    ok, err = aligned(T, Tgt)
+    for c in refs + curs:
+        c.close()
+    q_ms, q_n = prof.get("k_nn_query", (0.0, 0))
+    return {"pairs": n_pairs, "points_per_cloud": n_points, "iterations": iters, "ms_per_step": round(1000 * dt, 3),
+            "pairs_per_s": round(n_pairs / dt, 2), "aligned_fraction": round(float(ok.mean()), 4),
+            "rot_err_median_deg": round(float(np.median(err[:, 0])), 5),
+            "single_pair_calls_ms": round(1000 * dt_single, 3), "vs_single_pair_calls": round(dt_single / dt, 2),
+            "batch_equals_single": bool(np.array_equal(T.view(np.uint32), Ts.view(np.uint32))
+                                        and np.array_equal(T1.view(np.uint32), T.view(np.uint32))),
+            "k_nn_query_ms": round(q_ms / max(q_n, 1), 4),
+            "data": "seeded plane/box scenes, rot<=10deg, trans<=0.2m, sigma=2mm"}
+
+
 def calibrate_copy(mib):
     """A device-to-device copy of known size, for scaling rocprof's FETCH_SIZE / WRITE_SIZE."""
     from nicp_amd import capi
@@ -200,6 +241,9 @@ def main():
     ap_.add_argument("--plain-pairs", type=int, default=32)
     ap_.add_argument("--uneven-pairs", type=int, default=512, help="pairs of the uneven-resolution batch (0: skip)")
     ap_.add_argument("--uneven-steps", type=int, default=3)
+    ap_.add_argument("--nn-pairs", type=int, default=16, help="cloud pairs of the kNN batch (0: skip)")
+    ap_.add_argument("--nn-points", type=int, default=200000)
+    ap_.add_argument("--nn-steps", type=int, default=3)
     ap_.add_argument("--max-rot", type=float, default=MAX_ROT_DEG, help="motion range of the pairs (deg)")
     ap_.add_argument("--max-trans", type=float, default=MAX_TRANS_M, help="motion range of the pairs (m)")
     ap_.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -385,6 +429,12 @@ def main():
         uneven = uneven_batch(ctx, args.uneven_pairs, 15, args.uneven_steps)
         log("uneven batch: %s" % uneven)
 
+    # kNN configuration: 16 pairs of 200k-point unstructured clouds, 20 AlignerNN iterations
+    nn = None
+    if args.nn_pairs > 0:
+        nn = nn_batch(ctx, args.nn_pairs, args.nn_points, 20, args.nn_steps)
+        log("kNN batch: %s" % nn)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(np.arange(EVAL_SEED_BASE, EVAL_SEED_BASE + 16), max_rot, max_trans)
@@ -422,6 +472,7 @@ def main():
         "batched_identity_ms_per_step": round(1000.0 * t_plain_b / args.steps, 3),
         "batched_identity_aligned_pairs_per_s": round(n_ok_plain / t_plain_b, 2),
         "uneven_batch": uneven,
+        "nn_batch": nn,
         "roofline": roofline,
         "net_roofline": net_roofline,
         "kernels": kernels,

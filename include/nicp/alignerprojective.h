@@ -1,43 +1,25 @@
-// nicp::CorrespondenceFinderProjective (nicp/nicp/correspondencefinderprojective.h:17-167),
-// nicp::Linearizer (nicp/nicp/linearizer.h:19-155) and nicp::AlignerProjective
-// (nicp/nicp/aligner.h:16-429, alignerprojective.h): the reference's setters,
-// getters and defaults over the device loop of nicp_align().
+// nicp::CorrespondenceFinderProjective (nicp/nicp/correspondencefinderprojective.h:17-167) and
+// nicp::AlignerProjective (nicp/nicp/alignerprojective.h): the reference's setters, getters
+// and defaults over the device loop of nicp_align().
 #pragma once
 
-#include <cmath>
-
-#include "cloud.h"
-#include "definitions.h"
-#include "pinholepointprojector.h"
+#include "aligner.h"
 
 namespace nicp {
 
-class CorrespondenceFinderProjective {
+class CorrespondenceFinderProjective : public CorrespondenceFinder {
  public:
-  // defaults of nicp/nicp/correspondencefinder.cpp:9-18 and correspondencefinderprojective.cpp:9-14
-  float inlierDistanceThreshold() const { return _inlierDistanceThreshold; }
-  void setInlierDistanceThreshold(float v) { _inlierDistanceThreshold = v; }
-  float squaredThreshold() const { return _inlierDistanceThreshold * _inlierDistanceThreshold; }
-  float inlierNormalAngularThreshold() const { return _inlierNormalAngularThreshold; }
-  void setInlierNormalAngularThreshold(float v) { _inlierNormalAngularThreshold = v; }
-  float flatCurvatureThreshold() const { return _flatCurvatureThreshold; }
-  void setFlatCurvatureThreshold(float v) { _flatCurvatureThreshold = v; }
-  float inlierCurvatureRatioThreshold() const { return _inlierCurvatureRatioThreshold; }
-  void setInlierCurvatureRatioThreshold(float v) { _inlierCurvatureRatioThreshold = v; }
+  // defaults of nicp/nicp/correspondencefinderprojective.cpp:9-14
   int rowSearchRegion() const { return _rowSearchRegion; }
   void setRowSearchRegion(int v) { _rowSearchRegion = v; }
   int colSearchRegion() const { return _colSearchRegion; }
   void setColSearchRegion(int v) { _colSearchRegion = v; }
-  bool demotedToGICP() const { return _demotedToGICP; }
-  void setDemotedToGICP(bool v) { _demotedToGICP = v; }
   void setImageSize(int rows, int cols) {
     _rows = rows;
     _cols = cols;
   }
   int imageRows() const { return _rows; }
   int imageCols() const { return _cols; }
-  int numCorrespondences() const { return (int)_correspondences.size(); }
-  const CorrespondenceVector &correspondences() const { return _correspondences; }
 
   // CorrespondenceFinderProjective::compute for explicit index images
   void compute(const Cloud &reference, const Cloud &current, const IntImage &referenceIndexImage,
@@ -47,200 +29,44 @@ class CorrespondenceFinderProjective {
     check(nicp_correspondences(Context::get(), reference.handle(), current.handle(), referenceIndexImage.data.data(),
                                currentIndexImage.data.data(), referenceIndexImage.rows, referenceIndexImage.cols,
                                T.matrix(), &ap, pairs.data(), &n));
-    _correspondences.resize(n);
-    for (int i = 0; i < n; ++i) _correspondences[i] = Correspondence(pairs[2 * i], pairs[2 * i + 1]);
+    _setPairs(pairs, n);
   }
-  void setCorrespondences(const CorrespondenceVector &c) { _correspondences = c; }
 
  private:
-  float _inlierDistanceThreshold = 1.0f;
-  float _inlierNormalAngularThreshold = std::cos((float)M_PI / 6.0f);
-  float _flatCurvatureThreshold = 0.2f;
-  float _inlierCurvatureRatioThreshold = 1.3f;
   int _rowSearchRegion = 0, _colSearchRegion = 0;
-  bool _demotedToGICP = false;
   int _rows = 0, _cols = 0;
-  CorrespondenceVector _correspondences;
 };
 
-class AlignerProjective;
-
-class Linearizer {
- public:
-  // defaults of nicp/nicp/linearizer.cpp:9-19
-  float inlierMaxChi2() const { return _inlierMaxChi2; }
-  void setInlierMaxChi2(float v) { _inlierMaxChi2 = v; }
-  bool robustKernel() const { return _robustKernel; }
-  void setRobustKernel(bool v) { _robustKernel = v; }
-  bool demotedToGeneralizedICP() const { return _demotedToGeneralizedICP; }
-  void setDemotedToGeneralizedICP(bool v) { _demotedToGeneralizedICP = v; }
-  float scale() const { return _scale; }
-  void setScale(float v) { _scale = v; }
-  const Isometry3f &T() const { return _T; }
-  void setT(const Isometry3f &T) { _T = T; }
-  void setAligner(AlignerProjective *a) { _aligner = a; }
-  AlignerProjective *aligner() const { return _aligner; }
-  const Matrix6d &H() const { return _H; }
-  const Vector6d &b() const { return _b; }
-  float error() const { return _error; }
-  int inliers() const { return _inliers; }
-  // Linearizer::update over the aligner's clouds and its finder's correspondences
-  inline void update();
-
-  // internal: results of a device alignment
-  void setResults(const Matrix6d &H, const Vector6d &b, float e, int inl) {
-    _H = H;
-    _b = b;
-    _error = e;
-    _inliers = inl;
-  }
-
- private:
-  float _inlierMaxChi2 = 9e3f;
-  bool _robustKernel = true, _demotedToGeneralizedICP = false;
-  float _scale = 1.0f;
-  Isometry3f _T;
-  AlignerProjective *_aligner = nullptr;
-  Matrix6d _H;
-  Vector6d _b;
-  float _error = 0.0f;
-  int _inliers = 0;
-};
-
-class AlignerProjective {
+class AlignerProjective : public Aligner {
  public:
   AlignerProjective() {}
-  AlignerProjective(PointProjector *p, Linearizer *l, CorrespondenceFinderProjective *c)
-      : _projector(p), _linearizer(l), _correspondenceFinder(c) {
-    if (l) l->setAligner(this);
-  }
-  void setProjector(PointProjector *p) { _projector = p; }
-  PointProjector *projector() const { return _projector; }
-  // Aligner::setLinearizer writes the back-pointer (nicp/nicp/aligner.h:212-216)
-  void setLinearizer(Linearizer *l) {
-    _linearizer = l;
-    if (l) l->setAligner(this);
-  }
-  Linearizer *linearizer() const { return _linearizer; }
+  AlignerProjective(PointProjector *p, Linearizer *l, CorrespondenceFinderProjective *c) : Aligner(p, l, c) {}
   void setCorrespondenceFinder(CorrespondenceFinderProjective *c) { _correspondenceFinder = c; }
-  CorrespondenceFinderProjective *correspondenceFinder() const { return _correspondenceFinder; }
-  void setReferenceCloud(const Cloud *c) { _referenceCloud = c; }
-  const Cloud *referenceCloud() const { return _referenceCloud; }
-  void setCurrentCloud(const Cloud *c) { _currentCloud = c; }
-  const Cloud *currentCloud() const { return _currentCloud; }
-  int outerIterations() const { return _outerIterations; }
-  void setOuterIterations(int v) { _outerIterations = v; }
-  int innerIterations() const { return _innerIterations; }
-  void setInnerIterations(int v) { _innerIterations = v; }
-  float lambda() const { return _lambda; }
-  void setLambda(float v) { _lambda = v; }
-  int minInliers() const { return _minInliers; }
-  void setMinInliers(int v) { _minInliers = v; }
-  float translationalMinEigenRatio() const { return _translationalMinEigenRatio; }
-  void setTranslationalMinEigenRatio(float v) { _translationalMinEigenRatio = v; }
-  float rotationalMinEigenRatio() const { return _rotationalMinEigenRatio; }
-  void setRotationalMinEigenRatio(float v) { _rotationalMinEigenRatio = v; }
-  void setDebug(bool d) { _debug = d; }
-  // Aligner::setInitialGuess zeroes the bottom row of the guess
-  void setInitialGuess(const Isometry3f &g) {
-    _initialGuess = g;
-    _initialGuess.setBottomRow();
+  CorrespondenceFinderProjective *correspondenceFinder() const {
+    return static_cast<CorrespondenceFinderProjective *>(_correspondenceFinder);
   }
-  const Isometry3f &initialGuess() const { return _initialGuess; }
-  // only identity sensor offsets are supported by the device loop
-  void setSensorOffset(const Isometry3f &o) {
-    if (!o.isIdentity()) throw std::runtime_error("nicp_amd: AlignerProjective supports identity sensor offsets only");
-  }
-  const Isometry3f &T() const { return _T; }
-  float error() const { return _error; }
-  int inliers() const { return _inliers; }
-  float totalTime() const { return _totalTime; }
-  bool solutionValid() const { return _solutionValid; }
-  const Matrix6d &omega() const { return _omega; }
-  double translationalEigenRatio() const { return _result.translational_ratio; }
-  double rotationalEigenRatio() const { return _result.rotational_ratio; }
 
-  nicp_align_params params() const {
-    nicp_align_params a;
-    const CorrespondenceFinderProjective *c = _correspondenceFinder;
-    a.inlier_distance_threshold = c->inlierDistanceThreshold();
-    a.inlier_normal_angular_threshold = c->inlierNormalAngularThreshold();
-    a.flat_curvature_threshold = c->flatCurvatureThreshold();
-    a.inlier_curvature_ratio_threshold = c->inlierCurvatureRatioThreshold();
-    a.row_search_region = c->rowSearchRegion();
-    a.col_search_region = c->colSearchRegion();
-    a.demoted_to_gicp = c->demotedToGICP();
-    a.inlier_max_chi2 = _linearizer->inlierMaxChi2();
-    a.robust_kernel = _linearizer->robustKernel();
-    a.normal_omega_scale = _linearizer->scale();
-    a.outer_iterations = _outerIterations;
-    a.inner_iterations = _innerIterations;
-    a.lambda = _lambda;
-    a.translational_min_eigen_ratio = _translationalMinEigenRatio;
-    a.rotational_min_eigen_ratio = _rotationalMinEigenRatio;
+  nicp_align_params params() const override {
+    nicp_align_params a = Aligner::params();
+    a.row_search_region = correspondenceFinder()->rowSearchRegion();
+    a.col_search_region = correspondenceFinder()->colSearchRegion();
     return a;
   }
 
   // AlignerProjective::align (nicp/nicp/alignerprojective.cpp:21-136)
-  void align() {
-    if (!_projector || !_linearizer || !_correspondenceFinder || !_referenceCloud || !_currentCloud)
-      throw std::runtime_error("nicp_amd: AlignerProjective is missing a component");
+  void align() override {
+    if (!_projector) throw std::runtime_error("nicp_amd: AlignerProjective is missing a component");
+    _check("AlignerProjective");
     nicp_camera cam = _projector->camera();
     nicp_align_params ap = params();
+    nicp_align_result r{};
     check(nicp_align(Context::get(), _referenceCloud->handle(), _currentCloud->handle(), &cam,
-                     _projector->imageRows(), _projector->imageCols(), _initialGuess.matrix(), &ap, _T.matrix(),
-                     &_result));
-    _error = _result.error;
-    _inliers = _result.inliers;
-    _totalTime = _result.total_time_ms;
-    _solutionValid = _result.solution_valid != 0;
-    std::memcpy(_omega.m, _result.omega, sizeof(_omega.m));
-    Matrix6d H;
-    Vector6d b;
-    std::memcpy(H.m, _result.H, sizeof(H.m));
-    std::memcpy(b.v, _result.b, sizeof(b.v));
-    _linearizer->setResults(H, b, _error, _inliers);
-    _linearizer->setT(_T.inverse());
-    int n = 0;
-    check(nicp_last_correspondences(Context::get(), nullptr, 0, &n));
-    std::vector<int> pairs(2 * (size_t)std::max(n, 1));
-    check(nicp_last_correspondences(Context::get(), pairs.data(), n, &n));
-    CorrespondenceVector cv(n);
-    for (int i = 0; i < n; ++i) cv[i] = Correspondence(pairs[2 * i], pairs[2 * i + 1]);
+                     _projector->imageRows(), _projector->imageCols(), _initialGuess.matrix(), &ap, _T.matrix(), &r));
+    _adopt(r);
+    CorrespondenceVector cv;
+    _fetchCorrespondences(cv);
     _correspondenceFinder->setCorrespondences(cv);
   }
-
- private:
-  PointProjector *_projector = nullptr;
-  Linearizer *_linearizer = nullptr;
-  CorrespondenceFinderProjective *_correspondenceFinder = nullptr;
-  const Cloud *_referenceCloud = nullptr, *_currentCloud = nullptr;
-  // defaults of nicp/nicp/aligner.cpp:21-41
-  int _outerIterations = 10, _innerIterations = 1, _minInliers = 100;
-  float _lambda = 1e3f, _translationalMinEigenRatio = 50.0f, _rotationalMinEigenRatio = 50.0f;
-  bool _debug = false, _solutionValid = false;
-  Isometry3f _initialGuess, _T;
-  float _error = 0.0f, _totalTime = 0.0f;
-  int _inliers = 0;
-  Matrix6d _omega;
-  nicp_align_result _result{};
 };
-
-// Linearizer::update: one device linearisation of the finder's correspondences at _T
-inline void Linearizer::update() {
-  if (!_aligner) throw std::runtime_error("nicp_amd: Linearizer has no aligner");
-  const CorrespondenceVector &cv = _aligner->correspondenceFinder()->correspondences();
-  std::vector<int> pairs(2 * cv.size());
-  for (size_t i = 0; i < cv.size(); ++i) {
-    pairs[2 * i] = cv[i].referenceIndex;
-    pairs[2 * i + 1] = cv[i].currentIndex;
-  }
-  nicp_align_params ap = _aligner->params();
-  double e = 0.0;
-  check(nicp_linearize(Context::get(), _aligner->referenceCloud()->handle(), _aligner->currentCloud()->handle(),
-                       pairs.empty() ? nullptr : pairs.data(), (int)cv.size(), _T.matrix(), &ap, _H.m, _b.v, &e,
-                       &_inliers));
-  _error = (float)e;
-}
 
 }  // namespace nicp

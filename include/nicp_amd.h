@@ -239,6 +239,15 @@ int nicp_nn_correspondences(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_clo
 int nicp_align_nn(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const float init_guess[16],
                   const nicp_align_params *ap, float normal_scaling, float cell_size, float T_out[16],
                   nicp_align_result *result);
+/* Batched form of nicp_align_nn (no reference counterpart; it replaces one AlignerNN::align
+ * per cloud pair on host threads): num_pairs (1..65536) pairs of clouds of any sizes are
+ * aligned in the same launches, each pair's transform and result equal to nicp_align_nn on
+ * that pair alone. init_guesses: 16 floats per pair (column-major) or NULL for identities;
+ * T_out: 16 floats per pair; results: one per pair, total_time_ms being the whole call's.
+ * nicp_last_correspondences has nothing to return after it. */
+int nicp_align_nn_batch(nicp_ctx *ctx, int num_pairs, const nicp_cloud *const *refs, const nicp_cloud *const *curs,
+                        const float *init_guesses, const nicp_align_params *ap, float normal_scaling,
+                        float cell_size, float *T_out, nicp_align_result *results);
 
 /* Batched form (no reference counterpart; it replaces running one converter/aligner per
  * frame pair on host threads): frames of uneven size live in one device arena, depth

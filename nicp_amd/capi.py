@@ -78,7 +78,7 @@ EXPORTS = [
     "nicp_batch_align_learned", "nicp_device_alloc", "nicp_device_free", "nicp_device_copy",
     "nicp_initnet_train_step_rows", "nicp_initnet_set_input_mode", "nicp_cloud_gaussians", "nicp_cloud_set_gaussians",
     "nicp_cloud_download_gaussians", "nicp_cloud_transform", "nicp_cloud_add", "nicp_merge",
-    "nicp_batch_set_autotune", "nicp_batch_autotune_read",
+    "nicp_batch_set_autotune", "nicp_batch_autotune_read", "nicp_align_nn_batch",
 ]
 
 _lib = None
@@ -119,6 +119,7 @@ def lib():
             "nicp_ctx_profile": [vp, C.c_int], "nicp_ctx_profile_read": [vp, vp, vp],
             "nicp_nn_correspondences": [vp, vp, vp, vp, vp, C.c_float, C.c_float, vp, vp, vp],
             "nicp_align_nn": [vp, vp, vp, vp, vp, C.c_float, C.c_float, vp, vp],
+            "nicp_align_nn_batch": [vp, C.c_int, vp, vp, vp, vp, C.c_float, C.c_float, vp, vp],
             "nicp_cloud_from_depth_f32": [vp, vp, C.c_int, C.c_int, vp, vp, C.c_int, vp],
             "nicp_depth_scale": [vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, vp],
             "nicp_batch_render_synthetic": [vp, C.c_int, vp, vp, vp, vp, C.c_float, C.c_float],
@@ -346,6 +347,20 @@ class Context:
         _check(lib().nicp_align_nn(self.h, ref.h, cur.h, _p(np.ascontiguousarray(init16, np.float32)), C.byref(ap),
                                    normal_scaling, cell_size, _p(T), C.byref(res)))
         return T, res
+
+    def align_nn_batch(self, refs, curs, inits, ap, normal_scaling=1.0, cell_size=0.0):
+        """nicp_align_nn_batch over the pairs (refs[i], curs[i]); inits: (num, 16) or None.
+        Returns (T (num, 16), [AlignResult])."""
+        num = len(refs)
+        assert len(curs) == num
+        rh = (C.c_void_p * num)(*[c.h for c in refs])
+        ch = (C.c_void_p * num)(*[c.h for c in curs])
+        T = np.empty((num, 16), np.float32)
+        res = (AlignResult * num)()
+        ini = None if inits is None else np.ascontiguousarray(inits, np.float32).reshape(num, 16)
+        _check(lib().nicp_align_nn_batch(self.h, num, rh, ch, None if ini is None else _p(ini), C.byref(ap),
+                                         normal_scaling, cell_size, _p(T), res))
+        return T, list(res)
 
     def last_correspondences(self):
         n = C.c_int()

@@ -1828,78 +1828,119 @@ extern "C" int nicp_ctx_profile_read(nicp_ctx *ctx, double *total_ms, long long 
 namespace {
 
 constexpr float kDefaultCell = 0.03f;
-constexpr long long kMaxCells = 1ll << 25;
+constexpr long long kMaxCells = 1ll << 25;        // cells of one pair's grid
+constexpr long long kMaxArenaCells = 1ll << 26;   // cells (and sentinels) of all pairs: the scan's reach
+constexpr int kMaxNNPairs = 1 << 16;               // leaves every pair a share of >= 1023 cells
 
-// uniform grid over the current cloud: cell starts (exclusive scan of counts) and the
-// cloud sorted by cell
-int build_grid(nicp_ctx *ctx, const nicp_cloud *cur, float ns, float cell, GridDev &g) {
+// Uniform grids over the current clouds of np pairs (pd.cur of the plan's pairs): one cell
+// arena holding each pair's cell starts plus a sentinel (an exclusive scan of counts over
+// the whole arena, so every pair's sorted points follow the previous pair's), and the clouds
+// sorted by cell. The nearest neighbour the query finds is exact at any cell size, so a pair
+// whose grid would exceed its share of the arena gets coarser cells and the same result.
+int build_grids(nicp_ctx *ctx, int np, const nicp_cloud *const *curs, const PairDev *d_pairs,
+                const FrameDev *d_frames, float ns, float cell, GridDev **d_grids) {
   hipStream_t s = ctx->stream;
-  const int n = std::max(cur->n, 1);
-  CHK(ctx->grid_aux.reserve(64 * sizeof(int) + 16384 * sizeof(int)));
-  unsigned int init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  const size_t bbox_bytes = align_up((size_t)np * 6 * sizeof(unsigned int), 256);
+  const size_t grid_bytes = align_up((size_t)np * sizeof(GridDev), 256);
+  CHK(ctx->grid_aux.reserve(bbox_bytes + grid_bytes + 64 * sizeof(int) + 16384 * sizeof(int)));
   unsigned int *d_bbox = ctx->grid_aux.as<unsigned int>();
-  HIPCHK(hipMemcpyAsync(d_bbox, init, sizeof(init), hipMemcpyHostToDevice, s));
-  const int nb = std::min(1024, (n + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(k_nn_bbox, dim3(nb), dim3(kThreads), 0, s, cur->fd.pts, cur->fd.npoints, d_bbox);
-  unsigned int bb[6];
-  HIPCHK(hipMemcpyAsync(bb, d_bbox, sizeof(bb), hipMemcpyDeviceToHost, s));
+  GridDev *grids_d = (GridDev *)(ctx->grid_aux.as<char>() + bbox_bytes);
+  int *total = (int *)(ctx->grid_aux.as<char>() + bbox_bytes + grid_bytes);
+  int *sums = total + 64;
+  std::vector<unsigned int> bb((size_t)np * 6);
+  int nmax = 1;
+  size_t npts = 0;
+  for (int i = 0; i < np; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      bb[6 * i + k] = 0xffffffffu;
+      bb[6 * i + 3 + k] = 0u;
+    }
+    nmax = std::max(nmax, curs[i]->n);
+    npts += (size_t)curs[i]->n;
+  }
+  HIPCHK(hipMemcpyAsync(d_bbox, bb.data(), bb.size() * sizeof(unsigned int), hipMemcpyHostToDevice, s));
+  const int pb = (nmax + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(k_nn_bbox, dim3(std::min(1024, pb), np), dim3(kThreads), 0, s, d_pairs, d_frames, d_bbox);
+  HIPCHK(hipMemcpyAsync(bb.data(), d_bbox, bb.size() * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  float lo[3], hi[3];
-  for (int k = 0; k < 3; ++k) {
-    unsigned int a = bb[k], b = bb[3 + k];
-    a = (a & 0x80000000u) ? (a & 0x7fffffffu) : ~a;
-    b = (b & 0x80000000u) ? (b & 0x7fffffffu) : ~b;
-    lo[k] = __builtin_bit_cast(float, a);
-    hi[k] = __builtin_bit_cast(float, b);
-    if (cur->n == 0 || !std::isfinite(lo[k]) || !std::isfinite(hi[k])) { lo[k] = 0.0f; hi[k] = 0.0f; }
+  const long long share = std::min(kMaxCells, kMaxArenaCells / np - 1);
+  std::vector<GridDev> grids(np);
+  long long arena = 0;
+  for (int i = 0; i < np; ++i) {
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+      unsigned int a = bb[6 * i + k], b = bb[6 * i + 3 + k];
+      a = (a & 0x80000000u) ? (a & 0x7fffffffu) : ~a;
+      b = (b & 0x80000000u) ? (b & 0x7fffffffu) : ~b;
+      lo[k] = __builtin_bit_cast(float, a);
+      hi[k] = __builtin_bit_cast(float, b);
+      if (curs[i]->n == 0 || !std::isfinite(lo[k]) || !std::isfinite(hi[k])) { lo[k] = 0.0f; hi[k] = 0.0f; }
+    }
+    float h = cell > 0.0f ? cell : kDefaultCell;
+    long long cells = 0;
+    int dims[3];
+    // extents in double: a float difference of extreme coordinates can overflow
+    for (;;) {
+      double c = 1.0;
+      for (int k = 0; k < 3; ++k) {
+        const double d = std::floor(((double)hi[k] - (double)lo[k]) / (double)h) + 1.0;
+        dims[k] = (int)std::min(d, 2e9);
+        c *= d;
+      }
+      if (c <= (double)share) {
+        cells = (long long)c;
+        break;
+      }
+      h *= 1.25f;
+    }
+    GridDev &g = grids[i];
+    g.ox = lo[0]; g.oy = lo[1]; g.oz = lo[2];
+    g.h = h;
+    g.inv_h = 1.0f / h;
+    g.nx = dims[0]; g.ny = dims[1]; g.nz = dims[2];
+    g.cell_base = (int)arena;
+    arena += cells + 1;
   }
-  float h = cell > 0.0f ? cell : kDefaultCell;
-  long long cells = 0;
-  int dims[3];
-  for (;;) {
-    for (int k = 0; k < 3; ++k) dims[k] = (int)std::floor((hi[k] - lo[k]) / h) + 1;
-    cells = (long long)dims[0] * dims[1] * dims[2];
-    if (cells <= kMaxCells) break;
-    h *= 1.25f;
-  }
-  g.ox = lo[0]; g.oy = lo[1]; g.oz = lo[2];
-  g.h = h;
-  g.inv_h = 1.0f / h;
-  g.nx = dims[0]; g.ny = dims[1]; g.nz = dims[2];
-  const int m = (int)cells + 1;
+  const int m = (int)arena;
   CHK(ctx->grid_cells.reserve((size_t)m * sizeof(int)));
   CHK(ctx->grid_cursor.reserve((size_t)m * sizeof(int)));
-  CHK(ctx->grid_sorted.reserve((size_t)n * 2 * sizeof(float4)));
+  CHK(ctx->grid_sorted.reserve(std::max<size_t>(npts, 1) * 2 * sizeof(float4)));
+  HIPCHK(hipMemcpyAsync(grids_d, grids.data(), (size_t)np * sizeof(GridDev), hipMemcpyHostToDevice, s));
   int *cells_d = ctx->grid_cells.as<int>();
   HIPCHK(hipMemsetAsync(cells_d, 0, (size_t)m * sizeof(int), s));
-  const int pb = (n + kThreads - 1) / kThreads;
-  hipLaunchKernelGGL(k_nn_count, dim3(pb), dim3(kThreads), 0, s, cur->fd.pts, cur->fd.npoints, g, cells_d);
+  hipLaunchKernelGGL(k_nn_count, dim3(pb, np), dim3(kThreads), 0, s, d_pairs, d_frames, (const GridDev *)grids_d,
+                     cells_d);
   const int sb = (m + 4095) / 4096;
   if (sb > 16384) return set_err(NICP_ERR_INVALID, "grid too large");
-  int *sums = (int *)(ctx->grid_aux.as<char>() + 64 * sizeof(int));
-  int *total = ctx->grid_aux.as<int>() + 8;
   hipLaunchKernelGGL(k_scan_local, dim3(sb), dim3(1024), 0, s, cells_d, m, sums);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, sb, total);
   hipLaunchKernelGGL(k_scan_add, dim3(sb), dim3(1024), 0, s, cells_d, m, (const int *)sums, ctx->grid_cursor.as<int>());
-  hipLaunchKernelGGL(k_nn_scatter, dim3(pb), dim3(kThreads), 0, s, cur->fd.pts, cur->fd.nrm, cur->fd.npoints, g, ns,
+  hipLaunchKernelGGL(k_nn_scatter, dim3(pb, np), dim3(kThreads), 0, s, d_pairs, d_frames, (const GridDev *)grids_d, ns,
                      ctx->grid_cursor.as<int>(), ctx->grid_sorted.as<float4>());
   HIPCHK(hipGetLastError());
+  *d_grids = grids_d;
   return NICP_OK;
 }
 
-// plan of one NN pair: the pair's "image" is 1 x N_ref
-int nn_plan(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const float *init16, PairPlan &plan,
-            FrameDev **d_frames, PairDev **d_pairs, int **d_bp) {
-  std::vector<FrameDev> frames = {ref->fd, cur->fd};
-  frames[0].rows = 1; frames[0].cols = std::max(1, ref->n);
-  frames[1].rows = 1; frames[1].cols = std::max(1, ref->n);
-  CHK(upload_frames(ctx, frames, d_frames));
+// plan of np NN pairs: pair i's "image" is 1 x N_ref(i); frames 2i and 2i+1 are its clouds
+int nn_plan(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const nicp_cloud *const *curs, const float *init16,
+            PairPlan &plan, FrameDev **d_frames, PairDev **d_pairs, int **d_bp) {
+  std::vector<FrameDev> frames(2 * (size_t)np);
+  std::vector<int> ri(np), ci(np);
   nicp_camera cam{};
   cam.K[0] = cam.K[4] = cam.K[8] = 1.0f;
   cam.min_distance = cam.max_distance = 1.0f;
-  const nicp_camera *cams[1] = {&cam};
-  int r0 = 0, c1 = 1;
-  make_plan(frames, 1, &r0, &c1, cams, plan);
+  std::vector<const nicp_camera *> cams(np, &cam);
+  for (int i = 0; i < np; ++i) {
+    frames[2 * i] = refs[i]->fd;
+    frames[2 * i + 1] = curs[i]->fd;
+    frames[2 * i].rows = frames[2 * i + 1].rows = 1;
+    frames[2 * i].cols = frames[2 * i + 1].cols = std::max(1, refs[i]->n);
+    ri[i] = 2 * i;
+    ci[i] = 2 * i + 1;
+  }
+  CHK(upload_frames(ctx, frames, d_frames));
+  make_plan(frames, np, ri.data(), ci.data(), cams.data(), plan);
   plan.blk_ref.clear();
   plan.blk_cur.clear();
   plan.total_rtiles = plan.total_ctiles = 0;
@@ -1908,27 +1949,92 @@ int nn_plan(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const f
   return prepare_align(ctx->stream, plan, B, init16, d_bp, &d_br, &d_bc, d_pairs);
 }
 
+int check_nn_pairs(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const nicp_cloud *const *curs) {
+  if (!ctx || !refs || !curs) return set_err(NICP_ERR_INVALID, "null argument");
+  if (np <= 0 || np > kMaxNNPairs) return set_err(NICP_ERR_INVALID, "num_pairs out of range");
+  for (int i = 0; i < np; ++i) {
+    if (!refs[i] || !curs[i]) return set_err(NICP_ERR_INVALID, "null cloud");
+    if (refs[i]->ctx != ctx || curs[i]->ctx != ctx) return set_err(NICP_ERR_INVALID, "cloud of another context");
+  }
+  return NICP_OK;
+}
+
+// AlignerNN::align (alignernn.cpp:21-118) for np pairs in the same launches; T_out and the
+// results hold one entry per pair, the time is the whole call's
+int align_nn_pairs(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const nicp_cloud *const *curs,
+                   const float *init16, const nicp_align_params *ap, float normal_scaling, float cell_size,
+                   float *T_out, nicp_align_result *results) {
+  CHK(check_nn_pairs(ctx, np, refs, curs));
+  CHK(check_align_params(ap));
+  if (ap->inner_iterations != 1) return set_err(NICP_ERR_INVALID, "the kNN aligner supports one inner iteration");
+  HIPCHK(hipSetDevice(ctx->device));
+  ctx->last_valid = false;  // the pairs buffer is rewritten
+  PairPlan plan;
+  FrameDev *d_frames;
+  PairDev *d_pairs;
+  int *d_bp;
+  CHK(nn_plan(ctx, np, refs, curs, init16, plan, &d_frames, &d_pairs, &d_bp));
+  hipStream_t s = ctx->stream;
+  HIPCHK(hipEventRecord(ctx->ev0, s));
+  GridDev *d_grids;
+  CHK(build_grids(ctx, np, curs, d_pairs, d_frames, normal_scaling, cell_size, &d_grids));
+  const AlignParamsDev a = to_dev(ap);
+  Prof *pf = &ctx->prof;
+  pf->sink = &pf->eager;
+  const dim3 tb(kThreads);
+  const dim3 ntiles(plan.total_pix_tiles);
+  for (int it = 0; it < ap->outer_iterations; ++it) {
+    LAUNCH(pf, KID_NN_QUERY, k_nn_query, ntiles, tb, s, d_pairs, d_frames, d_bp, (const GridDev *)d_grids,
+           (const int *)ctx->grid_cells.as<int>(), (const float4 *)ctx->grid_sorted.as<float4>(), a, normal_scaling,
+           (int *)nullptr);
+    LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, ntiles, tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
+    LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(64), s, d_pairs, ap->lambda, 0, 0);
+  }
+  if (ap->outer_iterations > 0) {
+    LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, ntiles, tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
+    LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(64), s, d_pairs, 0.0f, 2, 0);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev1, s));
+  std::vector<float> st((size_t)np * kStateFloats);
+  std::vector<double> r((size_t)np * 2 * kPartial);
+  HIPCHK(hipMemcpyAsync(st.data(), ctx->state.p, st.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(r.data(), ctx->result.p, r.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (ap->outer_iterations == 0) std::fill(r.begin(), r.end(), 0.0);
+  for (int i = 0; i < np; ++i) {
+    const float *T = &st[(size_t)i * kStateFloats + kSlotT];
+    if (T_out) memcpy(T_out + (size_t)i * 16, T, 16 * sizeof(float));
+    if (results) fill_result(&r[(size_t)i * 2 * kPartial], T, ap, &results[i], ms);
+  }
+  return NICP_OK;
+}
+
 }  // namespace
 
 extern "C" int nicp_nn_correspondences(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const float T[16],
                                        const nicp_align_params *ap, float normal_scaling, float cell_size, int *pairs,
                                        int *num_pairs, int *nn_out) {
-  if (!ctx || !ref || !cur || !T || !num_pairs) return set_err(NICP_ERR_INVALID, "null argument");
+  CHK(check_nn_pairs(ctx, 1, &ref, &cur));
+  if (!T || !num_pairs) return set_err(NICP_ERR_INVALID, "null argument");
   CHK(check_align_params(ap));
   HIPCHK(hipSetDevice(ctx->device));
+  ctx->last_valid = false;  // the pairs buffer is rewritten
   PairPlan plan;
   FrameDev *d_frames;
   PairDev *d_pairs;
   int *d_bp;
-  CHK(nn_plan(ctx, ref, cur, nullptr, plan, &d_frames, &d_pairs, &d_bp));
+  CHK(nn_plan(ctx, 1, &ref, &cur, nullptr, plan, &d_frames, &d_pairs, &d_bp));
   HIPCHK(hipMemcpyAsync(ctx->state.as<float>() + kSlotTinv, T, 16 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-  GridDev g;
-  CHK(build_grid(ctx, cur, normal_scaling, cell_size, g));
+  GridDev *d_grids;
+  CHK(build_grids(ctx, 1, &cur, d_pairs, d_frames, normal_scaling, cell_size, &d_grids));
   const int nr = std::max(ref->n, 1);
   CHK(ctx->nn_out.reserve((size_t)nr * sizeof(int)));
-  hipLaunchKernelGGL(k_nn_query, dim3(plan.total_pix_tiles), dim3(kThreads), 0, ctx->stream, d_pairs, d_frames, d_bp, g,
-                     (const int *)ctx->grid_cells.as<int>(), (const float4 *)ctx->grid_sorted.as<float4>(), to_dev(ap),
-                     normal_scaling, ctx->nn_out.as<int>());
+  hipLaunchKernelGGL(k_nn_query, dim3(plan.total_pix_tiles), dim3(kThreads), 0, ctx->stream, d_pairs, d_frames, d_bp,
+                     (const GridDev *)d_grids, (const int *)ctx->grid_cells.as<int>(),
+                     (const float4 *)ctx->grid_sorted.as<float4>(), to_dev(ap), normal_scaling, ctx->nn_out.as<int>());
   HIPCHK(hipGetLastError());
   std::vector<int2> pp(nr);
   HIPCHK(hipMemcpyAsync(pp.data(), ctx->pairs_buf.p, (size_t)nr * sizeof(int2), hipMemcpyDeviceToHost, ctx->stream));
@@ -1951,52 +2057,19 @@ extern "C" int nicp_nn_correspondences(nicp_ctx *ctx, const nicp_cloud *ref, con
 extern "C" int nicp_align_nn(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const float init_guess[16],
                              const nicp_align_params *ap, float normal_scaling, float cell_size, float T_out[16],
                              nicp_align_result *result) {
-  if (!ctx || !ref || !cur) return set_err(NICP_ERR_INVALID, "null argument");
-  CHK(check_align_params(ap));
-  if (ap->inner_iterations != 1) return set_err(NICP_ERR_INVALID, "the kNN aligner supports one inner iteration");
-  HIPCHK(hipSetDevice(ctx->device));
-  PairPlan plan;
-  FrameDev *d_frames;
-  PairDev *d_pairs;
-  int *d_bp;
-  float I4[16];
-  set_identity4(I4);
-  CHK(nn_plan(ctx, ref, cur, init_guess ? init_guess : I4, plan, &d_frames, &d_pairs, &d_bp));
-  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  GridDev g;
-  CHK(build_grid(ctx, cur, normal_scaling, cell_size, g));
-  const AlignParamsDev a = to_dev(ap);
-  Prof *pf = &ctx->prof;
-  pf->sink = &pf->eager;
-  const dim3 tb(kThreads);
-  hipStream_t s = ctx->stream;
-  for (int it = 0; it < ap->outer_iterations; ++it) {
-    LAUNCH(pf, KID_NN_QUERY, k_nn_query, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_bp, g,
-           (const int *)ctx->grid_cells.as<int>(), (const float4 *)ctx->grid_sorted.as<float4>(), a, normal_scaling,
-           (int *)nullptr);
-    LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
-    LAUNCH(pf, KID_SOLVE, k_solve, dim3(1), dim3(64), s, d_pairs, ap->lambda, 0, 0);
-  }
-  if (ap->outer_iterations > 0) {
-    LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
-    LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(1), dim3(64), s, d_pairs, 0.0f, 2, 0);
-  }
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(ctx->ev1, s));
-  float T[16];
-  double r[2 * kPartial];
-  HIPCHK(hipMemcpyAsync(T, ctx->state.p, sizeof(T), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(r, ctx->result.p, sizeof(r), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  float ms = 0.0f;
-  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-  if (ap->outer_iterations == 0) memset(r, 0, sizeof(r));
-  if (T_out) memcpy(T_out, T, sizeof(T));
-  if (result) fill_result(r, T, ap, result, ms);
+  CHK(align_nn_pairs(ctx, 1, &ref, &cur, init_guess, ap, normal_scaling, cell_size, T_out, result));
+  // the pair's correspondences stay in the pairs buffer at offset 0 (nicp_last_correspondences)
   ctx->last_rows = 1;
   ctx->last_cols = std::max(1, ref->n);
   ctx->last_valid = ap->outer_iterations > 0;
   return NICP_OK;
+}
+
+extern "C" int nicp_align_nn_batch(nicp_ctx *ctx, int num_pairs, const nicp_cloud *const *refs,
+                                   const nicp_cloud *const *curs, const float *init_guesses,
+                                   const nicp_align_params *ap, float normal_scaling, float cell_size, float *T_out,
+                                   nicp_align_result *results) {
+  return align_nn_pairs(ctx, num_pairs, refs, curs, init_guesses, ap, normal_scaling, cell_size, T_out, results);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -1152,10 +1152,15 @@ __device__ __forceinline__ unsigned int f2ord(float f) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// bbox[0..2] = min keys (init 0xffffffff), bbox[3..5] = max keys (init 0)
-extern "C" __global__ __launch_bounds__(kThreads) void k_nn_bbox(const float4 *__restrict__ pts, const int *__restrict__ npts,
+// bounding box of the current cloud of pair blockIdx.y: bbox[6 y + 0..2] = min keys
+// (init 0xffffffff), bbox[6 y + 3..5] = max keys (init 0)
+extern "C" __global__ __launch_bounds__(kThreads) void k_nn_bbox(const PairDev *__restrict__ Pp,
+                                                                const FrameDev *__restrict__ F,
                                                                 unsigned int *__restrict__ bbox) {
-  const int n = *npts;
+  const FrameDev &fc = F[Pp[blockIdx.y].cur];
+  const int n = *fc.npoints;
+  const float4 *__restrict__ pts = fc.pts;
+  bbox += 6 * blockIdx.y;
   float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     const float4 p = pts[i];
@@ -1178,22 +1183,32 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_nn_bbox(const float4 *_
   }
 }
 
+// one pair's grid; its nx*ny*nz cell starts and a sentinel live at cell_base of the
+// batch's cell arena
 struct GridDev {
   float ox, oy, oz, h, inv_h;
   int nx, ny, nz;
+  int cell_base;
 };
 
 __device__ __forceinline__ int cell_coord(float v, float o, float inv_h) { return (int)floorf((v - o) * inv_h); }
 
-extern "C" __global__ __launch_bounds__(kThreads) void k_nn_count(const float4 *__restrict__ pts, const int *__restrict__ npts,
-                                                                 GridDev g, int *__restrict__ counts) {
+__device__ __forceinline__ int grid_cell(const GridDev &g, const float4 &p) {
+  const int cx = min(max(cell_coord(p.x, g.ox, g.inv_h), 0), g.nx - 1);
+  const int cy = min(max(cell_coord(p.y, g.oy, g.inv_h), 0), g.ny - 1);
+  const int cz = min(max(cell_coord(p.z, g.oz, g.inv_h), 0), g.nz - 1);
+  return g.cell_base + (cz * g.ny + cy) * g.nx + cx;
+}
+
+// points per cell of pair blockIdx.y's current cloud
+extern "C" __global__ __launch_bounds__(kThreads) void k_nn_count(const PairDev *__restrict__ Pp,
+                                                                 const FrameDev *__restrict__ F,
+                                                                 const GridDev *__restrict__ grids,
+                                                                 int *__restrict__ counts) {
+  const FrameDev &fc = F[Pp[blockIdx.y].cur];
   const int i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= *npts) return;
-  const float4 p = pts[i];
-  int cx = min(max(cell_coord(p.x, g.ox, g.inv_h), 0), g.nx - 1);
-  int cy = min(max(cell_coord(p.y, g.oy, g.inv_h), 0), g.ny - 1);
-  int cz = min(max(cell_coord(p.z, g.oz, g.inv_h), 0), g.nz - 1);
-  atomicAdd(counts + ((size_t)cz * g.ny + cy) * g.nx + cx, 1);
+  if (i >= *fc.npoints) return;
+  atomicAdd(counts + grid_cell(grids[blockIdx.y], fc.pts[i]), 1);
 }
 
 // exclusive scan of n ints, pass 1: each block of 1024 threads scans 4096 entries
@@ -1258,18 +1273,18 @@ extern "C" __global__ __launch_bounds__(1024) void k_scan_add(int *__restrict__ 
     }
 }
 
-// sorted current cloud: (x, y, z, index bits) and (s nx, s ny, s nz, 0) per point, in cell order
-extern "C" __global__ __launch_bounds__(kThreads) void k_nn_scatter(const float4 *__restrict__ pts, const float4 *__restrict__ nrm,
-                                                                   const int *__restrict__ npts, GridDev g, float ns,
+// sorted current clouds: (x, y, z, index bits) and (s nx, s ny, s nz, 0) per point, in cell
+// order; the cell arena's exclusive scan puts every pair's points after the previous pair's
+extern "C" __global__ __launch_bounds__(kThreads) void k_nn_scatter(const PairDev *__restrict__ Pp,
+                                                                   const FrameDev *__restrict__ F,
+                                                                   const GridDev *__restrict__ grids, float ns,
                                                                    int *__restrict__ cursor, float4 *__restrict__ sorted) {
+  const FrameDev &fc = F[Pp[blockIdx.y].cur];
   const int i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= *npts) return;
-  const float4 p = pts[i];
-  const float4 n = nrm[i];
-  int cx = min(max(cell_coord(p.x, g.ox, g.inv_h), 0), g.nx - 1);
-  int cy = min(max(cell_coord(p.y, g.oy, g.inv_h), 0), g.ny - 1);
-  int cz = min(max(cell_coord(p.z, g.oz, g.inv_h), 0), g.nz - 1);
-  const int slot = atomicAdd(cursor + ((size_t)cz * g.ny + cy) * g.nx + cx, 1);
+  if (i >= *fc.npoints) return;
+  const float4 p = fc.pts[i];
+  const float4 n = fc.nrm[i];
+  const int slot = atomicAdd(cursor + grid_cell(grids[blockIdx.y], p), 1);
   sorted[2 * (size_t)slot] = make_float4(p.x, p.y, p.z, __int_as_float(i));
   sorted[2 * (size_t)slot + 1] = make_float4(ns * n.x, ns * n.y, ns * n.z, 0.0f);
 }
@@ -1278,12 +1293,16 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_nn_scatter(const float4
 // The pair's "image" is 1 x N_ref; pairs[i] = (i, j) or (-1, -1).
 extern "C" __global__ __launch_bounds__(kThreads) void k_nn_query(const PairDev *__restrict__ Pp,
                                                                  const FrameDev *__restrict__ F,
-                                                                 const int *__restrict__ blk_pair, GridDev g,
-                                                                 const int *__restrict__ cell_start,
+                                                                 const int *__restrict__ blk_pair,
+                                                                 const GridDev *__restrict__ grids,
+                                                                 const int *__restrict__ cell_arena,
                                                                  const float4 *__restrict__ sorted, AlignParamsDev ap,
                                                                  float ns, int *__restrict__ nn_out) {
   const int gb = blockIdx.x;
-  const PairDev &pd = Pp[blk_pair[gb]];
+  const int pi = blk_pair[gb];
+  const PairDev &pd = Pp[pi];
+  const GridDev g = grids[pi];
+  const int *__restrict__ cell_start = cell_arena + g.cell_base;
   const FrameDev &fr = F[pd.ref];
   const FrameDev &fc = F[pd.cur];
   const int nref = *fr.npoints;
