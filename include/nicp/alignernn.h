@@ -14,7 +14,7 @@ class CorrespondenceFinderNN : public CorrespondenceFinder {
   // _normal_scaling = 1 (nicp/nicp/correspondencefindernn.cpp:8)
   float normalScaling() const { return _normal_scaling; }
   void setNormalScaling(float v) { _normal_scaling = v; }
-  // edge of the search grid's cells in metres; <= 0 picks the default (3 cm)
+  // edge of the search grid's cells in metres; <= 0 picks the default (2 cm)
   float cellSize() const { return _cellSize; }
   void setCellSize(float v) { _cellSize = v; }
 

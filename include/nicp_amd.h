@@ -228,8 +228,8 @@ int nicp_last_correspondences(nicp_ctx *ctx, int *pairs, int capacity, int *num_
 /* Replaces CorrespondenceFinderNN::init + compute (nicp/nicp/correspondencefindernn.cpp:44-97):
  * the nearest neighbour of each transformed reference point [T p; s R n] among the current
  * points [p; s n] within the squared inlier distance (exact, on a uniform grid of cell_size
- * metres, <= 0 for the default; FLANN's approximate kd-tree is not reproduced; ties go to the
- * lower index), then the reference's gates. pairs (capacity: reference points) are in
+ * metres, <= 0 for the default of 2 cm, which changes the speed only; FLANN's approximate
+ * kd-tree is not reproduced; ties go to the lower index), then the reference's gates. pairs (capacity: reference points) are in
  * reference-index order; nn_out (optional, one int per reference point) receives the raw
  * neighbour or -1. */
 int nicp_nn_correspondences(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const float T[16],

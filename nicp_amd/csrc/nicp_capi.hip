@@ -224,7 +224,7 @@ struct nicp_ctx {
   Prof prof;
   std::vector<ProfEntry> *pending_graph = nullptr;  // graph events not yet read
   // kNN grid of the current cloud
-  DevBuf grid_cells, grid_cursor, grid_sorted, grid_aux, nn_out;
+  DevBuf grid_cells, grid_cursor, grid_sorted, grid_aux, nn_out, nn_order;
 };
 
 struct nicp_cloud {
@@ -877,7 +877,7 @@ extern "C" int nicp_ctx_destroy(nicp_ctx *c) {
   (void)hipStreamSynchronize(c->stream);
   DevBuf *bufs[] = {&c->frames_tbl, &c->pairs_tbl, &c->maps, &c->zbuf, &c->cur_index, &c->pairs_buf,
                     &c->state, &c->partials, &c->result, &c->depth_tmp, &c->scratch_in, &c->grid_cells,
-                    &c->grid_cursor, &c->grid_sorted, &c->grid_aux, &c->nn_out};
+                    &c->grid_cursor, &c->grid_sorted, &c->grid_aux, &c->nn_out, &c->nn_order};
   for (DevBuf *b : bufs) b->release();
   c->prof.release();
   (void)hipEventDestroy(c->ev0);
@@ -1827,55 +1827,80 @@ extern "C" int nicp_ctx_profile_read(nicp_ctx *ctx, double *total_ms, long long 
 // ---------------------------------------------------------------------------------------
 namespace {
 
-constexpr float kDefaultCell = 0.03f;
+constexpr float kDefaultCell = 0.02f;
 constexpr long long kMaxCells = 1ll << 25;        // cells of one pair's grid
 constexpr long long kMaxArenaCells = 1ll << 26;   // cells (and sentinels) of all pairs: the scan's reach
 constexpr int kMaxNNPairs = 1 << 16;               // leaves every pair a share of >= 1023 cells
+
+// decodes k_nn_bbox's order-preserving keys; an empty or non-finite box becomes the origin
+void decode_bbox(const unsigned int *bb, int n, float lo[3], float hi[3]) {
+  for (int k = 0; k < 3; ++k) {
+    unsigned int a = bb[k], b = bb[3 + k];
+    a = (a & 0x80000000u) ? (a & 0x7fffffffu) : ~a;
+    b = (b & 0x80000000u) ? (b & 0x7fffffffu) : ~b;
+    lo[k] = __builtin_bit_cast(float, a);
+    hi[k] = __builtin_bit_cast(float, b);
+    if (n == 0 || !std::isfinite(lo[k]) || !std::isfinite(hi[k])) { lo[k] = 0.0f; hi[k] = 0.0f; }
+  }
+}
+
+// exclusive scan of the m ints at cells (m <= kMaxArenaCells), copied to cursor as well
+int scan_arena(nicp_ctx *ctx, int *cells, int m, int *cursor, int *sums, int *total) {
+  const int sb = (m + 4095) / 4096;
+  if (sb > 16384) return set_err(NICP_ERR_INVALID, "grid too large");
+  hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(k_scan_local, dim3(sb), dim3(1024), 0, s, cells, m, sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, sb, total);
+  hipLaunchKernelGGL(k_scan_add, dim3(sb), dim3(1024), 0, s, cells, m, (const int *)sums, cursor);
+  return NICP_OK;
+}
 
 // Uniform grids over the current clouds of np pairs (pd.cur of the plan's pairs): one cell
 // arena holding each pair's cell starts plus a sentinel (an exclusive scan of counts over
 // the whole arena, so every pair's sorted points follow the previous pair's), and the clouds
 // sorted by cell. The nearest neighbour the query finds is exact at any cell size, so a pair
 // whose grid would exceed its share of the arena gets coarser cells and the same result.
-int build_grids(nicp_ctx *ctx, int np, const nicp_cloud *const *curs, const PairDev *d_pairs,
-                const FrameDev *d_frames, float ns, float cell, GridDev **d_grids) {
+// Also the query order: each pair's reference indices sorted by the Morton code of a coarse
+// cubic grid over the reference cloud (ctx->nn_order, pair p from grids[p].order_base).
+int build_grids(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const nicp_cloud *const *curs,
+                const PairDev *d_pairs, const FrameDev *d_frames, float ns, float cell, GridDev **d_grids) {
   hipStream_t s = ctx->stream;
-  const size_t bbox_bytes = align_up((size_t)np * 6 * sizeof(unsigned int), 256);
+  const size_t bbox_bytes = align_up((size_t)np * 12 * sizeof(unsigned int), 256);
   const size_t grid_bytes = align_up((size_t)np * sizeof(GridDev), 256);
-  CHK(ctx->grid_aux.reserve(bbox_bytes + grid_bytes + 64 * sizeof(int) + 16384 * sizeof(int)));
+  CHK(ctx->grid_aux.reserve(bbox_bytes + 2 * grid_bytes + 64 * sizeof(int) + 16384 * sizeof(int)));
   unsigned int *d_bbox = ctx->grid_aux.as<unsigned int>();
   GridDev *grids_d = (GridDev *)(ctx->grid_aux.as<char>() + bbox_bytes);
-  int *total = (int *)(ctx->grid_aux.as<char>() + bbox_bytes + grid_bytes);
+  GridDev *ogrids_d = (GridDev *)(ctx->grid_aux.as<char>() + bbox_bytes + grid_bytes);
+  int *total = (int *)(ctx->grid_aux.as<char>() + bbox_bytes + 2 * grid_bytes);
   int *sums = total + 64;
-  std::vector<unsigned int> bb((size_t)np * 6);
+  std::vector<unsigned int> bb((size_t)np * 12);
   int nmax = 1;
-  size_t npts = 0;
+  size_t ncur = 0, nref = 0;
   for (int i = 0; i < np; ++i) {
-    for (int k = 0; k < 3; ++k) {
-      bb[6 * i + k] = 0xffffffffu;
-      bb[6 * i + 3 + k] = 0u;
-    }
-    nmax = std::max(nmax, curs[i]->n);
-    npts += (size_t)curs[i]->n;
+    for (int w = 0; w < 2; ++w)
+      for (int k = 0; k < 3; ++k) {
+        bb[12 * i + 6 * w + k] = 0xffffffffu;
+        bb[12 * i + 6 * w + 3 + k] = 0u;
+      }
+    nmax = std::max(nmax, std::max(curs[i]->n, refs[i]->n));
+    ncur += (size_t)curs[i]->n;
+    nref += (size_t)refs[i]->n;
   }
   HIPCHK(hipMemcpyAsync(d_bbox, bb.data(), bb.size() * sizeof(unsigned int), hipMemcpyHostToDevice, s));
   const int pb = (nmax + kThreads - 1) / kThreads;
-  hipLaunchKernelGGL(k_nn_bbox, dim3(std::min(1024, pb), np), dim3(kThreads), 0, s, d_pairs, d_frames, d_bbox);
+  hipLaunchKernelGGL(k_nn_bbox, dim3(std::min(1024, pb), np, 2), dim3(kThreads), 0, s, d_pairs, d_frames, d_bbox);
   HIPCHK(hipMemcpyAsync(bb.data(), d_bbox, bb.size() * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   const long long share = std::min(kMaxCells, kMaxArenaCells / np - 1);
-  std::vector<GridDev> grids(np);
-  long long arena = 0;
+  // Morton bits per axis of the order grids: 2^(3 bits) cells per pair within the arena
+  int obits = 7;
+  while (obits > 1 && (long long)np << (3 * obits) > kMaxArenaCells - 1) --obits;
+  std::vector<GridDev> grids(np), ogrids(np);
+  long long arena = 0, oarena = 0;
+  size_t obase = 0;
   for (int i = 0; i < np; ++i) {
     float lo[3], hi[3];
-    for (int k = 0; k < 3; ++k) {
-      unsigned int a = bb[6 * i + k], b = bb[6 * i + 3 + k];
-      a = (a & 0x80000000u) ? (a & 0x7fffffffu) : ~a;
-      b = (b & 0x80000000u) ? (b & 0x7fffffffu) : ~b;
-      lo[k] = __builtin_bit_cast(float, a);
-      hi[k] = __builtin_bit_cast(float, b);
-      if (curs[i]->n == 0 || !std::isfinite(lo[k]) || !std::isfinite(hi[k])) { lo[k] = 0.0f; hi[k] = 0.0f; }
-    }
+    decode_bbox(&bb[12 * i], curs[i]->n, lo, hi);
     float h = cell > 0.0f ? cell : kDefaultCell;
     long long cells = 0;
     int dims[3];
@@ -1899,24 +1924,46 @@ int build_grids(nicp_ctx *ctx, int np, const nicp_cloud *const *curs, const Pair
     g.inv_h = 1.0f / h;
     g.nx = dims[0]; g.ny = dims[1]; g.nz = dims[2];
     g.cell_base = (int)arena;
+    g.order_base = (int)obase;
     arena += cells + 1;
+    obase += (size_t)refs[i]->n;
+    // order grid: cubic cells over the reference box
+    decode_bbox(&bb[12 * i + 6], refs[i]->n, lo, hi);
+    double ext = 0.0;
+    for (int k = 0; k < 3; ++k) ext = std::max(ext, (double)hi[k] - (double)lo[k]);
+    GridDev &o = ogrids[i];
+    o.ox = lo[0]; o.oy = lo[1]; o.oz = lo[2];
+    const double oh = std::max(ext, 1e-30) / (double)(1 << obits) * (1.0 + 1e-6);
+    o.h = (float)oh;
+    o.inv_h = std::isfinite(1.0 / oh) ? (float)(1.0 / oh) : 0.0f;
+    o.nx = obits;
+    o.ny = o.nz = 0;
+    o.cell_base = (int)oarena;
+    o.order_base = 0;
+    oarena += 1ll << (3 * obits);
   }
-  const int m = (int)arena;
-  CHK(ctx->grid_cells.reserve((size_t)m * sizeof(int)));
-  CHK(ctx->grid_cursor.reserve((size_t)m * sizeof(int)));
-  CHK(ctx->grid_sorted.reserve(std::max<size_t>(npts, 1) * 2 * sizeof(float4)));
+  const int m = (int)arena, mo = (int)oarena + 1;
+  CHK(ctx->grid_cells.reserve((size_t)std::max(m, mo) * sizeof(int)));
+  CHK(ctx->grid_cursor.reserve((size_t)std::max(m, mo) * sizeof(int)));
+  CHK(ctx->grid_sorted.reserve(std::max<size_t>(ncur, 1) * 2 * sizeof(float4)));
+  CHK(ctx->nn_order.reserve(std::max<size_t>(nref, 1) * sizeof(int)));
   HIPCHK(hipMemcpyAsync(grids_d, grids.data(), (size_t)np * sizeof(GridDev), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(ogrids_d, ogrids.data(), (size_t)np * sizeof(GridDev), hipMemcpyHostToDevice, s));
   int *cells_d = ctx->grid_cells.as<int>();
+  int *cursor_d = ctx->grid_cursor.as<int>();
+  // query order first (its arena is then reused for the grid)
+  HIPCHK(hipMemsetAsync(cells_d, 0, (size_t)mo * sizeof(int), s));
+  hipLaunchKernelGGL(k_nn_order_count, dim3(pb, np), dim3(kThreads), 0, s, d_pairs, d_frames,
+                     (const GridDev *)ogrids_d, cells_d);
+  CHK(scan_arena(ctx, cells_d, mo, cursor_d, sums, total));
+  hipLaunchKernelGGL(k_nn_order_scatter, dim3(pb, np), dim3(kThreads), 0, s, d_pairs, d_frames,
+                     (const GridDev *)ogrids_d, cursor_d, ctx->nn_order.as<int>());
   HIPCHK(hipMemsetAsync(cells_d, 0, (size_t)m * sizeof(int), s));
   hipLaunchKernelGGL(k_nn_count, dim3(pb, np), dim3(kThreads), 0, s, d_pairs, d_frames, (const GridDev *)grids_d,
                      cells_d);
-  const int sb = (m + 4095) / 4096;
-  if (sb > 16384) return set_err(NICP_ERR_INVALID, "grid too large");
-  hipLaunchKernelGGL(k_scan_local, dim3(sb), dim3(1024), 0, s, cells_d, m, sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, sb, total);
-  hipLaunchKernelGGL(k_scan_add, dim3(sb), dim3(1024), 0, s, cells_d, m, (const int *)sums, ctx->grid_cursor.as<int>());
+  CHK(scan_arena(ctx, cells_d, m, cursor_d, sums, total));
   hipLaunchKernelGGL(k_nn_scatter, dim3(pb, np), dim3(kThreads), 0, s, d_pairs, d_frames, (const GridDev *)grids_d, ns,
-                     ctx->grid_cursor.as<int>(), ctx->grid_sorted.as<float4>());
+                     cursor_d, ctx->grid_sorted.as<float4>());
   HIPCHK(hipGetLastError());
   *d_grids = grids_d;
   return NICP_OK;
@@ -1977,7 +2024,7 @@ int align_nn_pairs(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const n
   hipStream_t s = ctx->stream;
   HIPCHK(hipEventRecord(ctx->ev0, s));
   GridDev *d_grids;
-  CHK(build_grids(ctx, np, curs, d_pairs, d_frames, normal_scaling, cell_size, &d_grids));
+  CHK(build_grids(ctx, np, refs, curs, d_pairs, d_frames, normal_scaling, cell_size, &d_grids));
   const AlignParamsDev a = to_dev(ap);
   Prof *pf = &ctx->prof;
   pf->sink = &pf->eager;
@@ -1985,8 +2032,8 @@ int align_nn_pairs(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const n
   const dim3 ntiles(plan.total_pix_tiles);
   for (int it = 0; it < ap->outer_iterations; ++it) {
     LAUNCH(pf, KID_NN_QUERY, k_nn_query, ntiles, tb, s, d_pairs, d_frames, d_bp, (const GridDev *)d_grids,
-           (const int *)ctx->grid_cells.as<int>(), (const float4 *)ctx->grid_sorted.as<float4>(), a, normal_scaling,
-           (int *)nullptr);
+           (const int *)ctx->grid_cells.as<int>(), (const float4 *)ctx->grid_sorted.as<float4>(),
+           (const int *)ctx->nn_order.as<int>(), a, normal_scaling, (int *)nullptr);
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, ntiles, tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
     LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(64), s, d_pairs, ap->lambda, 0, 0);
   }
@@ -2029,12 +2076,13 @@ extern "C" int nicp_nn_correspondences(nicp_ctx *ctx, const nicp_cloud *ref, con
   CHK(nn_plan(ctx, 1, &ref, &cur, nullptr, plan, &d_frames, &d_pairs, &d_bp));
   HIPCHK(hipMemcpyAsync(ctx->state.as<float>() + kSlotTinv, T, 16 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
   GridDev *d_grids;
-  CHK(build_grids(ctx, 1, &cur, d_pairs, d_frames, normal_scaling, cell_size, &d_grids));
+  CHK(build_grids(ctx, 1, &ref, &cur, d_pairs, d_frames, normal_scaling, cell_size, &d_grids));
   const int nr = std::max(ref->n, 1);
   CHK(ctx->nn_out.reserve((size_t)nr * sizeof(int)));
   hipLaunchKernelGGL(k_nn_query, dim3(plan.total_pix_tiles), dim3(kThreads), 0, ctx->stream, d_pairs, d_frames, d_bp,
                      (const GridDev *)d_grids, (const int *)ctx->grid_cells.as<int>(),
-                     (const float4 *)ctx->grid_sorted.as<float4>(), to_dev(ap), normal_scaling, ctx->nn_out.as<int>());
+                     (const float4 *)ctx->grid_sorted.as<float4>(), (const int *)ctx->nn_order.as<int>(), to_dev(ap),
+                     normal_scaling, ctx->nn_out.as<int>());
   HIPCHK(hipGetLastError());
   std::vector<int2> pp(nr);
   HIPCHK(hipMemcpyAsync(pp.data(), ctx->pairs_buf.p, (size_t)nr * sizeof(int2), hipMemcpyDeviceToHost, ctx->stream));

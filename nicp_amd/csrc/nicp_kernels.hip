@@ -1152,15 +1152,15 @@ __device__ __forceinline__ unsigned int f2ord(float f) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// bounding box of the current cloud of pair blockIdx.y: bbox[6 y + 0..2] = min keys
-// (init 0xffffffff), bbox[6 y + 3..5] = max keys (init 0)
+// bounding boxes of pair blockIdx.y's current (blockIdx.z = 0) and reference (1) clouds:
+// bbox[12 y + 6 z + 0..2] = min keys (init 0xffffffff), bbox[12 y + 6 z + 3..5] = max keys (init 0)
 extern "C" __global__ __launch_bounds__(kThreads) void k_nn_bbox(const PairDev *__restrict__ Pp,
                                                                 const FrameDev *__restrict__ F,
                                                                 unsigned int *__restrict__ bbox) {
-  const FrameDev &fc = F[Pp[blockIdx.y].cur];
+  const FrameDev &fc = F[blockIdx.z ? Pp[blockIdx.y].ref : Pp[blockIdx.y].cur];
   const int n = *fc.npoints;
   const float4 *__restrict__ pts = fc.pts;
-  bbox += 6 * blockIdx.y;
+  bbox += 12 * blockIdx.y + 6 * blockIdx.z;
   float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     const float4 p = pts[i];
@@ -1189,6 +1189,7 @@ struct GridDev {
   float ox, oy, oz, h, inv_h;
   int nx, ny, nz;
   int cell_base;
+  int order_base;  // the pair's first entry in the query order
 };
 
 __device__ __forceinline__ int cell_coord(float v, float o, float inv_h) { return (int)floorf((v - o) * inv_h); }
@@ -1289,14 +1290,225 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_nn_scatter(const PairDe
   sorted[2 * (size_t)slot + 1] = make_float4(ns * n.x, ns * n.y, ns * n.z, 0.0f);
 }
 
+// Query order: the reference points of every pair sorted by a Morton code of a coarse cubic
+// grid (2^bits cells per axis, bits in nx) over the reference cloud's bounding box, so that the
+// 64 queries of a wavefront lie close together. The order changes the speed only.
+__device__ __forceinline__ unsigned int spread3(unsigned int v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ __forceinline__ int order_key(const GridDev &o, const float4 &p) {
+  const int top = (1 << o.nx) - 1;
+  const unsigned int cx = (unsigned int)min(max(cell_coord(p.x, o.ox, o.inv_h), 0), top);
+  const unsigned int cy = (unsigned int)min(max(cell_coord(p.y, o.oy, o.inv_h), 0), top);
+  const unsigned int cz = (unsigned int)min(max(cell_coord(p.z, o.oz, o.inv_h), 0), top);
+  return o.cell_base + (int)(spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2));
+}
+extern "C" __global__ __launch_bounds__(kThreads) void k_nn_order_count(const PairDev *__restrict__ Pp,
+                                                                       const FrameDev *__restrict__ F,
+                                                                       const GridDev *__restrict__ ogrids,
+                                                                       int *__restrict__ counts) {
+  const FrameDev &fr = F[Pp[blockIdx.y].ref];
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= *fr.npoints) return;
+  atomicAdd(counts + order_key(ogrids[blockIdx.y], fr.pts[i]), 1);
+}
+extern "C" __global__ __launch_bounds__(kThreads) void k_nn_order_scatter(const PairDev *__restrict__ Pp,
+                                                                         const FrameDev *__restrict__ F,
+                                                                         const GridDev *__restrict__ ogrids,
+                                                                         int *__restrict__ cursor,
+                                                                         int *__restrict__ order) {
+  const FrameDev &fr = F[Pp[blockIdx.y].ref];
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= *fr.npoints) return;
+  order[atomicAdd(cursor + order_key(ogrids[blockIdx.y], fr.pts[i]), 1)] = i;
+}
+
+// 6D squared distance in FLANN's L2 functor order for 6 dimensions
+__device__ __forceinline__ float nn_dist6(const float q[6], float ax, float ay, float az, float bx, float by,
+                                          float bz) {
+  const float d0 = q[0] - ax, d1 = q[1] - ay, d2 = q[2] - az, d3 = q[3] - bx, d4 = q[4] - by, d5 = q[5] - bz;
+  float d = 0.0f + (d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3);
+  d += d4 * d4;
+  d += d5 * d5;
+  return d;
+}
+// keeps the smallest distance below the initial bound, ties to the lower index: the result
+// does not depend on the order in which candidates are seen
+__device__ __forceinline__ void nn_take(float d, int idx, float &best, int &bi) {
+  // non-short-circuit form: two selects instead of branches
+  const bool take = (d < best) | ((d == best) & (bi >= 0) & (idx < bi));
+  best = take ? d : best;
+  bi = take ? idx : bi;
+}
+// shell k >= 1 may be skipped once every point outside shells 0..k-1 is farther than best (a
+// point in a cell at Chebyshev cell distance m >= 1 is at least (m - 1) h away)
+__device__ __forceinline__ bool nn_done(int k, float h, float r2, float best) {
+  if (k < 1) return false;
+  const float lb = (float)(k - 1) * h;
+  const float lb2 = lb * lb * (1.0f - 1e-5f);
+  return lb2 >= r2 || best < lb2;
+}
+
+// one lane's search: shells of cells around its own cell (cx, cy, cz)
+__device__ void nn_walk(const float q[6], int cx, int cy, int cz, const GridDev &g, const int *__restrict__ cell_start,
+                        const float4 *__restrict__ sorted, float r2, float &best, int &bi) {
+  // Chebyshev distance from the query's cell to the farthest grid cell bounds the shells
+  const int kcap = max(max(max(abs(cx), abs(cx - (g.nx - 1))), max(abs(cy), abs(cy - (g.ny - 1)))),
+                       max(abs(cz), abs(cz - (g.nz - 1))));
+  for (int k = 0; k <= kcap; ++k) {
+    if (nn_done(k, g.h, r2, best)) break;
+    for (int dz = -k; dz <= k; ++dz) {
+      const int z = cz + dz;
+      if (z < 0 || z >= g.nz) continue;
+      for (int dy = -k; dy <= k; ++dy) {
+        const int y = cy + dy;
+        if (y < 0 || y >= g.ny) continue;
+        const bool face = (dz == -k || dz == k || dy == -k || dy == k);
+        const int step = face ? 1 : 2 * k;
+        for (int dx = -k; dx <= k; dx += (step > 0 ? step : 1)) {
+          const int x = cx + dx;
+          if (x >= 0 && x < g.nx) {
+            const size_t cell = ((size_t)z * g.ny + y) * g.nx + x;
+            const int s = cell_start[cell], e = cell_start[cell + 1];
+            for (int j = s; j < e; ++j) {
+              const float4 a = sorted[2 * (size_t)j], b = sorted[2 * (size_t)j + 1];
+              nn_take(nn_dist6(q, a.x, a.y, a.z, b.x, b.y, b.z), __float_as_int(a.w), best, bi);
+            }
+          }
+          if (k == 0) break;
+        }
+      }
+    }
+  }
+}
+
+// The points of up to two ranges per lane ([s1, s1 + n1) and [s2, s2 + n2) of the sorted
+// arena) against every lane's query. The ranges are packed into rounds of 64 points: point
+// `pos` of the concatenation belongs to the last lane whose exclusive offset is <= pos (a
+// binary search over the lanes' offsets). Each lane loads one point per round into the
+// wavefront's LDS slots, from where every lane reads each point at one (broadcast) address.
+// The running best is the key (distance bits << 32 | index): distances are >= +0, so one
+// unsigned compare orders by distance and then by index, as nn_take does.
+__device__ __forceinline__ void nn_scan_packed(const float q[6], const float4 *__restrict__ sorted, int s1, int n1,
+                                               int s2, int n2, int lane, float4 (*sh)[64],
+                                               unsigned long long &key) {
+  const int len = n1 + n2;
+  const int incl = wave_incl_scan(len);
+  const int total = __builtin_amdgcn_readlane(incl, 63);
+  const int excl = incl - len;
+  for (int b0 = 0; b0 < total; b0 += 64) {
+    const int pos = b0 + lane;
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+      const int v = __shfl(excl, lo + step, 64);
+      lo = v <= pos ? lo + step : lo;
+    }
+    // every shuffle outside any branch: ds_bpermute reads 0 from a lane that EXEC disables
+    const int o = pos - __shfl(excl, lo, 64);
+    const int m1 = __shfl(n1, lo, 64);
+    const int a1 = __shfl(s1, lo, 64), a2 = __shfl(s2, lo, 64);
+    const int j = o < m1 ? a1 + o : a2 + (o - m1);
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a;
+    if (pos < total) {
+      a = sorted[2 * (size_t)j];
+      b = sorted[2 * (size_t)j + 1];
+    }
+    sh[0][lane] = a;
+    sh[1][lane] = b;
+    // one wavefront writes and reads its own slots: wait for the writes, keep the order
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+    const int cnt = min(64, total - b0);
+#pragma unroll 4
+    for (int t = 0; t < cnt; ++t) {
+      const float4 pa = sh[0][t], pb = sh[1][t];
+      const float d = nn_dist6(q, pa.x, pa.y, pa.z, pb.x, pb.y, pb.z);
+      const unsigned long long c = ((unsigned long long)__float_as_uint(d) << 32) | __float_as_uint(pa.w);
+      key = c < key ? c : key;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+constexpr int kNNWaveBox = 8;  // widest box of query cells (per axis) a wavefront searches together
+
+// the wavefront's search: shells of cells around the box [lo, hi] of its active queries' cells.
+// Every cell within Chebyshev distance k of a lane's cell lies in the box grown by k, so after
+// shells 0..k-1 a lane has seen every cell the per-lane walk sees up to the same k, and stops
+// by the same bound. Lanes keep scanning (with no effect on their exact minimum) until all
+// lanes are done. 64 rows of a shell look up their cell ranges at once and scan them packed.
+__device__ void nn_wave(const float q[6], int lx, int ly, int lz, int hx, int hy, int hz, const GridDev &g,
+                        const int *__restrict__ cell_start, const float4 *__restrict__ sorted, float r2, bool active,
+                        int lane, float4 (*sh)[64], float &best, int &bi) {
+  const int kcap = max(max(max(lx, g.nx - 1 - hx), max(ly, g.ny - 1 - hy)), max(max(lz, g.nz - 1 - hz), 0));
+  // the initial key admits exactly the distances below r2 (none when r2 is NaN)
+  const unsigned long long key0 = r2 == r2 ? (unsigned long long)__float_as_uint(r2) << 32 : 0ull;
+  unsigned long long key = key0;
+  for (int k = 0; k <= kcap; ++k) {
+    best = __uint_as_float((unsigned int)(key >> 32));
+    if (__ballot(active && !nn_done(k, g.h, r2, best)) == 0) break;
+    const int zlo = max(lz - k, 0), zhi = min(hz + k, g.nz - 1), ylo = max(ly - k, 0), yhi = min(hy + k, g.ny - 1);
+    if (zlo > zhi || ylo > yhi) continue;
+    const int yc = yhi - ylo + 1, rows = yc * (zhi - zlo + 1);
+    for (int r0 = 0; r0 < rows; r0 += 64) {
+      const int r = r0 + lane;
+      int s1 = 0, e1 = 0, s2 = 0, e2 = 0;
+      if (r < rows) {
+        const int y = ylo + r % yc, z = zlo + r / yc;
+        const int row = (z * g.ny + y) * g.nx;
+        if (k == 0 || z == lz - k || z == hz + k || y == ly - k || y == hy + k) {
+          const int x0 = max(lx - k, 0), x1 = min(hx + k, g.nx - 1);
+          if (x0 <= x1) {
+            s1 = cell_start[row + x0];
+            e1 = cell_start[row + x1 + 1];
+          }
+        } else {
+          const int xa = lx - k, xb = hx + k;
+          if (xa >= 0 && xa < g.nx) {
+            s1 = cell_start[row + xa];
+            e1 = cell_start[row + xa + 1];
+          }
+          if (xb >= 0 && xb < g.nx) {
+            s2 = cell_start[row + xb];
+            e2 = cell_start[row + xb + 1];
+          }
+        }
+      }
+      nn_scan_packed(q, sorted, s1, max(e1 - s1, 0), s2, max(e2 - s2, 0), lane, sh, key);
+    }
+  }
+  best = __uint_as_float((unsigned int)(key >> 32));
+  bi = key < key0 ? (int)(unsigned int)key : -1;
+}
+
 // one thread per reference point of pair blockIdx-mapped pd: nearest neighbour + gates.
-// The pair's "image" is 1 x N_ref; pairs[i] = (i, j) or (-1, -1).
+// The pair's "image" is 1 x N_ref; pairs[i] = (i, j) or (-1, -1). Threads take the queries in
+// the pair's order (identity when order is null); every lane of a wavefront stays in the loop,
+// as the wave search needs all of them.
 extern "C" __global__ __launch_bounds__(kThreads) void k_nn_query(const PairDev *__restrict__ Pp,
                                                                  const FrameDev *__restrict__ F,
                                                                  const int *__restrict__ blk_pair,
                                                                  const GridDev *__restrict__ grids,
                                                                  const int *__restrict__ cell_arena,
-                                                                 const float4 *__restrict__ sorted, AlignParamsDev ap,
+                                                                 const float4 *__restrict__ sorted,
+                                                                 const int *__restrict__ order, AlignParamsDev ap,
                                                                  float ns, int *__restrict__ nn_out) {
   const int gb = blockIdx.x;
   const int pi = blk_pair[gb];
@@ -1307,6 +1519,8 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_nn_query(const PairDev 
   const FrameDev &fc = F[pd.cur];
   const int nref = *fr.npoints;
   const int base = (gb - pd.tile_base) * kTile;
+  const int lane = threadIdx.x & 63;
+  __shared__ float4 nn_sh[kThreads / 64][2][64];  // per wavefront: the points of one scan round
   float T[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) T[k] = pd.state[kSlotTinv + k];
@@ -1318,64 +1532,39 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_nn_query(const PairDev 
     for (int r = 0; r < 3; ++r) sR[c * 3 + r] = mc(T, r, c) * ns;
   const float r2 = ap.sq_thr;
   for (int kk = 0; kk < 4; ++kk) {
-    const int i = base + kk * kThreads + threadIdx.x;
-    if (i >= pd.cols) break;
-    if (i >= nref) {
-      pd.pairs[i] = make_int2(-1, -1);
-      continue;
+    const int j = base + kk * kThreads + threadIdx.x;
+    const bool active = j < nref;
+    const int i = active && order ? order[g.order_base + j] : j;
+    float4 p = make_float4(0.0f, 0.0f, 0.0f, 1.0f), n = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (active) {
+      p = fr.pts[i];
+      n = fr.nrm[i];
     }
-    const float4 p = fr.pts[i];
-    const float4 n = fr.nrm[i];
     float q[6], pt[3];
     iso_mul3(T, p.x, p.y, p.z, p.w, pt);
     q[0] = pt[0]; q[1] = pt[1]; q[2] = pt[2];
 #pragma unroll
     for (int r = 0; r < 3; ++r) q[3 + r] = red3(sR[r] * n.x, sR[3 + r] * n.y, sR[6 + r] * n.z);
-    const int cx = cell_coord(q[0], g.ox, g.inv_h), cy = cell_coord(q[1], g.oy, g.inv_h),
-              cz = cell_coord(q[2], g.oz, g.inv_h);
+    // cell coordinates, held within +-2^20 (a farther query only searches more shells than it
+    // needs: its true cell distances are larger than the held ones)
+    constexpr int kFar = 1 << 20;
+    const int cx = min(max(cell_coord(q[0], g.ox, g.inv_h), -kFar), kFar);
+    const int cy = min(max(cell_coord(q[1], g.oy, g.inv_h), -kFar), kFar);
+    const int cz = min(max(cell_coord(q[2], g.oz, g.inv_h), -kFar), kFar);
     float best = r2;
     int bi = -1;
-    // Chebyshev distance from the query's cell to the farthest grid cell bounds the shells
-    const int kcap = max(max(max(abs(cx), abs(cx - (g.nx - 1))), max(abs(cy), abs(cy - (g.ny - 1)))),
-                         max(abs(cz), abs(cz - (g.nz - 1))));
-    for (int k = 0; k <= kcap; ++k) {
-      if (k >= 1) {
-        const float lb = (float)(k - 1) * g.h;
-        const float lb2 = lb * lb * (1.0f - 1e-5f);
-        if (lb2 >= r2 || best < lb2) break;
-      }
-      for (int dz = -k; dz <= k; ++dz) {
-        const int z = cz + dz;
-        if (z < 0 || z >= g.nz) continue;
-        for (int dy = -k; dy <= k; ++dy) {
-          const int y = cy + dy;
-          if (y < 0 || y >= g.ny) continue;
-          const bool face = (dz == -k || dz == k || dy == -k || dy == k);
-          const int step = face ? 1 : 2 * k;
-          for (int dx = -k; dx <= k; dx += (step > 0 ? step : 1)) {
-            const int x = cx + dx;
-            if (x >= 0 && x < g.nx) {
-              const size_t cell = ((size_t)z * g.ny + y) * g.nx + x;
-              const int s = cell_start[cell], e = cell_start[cell + 1];
-              for (int j = s; j < e; ++j) {
-                const float4 a = sorted[2 * (size_t)j], b = sorted[2 * (size_t)j + 1];
-                const float d0 = q[0] - a.x, d1 = q[1] - a.y, d2 = q[2] - a.z, d3 = q[3] - b.x, d4 = q[4] - b.y,
-                            d5 = q[5] - b.z;
-                // FLANN's L2 functor order for 6 dimensions
-                float d = 0.0f + (d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3);
-                d += d4 * d4;
-                d += d5 * d5;
-                const int idx = __float_as_int(a.w);
-                if (d < best || (d == best && bi >= 0 && idx < bi)) {
-                  best = d;
-                  bi = idx;
-                }
-              }
-            }
-            if (k == 0) break;
-          }
-        }
-      }
+    const int lx = wave_min_i(active ? cx : INT_MAX), hx = wave_max_i(active ? cx : INT_MIN);
+    if (lx <= hx) {  // some lane has a query
+      const int ly = wave_min_i(active ? cy : INT_MAX), hy = wave_max_i(active ? cy : INT_MIN);
+      const int lz = wave_min_i(active ? cz : INT_MAX), hz = wave_max_i(active ? cz : INT_MIN);
+      if (hx - lx < kNNWaveBox && hy - ly < kNNWaveBox && hz - lz < kNNWaveBox)
+        nn_wave(q, lx, ly, lz, hx, hy, hz, g, cell_start, sorted, r2, active, lane, nn_sh[threadIdx.x >> 6], best, bi);
+      else if (active)
+        nn_walk(q, cx, cy, cz, g, cell_start, sorted, r2, best, bi);
+    }
+    if (!active) {
+      if (j < pd.cols) pd.pairs[j] = make_int2(-1, -1);
+      continue;
     }
     if (nn_out) nn_out[i] = bi;
     bool ok = bi >= 0;
