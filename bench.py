@@ -76,6 +76,9 @@ def kernel_bytes(name, pairs_info, frames_info):
         return 42 * P
     if name == "k_integral_y":
         return 80 * P
+    if name == "k_integral_frame":
+        # both prefix passes at once: depth in (2 B), the 40 B integral image out, per pixel
+        return 42 * P
     if name == "k_unproject":
         return 6 * P + 16 * N
     if name == "k_count":

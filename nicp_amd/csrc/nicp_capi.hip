@@ -102,7 +102,7 @@ struct Carver {
 enum KernelId {
   KID_COUNT, KID_SCAN, KID_UNPROJECT, KID_INTEGRAL_X, KID_INTEGRAL_Y, KID_STATS, KID_ZFILL, KID_ZPROJECT_CUR,
   KID_ZRESOLVE, KID_ZPROJECT_REF, KID_CORR_LIN, KID_SOLVE, KID_LIN_PAIRS, KID_SOLVE_STATS, KID_NN_QUERY,
-  KID_NET_INPUT, KID_NET_FORWARD, KID_START_PROBE, KID_N
+  KID_NET_INPUT, KID_NET_FORWARD, KID_START_PROBE, KID_INTEGRAL_FRAME, KID_N
 };
 // the last three are stages: the network input kernel, the network's forward pass (its six
 // GEMM launches) and the one-iteration probe of both starts of the learned-start aligner
@@ -110,7 +110,7 @@ static const char *kKernelNames[KID_N] = {"k_count", "k_scan", "k_unproject", "k
                                           "k_stats", "k_zfill", "k_zproject(current)", "k_zresolve",
                                           "k_zproject(reference)", "k_corr_lin", "k_solve", "k_lin_pairs",
                                           "k_solve(statistics)", "k_nn_query", "k_net_input", "initnet_forward",
-                                          "start_probe"};
+                                          "start_probe", "k_integral_frame"};
 
 namespace {
 struct ProfEntry {
@@ -422,6 +422,11 @@ void frame_pointers(FrameDev &f, char *base, const FrameLayout &L, bool with_sta
   f.tile_off = (int *)(base + L.toff);
 }
 
+// Frames per conversion from which both prefix passes run as one workgroup per frame
+// (k_integral_frame, the same values): with fewer frames the per-row and per-column
+// kernels spread one frame over more compute units.
+constexpr int kFusedIntegralFrames = 16;
+
 // Launch sequence of the conversion for a frame table already on the device.
 // All frames of one launch share the projector model.
 int launch_convert(Prof *pf, hipStream_t s, const FrameDev *d_frames, int num_frames, const int *d_blk, int total_tiles,
@@ -434,9 +439,14 @@ int launch_convert(Prof *pf, hipStream_t s, const FrameDev *d_frames, int num_fr
   LAUNCH(pf, KID_SCAN, k_scan, dim3(num_frames), dim3(1024), s, d_frames);
   if (sph) LAUNCH(pf, KID_UNPROJECT, k_unproject_sph, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
   else LAUNCH(pf, KID_UNPROJECT, k_unproject, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
-  if (sph) LAUNCH(pf, KID_INTEGRAL_X, k_integral_x_sph, dim3(total_x), dim3(kThreads), s, d_frames, d_x);
-  else LAUNCH(pf, KID_INTEGRAL_X, k_integral_x, dim3(total_x), dim3(kThreads), s, d_frames, d_x);
-  LAUNCH(pf, KID_INTEGRAL_Y, k_integral_y, dim3(total_y), dim3(kThreads), s, d_frames, d_y);
+  if (num_frames >= kFusedIntegralFrames) {
+    if (sph) LAUNCH(pf, KID_INTEGRAL_FRAME, k_integral_frame_sph, dim3(num_frames), dim3(kIFrameThreads), s, d_frames);
+    else LAUNCH(pf, KID_INTEGRAL_FRAME, k_integral_frame, dim3(num_frames), dim3(kIFrameThreads), s, d_frames);
+  } else {
+    if (sph) LAUNCH(pf, KID_INTEGRAL_X, k_integral_x_sph, dim3(total_x), dim3(kThreads), s, d_frames, d_x);
+    else LAUNCH(pf, KID_INTEGRAL_X, k_integral_x, dim3(total_x), dim3(kThreads), s, d_frames, d_x);
+    LAUNCH(pf, KID_INTEGRAL_Y, k_integral_y, dim3(total_y), dim3(kThreads), s, d_frames, d_y);
+  }
   if (sph) LAUNCH(pf, KID_STATS, k_stats_sph, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk, sp);
   else LAUNCH(pf, KID_STATS, k_stats, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk, sp);
   HIPCHK(hipGetLastError());

@@ -347,6 +347,184 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_integral_y(const FrameD
   }
 }
 
+// Both prefix passes in one workgroup per frame (for batches of many frames). The frame is
+// walked in bands of kIBand rows and, within a band, chunks of kIChunk columns. Waves 0-7
+// compute a chunk's x-prefixes exactly as k_integral_x does (one thread per (row, channel),
+// sequential along the row, the carry kept across the band's chunks) into one of two LDS
+// tiles, while waves 8-15 take the previous chunk's tile and carry its columns down the band
+// exactly as k_integral_y does (one thread per (column, channel), sequential down the rows,
+// starting from the integral image's row above the band, which the same thread wrote), and
+// write every row segment of the integral image once, coalesced. The values are those of the
+// two-kernel path, bit for bit; no workgroup waits for another.
+constexpr int kIBand = 40, kIChunk = 48;
+constexpr int kIStride = kIChunk * 10 + 1;  // LDS row stride (floats): rows fall in different banks
+constexpr int kIFrameThreads = 1024;
+
+// global-address-space view of a device pointer held in a frame table (flat loads would
+// also count as LDS operations and wait with them)
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gview(const T *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+
+// rows ci and cj of iKRt: a thread's channel p[ci] * p[cj] of the point p = iKRt (c d, r d, d, 1)
+// needs only those two rows of mul4_packet, in its association order
+struct XRows {
+  float a[4], b[4];
+};
+__device__ __forceinline__ XRows x_rows(const float *iK, int ci, int cj) {
+  XRows x;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    x.a[k] = iK[k * 4 + ci];
+    x.b[k] = iK[k * 4 + cj];
+  }
+  return x;
+}
+// the channel at (row, col) from depth d (0 when invalid), bit for bit k_integral_x's value
+__device__ __forceinline__ float x_channel_depth(const FrameDev &fd, const XRows &x, int row, int col, float d) {
+  const float v0 = (float)col * d, v1 = (float)row * d;
+  float a = x.a[0] * v0;
+  a = x.a[1] * v1 + a;
+  a = x.a[2] * d + a;
+  a = x.a[3] * 1.0f + a;
+  float b = x.b[0] * v0;
+  b = x.b[1] * v1 + b;
+  b = x.b[2] * d + b;
+  b = x.b[3] * 1.0f + b;
+  return depth_valid(d, fd.min_d, fd.max_d) ? a * b : 0.0f;
+}
+// component i of p as a chain of selects
+__device__ __forceinline__ float sel4(int i, const float4 &p) {
+  float a = p.x;
+  a = i == 1 ? p.y : a;
+  a = i == 2 ? p.z : a;
+  a = i == 3 ? p.w : a;
+  return a;
+}
+
+// eight consecutive channel values of a row, the loads first
+template <int M>
+__device__ __forceinline__ void x_channels8(const FrameDev &fd, const XRows &x, int row, int col, int ci, int cj,
+                                            float v[8]) {
+  const size_t q = (size_t)row * fd.cols + col;
+  if (M == kSpherical) {
+    int idx[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) idx[u] = gview(fd.index)[q + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const float4 p = fd.pts[idx[u] >= 0 ? idx[u] : 0];
+      v[u] = idx[u] >= 0 ? sel4(ci, p) * sel4(cj, p) : 0.0f;
+    }
+    return;
+  }
+  float d[8];
+  if (fd.depthf) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) d[u] = gview(fd.depthf)[q + u];
+  } else {
+    uint16_t raw[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) raw[u] = gview(fd.depth)[q + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) d[u] = depth_of(raw[u], fd.depth_scale);
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u) v[u] = x_channel_depth(fd, x, row, col + u, d[u]);
+}
+template <int M>
+__device__ __forceinline__ float x_channel1(const FrameDev &fd, const XRows &x, int row, int col, int ci, int cj) {
+  const size_t q = (size_t)row * fd.cols + col;
+  if (M == kSpherical) {
+    const int idx = gview(fd.index)[q];
+    const float4 p = fd.pts[idx >= 0 ? idx : 0];
+    return idx >= 0 ? sel4(ci, p) * sel4(cj, p) : 0.0f;
+  }
+  const float d = fd.depthf ? gview(fd.depthf)[q] : depth_of(gview(fd.depth)[q], fd.depth_scale);
+  return x_channel_depth(fd, x, row, col, d);
+}
+
+template <int M>
+__device__ __forceinline__ void integral_frame_body(const FrameDev *__restrict__ F) {
+  __shared__ float tile[2][kIBand * kIStride];
+  const FrameDev &fd = F[blockIdx.x];
+  const int rows = fd.rows, cols = fd.cols;
+  const int nchunks = (cols + kIChunk - 1) / kIChunk, nbands = (rows + kIBand - 1) / kIBand;
+  const int steps = nbands * nchunks;
+  const int t = threadIdx.x;
+  const bool xside = t < kIFrameThreads / 2;
+  // x side: (row in band, channel); y side: (column in chunk, channel) as one slot index
+  const int xr = t / 10, xch = t - xr * 10;
+  const int ys = t - kIFrameThreads / 2, ycc = ys / 10;
+  const bool xact = xside && t < kIBand * 10, yact = !xside && ys < kIChunk * 10;
+  const int ci = c_ch_i[xch < 10 ? xch : 0], cj = c_ch_j[xch < 10 ? xch : 0];
+  __attribute__((address_space(1))) float *ii = (__attribute__((address_space(1))) float *)fd.ii;
+  const XRows xrw = x_rows(fd.iKRt, ci, cj);
+  float xacc = 0.0f;
+  for (int st = 0; st <= steps; ++st) {
+    if (xact && st < steps) {
+      const int b = st / nchunks, k = st - b * nchunks;
+      const int row = b * kIBand + xr, c0 = k * kIChunk;
+      if (k == 0) xacc = 0.0f;
+      float *out = &tile[st & 1][xr * kIStride + xch];
+      if (row < rows) {
+        const int cn = min(kIChunk, cols - c0);
+        int cc = 0;
+        // eight columns' loads in flight at a time; the prefix itself stays sequential
+        for (; cc + 8 <= cn; cc += 8) {
+          float v[8];
+          x_channels8<M>(fd, xrw, row, c0 + cc, ci, cj, v);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            xacc = v[u] + xacc;
+            out[(cc + u) * 10] = xacc;
+          }
+        }
+        for (; cc < cn; ++cc) {
+          xacc = x_channel1<M>(fd, xrw, row, c0 + cc, ci, cj) + xacc;
+          out[cc * 10] = xacc;
+        }
+      }
+    }
+    if (yact && st >= 1) {
+      const int sp = st - 1, b = sp / nchunks, k = sp - b * nchunks;
+      const int col = k * kIChunk + ycc, row0 = b * kIBand;
+      if (col < cols) {
+        // slot ys of the chunk is column c0 + ycc, channel ys % 10: consecutive threads write
+        // consecutive floats of the row segment
+        const size_t rowstride = (size_t)cols * 10;
+        const size_t o = (size_t)row0 * rowstride + (size_t)k * kIChunk * 10 + ys;
+        float acc = row0 > 0 ? ii[o - rowstride] : 0.0f;
+        const float *src = &tile[sp & 1][ys];
+        const int rn = min(kIBand, rows - row0);
+        int r = 0;
+        for (; r + 8 <= rn; r += 8) {
+          float v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) v[u] = src[(r + u) * kIStride];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            acc = v[u] + acc;
+            ii[o + (size_t)(r + u) * rowstride] = acc;
+          }
+        }
+        for (; r < rn; ++r) {
+          acc = src[r * kIStride] + acc;
+          ii[o + (size_t)r * rowstride] = acc;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+extern "C" __global__ __launch_bounds__(kIFrameThreads) void k_integral_frame(const FrameDev *__restrict__ F) {
+  integral_frame_body<kPinhole>(F);
+}
+extern "C" __global__ __launch_bounds__(kIFrameThreads) void k_integral_frame_sph(const FrameDev *__restrict__ F) {
+  integral_frame_body<kSpherical>(F);
+}
+
 // ---- Eigen SelfAdjointEigenSolver<Matrix3f>::computeDirect, restated (3x3 FP32) ----
 namespace {
 
@@ -504,14 +682,23 @@ __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const
   const FrameDev &fd = F[blk_frame[g]];
   const int rows = fd.rows, cols = fd.cols, P = rows * cols;
   const int base = (g - fd.tile_base) * kTile;
+  // stage 1, all four pixels at once: index and depth loads, window radius and the row-major
+  // offsets of the window's corners, so that the corner reads below wait on one load only
+  int idxs[4], offs[4][4];
+  float ds[4];
+  bool oks[4];
+#pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int p = base + k * kThreads + threadIdx.x;
-    if (p >= P) break;
-    const int idx = fd.index[p];
-    if (idx < 0) continue;
+    idxs[k] = p < P ? fd.index[p] : -1;
+    ds[k] = p < P ? frame_depth(fd, p) : 0.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int p = base + k * kThreads + threadIdx.x;
     const int r = p / cols, c = p - r * cols;
-    const float d = frame_depth(fd, p);
-    bool ok = true;
+    const float d = ds[k];
+    bool ok = idxs[k] >= 0;
     int R0 = -1, R1 = -1;
     if (d < fd.min_d || d > fd.max_d) {
       ok = false;
@@ -531,15 +718,29 @@ __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const
       if (R > sp.max_r) R = sp.max_r;
       R0 = R1 = R;
     }
+    const int xmin = min(max(c - R0 - 1, 0), cols - 1), xmax = min(max(c + R0 - 1, 0), cols - 1);
+    const int ymin = min(max(r - R1 - 1, 0), rows - 1), ymax = min(max(r + R1 - 1, 0), rows - 1);
+    offs[k][0] = ymax * cols + xmax;  // A
+    offs[k][1] = ymin * cols + xmin;  // D
+    offs[k][2] = ymax * cols + xmin;  // B
+    offs[k][3] = ymin * cols + xmax;  // C
+    oks[k] = ok;
+  }
+  // stage 2: per pixel, the window sums, the eigensolve and the outputs
+  for (int k = 0; k < 4; ++k) {
+    const int p = base + k * kThreads + threadIdx.x;
+    const int idx = idxs[k];
+    if (idx < 0) continue;
+    const int r = p / cols, c = p - r * cols;
+    const float d = ds[k];
+    bool ok = oks[k];
     float acc[10];
     int n = 0;
     if (ok) {
-      int xmin = min(max(c - R0 - 1, 0), cols - 1), xmax = min(max(c + R0 - 1, 0), cols - 1);
-      int ymin = min(max(r - R1 - 1, 0), rows - 1), ymax = min(max(r + R1 - 1, 0), rows - 1);
-      const float2 *A = (const float2 *)(fd.ii + ((size_t)ymax * cols + xmax) * 10);
-      const float2 *D = (const float2 *)(fd.ii + ((size_t)ymin * cols + xmin) * 10);
-      const float2 *B = (const float2 *)(fd.ii + ((size_t)ymax * cols + xmin) * 10);
-      const float2 *C = (const float2 *)(fd.ii + ((size_t)ymin * cols + xmax) * 10);
+      const float2 *A = (const float2 *)(fd.ii + (size_t)offs[k][0] * 10);
+      const float2 *D = (const float2 *)(fd.ii + (size_t)offs[k][1] * 10);
+      const float2 *B = (const float2 *)(fd.ii + (size_t)offs[k][2] * 10);
+      const float2 *C = (const float2 *)(fd.ii + (size_t)offs[k][3] * 10);
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
         float2 a = A[q], dd = D[q], bb = B[q], cc = C[q];

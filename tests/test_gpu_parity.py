@@ -249,3 +249,35 @@ def test_depth_pyramid_and_float_input(ctx):
     assert a.n == b.n
     for x, y in zip(a.download(), b.download()):
         np.testing.assert_array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+def test_batch_conversion_equals_single(ctx):
+    """A batch of many frames (>= 16: both prefix passes then run as one workgroup per frame)
+    converts every frame to the clouds of single-frame conversion (the per-row and per-column
+    prefix kernels), bit for bit: index images, points, normals, curvature and information
+    matrices. Odd sizes exercise partial bands and chunks."""
+    say("case batch_conversion_equals_single")
+    shapes = [(120, 160), (480, 640), (37, 53), (1, 1), (240, 320), (41, 49), (81, 96), (960, 1280)] * 2
+    _, dsp = params(min_image_radius=1, min_points=3)
+    frames, cam_list = [], []
+    for i, (rows, cols) in enumerate(shapes):
+        if rows * cols >= 60 * 80:
+            d = _frames(60 + i, rows, cols)[0]
+        else:
+            rng = np.random.default_rng(60 + i)
+            d = rng.uniform(800, 3000, (rows, cols))
+            d[rng.uniform(size=d.shape) < 0.1] = 0
+            d = d.astype(np.uint16)
+        frames.append(d)
+        cam_list.append(cams(rows, cols)[1])
+    b = capi.Batch(ctx, shapes, cam_list, dsp)
+    for k, f in enumerate(frames):
+        b.upload(k, f)
+    b.convert()
+    for k, (rows, cols) in enumerate(shapes):
+        s = ctx.cloud_from_depth(frames[k], cam_list[k], dsp)
+        bc = b.frame_cloud(k, (rows, cols))
+        assert bc.n == s.n, k
+        np.testing.assert_array_equal(bc.index_image(), s.index_image())
+        for x, y in zip(bc.download(), s.download()):
+            np.testing.assert_array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))

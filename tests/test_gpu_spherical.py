@@ -174,3 +174,23 @@ def test_front_end_spherical_projector(ctx, tmp_path):
     T, res = ctx.align(cr, cc, dc, rows, cols, identity16(), capi.align_params(outer_iterations=20))
     assert np.array_equal(Tf, T)
     assert int(lines["inliers"][0]) == res.inliers
+
+
+def test_batch_conversion_equals_single_spherical(ctx):
+    """16 panoramic frames in one batch (both prefix passes as one workgroup per frame, the
+    points read through the index image) convert bit for bit as single frames do."""
+    rows, cols = 90, 180
+    print("case spherical batch conversion", flush=True)
+    oc, dc = spherical_cams(rows, cols)
+    dsp = capi.stats_params(**SP)
+    frames = [spherical_room(rows, cols, oc, seed=40 + k) for k in range(16)]
+    b = capi.Batch(ctx, [(rows, cols)] * 16, [dc] * 16, dsp)
+    for i, d in enumerate(frames):
+        b.upload(i, d)
+    b.convert()
+    for k in range(16):
+        s = ctx.cloud_from_depth(frames[k], dc, dsp)
+        bc = b.frame_cloud(k, (rows, cols))
+        assert bc.n == s.n
+        for x, y in zip(bc.download(), s.download()):
+            assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32)), k
