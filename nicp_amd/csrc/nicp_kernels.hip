@@ -394,13 +394,12 @@ __device__ __forceinline__ float x_channel_depth(const FrameDev &fd, const XRows
   b = x.b[3] * 1.0f + b;
   return depth_valid(d, fd.min_d, fd.max_d) ? a * b : 0.0f;
 }
-// component i of p as a chain of selects
+// component i of p by two levels of selects on the bits of i (a chain of compares is turned
+// into an indexed stack copy)
 __device__ __forceinline__ float sel4(int i, const float4 &p) {
-  float a = p.x;
-  a = i == 1 ? p.y : a;
-  a = i == 2 ? p.z : a;
-  a = i == 3 ? p.w : a;
-  return a;
+  const bool odd = (i & 1) != 0, high = (i & 2) != 0;
+  const float lo = odd ? p.y : p.x, hi = odd ? p.w : p.z;
+  return high ? hi : lo;
 }
 
 // eight consecutive channel values of a row, the loads first
@@ -462,15 +461,60 @@ __device__ __forceinline__ void integral_frame_body(const FrameDev *__restrict__
   __attribute__((address_space(1))) float *ii = (__attribute__((address_space(1))) float *)fd.ii;
   const XRows xrw = x_rows(fd.iKRt, ci, cj);
   float xacc = 0.0f;
+  // the y side's carry for its next chunk, loaded a step ahead (after this step's writes,
+  // which hold it when the frame has a single chunk): the row above a band was written
+  // nchunks steps earlier and has usually left the cache
+  float ycarry = 0.0f;
+  // Pinhole frames with uint16 depth and a width that is a multiple of 8 (every row then
+  // starts 16-byte aligned): the x side loads the depths of its next chunk's full groups of
+  // eight during the current step
+  const bool pre = M == kPinhole && !fd.depthf && (cols & 7) == 0;
+  const __attribute__((address_space(1))) unsigned int *d16 =
+      (const __attribute__((address_space(1))) unsigned int *)fd.depth;
+  uint4 dnext[kIChunk / 8];
+#pragma unroll
+  for (int j = 0; j < kIChunk / 8; ++j) dnext[j] = make_uint4(0u, 0u, 0u, 0u);
+  auto prefetch = [&](int st2) {
+    const int b2 = st2 / nchunks, k2 = st2 - b2 * nchunks;
+    const int row2 = b2 * kIBand + xr, c2 = k2 * kIChunk;
+#pragma unroll
+    for (int j = 0; j < kIChunk / 8; ++j)
+      if (row2 < rows && c2 + 8 * j + 8 <= cols) {
+        const size_t w = ((size_t)row2 * cols + c2) / 2 + 4 * j;  // 32-bit words, 16-byte aligned
+        dnext[j] = make_uint4(d16[w], d16[w + 1], d16[w + 2], d16[w + 3]);
+      }
+  };
+  if (pre && xact && steps > 0) prefetch(0);
   for (int st = 0; st <= steps; ++st) {
     if (xact && st < steps) {
       const int b = st / nchunks, k = st - b * nchunks;
       const int row = b * kIBand + xr, c0 = k * kIChunk;
       if (k == 0) xacc = 0.0f;
       float *out = &tile[st & 1][xr * kIStride + xch];
+      uint4 dcur[kIChunk / 8];
+      if constexpr (M == kPinhole) {
+#pragma unroll
+        for (int j = 0; j < kIChunk / 8; ++j) dcur[j] = dnext[j];
+        if (pre && st + 1 < steps) prefetch(st + 1);
+      }
       if (row < rows) {
         const int cn = min(kIChunk, cols - c0);
         int cc = 0;
+        if (M == kPinhole && pre) {
+#pragma unroll
+          for (int j = 0; j < kIChunk / 8; ++j) {
+            if (8 * j + 8 <= cn) {
+              const unsigned int w[4] = {dcur[j].x, dcur[j].y, dcur[j].z, dcur[j].w};
+#pragma unroll
+              for (int u = 0; u < 8; ++u) {
+                const float d = depth_of((uint16_t)(w[u >> 1] >> (16 * (u & 1))), fd.depth_scale);
+                xacc = x_channel_depth(fd, xrw, row, c0 + 8 * j + u, d) + xacc;
+                out[(8 * j + u) * 10] = xacc;
+              }
+              cc = 8 * j + 8;
+            }
+          }
+        }
         // eight columns' loads in flight at a time; the prefix itself stays sequential
         for (; cc + 8 <= cn; cc += 8) {
           float v[8];
@@ -495,7 +539,7 @@ __device__ __forceinline__ void integral_frame_body(const FrameDev *__restrict__
         // consecutive floats of the row segment
         const size_t rowstride = (size_t)cols * 10;
         const size_t o = (size_t)row0 * rowstride + (size_t)k * kIChunk * 10 + ys;
-        float acc = row0 > 0 ? ii[o - rowstride] : 0.0f;
+        float acc = ycarry;
         const float *src = &tile[sp & 1][ys];
         const int rn = min(kIBand, rows - row0);
         int r = 0;
@@ -514,6 +558,11 @@ __device__ __forceinline__ void integral_frame_body(const FrameDev *__restrict__
           ii[o + (size_t)r * rowstride] = acc;
         }
       }
+    }
+    if (yact && st < steps) {
+      const int b = st / nchunks, k = st - b * nchunks;
+      const int col = k * kIChunk + ycc, row0 = b * kIBand;
+      ycarry = col < cols && row0 > 0 ? ii[(size_t)(row0 - 1) * cols * 10 + (size_t)k * kIChunk * 10 + ys] : 0.0f;
     }
     __syncthreads();
   }

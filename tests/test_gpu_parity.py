@@ -257,7 +257,7 @@ def test_batch_conversion_equals_single(ctx):
     prefix kernels), bit for bit: index images, points, normals, curvature and information
     matrices. Odd sizes exercise partial bands and chunks."""
     say("case batch_conversion_equals_single")
-    shapes = [(120, 160), (480, 640), (37, 53), (1, 1), (240, 320), (41, 49), (81, 96), (960, 1280)] * 2
+    shapes = [(120, 160), (480, 640), (37, 53), (1, 1), (240, 320), (41, 49), (81, 96), (960, 1280), (130, 20)] * 2
     _, dsp = params(min_image_radius=1, min_points=3)
     frames, cam_list = [], []
     for i, (rows, cols) in enumerate(shapes):
