@@ -224,7 +224,7 @@ struct nicp_ctx {
   Prof prof;
   std::vector<ProfEntry> *pending_graph = nullptr;  // graph events not yet read
   // kNN grid of the current cloud
-  DevBuf grid_cells, grid_cursor, grid_sorted, grid_aux, nn_out, nn_order;
+  DevBuf grid_cells, grid_cursor, grid_sorted, grid_aux, nn_out, nn_order, grid_cells_c, grid_sorted_c;
 };
 
 struct nicp_cloud {
@@ -887,7 +887,8 @@ extern "C" int nicp_ctx_destroy(nicp_ctx *c) {
   (void)hipStreamSynchronize(c->stream);
   DevBuf *bufs[] = {&c->frames_tbl, &c->pairs_tbl, &c->maps, &c->zbuf, &c->cur_index, &c->pairs_buf,
                     &c->state, &c->partials, &c->result, &c->depth_tmp, &c->scratch_in, &c->grid_cells,
-                    &c->grid_cursor, &c->grid_sorted, &c->grid_aux, &c->nn_out, &c->nn_order};
+                    &c->grid_cursor, &c->grid_sorted, &c->grid_aux, &c->nn_out, &c->nn_order, &c->grid_cells_c,
+                    &c->grid_sorted_c};
   for (DevBuf *b : bufs) b->release();
   c->prof.release();
   (void)hipEventDestroy(c->ev0);
@@ -1869,19 +1870,32 @@ int scan_arena(nicp_ctx *ctx, int *cells, int m, int *cursor, int *sums, int *to
 // arena holding each pair's cell starts plus a sentinel (an exclusive scan of counts over
 // the whole arena, so every pair's sorted points follow the previous pair's), and the clouds
 // sorted by cell. The nearest neighbour the query finds is exact at any cell size, so a pair
-// whose grid would exceed its share of the arena gets coarser cells and the same result.
-// Also the query order: each pair's reference indices sorted by the Morton code of a coarse
-// cubic grid over the reference cloud (ctx->nn_order, pair p from grids[p].order_base).
+// whose grid would exceed its share of the arena gets coarser cells and the same result, and
+// a call may search with either of two grids: the fine one (cell) and, when asked for, a
+// coarse one (kCoarseCells x cell) for searches over a wide radius (the first iteration
+// from an initial guess). Also the query order: each pair's reference indices sorted by the
+// Morton code of a coarse cubic grid over the reference cloud (ctx->nn_order, pair p from
+// grids[p].order_base).
+constexpr float kCoarseCells = 3.0f;
+
+struct NNGrids {
+  const GridDev *fine = nullptr, *coarse = nullptr;  // device tables of np grids
+  const int *cells_fine = nullptr, *cells_coarse = nullptr;
+  const float4 *sorted_fine = nullptr, *sorted_coarse = nullptr;
+};
+
 int build_grids(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const nicp_cloud *const *curs,
-                const PairDev *d_pairs, const FrameDev *d_frames, float ns, float cell, GridDev **d_grids) {
+                const PairDev *d_pairs, const FrameDev *d_frames, float ns, float cell, bool with_coarse,
+                NNGrids &out) {
   hipStream_t s = ctx->stream;
   const size_t bbox_bytes = align_up((size_t)np * 12 * sizeof(unsigned int), 256);
   const size_t grid_bytes = align_up((size_t)np * sizeof(GridDev), 256);
-  CHK(ctx->grid_aux.reserve(bbox_bytes + 2 * grid_bytes + 64 * sizeof(int) + 16384 * sizeof(int)));
+  CHK(ctx->grid_aux.reserve(bbox_bytes + 3 * grid_bytes + 64 * sizeof(int) + 16384 * sizeof(int)));
   unsigned int *d_bbox = ctx->grid_aux.as<unsigned int>();
   GridDev *grids_d = (GridDev *)(ctx->grid_aux.as<char>() + bbox_bytes);
-  GridDev *ogrids_d = (GridDev *)(ctx->grid_aux.as<char>() + bbox_bytes + grid_bytes);
-  int *total = (int *)(ctx->grid_aux.as<char>() + bbox_bytes + 2 * grid_bytes);
+  GridDev *cgrids_d = (GridDev *)(ctx->grid_aux.as<char>() + bbox_bytes + grid_bytes);
+  GridDev *ogrids_d = (GridDev *)(ctx->grid_aux.as<char>() + bbox_bytes + 2 * grid_bytes);
+  int *total = (int *)(ctx->grid_aux.as<char>() + bbox_bytes + 3 * grid_bytes);
   int *sums = total + 64;
   std::vector<unsigned int> bb((size_t)np * 12);
   int nmax = 1;
@@ -1898,46 +1912,22 @@ int build_grids(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const nicp
   }
   HIPCHK(hipMemcpyAsync(d_bbox, bb.data(), bb.size() * sizeof(unsigned int), hipMemcpyHostToDevice, s));
   const int pb = (nmax + kThreads - 1) / kThreads;
-  hipLaunchKernelGGL(k_nn_bbox, dim3(std::min(1024, pb), np, 2), dim3(kThreads), 0, s, d_pairs, d_frames, d_bbox);
+  hipLaunchKernelGGL(k_nn_bbox, dim3(std::min(64, pb), np, 2), dim3(kThreads), 0, s, d_pairs, d_frames, d_bbox);
   HIPCHK(hipMemcpyAsync(bb.data(), d_bbox, bb.size() * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   const long long share = std::min(kMaxCells, kMaxArenaCells / np - 1);
   // Morton bits per axis of the order grids: 2^(3 bits) cells per pair within the arena
   int obits = 7;
   while (obits > 1 && (long long)np << (3 * obits) > kMaxArenaCells - 1) --obits;
-  std::vector<GridDev> grids(np), ogrids(np);
-  long long arena = 0, oarena = 0;
+  std::vector<GridDev> ogrids(np);
+  long long oarena = 0;
   size_t obase = 0;
+  std::vector<int> order_base(np);
   for (int i = 0; i < np; ++i) {
-    float lo[3], hi[3];
-    decode_bbox(&bb[12 * i], curs[i]->n, lo, hi);
-    float h = cell > 0.0f ? cell : kDefaultCell;
-    long long cells = 0;
-    int dims[3];
-    // extents in double: a float difference of extreme coordinates can overflow
-    for (;;) {
-      double c = 1.0;
-      for (int k = 0; k < 3; ++k) {
-        const double d = std::floor(((double)hi[k] - (double)lo[k]) / (double)h) + 1.0;
-        dims[k] = (int)std::min(d, 2e9);
-        c *= d;
-      }
-      if (c <= (double)share) {
-        cells = (long long)c;
-        break;
-      }
-      h *= 1.25f;
-    }
-    GridDev &g = grids[i];
-    g.ox = lo[0]; g.oy = lo[1]; g.oz = lo[2];
-    g.h = h;
-    g.inv_h = 1.0f / h;
-    g.nx = dims[0]; g.ny = dims[1]; g.nz = dims[2];
-    g.cell_base = (int)arena;
-    g.order_base = (int)obase;
-    arena += cells + 1;
+    order_base[i] = (int)obase;
     obase += (size_t)refs[i]->n;
     // order grid: cubic cells over the reference box
+    float lo[3], hi[3];
     decode_bbox(&bb[12 * i + 6], refs[i]->n, lo, hi);
     double ext = 0.0;
     for (int k = 0; k < 3; ++k) ext = std::max(ext, (double)hi[k] - (double)lo[k]);
@@ -1952,30 +1942,81 @@ int build_grids(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const nicp
     o.order_base = 0;
     oarena += 1ll << (3 * obits);
   }
-  const int m = (int)arena, mo = (int)oarena + 1;
-  CHK(ctx->grid_cells.reserve((size_t)std::max(m, mo) * sizeof(int)));
-  CHK(ctx->grid_cursor.reserve((size_t)std::max(m, mo) * sizeof(int)));
-  CHK(ctx->grid_sorted.reserve(std::max<size_t>(ncur, 1) * 2 * sizeof(float4)));
+  // grids of the current clouds at cell edge h0 into (cells, sorted, table)
+  auto cur_grid = [&](float h0, DevBuf &cells_buf, DevBuf &sorted_buf, GridDev *table) -> int {
+    std::vector<GridDev> grids(np);
+    long long arena = 0;
+    for (int i = 0; i < np; ++i) {
+      float lo[3], hi[3];
+      decode_bbox(&bb[12 * i], curs[i]->n, lo, hi);
+      float h = h0;
+      long long cells = 0;
+      int dims[3];
+      // extents in double: a float difference of extreme coordinates can overflow
+      for (;;) {
+        double c = 1.0;
+        for (int k = 0; k < 3; ++k) {
+          const double d = std::floor(((double)hi[k] - (double)lo[k]) / (double)h) + 1.0;
+          dims[k] = (int)std::min(d, 2e9);
+          c *= d;
+        }
+        if (c <= (double)share) {
+          cells = (long long)c;
+          break;
+        }
+        h *= 1.25f;
+      }
+      GridDev &g = grids[i];
+      g.ox = lo[0]; g.oy = lo[1]; g.oz = lo[2];
+      g.h = h;
+      g.inv_h = 1.0f / h;
+      g.nx = dims[0]; g.ny = dims[1]; g.nz = dims[2];
+      g.cell_base = (int)arena;
+      g.order_base = order_base[i];
+      arena += cells + 1;
+    }
+    const int m = (int)arena;
+    CHK(cells_buf.reserve((size_t)m * sizeof(int)));
+    CHK(ctx->grid_cursor.reserve((size_t)m * sizeof(int)));
+    CHK(sorted_buf.reserve(std::max<size_t>(ncur, 1) * 2 * sizeof(float4)));
+    HIPCHK(hipMemcpyAsync(table, grids.data(), (size_t)np * sizeof(GridDev), hipMemcpyHostToDevice, s));
+    int *cells_d = cells_buf.as<int>();
+    HIPCHK(hipMemsetAsync(cells_d, 0, (size_t)m * sizeof(int), s));
+    hipLaunchKernelGGL(k_nn_count, dim3(pb, np), dim3(kThreads), 0, s, d_pairs, d_frames, (const GridDev *)table,
+                       cells_d);
+    CHK(scan_arena(ctx, cells_d, m, ctx->grid_cursor.as<int>(), sums, total));
+    hipLaunchKernelGGL(k_nn_scatter, dim3(pb, np), dim3(kThreads), 0, s, d_pairs, d_frames, (const GridDev *)table,
+                       ns, ctx->grid_cursor.as<int>(), sorted_buf.as<float4>());
+    return NICP_OK;
+  };
+  // query order first (its arena is then reused by the grids' scans)
+  const int mo = (int)oarena + 1;
+  CHK(ctx->grid_cells.reserve((size_t)mo * sizeof(int)));
+  CHK(ctx->grid_cursor.reserve((size_t)mo * sizeof(int)));
   CHK(ctx->nn_order.reserve(std::max<size_t>(nref, 1) * sizeof(int)));
-  HIPCHK(hipMemcpyAsync(grids_d, grids.data(), (size_t)np * sizeof(GridDev), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(ogrids_d, ogrids.data(), (size_t)np * sizeof(GridDev), hipMemcpyHostToDevice, s));
-  int *cells_d = ctx->grid_cells.as<int>();
-  int *cursor_d = ctx->grid_cursor.as<int>();
-  // query order first (its arena is then reused for the grid)
-  HIPCHK(hipMemsetAsync(cells_d, 0, (size_t)mo * sizeof(int), s));
+  int *ocells = ctx->grid_cells.as<int>();
+  HIPCHK(hipMemsetAsync(ocells, 0, (size_t)mo * sizeof(int), s));
   hipLaunchKernelGGL(k_nn_order_count, dim3(pb, np), dim3(kThreads), 0, s, d_pairs, d_frames,
-                     (const GridDev *)ogrids_d, cells_d);
-  CHK(scan_arena(ctx, cells_d, mo, cursor_d, sums, total));
+                     (const GridDev *)ogrids_d, ocells);
+  CHK(scan_arena(ctx, ocells, mo, ctx->grid_cursor.as<int>(), sums, total));
   hipLaunchKernelGGL(k_nn_order_scatter, dim3(pb, np), dim3(kThreads), 0, s, d_pairs, d_frames,
-                     (const GridDev *)ogrids_d, cursor_d, ctx->nn_order.as<int>());
-  HIPCHK(hipMemsetAsync(cells_d, 0, (size_t)m * sizeof(int), s));
-  hipLaunchKernelGGL(k_nn_count, dim3(pb, np), dim3(kThreads), 0, s, d_pairs, d_frames, (const GridDev *)grids_d,
-                     cells_d);
-  CHK(scan_arena(ctx, cells_d, m, cursor_d, sums, total));
-  hipLaunchKernelGGL(k_nn_scatter, dim3(pb, np), dim3(kThreads), 0, s, d_pairs, d_frames, (const GridDev *)grids_d, ns,
-                     cursor_d, ctx->grid_sorted.as<float4>());
+                     (const GridDev *)ogrids_d, ctx->grid_cursor.as<int>(), ctx->nn_order.as<int>());
+  const float h0 = cell > 0.0f ? cell : kDefaultCell;
+  CHK(cur_grid(h0, ctx->grid_cells, ctx->grid_sorted, grids_d));
+  out.fine = grids_d;
+  out.cells_fine = ctx->grid_cells.as<int>();
+  out.sorted_fine = ctx->grid_sorted.as<float4>();
+  out.coarse = grids_d;
+  out.cells_coarse = out.cells_fine;
+  out.sorted_coarse = out.sorted_fine;
+  if (with_coarse) {
+    CHK(cur_grid(h0 * kCoarseCells, ctx->grid_cells_c, ctx->grid_sorted_c, cgrids_d));
+    out.coarse = cgrids_d;
+    out.cells_coarse = ctx->grid_cells_c.as<int>();
+    out.sorted_coarse = ctx->grid_sorted_c.as<float4>();
+  }
   HIPCHK(hipGetLastError());
-  *d_grids = grids_d;
   return NICP_OK;
 }
 
@@ -2033,16 +2074,18 @@ int align_nn_pairs(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const n
   CHK(nn_plan(ctx, np, refs, curs, init16, plan, &d_frames, &d_pairs, &d_bp));
   hipStream_t s = ctx->stream;
   HIPCHK(hipEventRecord(ctx->ev0, s));
-  GridDev *d_grids;
-  CHK(build_grids(ctx, np, refs, curs, d_pairs, d_frames, normal_scaling, cell_size, &d_grids));
+  NNGrids G;
+  CHK(build_grids(ctx, np, refs, curs, d_pairs, d_frames, normal_scaling, cell_size, ap->outer_iterations > 0, G));
   const AlignParamsDev a = to_dev(ap);
   Prof *pf = &ctx->prof;
   pf->sink = &pf->eager;
   const dim3 tb(kThreads);
   const dim3 ntiles(plan.total_pix_tiles);
   for (int it = 0; it < ap->outer_iterations; ++it) {
-    LAUNCH(pf, KID_NN_QUERY, k_nn_query, ntiles, tb, s, d_pairs, d_frames, d_bp, (const GridDev *)d_grids,
-           (const int *)ctx->grid_cells.as<int>(), (const float4 *)ctx->grid_sorted.as<float4>(),
+    // the first iteration searches from the initial guess, usually over a wide radius
+    const bool coarse = it == 0;
+    LAUNCH(pf, KID_NN_QUERY, k_nn_query, ntiles, tb, s, d_pairs, d_frames, d_bp, coarse ? G.coarse : G.fine,
+           coarse ? G.cells_coarse : G.cells_fine, coarse ? G.sorted_coarse : G.sorted_fine,
            (const int *)ctx->nn_order.as<int>(), a, normal_scaling, (int *)nullptr);
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, ntiles, tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
     LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(64), s, d_pairs, ap->lambda, 0, 0);
@@ -2085,13 +2128,12 @@ extern "C" int nicp_nn_correspondences(nicp_ctx *ctx, const nicp_cloud *ref, con
   int *d_bp;
   CHK(nn_plan(ctx, 1, &ref, &cur, nullptr, plan, &d_frames, &d_pairs, &d_bp));
   HIPCHK(hipMemcpyAsync(ctx->state.as<float>() + kSlotTinv, T, 16 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-  GridDev *d_grids;
-  CHK(build_grids(ctx, 1, &ref, &cur, d_pairs, d_frames, normal_scaling, cell_size, &d_grids));
+  NNGrids G;
+  CHK(build_grids(ctx, 1, &ref, &cur, d_pairs, d_frames, normal_scaling, cell_size, false, G));
   const int nr = std::max(ref->n, 1);
   CHK(ctx->nn_out.reserve((size_t)nr * sizeof(int)));
   hipLaunchKernelGGL(k_nn_query, dim3(plan.total_pix_tiles), dim3(kThreads), 0, ctx->stream, d_pairs, d_frames, d_bp,
-                     (const GridDev *)d_grids, (const int *)ctx->grid_cells.as<int>(),
-                     (const float4 *)ctx->grid_sorted.as<float4>(), (const int *)ctx->nn_order.as<int>(), to_dev(ap),
+                     G.fine, G.cells_fine, G.sorted_fine, (const int *)ctx->nn_order.as<int>(), to_dev(ap),
                      normal_scaling, ctx->nn_out.as<int>());
   HIPCHK(hipGetLastError());
   std::vector<int2> pp(nr);

@@ -1424,11 +1424,28 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_nn_bbox(const PairDev *
       mx[k] = fmaxf(mx[k], __shfl_xor(mx[k], o, 64));
     }
   }
+  // the block's waves meet in LDS; one set of atomics per block (all blocks of a cloud
+  // update the same six words)
+  __shared__ float red[kThreads / 64][6];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      atomicMin(bbox + k, f2ord(mn[k]));
-      atomicMax(bbox + 3 + k, f2ord(mx[k]));
+      red[w][k] = mn[k];
+      red[w][3 + k] = mx[k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float a = red[0][k], b = red[0][3 + k];
+      for (int v = 1; v < kThreads / 64; ++v) {
+        a = fminf(a, red[v][k]);
+        b = fmaxf(b, red[v][3 + k]);
+      }
+      atomicMin(bbox + k, f2ord(a));
+      atomicMax(bbox + 3 + k, f2ord(b));
     }
   }
 }
