@@ -29,7 +29,7 @@
 namespace initnet {
 
 constexpr int kLayers = 6;
-constexpr int kProbeSub = 2;  // pixel stride of the start probe's linearisation
+constexpr int kProbeSub = 4;  // pixel stride of the start probe's linearisation
 constexpr int kInRows = 60, kInCols = 80, kInCh = 2;
 
 struct LayerShape {
