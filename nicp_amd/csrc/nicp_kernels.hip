@@ -898,6 +898,9 @@ __device__ __forceinline__ void zproject_body(const PairDev *__restrict__ Pp, co
   for (int k = 0; k < 16; ++k) KRt[k] = S[k];
   const int cols = pd.cols, rows = pd.rows;
   const int lane = threadIdx.x & 63;
+  // a subsampled pair (the start probe) reads only the cells of every sub-th row and column:
+  // points landing elsewhere are dropped before their atomics
+  const int sub = which ? 1 : pd.sub;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int i = base + k * kThreads + threadIdx.x;
@@ -908,7 +911,8 @@ __device__ __forceinline__ void zproject_body(const PairDev *__restrict__ Pp, co
       if (M == kSpherical) {
         int x, y;
         float d;
-        if (sph_project(pd.sph, KRt, p, pd.min_d, pd.max_d, x, y, d) && x >= 0 && x < cols && y >= 0 && y < rows) {
+        if (sph_project(pd.sph, KRt, p, pd.min_d, pd.max_d, x, y, d) && x >= 0 && x < cols && y >= 0 && y < rows &&
+            (sub == 1 || (x % sub == 0 && y % sub == 0))) {
           cell = y * cols + x;
           key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
         }
@@ -919,7 +923,7 @@ __device__ __forceinline__ void zproject_body(const PairDev *__restrict__ Pp, co
         if (!(d < pd.min_d || d > pd.max_d)) {
           const float inv = 1.0f / d;
           const int x = (int)roundf(ip[0] * inv), y = (int)roundf(ip[1] * inv);
-          if (x >= 0 && x < cols && y >= 0 && y < rows) {
+          if (x >= 0 && x < cols && y >= 0 && y < rows && (sub == 1 || (x % sub == 0 && y % sub == 0))) {
             cell = y * cols + x;
             key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
           }
