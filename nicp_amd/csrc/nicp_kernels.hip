@@ -869,7 +869,20 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_zfill(const PairDev *__
   const int g = blockIdx.x;
   const PairDev &pd = Pp[blk_pair[g]];
   const int P = pd.rows * pd.cols;
-  // a subsampled pair has fewer blocks than its z-buffer has tiles: stride over them
+  if (pd.sub > 1) {
+    // a subsampled pair (the start probe) projects into and reads only the cells of every
+    // sub-th row and column: one block per tile of that grid clears them
+    const int sub = pd.sub, gcols = (pd.cols + sub - 1) / sub, P2 = ((pd.rows + sub - 1) / sub) * gcols;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int q = (g - pd.tile_base) * kTile + k * kThreads + threadIdx.x;
+      if (q < P2) {
+        const int gr = q / gcols;
+        pd.zbuf[gr * sub * pd.cols + (q - gr * gcols) * sub] = kZInit;
+      }
+    }
+    return;
+  }
   const int stride = pd.n_tiles * kTile;
   for (int base = (g - pd.tile_base) * kTile; base < P; base += stride) {
 #pragma unroll
