@@ -641,18 +641,18 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
     }
     LAUNCH(pf, KID_CORR_LIN, k_corr_lin, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
            (int)(last || inner > 1));
-    LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(64), s, d_pairs, ap->lambda, inner > 1 ? 1 : 0, 0);
+    LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, ap->lambda, inner > 1 ? 1 : 0, 0);
     for (int k = 1; k < inner; ++k) {
       LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
              kSlotInvTInner, 0);
-      LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(64), s, d_pairs, ap->lambda, k == inner - 1 ? 0 : 1, 1);
+      LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, ap->lambda, k == inner - 1 ? 0 : 1, 1);
     }
   }
   // Aligner::_computeStatistics: one more linearisation at the final T over the last correspondences
   if (ap->outer_iterations > 0 && with_stats) {
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
            kSlotTinv, 0);
-    LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(64), s, d_pairs, 0.0f, 2, 0);
+    LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, 0.0f, 2, 0);
   }
   HIPCHK(hipGetLastError());
   return NICP_OK;
@@ -1370,7 +1370,7 @@ extern "C" int nicp_linearize(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_c
   HIPCHK(hipMemcpyAsync(ctx->pairs_buf.p, pp.data(), (size_t)npx * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_lin_pairs, dim3(plan.total_pix_tiles), dim3(kThreads), 0, ctx->stream, d_pairs, d_frames, d_bp,
                      to_dev(ap), kSlotTinv, npx);
-  hipLaunchKernelGGL(k_solve, dim3(1), dim3(64), 0, ctx->stream, d_pairs, 0.0f, 2, 0);
+  hipLaunchKernelGGL(k_solve, dim3(1), dim3(kSolveThreads), 0, ctx->stream, d_pairs, 0.0f, 2, 0);
   HIPCHK(hipGetLastError());
   double r[2 * kPartial];
   HIPCHK(hipMemcpyAsync(r, ctx->result.p, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
@@ -2088,11 +2088,11 @@ int align_nn_pairs(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const n
            coarse ? G.cells_coarse : G.cells_fine, coarse ? G.sorted_coarse : G.sorted_fine,
            (const int *)ctx->nn_order.as<int>(), a, normal_scaling, (int *)nullptr);
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, ntiles, tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
-    LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(64), s, d_pairs, ap->lambda, 0, 0);
+    LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, ap->lambda, 0, 0);
   }
   if (ap->outer_iterations > 0) {
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, ntiles, tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
-    LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(64), s, d_pairs, 0.0f, 2, 0);
+    LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, 0.0f, 2, 0);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev1, s));

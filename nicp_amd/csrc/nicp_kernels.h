@@ -9,6 +9,8 @@ constexpr int kTile = 1024;          // pixels (or points) per per-pixel block
 constexpr int kThreads = 256;        // threads per block (4 wavefronts of 64)
 constexpr int kPartial = 32;         // doubles per block partial of the linearizer
 constexpr int kVals = 31;            // used partial slots: H (21), b (6), chi2, inliers, count, both-valid pixels
+constexpr int kSolveThreads = 256;   // threads per pair of k_solve (kSolveLanes per partial slot)
+constexpr int kSolveLanes = 8;
 constexpr uint64_t kZInit = ~0ull;   // empty z-buffer cell
 
 // per-pixel blocks of a pair: 1024-pixel tiles of its projector image

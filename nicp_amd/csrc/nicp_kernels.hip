@@ -1337,22 +1337,38 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_lin_pairs(const PairDev
 }
 
 namespace {
+// The pair's FP64 totals: kSolveLanes consecutive threads per slot v, thread `part` adding
+// tiles part, part + kSolveLanes, ... in order with eight loads in flight, then a fixed tree
+// over the kSolveLanes sums. The order is fixed, so every call on the same partials gives
+// the same bits.
 __device__ void reduce_partials(const PairDev &pd, double *tot /* shared, kPartial */) {
-  const int lane = threadIdx.x;
-  for (int v = 0; v < kVals; ++v) {
-    double s = 0.0;
-    for (int i = lane; i < pd.n_tiles; i += 64) s += pd.partials[(size_t)i * kPartial + v];
-    s = wave_sum_d(s);
-    if (lane == 0) tot[v] = s;
+  const int v = threadIdx.x / kSolveLanes, part = threadIdx.x - v * kSolveLanes;
+  double s = 0.0;
+  if (v < kVals) {
+    const double *__restrict__ src = pd.partials + v;
+    int i = part;
+    for (; i + 7 * kSolveLanes < pd.n_tiles; i += 8 * kSolveLanes) {
+      double x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) x[u] = src[(size_t)(i + u * kSolveLanes) * kPartial];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += x[u];
+    }
+    for (; i < pd.n_tiles; i += kSolveLanes) s += src[(size_t)i * kPartial];
   }
+  // the slot's lanes are consecutive lanes of one wavefront
+#pragma unroll
+  for (int o = kSolveLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (v < kVals && part == 0) tot[v] = s;
 }
 }  // namespace
 
-// One wavefront per pair: FP64 totals, H + lambda I, LDLT, update of the SE(3) state.
+// One workgroup per pair: the FP64 totals by all its threads, then on thread 0 H + lambda I,
+// LDLT and the update of the SE(3) state.
 // mode 0: last inner iteration (updates T, Tinv, KRt); 1: inner iteration (updates invT_inner);
 // 2: statistics linearisation (stores the totals only).
-extern "C" __global__ __launch_bounds__(64) void k_solve(const PairDev *__restrict__ Pp, float lambda, int mode,
-                                                        int from_inner) {
+extern "C" __global__ __launch_bounds__(kSolveThreads) void k_solve(const PairDev *__restrict__ Pp, float lambda,
+                                                                   int mode, int from_inner) {
   const PairDev &pd = Pp[blockIdx.x];
   __shared__ double tot[kPartial];
   reduce_partials(pd, tot);
