@@ -204,35 +204,53 @@ NICP_HD void t2v(const S *T, S *x) {
 
 // LDLT with diagonal pivoting and the pseudo-inverse-of-D solve (Eigen's
 // LDLT<Matrix6> algorithm, nicp/nicp/alignerprojective.cpp:92), for one 6x6
-// column-major system. Returns 0 when the whole diagonal is zero.
+// column-major system. Returns 0 when the whole diagonal is zero. Every loop is unrolled and
+// every swap with the pivot row is written as a swap with each candidate row under the
+// condition that it is the pivot, so all array indices are constants (a GPU thread keeps
+// the matrix in registers); the arithmetic and its order are those of the plain loops.
 template <typename S>
 NICP_HD int ldlt_solve6(const S *Hin, const S *rhs, S *x) {
   const int n = 6;
   S m[36];
+#pragma unroll
   for (int i = 0; i < 36; ++i) m[i] = Hin[i];
 #define A(r, c) m[(c)*6 + (r)]
   int tr[6];
   S temp[6];
   int ok = 1;
+  bool stop = false;
+#pragma unroll
   for (int k = 0; k < n; ++k) {
+    if (stop) break;
     int idx = k;
     S best = fabs(A(k, k));
+#pragma unroll
     for (int i = k + 1; i < n; ++i)
       if (fabs(A(i, i)) > best) { best = fabs(A(i, i)); idx = i; }
     tr[k] = idx;
-    if (k != idx) {
-      for (int j = 0; j < k; ++j) { S t = A(k, j); A(k, j) = A(idx, j); A(idx, j) = t; }
-      for (int i = idx + 1; i < n; ++i) { S t = A(i, k); A(i, k) = A(i, idx); A(i, idx) = t; }
-      { S t = A(k, k); A(k, k) = A(idx, idx); A(idx, idx) = t; }
-      for (int i = k + 1; i < idx; ++i) { S t = A(i, k); A(i, k) = A(idx, i); A(idx, i) = t; }
+#pragma unroll
+    for (int c = k + 1; c < n; ++c) {
+      if (c == idx) {
+#pragma unroll
+        for (int j = 0; j < k; ++j) { S t = A(k, j); A(k, j) = A(c, j); A(c, j) = t; }
+#pragma unroll
+        for (int i = c + 1; i < n; ++i) { S t = A(i, k); A(i, k) = A(i, c); A(i, c) = t; }
+        { S t = A(k, k); A(k, k) = A(c, c); A(c, c) = t; }
+#pragma unroll
+        for (int i = k + 1; i < c; ++i) { S t = A(i, k); A(i, k) = A(c, i); A(c, i) = t; }
+      }
     }
     if (k > 0) {
+#pragma unroll
       for (int j = 0; j < k; ++j) temp[j] = A(j, j) * A(k, j);
       S s = A(k, 0) * temp[0];
+#pragma unroll
       for (int j = 1; j < k; ++j) s = s + A(k, j) * temp[j];
       A(k, k) -= s;
+#pragma unroll
       for (int i = k + 1; i < n; ++i) {
         S t = A(i, 0) * temp[0];
+#pragma unroll
         for (int j = 1; j < k; ++j) t = t + A(i, j) * temp[j];
         A(i, k) -= t;
       }
@@ -240,25 +258,48 @@ NICP_HD int ldlt_solve6(const S *Hin, const S *rhs, S *x) {
     S akk = A(k, k);
     bool valid = fabs(akk) > S(0);
     if (k == 0 && !valid) {
+#pragma unroll
       for (int j = 0; j < n; ++j) tr[j] = j;
       ok = 0;
-      break;
+      stop = true;
+      continue;
     }
-    if (k + 1 < n && valid)
+    if (k + 1 < n && valid) {
+#pragma unroll
       for (int i = k + 1; i < n; ++i) A(i, k) /= akk;
+    }
   }
   S d[6];
+#pragma unroll
   for (int i = 0; i < n; ++i) d[i] = rhs[i];
-  for (int i = 0; i < n; ++i) { S t = d[i]; d[i] = d[tr[i]]; d[tr[i]] = t; }
+  // d[i] <-> d[tr[i]] for i ascending (tr[i] >= i)
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+#pragma unroll
+    for (int c = i + 1; c < n; ++c)
+      if (c == tr[i]) { S t = d[i]; d[i] = d[c]; d[c] = t; }
+  }
+#pragma unroll
   for (int i = 0; i < n; ++i)
+#pragma unroll
     for (int j = 0; j < i; ++j) d[i] -= A(i, j) * d[j];
   // Eigen uses numeric_limits<S>::min() as the zero-pivot tolerance
   const S tol = sizeof(S) == 8 ? S(2.2250738585072014e-308) : S(1.17549435e-38f);
+#pragma unroll
   for (int i = 0; i < n; ++i) d[i] = fabs(A(i, i)) > tol ? d[i] / A(i, i) : S(0);
+#pragma unroll
   for (int i = n - 1; i >= 0; --i)
+#pragma unroll
     for (int j = i + 1; j < n; ++j) d[i] -= A(j, i) * d[j];
-  for (int i = n - 1; i >= 0; --i) { S t = d[i]; d[i] = d[tr[i]]; d[tr[i]] = t; }
+  // d[i] <-> d[tr[i]] for i descending
+#pragma unroll
+  for (int i = n - 1; i >= 0; --i) {
+#pragma unroll
+    for (int c = i + 1; c < n; ++c)
+      if (c == tr[i]) { S t = d[i]; d[i] = d[c]; d[c] = t; }
+  }
 #undef A
+#pragma unroll
   for (int i = 0; i < n; ++i) x[i] = d[i];
   return ok;
 }
