@@ -442,6 +442,7 @@ int adopt_store(nicp_cloud *cl, PointStore &ps, int n, int cap) {
   if (cl->alloc) (void)hipFree(cl->alloc);
   if (cl->gauss_alloc) (void)hipFree(cl->gauss_alloc);
   cl->alloc = ps.mem;
+  cl->alloc_bytes = 0;  // not from the pool's sizes: freed, not pooled, on destroy
   cl->gauss_alloc = nullptr;
   cl->cap = std::max(cap, 1);
   cl->n = n;

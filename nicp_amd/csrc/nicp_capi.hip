@@ -225,6 +225,9 @@ struct nicp_ctx {
   std::vector<ProfEntry> *pending_graph = nullptr;  // graph events not yet read
   // kNN grid of the current cloud
   DevBuf grid_cells, grid_cursor, grid_sorted, grid_aux, nn_out, nn_order, grid_cells_c, grid_sorted_c;
+  // allocations of destroyed clouds (size, pointer), reused for new clouds of the same size;
+  // every user of them runs on `stream`, so a reuse is ordered after the last use
+  std::vector<std::pair<size_t, void *>> cloud_pool;
 };
 
 struct nicp_cloud {
@@ -234,6 +237,7 @@ struct nicp_cloud {
   int has_index = 0;
   int has_stats = 0;
   void *alloc = nullptr;  // owned allocation (null for batch views)
+  size_t alloc_bytes = 0; // its size when it may go back to the context's pool (else 0)
   int cap = 0;            // points the arrays hold
   float4 *gauss = nullptr;        // nicp_gaussian records (7 float4 each) or null
   void *gauss_alloc = nullptr;    // owned allocation of gauss when it is not inside alloc
@@ -890,6 +894,8 @@ extern "C" int nicp_ctx_destroy(nicp_ctx *c) {
                     &c->grid_cursor, &c->grid_sorted, &c->grid_aux, &c->nn_out, &c->nn_order, &c->grid_cells_c,
                     &c->grid_sorted_c};
   for (DevBuf *b : bufs) b->release();
+  for (auto &pe : c->cloud_pool) (void)hipFree(pe.second);
+  c->cloud_pool.clear();
   c->prof.release();
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
@@ -942,6 +948,27 @@ extern "C" int nicp_device_copy(nicp_ctx *c, void *dst, const void *src, size_t 
 // clouds
 // ---------------------------------------------------------------------------------------
 namespace {
+constexpr size_t kCloudPoolEntries = 8;
+
+// a cloud allocation of exactly `bytes`: from the context's pool when it holds one
+hipError_t cloud_alloc(nicp_ctx *ctx, size_t bytes, void **p) {
+  for (size_t i = 0; i < ctx->cloud_pool.size(); ++i)
+    if (ctx->cloud_pool[i].first == bytes) {
+      *p = ctx->cloud_pool[i].second;
+      ctx->cloud_pool.erase(ctx->cloud_pool.begin() + (long)i);
+      return hipSuccess;
+    }
+  return hipMalloc(p, bytes);
+}
+// back to the pool; the oldest entry is freed when the pool is full
+void cloud_release(nicp_ctx *ctx, void *p, size_t bytes) {
+  if (ctx->cloud_pool.size() >= kCloudPoolEntries) {
+    (void)hipFree(ctx->cloud_pool.front().second);
+    ctx->cloud_pool.erase(ctx->cloud_pool.begin());
+  }
+  ctx->cloud_pool.emplace_back(bytes, p);
+}
+
 int cloud_from_depth(nicp_ctx *ctx, const uint16_t *depth, const float *depthf, int rows, int cols, float depth_scale,
                      const nicp_camera *cam, const nicp_stats_params *sp, int keep_stats, nicp_cloud **out) {
   if (!ctx || (!depth && !depthf) || !sp || !out) return set_err(NICP_ERR_INVALID, "null argument");
@@ -954,12 +981,13 @@ int cloud_from_depth(nicp_ctx *ctx, const uint16_t *depth, const float *depthf, 
   cl->ctx = ctx;
   FrameLayout L = frame_layout(rows, cols, keep_stats != 0, true, 0, depthf != nullptr);
   void *mem = nullptr;
-  hipError_t e = hipMalloc(&mem, L.total);
+  hipError_t e = cloud_alloc(ctx, L.total, &mem);
   if (e != hipSuccess) {
     delete cl;
     return set_err(NICP_ERR_OOM, "hipMalloc(%zu): %s", L.total, hipGetErrorString(e));
   }
   cl->alloc = mem;
+  cl->alloc_bytes = L.total;
   cl->cap = rows * cols;
   FrameDev &f = cl->fd;
   frame_geometry(f, rows, cols, cam, sp->world_radius, depth_scale);
@@ -1051,12 +1079,13 @@ extern "C" int nicp_cloud_from_host(nicp_ctx *ctx, int n, const float *points4, 
   size_t o_pts = cv.take<float4>(cap), o_nrm = cv.take<float4>(cap), o_omg = cv.take<float4>(cap * 3),
          o_np = cv.take<int>(1);
   void *mem = nullptr;
-  hipError_t e = hipMalloc(&mem, cv.off);
+  hipError_t e = cloud_alloc(ctx, cv.off, &mem);
   if (e != hipSuccess) {
     delete cl;
     return set_err(NICP_ERR_OOM, "hipMalloc: %s", hipGetErrorString(e));
   }
   cl->alloc = mem;
+  cl->alloc_bytes = cv.off;
   cl->n = n;
   cl->cap = (int)cap;
   FrameDev &f = cl->fd;
@@ -1100,9 +1129,16 @@ extern "C" int nicp_cloud_destroy(nicp_cloud *cl) {
   if (!cl) return NICP_OK;
   if (cl->alloc || cl->gauss_alloc) {
     (void)hipSetDevice(cl->ctx->device);
-    (void)hipStreamSynchronize(cl->ctx->stream);
-    if (cl->alloc) (void)hipFree(cl->alloc);
-    if (cl->gauss_alloc) (void)hipFree(cl->gauss_alloc);
+    if (cl->alloc && cl->alloc_bytes) {
+      cloud_release(cl->ctx, cl->alloc, cl->alloc_bytes);
+    } else if (cl->alloc) {
+      (void)hipStreamSynchronize(cl->ctx->stream);
+      (void)hipFree(cl->alloc);
+    }
+    if (cl->gauss_alloc) {
+      (void)hipStreamSynchronize(cl->ctx->stream);
+      (void)hipFree(cl->gauss_alloc);
+    }
   }
   delete cl;
   return NICP_OK;
