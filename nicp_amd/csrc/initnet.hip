@@ -29,6 +29,7 @@
 namespace initnet {
 
 constexpr int kLayers = 6;
+constexpr int kFcSplits = 16;  // most reduction chunks of a forward GEMM (fwd_splits)
 constexpr int kProbeSub = 4;  // pixel stride of the start probe's linearisation
 constexpr int kInRows = 60, kInCols = 80, kInCh = 2;
 
@@ -114,6 +115,13 @@ struct StoreAct {  // forward: y = relu?(v)
     y[(size_t)m * N + n] = relu ? fmaxf(v, 0.0f) : v;
   }
 };
+struct StorePartial {  // one chunk of a split reduction: p[chunk][m][n] = v (summed by k_sum_chunks)
+  float *p;
+  int M, N;
+  __device__ __forceinline__ void operator()(int m, int n, float v) const {
+    p[((size_t)blockIdx.z * M + m) * N + n] = v;
+  }
+};
 struct StoreAtomic {  // weight gradient, split over the reduction: dW += v
   float *g;
   int N;
@@ -196,10 +204,14 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, L
       }
 }
 
+// the reduction chunk of a GEMM split `splits` ways (a multiple of 16) and the chunk count
+inline int gemm_kchunk(int K, int splits) { return ((K + splits - 1) / splits + 15) / 16 * 16; }
+inline int gemm_chunks(int K, int splits) { return (K + gemm_kchunk(K, splits) - 1) / gemm_kchunk(K, splits); }
+
 template <class LA, class LB, class ST>
 hipError_t gemm(hipStream_t s, int M, int N, int K, int splits, LA la, LB lb, ST st) {
-  const int kchunk = ((K + splits - 1) / splits + 15) / 16 * 16;
-  const int z = (K + kchunk - 1) / kchunk;
+  const int kchunk = gemm_kchunk(K, splits);
+  const int z = gemm_chunks(K, splits);
   dim3 grid((N + 63) / 64, (M + 63) / 64, z);
   hipLaunchKernelGGL((k_gemm<LA, LB, ST>), grid, dim3(256), 0, s, M, N, K, kchunk, la, lb, st);
   return hipGetLastError();
@@ -208,6 +220,15 @@ hipError_t gemm(hipStream_t s, int M, int N, int K, int splits, LA la, LB lb, ST
 // ---------------------------------------------------------------------------------
 // elementwise kernels
 // ---------------------------------------------------------------------------------
+// y[m][n] = relu?(sum over chunks c = 0, 1, .. of p[c][m][n]), the chunks added in order
+__global__ void k_sum_chunks(const float *__restrict__ p, int chunks, int M, int N, float *__restrict__ y, int relu) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x, mn = (size_t)M * N;
+  if (i >= mn) return;
+  float v = p[i];
+  for (int c = 1; c < chunks; ++c) v += p[c * mn + i];
+  y[i] = relu ? fmaxf(v, 0.0f) : v;
+}
+
 __global__ void k_fill(float *p, size_t n, float v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -312,6 +333,7 @@ struct nicp_initnet {
   int rows_cap = 0;
   int input_mode = 0;       // 0: depth (m) into the first layer, 1: inverse depth
   int adam_t = 0;
+  float *chunks = nullptr;  // partial sums of the fully connected layers' split reductions
 };
 
 namespace {
@@ -322,6 +344,14 @@ size_t act_count(int l, int B) {
   if (l == 0) return (size_t)B * kInRows * kInCols * kInCh;
   const LayerShape s = layer_shape(l - 1);
   return (size_t)B * s.oh * s.ow * s.oc;
+}
+
+// reduction chunks of a layer's forward GEMM: a layer with fewer than 256 output tiles of 64 x 64
+// (the fully connected ones and the last convolutions) splits its reduction over more
+// workgroups, at most kFcSplits ways; the chunks are then added in order
+int fwd_splits(int M, int N) {
+  const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  return tiles >= 256 ? 1 : std::min(kFcSplits, 256 / tiles);
 }
 
 int net_reserve(nicp_initnet *n, int B) {
@@ -340,6 +370,15 @@ int net_reserve(nicp_initnet *n, int B) {
     HIPCHK(hipMalloc(&n->dact[l], act_count(l, B) * sizeof(float)));
   }
   HIPCHK(hipMalloc(&n->target, (size_t)B * 6 * sizeof(float)));
+  if (n->chunks) (void)hipFree(n->chunks);
+  n->chunks = nullptr;
+  size_t most = 1;
+  for (int l = 0; l < kLayers; ++l) {
+    const LayerShape sh = layer_shape(l);
+    const int M = B * sh.oh * sh.ow, sp = fwd_splits(M, sh.oc);
+    if (sp > 1) most = std::max(most, (size_t)gemm_chunks(layer_k(l) + 1, sp) * M * sh.oc);
+  }
+  HIPCHK(hipMalloc(&n->chunks, most * sizeof(float)));
   n->cap = B;
   return NICP_OK;
 }
@@ -357,8 +396,17 @@ int net_forward(nicp_initnet *n, int B) {
     const LayerShape sh = layer_shape(l);
     const int M = B * sh.oh * sh.ow, K = layer_k(l);
     ConvIn a{n->act[l], sh.ih, sh.iw, sh.ic, sh.oh, sh.ow, K, sh.conv, l == 0 ? n->input_mode : 0};
-    HIPCHK(gemm(s, M, sh.oc, K + 1, 1, a, Weights{n->params + n->poff[l], sh.oc},
-                StoreAct{n->act[l + 1], sh.oc, sh.relu}));
+    const int sp = fwd_splits(M, sh.oc);
+    if (sp == 1) {
+      HIPCHK(gemm(s, M, sh.oc, K + 1, 1, a, Weights{n->params + n->poff[l], sh.oc},
+                  StoreAct{n->act[l + 1], sh.oc, sh.relu}));
+    } else {
+      HIPCHK(gemm(s, M, sh.oc, K + 1, sp, a, Weights{n->params + n->poff[l], sh.oc}, StorePartial{n->chunks, M, sh.oc}));
+      const size_t mn = (size_t)M * sh.oc;
+      hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, (const float *)n->chunks,
+                         gemm_chunks(K + 1, sp), M, sh.oc, n->act[l + 1], sh.relu);
+      HIPCHK(hipGetLastError());
+    }
   }
   return NICP_OK;
 }
@@ -476,6 +524,7 @@ extern "C" int nicp_initnet_destroy(nicp_initnet *n) {
     if (b) (void)hipFree(b);
   if (n->jobs) (void)hipFree(n->jobs);
   if (n->rows_dev) (void)hipFree(n->rows_dev);
+  if (n->chunks) (void)hipFree(n->chunks);
   delete n;
   return NICP_OK;
 }
