@@ -1074,15 +1074,30 @@ __device__ __forceinline__ void lin_terms(const float *rp, const float *rn, floa
   acc_add(acc[28], 1.0f);
 }
 
-// block-wide FP64 reduction of the kVals accumulators into one partial
+// block-wide FP64 reduction of the kVals accumulators into one partial. Within a wavefront a
+// reduce-scatter: the kPartial (32) slots are halved five times; at partner offset o a lane
+// keeps the half of its slots selected by its lane bit o and adds its partner's copy of it
+// (32 FP64 exchanges instead of 6 per slot). Lane l then holds slot l >> 1 summed over
+// the half of the wavefront with its lane bit 0, and one more exchange with lane l ^ 1
+// completes it. The order of the additions is fixed.
 __device__ void block_partial(Acc2 *acc, double *out) {
+  static_assert(kPartial == 32, "the reduce-scatter halves 32 slots five times");
   __shared__ double red[kThreads / 64][kPartial];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double v[kPartial];
 #pragma unroll
-  for (int v = 0; v < kVals; ++v) {
-    double x = wave_sum_d(acc[v]);
-    if (lane == 0) red[wave][v] = x;
+  for (int i = 0; i < kPartial; ++i) v[i] = i < kVals ? acc[i] : 0.0;
+#pragma unroll
+  for (int h = kPartial / 2, o = 32; h >= 1; h >>= 1, o >>= 1) {
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int i = 0; i < h; ++i) {
+      const double keep = up ? v[h + i] : v[i], send = up ? v[i] : v[h + i];
+      v[i] = keep + __shfl_xor(send, o, 64);
+    }
   }
+  v[0] += __shfl_xor(v[0], 1, 64);
+  if ((lane & 1) == 0) red[wave][lane >> 1] = v[0];
   __syncthreads();
   if (threadIdx.x < kPartial) {
     double s = 0.0;
