@@ -1002,11 +1002,13 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_zresolve(const PairDev 
 
 namespace {
 
-// Per-thread accumulation of the FP32 per-correspondence terms in FP64: each term is
-// exact in double, and on CDNA4 one DADD costs less than the six FP32 operations of a
-// TwoSum step while leaving a smaller error than compensated FP32.
-typedef double Acc2;
-__device__ __forceinline__ void acc_add(Acc2 &a, float x) { a += (double)x; }
+// Per-thread accumulation of the FP32 per-correspondence terms: a thread sums at most the
+// four correspondences of its pixels in FP32 (relative error of a slot at most 3 ulp of
+// its terms' magnitudes), and every sum beyond the thread is FP64 (block_partial, then the
+// per-pair reduction of k_solve). FP32 slots take 31 registers instead of 62, which lets
+// the linearising kernels run three wavefronts per SIMD instead of two.
+typedef float Acc2;
+__device__ __forceinline__ void acc_add(Acc2 &a, float x) { a += x; }
 
 // The per-correspondence terms of Linearizer::update (linearizer.cpp:55-112) for
 // J_p = [I | S(p)], J_n = [0 | S(n)], S(v) = -2[v]x (bm_se3.h:55-66).
@@ -1086,7 +1088,7 @@ __device__ void block_partial(Acc2 *acc, double *out) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double v[kPartial];
 #pragma unroll
-  for (int i = 0; i < kPartial; ++i) v[i] = i < kVals ? acc[i] : 0.0;
+  for (int i = 0; i < kPartial; ++i) v[i] = i < kVals ? (double)acc[i] : 0.0;
 #pragma unroll
   for (int h = kPartial / 2, o = 32; h >= 1; h >>= 1, o >>= 1) {
     const bool up = (lane & o) != 0;
@@ -1201,10 +1203,9 @@ __device__ int window_search(const FrameDev &fr, const FrameDev &fc, const int *
 
 // Fused correspondence + linearisation over the projector image of each pair.
 // Reads and resets the z-buffer of the reference projection.
-extern "C" __global__ __launch_bounds__(kThreads) void k_corr_lin(const PairDev *__restrict__ Pp,
-                                                                 const FrameDev *__restrict__ F,
-                                                                 const int *__restrict__ blk_pair, AlignParamsDev ap,
-                                                                 int store_pairs) {
+extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_corr_lin(
+    const PairDev *__restrict__ Pp, const FrameDev *__restrict__ F, const int *__restrict__ blk_pair, AlignParamsDev ap,
+    int store_pairs) {
   const int g = xcd_tile(blockIdx.x, gridDim.x);
   const PairDev &pd = Pp[blk_pair[g]];
   const FrameDev &fr = F[pd.ref];
@@ -1221,7 +1222,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_corr_lin(const PairDev 
   const bool multi = ap.row_region != 0 || ap.col_region != 0;
   Acc2 acc[kVals];
 #pragma unroll
-  for (int v = 0; v < kVals; ++v) acc[v] = 0.0;
+  for (int v = 0; v < kVals; ++v) acc[v] = 0.0f;
   if (!multi) {
     // all loads of the thread's four pixels are issued before any of them is used
     int ri[4], ci[4], pix[4];
@@ -1304,10 +1305,9 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_corr_lin(const PairDev 
 }
 
 // Linearizer::update over stored pairs at the transform in state slot `slot`.
-extern "C" __global__ __launch_bounds__(kThreads) void k_lin_pairs(const PairDev *__restrict__ Pp,
-                                                                  const FrameDev *__restrict__ F,
-                                                                  const int *__restrict__ blk_pair, AlignParamsDev ap,
-                                                                  int slot, int num_override) {
+extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_lin_pairs(
+    const PairDev *__restrict__ Pp, const FrameDev *__restrict__ F, const int *__restrict__ blk_pair, AlignParamsDev ap,
+    int slot, int num_override) {
   const int g = blockIdx.x;
   const PairDev &pd = Pp[blk_pair[g]];
   const FrameDev &fr = F[pd.ref];
@@ -1320,7 +1320,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_lin_pairs(const PairDev
   for (int k = 0; k < 16; ++k) T[k] = pd.state[slot + k];
   Acc2 acc[kVals];
 #pragma unroll
-  for (int v = 0; v < kVals; ++v) acc[v] = 0.0;
+  for (int v = 0; v < kVals; ++v) acc[v] = 0.0f;
   int2 pr[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
