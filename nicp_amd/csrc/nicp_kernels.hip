@@ -31,11 +31,33 @@ using namespace nicp_math;
 
 namespace {
 
+// global-address-space view of a device pointer held in a frame table (flat loads would
+// also count as LDS operations and wait with them)
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gview(const T *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmut(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+// 16-byte global load of a float4 (through a clang vector type: HIP's float4 class has no
+// address-space-qualified copy)
+typedef float gvec4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gload4(const float4 *p) {
+  const gvec4 v = *gview((const gvec4 *)p);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gstore4(float4 *p, float4 v) {
+  *gmut((gvec4 *)p) = gvec4{v.x, v.y, v.z, v.w};
+}
+typedef float gvec2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ float depth_of(uint16_t raw, float scale) { return raw ? scale * (float)raw : 0.0f; }
 __device__ __forceinline__ bool depth_valid(float d, float mn, float mx) { return !(d == 0.0f || d < mn || d > mx); }
 // the depth of pixel p of a frame: float input, or uint16 times the scale (imageutils.cpp:85-99)
 __device__ __forceinline__ float frame_depth(const FrameDev &fd, int p) {
-  return fd.depthf ? fd.depthf[p] : depth_of(fd.depth[p], fd.depth_scale);
+  return fd.depthf ? *gview(fd.depthf + p) : depth_of(*gview(fd.depth + p), fd.depth_scale);
 }
 
 // PinholePointProjector::_unProject (pinholepointprojector.h:275-280): x = column, y = row
@@ -360,12 +382,6 @@ constexpr int kIBand = 40, kIChunk = 48;
 constexpr int kIStride = kIChunk * 10 + 1;  // LDS row stride (floats): rows fall in different banks
 constexpr int kIFrameThreads = 1024;
 
-// global-address-space view of a device pointer held in a frame table (flat loads would
-// also count as LDS operations and wait with them)
-template <typename T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T *gview(const T *p) {
-  return (const __attribute__((address_space(1))) T *)p;
-}
 
 // rows ci and cj of iKRt: a thread's channel p[ci] * p[cj] of the point p = iKRt (c d, r d, d, 1)
 // needs only those two rows of mul4_packet, in its association order
@@ -739,7 +755,7 @@ __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int p = base + k * kThreads + threadIdx.x;
-    idxs[k] = p < P ? fd.index[p] : -1;
+    idxs[k] = p < P ? *gview(fd.index + p) : -1;
     ds[k] = p < P ? frame_depth(fd, p) : 0.0f;
   }
 #pragma unroll
@@ -786,13 +802,13 @@ __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const
     float acc[10];
     int n = 0;
     if (ok) {
-      const float2 *A = (const float2 *)(fd.ii + (size_t)offs[k][0] * 10);
-      const float2 *D = (const float2 *)(fd.ii + (size_t)offs[k][1] * 10);
-      const float2 *B = (const float2 *)(fd.ii + (size_t)offs[k][2] * 10);
-      const float2 *C = (const float2 *)(fd.ii + (size_t)offs[k][3] * 10);
+      const auto *A = gview((const gvec2 *)(fd.ii + (size_t)offs[k][0] * 10));
+      const auto *D = gview((const gvec2 *)(fd.ii + (size_t)offs[k][1] * 10));
+      const auto *B = gview((const gvec2 *)(fd.ii + (size_t)offs[k][2] * 10));
+      const auto *C = gview((const gvec2 *)(fd.ii + (size_t)offs[k][3] * 10));
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
-        float2 a = A[q], dd = D[q], bb = B[q], cc = C[q];
+        const gvec2 a = A[q], dd = D[q], bb = B[q], cc = C[q];
         acc[2 * q] = ((a.x + dd.x) - bb.x) - cc.x;
         acc[2 * q + 1] = ((a.y + dd.y) - bb.y) - cc.y;
       }
@@ -837,17 +853,17 @@ __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const
         nrm = make_float4(0.0f, 0.0f, 0.0f, curv);
       }
     }
-    fd.nrm[idx] = nrm;
+    gstore4(fd.nrm + idx, nrm);
     float4 *o = fd.omg + 3 * (size_t)idx;
-    o[0] = make_float4(op[0], op[1], op[2], op[3]);
-    o[1] = make_float4(op[4], op[5], on[0], on[1]);
-    o[2] = make_float4(on[2], on[3], on[4], on[5]);
+    gstore4(o, make_float4(op[0], op[1], op[2], op[3]));
+    gstore4(o + 1, make_float4(op[4], op[5], on[0], on[1]));
+    gstore4(o + 2, make_float4(on[2], on[3], on[4], on[5]));
     if (sp.keep_stats) {
       float4 *s = fd.stats + 4 * (size_t)idx;
-      s[0] = make_float4(V[0], V[1], V[2], V[3]);
-      s[1] = make_float4(V[4], V[5], V[6], V[7]);
-      s[2] = make_float4(V[8], ev[0], ev[1], ev[2]);
-      s[3] = make_float4(mean[0], mean[1], mean[2], __int_as_float(ok ? n : 0));
+      gstore4(s, make_float4(V[0], V[1], V[2], V[3]));
+      gstore4(s + 1, make_float4(V[4], V[5], V[6], V[7]));
+      gstore4(s + 2, make_float4(V[8], ev[0], ev[1], ev[2]));
+      gstore4(s + 3, make_float4(mean[0], mean[1], mean[2], __int_as_float(ok ? n : 0)));
     }
   }
 }
@@ -903,12 +919,12 @@ __device__ __forceinline__ void zproject_body(const PairDev *__restrict__ Pp, co
   const int g = blockIdx.x;
   const PairDev &pd = Pp[blk_pair[g]];
   const FrameDev &fd = F[which ? pd.cur : pd.ref];
-  const int n = *fd.npoints;
+  const int n = *gview(fd.npoints);
   const int base = (g - (which ? pd.ctile_base : pd.rtile_base)) * kTile;
   const float *S = pd.state + (which ? kSlotKRtCur : kSlotKRt);
   float KRt[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) KRt[k] = S[k];
+  for (int k = 0; k < 16; ++k) KRt[k] = *gview(S + k);
   const int cols = pd.cols, rows = pd.rows;
   const int lane = threadIdx.x & 63;
   // a subsampled pair (the start probe) reads only the cells of every sub-th row and column:
@@ -920,7 +936,7 @@ __device__ __forceinline__ void zproject_body(const PairDev *__restrict__ Pp, co
     int cell = -1;
     unsigned long long key = kZInit;
     if (i < n) {
-      const float4 p = fd.pts[i];
+      const float4 p = gload4(fd.pts + i);
       if (M == kSpherical) {
         int x, y;
         float d;
@@ -960,9 +976,11 @@ __device__ __forceinline__ void zproject_body(const PairDev *__restrict__ Pp, co
       m = other < m ? other : m;
     }
     if (cell >= 0 && !same) {
-      if (strategy == 0 || pd.zbuf[cell] > key) atomicMin(pd.zbuf + cell, key);
+      if (strategy == 0 || *gview(pd.zbuf + cell) > key)
+        __hip_atomic_fetch_min(gmut(pd.zbuf + cell), key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else if (lane == lead) {
-      if (strategy == 0 || pd.zbuf[cell] > m) atomicMin(pd.zbuf + cell, m);
+      if (strategy == 0 || *gview(pd.zbuf + cell) > m)
+        __hip_atomic_fetch_min(gmut(pd.zbuf + cell), m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -1105,7 +1123,7 @@ __device__ void block_partial(Acc2 *acc, double *out) {
     double s = 0.0;
     if (threadIdx.x < kVals)
       for (int w = 0; w < kThreads / 64; ++w) s += red[w][threadIdx.x];
-    out[threadIdx.x] = s;
+    *gmut(out + threadIdx.x) = s;
   }
 }
 
@@ -1218,7 +1236,7 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
   const int base = t * kTile;
   float T[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) T[k] = pd.state[kSlotTinv + k];
+  for (int k = 0; k < 16; ++k) T[k] = *gview(pd.state + kSlotTinv + k);
   const bool multi = ap.row_region != 0 || ap.col_region != 0;
   Acc2 acc[kVals];
 #pragma unroll
@@ -1235,21 +1253,21 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
         const int gr = p / gcols;
         pix[k] = p < P ? gr * sub * pd.cols + (p - gr * gcols) * sub : -1;
       }
-      const unsigned long long key = pix[k] >= 0 ? pd.zbuf[pix[k]] : kZInit;
+      const unsigned long long key = pix[k] >= 0 ? *gview(pd.zbuf + pix[k]) : kZInit;
       ri[k] = key == kZInit ? -1 : (int)(unsigned int)(key & 0xffffffffull);
-      ci[k] = pix[k] >= 0 ? pd.cur_index[pix[k]] : -1;
+      ci[k] = pix[k] >= 0 ? *gview(pd.cur_index + pix[k]) : -1;
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      if (pix[k] >= 0) pd.zbuf[pix[k]] = kZInit;
+      if (pix[k] >= 0) *gmut(pd.zbuf + pix[k]) = kZInit;
     float4 cn[4], rn0[4], cp[4], rp0[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (ri[k] >= 0 && ci[k] >= 0) {
-        cn[k] = fc.nrm[ci[k]];
-        rn0[k] = fr.nrm[ri[k]];
-        cp[k] = fc.pts[ci[k]];
-        rp0[k] = fr.pts[ri[k]];
+        cn[k] = gload4(fc.nrm + ci[k]);
+        rn0[k] = gload4(fr.nrm + ri[k]);
+        cp[k] = gload4(fc.pts + ci[k]);
+        rp0[k] = gload4(fr.pts + ri[k]);
       }
     }
     bool ok[4];
@@ -1267,13 +1285,15 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
     for (int k = 0; k < 4; ++k)
       if (ok[k]) {
         const float4 *o = fc.omg + 3 * (size_t)ci[k];
-        om[k][0] = o[0];
-        om[k][1] = o[1];
-        om[k][2] = o[2];
+        om[k][0] = gload4(o);
+        om[k][1] = gload4(o + 1);
+        om[k][2] = gload4(o + 2);
       }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      if (store_pairs && pix[k] >= 0) pd.pairs[pix[k]] = ok[k] ? make_int2(ri[k], ci[k]) : make_int2(-1, -1);
+      if (store_pairs && pix[k] >= 0)
+        *gmut((unsigned long long *)pd.pairs + pix[k]) =
+            ok[k] ? ((unsigned long long)(unsigned int)ci[k] << 32) | (unsigned int)ri[k] : ~0ull;
       if (ok[k]) lin_terms(rp[k], rn[k], cp[k], cn[k], om[k], ap, acc);
     }
   } else {
@@ -1317,7 +1337,7 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
   const int base = t * kTile;
   float T[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) T[k] = pd.state[slot + k];
+  for (int k = 0; k < 16; ++k) T[k] = *gview(pd.state + slot + k);
   Acc2 acc[kVals];
 #pragma unroll
   for (int v = 0; v < kVals; ++v) acc[v] = 0.0f;
@@ -1325,20 +1345,25 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int p = base + k * kThreads + threadIdx.x;
-    pr[k] = p < P ? pd.pairs[p] : make_int2(-1, -1);
+    if (p < P) {
+      const unsigned long long v = *gview((const unsigned long long *)pd.pairs + p);
+      pr[k] = make_int2((int)(unsigned int)v, (int)(unsigned int)(v >> 32));
+    } else {
+      pr[k] = make_int2(-1, -1);
+    }
   }
   float4 rp0[4], rn0[4], cp[4], cn[4], om[4][3];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
     if (pr[k].x >= 0) {
-      rp0[k] = fr.pts[pr[k].x];
-      rn0[k] = fr.nrm[pr[k].x];
-      cp[k] = fc.pts[pr[k].y];
-      cn[k] = fc.nrm[pr[k].y];
+      rp0[k] = gload4(fr.pts + pr[k].x);
+      rn0[k] = gload4(fr.nrm + pr[k].x);
+      cp[k] = gload4(fc.pts + pr[k].y);
+      cn[k] = gload4(fc.nrm + pr[k].y);
       const float4 *o = fc.omg + 3 * (size_t)pr[k].y;
-      om[k][0] = o[0];
-      om[k][1] = o[1];
-      om[k][2] = o[2];
+      om[k][0] = gload4(o);
+      om[k][1] = gload4(o + 1);
+      om[k][2] = gload4(o + 2);
     }
 #pragma unroll
   for (int k = 0; k < 4; ++k)
@@ -1838,7 +1863,7 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_nn_query(const PairDev 
   __shared__ float4 nn_sh[kThreads / 64][2][64];  // per wavefront: the points of one scan round
   float T[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) T[k] = pd.state[kSlotTinv + k];
+  for (int k = 0; k < 16; ++k) T[k] = *gview(pd.state + kSlotTinv + k);
   // sR = T.linear() * ns
   float sR[9];
 #pragma unroll
