@@ -1094,12 +1094,64 @@ __device__ __forceinline__ void lin_terms(const float *rp, const float *rn, floa
   acc_add(acc[28], 1.0f);
 }
 
+// the value of lane ^ O (O = 1, 2, 4, 8: within a DPP row of 16 lanes)
+template <int O>
+__device__ __forceinline__ int dpp_xor(int x) {
+  if (O == 1) return __builtin_amdgcn_update_dpp(x, x, 0xb1, 0xf, 0xf, false);  // quad_perm [1, 0, 3, 2]
+  if (O == 2) return __builtin_amdgcn_update_dpp(x, x, 0x4e, 0xf, 0xf, false);  // quad_perm [2, 3, 0, 1]
+  if (O == 8) return __builtin_amdgcn_update_dpp(x, x, 0x128, 0xf, 0xf, false);  // row_ror:8
+  const int from_below = __builtin_amdgcn_update_dpp(x, x, 0x114, 0xf, 0xf, false);  // row_shr:4, lane - 4
+  const int from_above = __builtin_amdgcn_update_dpp(x, x, 0x104, 0xf, 0xf, false);  // row_shl:4, lane + 4
+  return (threadIdx.x & 4) ? from_below : from_above;
+}
+template <int O>
+__device__ __forceinline__ double dpp_xor_f64(double x) {
+  const unsigned long long b = __double_as_longlong(x);
+  const unsigned int lo = (unsigned int)dpp_xor<O>((int)(unsigned int)b);
+  const unsigned int hi = (unsigned int)dpp_xor<O>((int)(unsigned int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// one halving step of the reduce-scatter for partner offsets 32 and 16: a permlane swap of
+// (v[i], v[h + i]) leaves each lane its kept slot and its partner's copy of it, one in each
+// register, whichever half the lane keeps
+template <int O>
+__device__ __forceinline__ double swap_add(double x, double y) {
+  const unsigned long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+  const unsigned int xl = (unsigned int)bx, xh = (unsigned int)(bx >> 32);
+  const unsigned int yl = (unsigned int)by, yh = (unsigned int)(by >> 32);
+  unsigned int nxl, nxh, nyl, nyh;
+  if (O == 32) {
+    const auto l = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+    nxl = l[0]; nyl = l[1]; nxh = h[0]; nyh = h[1];
+  } else {
+    const auto l = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+    nxl = l[0]; nyl = l[1]; nxh = h[0]; nyh = h[1];
+  }
+  return __longlong_as_double((long long)(((unsigned long long)nxh << 32) | nxl)) +
+         __longlong_as_double((long long)(((unsigned long long)nyh << 32) | nyl));
+}
+
+// halving step for partner offsets 8, 4, 2 (DPP within a row)
+template <int O, int H>
+__device__ __forceinline__ void dpp_halve(double *v) {
+  const bool up = (threadIdx.x & O) != 0;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const double keep = up ? v[H + i] : v[i], send = up ? v[i] : v[H + i];
+    v[i] = keep + dpp_xor_f64<O>(send);
+  }
+}
+
 // block-wide FP64 reduction of the kVals accumulators into one partial. Within a wavefront a
 // reduce-scatter: the kPartial (32) slots are halved five times; at partner offset o a lane
 // keeps the half of its slots selected by its lane bit o and adds its partner's copy of it
 // (32 FP64 exchanges instead of 6 per slot). Lane l then holds slot l >> 1 summed over
 // the half of the wavefront with its lane bit 0, and one more exchange with lane l ^ 1
-// completes it. The order of the additions is fixed.
+// completes it. The order of the additions is fixed. The exchanges run on the vector ALU:
+// permlane swaps for offsets 32 and 16, DPP moves below.
 __device__ void block_partial(Acc2 *acc, double *out) {
   static_assert(kPartial == 32, "the reduce-scatter halves 32 slots five times");
   __shared__ double red[kThreads / 64][kPartial];
@@ -1108,15 +1160,13 @@ __device__ void block_partial(Acc2 *acc, double *out) {
 #pragma unroll
   for (int i = 0; i < kPartial; ++i) v[i] = i < kVals ? (double)acc[i] : 0.0;
 #pragma unroll
-  for (int h = kPartial / 2, o = 32; h >= 1; h >>= 1, o >>= 1) {
-    const bool up = (lane & o) != 0;
+  for (int i = 0; i < 16; ++i) v[i] = swap_add<32>(v[i], v[16 + i]);
 #pragma unroll
-    for (int i = 0; i < h; ++i) {
-      const double keep = up ? v[h + i] : v[i], send = up ? v[i] : v[h + i];
-      v[i] = keep + __shfl_xor(send, o, 64);
-    }
-  }
-  v[0] += __shfl_xor(v[0], 1, 64);
+  for (int i = 0; i < 8; ++i) v[i] = swap_add<16>(v[i], v[8 + i]);
+  dpp_halve<8, 4>(v);
+  dpp_halve<4, 2>(v);
+  dpp_halve<2, 1>(v);
+  v[0] += dpp_xor_f64<1>(v[0]);
   if ((lane & 1) == 0) red[wave][lane >> 1] = v[0];
   __syncthreads();
   if (threadIdx.x < kPartial) {
