@@ -199,42 +199,51 @@ extern "C" __global__ __launch_bounds__(1024) void k_scan(const FrameDev *__rest
     }
 }
 
-// Each thread owns 4 consecutive pixels, so the block's numbering is row-major.
+// Compaction of the valid pixels of a tile into the cloud, numbered in raster order.
 template <int M>
 __device__ __forceinline__ void unproject_body(const FrameDev *__restrict__ F, const int *__restrict__ blk_frame) {
   const int g = blockIdx.x;
   const FrameDev &fd = F[blk_frame[g]];
   const int P = fd.rows * fd.cols;
   const int t = g - fd.tile_base;
-  const int p0 = t * kTile + threadIdx.x * 4;
-  float dv[4];
-  int valid[4], cnt = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int p = p0 + k;
-    dv[k] = p < P ? frame_depth(fd, p) : 0.0f;
-    valid[k] = p < P && pixel_valid<M>(fd, p, dv[k]);
-    cnt += valid[k];
-  }
-  __shared__ int wsum[kThreads / 64];
-  int incl = wave_incl_scan(cnt);
+  const int base = t * kTile;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (lane == 63) wsum[wave] = incl;
-  __syncthreads();
-  int before = 0;
-  for (int w = 0; w < wave; ++w) before += wsum[w];
-  int idx = fd.tile_off[t] + before + incl - cnt;
+  // pixel base + k * kThreads + threadIdx.x: each load and store instruction of a wavefront
+  // covers 64 consecutive pixels, and the valid ones take consecutive cloud indices
+  float dv[4];
+  unsigned long long bal[4];
+  __shared__ int wsum[4][kThreads / 64];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    int p = p0 + k;
-    if (p >= P) break;
-    if (valid[k]) {
-      int r = p / fd.cols, c = p - r * fd.cols;
-      fd.pts[idx] = M == kSpherical ? sph_point(fd.sph, fd.iKRt, c, r, dv[k]) : unproject_point(fd.iKRt, c, r, dv[k]);
-      fd.index[p] = idx;
-      ++idx;
+    const int p = base + k * kThreads + threadIdx.x;
+    dv[k] = p < P ? frame_depth(fd, p) : 0.0f;
+    bal[k] = __ballot(p < P && pixel_valid<M>(fd, p, dv[k]));
+    if (lane == 0) wsum[k][wave] = __popcll(bal[k]);
+  }
+  __syncthreads();
+  // the cloud index of a valid pixel is its rank in raster order: the tile's offset, the
+  // valid pixels of the earlier rounds k, of the earlier wavefronts of round k, and of the
+  // lower lanes of its own wavefront
+  const unsigned long long below = lane ? ~0ull >> (64 - lane) : 0ull;
+  int before = *gview(fd.tile_off + t);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int p = base + k * kThreads + threadIdx.x;
+    int at = before, round = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+      if (w < wave) at += wsum[k][w];
+      round += wsum[k][w];
+    }
+    before += round;
+    if (p >= P) continue;
+    if ((bal[k] >> lane) & 1ull) {
+      const int idx = at + __popcll(bal[k] & below);
+      const int r = p / fd.cols, c = p - r * fd.cols;
+      gstore4(fd.pts + idx, M == kSpherical ? sph_point(fd.sph, fd.iKRt, c, r, dv[k]) : unproject_point(fd.iKRt, c, r, dv[k]));
+      *gmut(fd.index + p) = idx;
     } else {
-      fd.index[p] = -1;
+      *gmut(fd.index + p) = -1;
     }
   }
 }
