@@ -765,9 +765,10 @@ extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_
   std::vector<float> pred((size_t)num_pairs * 6);
   HIPCHK(hipMemcpyAsync(pred.data(), n->act[kLayers], pred.size() * sizeof(float), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  // 2. both starts judged by one linearisation over every second row and column (the whole
-  //    reference cloud is projected, so the correspondences there are the exact ones):
-  //    pairs [0, n) start at the prediction, [n, 2n) at the fallback guess
+  // 2. both starts judged by one linearisation over every kProbeSub-th row and column (the
+  //    whole reference cloud is projected, so the correspondences there are the exact ones):
+  //    pairs [0, n) start at the prediction, [n, 2n) at the fallback guess; pairs i and n + i
+  //    share their frames, so one pass reads each reference point for both projections
   std::vector<float> starts((size_t)2 * num_pairs * 16);
   std::vector<int> ref2(2 * (size_t)num_pairs), cur2(2 * (size_t)num_pairs);
   for (int i = 0; i < num_pairs; ++i) {
@@ -798,7 +799,7 @@ extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_
     if (p) pf.begin(KID_START_PROBE, s, en);
     pf.on = false;
     const int rc = batch_align(b, 2 * num_pairs, ref2.data(), cur2.data(), starts.data(), &probe, 0, nullptr, nullptr,
-                               false, &raw, kProbeSub);
+                               false, &raw, kProbeSub, num_pairs);
     pf.on = p;
     CHK(rc);
     if (p) {

@@ -465,6 +465,9 @@ struct PairPlan {
   std::vector<int> blk_pair, blk_ref, blk_cur;  // block maps
   int total_pix_tiles = 0, total_rtiles = 0, total_ctiles = 0;
   size_t total_pix = 0;
+  // > 0: pairs q and q + twin share both frames (the two starts of the start probe), and
+  // one k_zproject_twin pass over the first half's reference tiles projects both
+  int twin = 0;
 };
 
 // hosts frames[] (host copies) and pair list -> plan with block maps and buffer offsets
@@ -640,6 +643,9 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
         HIPCHK(hipEventRecord(e0, s));
       }
       if (sph) LAUNCH(pf, KID_ZPROJECT_REF, k_zproject_sph, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0, zs);
+      else if (plan.twin > 0)
+        LAUNCH(pf, KID_ZPROJECT_REF, k_zproject_twin, dim3(plan.pairs[plan.twin].rtile_base), tb, s, d_pairs, d_frames,
+               d_blk_ref, plan.twin);
       else LAUNCH(pf, KID_ZPROJECT_REF, k_zproject, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0, zs);
       if (tune) HIPCHK(hipEventRecord(tune->ev.back(), s));
     }
@@ -1619,7 +1625,7 @@ namespace {
 // sub > 1 linearises over every sub-th row and column only (the start probe).
 int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *cur_frame, const float *init_guesses16,
                 const nicp_align_params *ap, int use_graph, float *T_out16, nicp_align_result *results,
-                bool with_stats, std::vector<double> *raw, int sub = 1) {
+                bool with_stats, std::vector<double> *raw, int sub = 1, int twin = 0) {
   if (!b || num_pairs <= 0 || !ref_frame || !cur_frame) return set_err(NICP_ERR_INVALID, "bad arguments");
   CHK(check_align_params(ap));
   for (int i = 0; i < num_pairs; ++i)
@@ -1659,6 +1665,11 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
       for (int t = 0; t < pd.n_tiles; ++t) plan.blk_pair.push_back(i);
     }
     plan.total_pix_tiles = (int)plan.blk_pair.size();
+    // the twin pass needs pairs q and q + twin on the same frames (pinhole)
+    bool twins = twin > 0 && num_pairs == 2 * twin && plan.pairs[0].model != NICP_SPHERICAL;
+    for (int i = 0; twins && i < twin; ++i)
+      twins = ref_frame[i] == ref_frame[i + twin] && cur_frame[i] == cur_frame[i + twin];
+    plan.twin = twins ? twin : 0;
   }
   AlignBuffers B{&b->pairs_tbl, &b->maps, &b->zbuf, &b->cur_index, &b->pairs_buf, &b->state, &b->partials, &b->result};
   int *d_bp, *d_br, *d_bc;

@@ -918,23 +918,85 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_zfill(const PairDev *__
   }
 }
 
+// the z-buffer cell and key of cloud point i (p) under KRt, or cell -1 when the point
+// falls outside the pair's depth range or image (or off the sampled cells, sub > 1)
+template <int M>
+__device__ __forceinline__ void zproject_point(const PairDev &pd, const float *KRt, float4 p, int i, int sub,
+                                               int &cell, unsigned long long &key) {
+  const int cols = pd.cols, rows = pd.rows;
+  if (M == kSpherical) {
+    int x, y;
+    float d;
+    if (sph_project(pd.sph, KRt, p, pd.min_d, pd.max_d, x, y, d) && x >= 0 && x < cols && y >= 0 && y < rows &&
+        (sub == 1 || (x % sub == 0 && y % sub == 0))) {
+      cell = y * cols + x;
+      key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
+    }
+  } else {
+    float ip[4];
+    mul4_packet(KRt, p.x, p.y, p.z, p.w, ip);
+    const float d = ip[2];
+    if (!(d < pd.min_d || d > pd.max_d)) {
+      const float inv = 1.0f / d;
+      const int x = (int)roundf(ip[0] * inv), y = (int)roundf(ip[1] * inv);
+      if (x >= 0 && x < cols && y >= 0 && y < rows && (sub == 1 || (x % sub == 0 && y % sub == 0))) {
+        cell = y * cols + x;
+        key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
+      }
+    }
+  }
+}
+
+// the wavefront's atomics for its lanes' (cell, key)
+__device__ __forceinline__ void zmerge(const PairDev &pd, int cell, unsigned long long key, int lane, int strategy) {
+  // The lanes that hit the cell of the wavefront's first active lane send one atomicMin
+  // with their smallest key. Device-scope atomics run one by one on the memory side, so
+  // this keeps a pair whose points pile onto a few pixels (a diverging transform) from
+  // stalling the whole launch; the z-buffer is the same.
+  const unsigned long long active = __ballot(cell >= 0);
+  if (active == 0) return;
+  const int lead = __ffsll((long long)active) - 1;
+  const int lead_cell = __shfl(cell, lead, 64);
+  const bool same = cell >= 0 && cell == lead_cell;
+  unsigned long long m = same ? key : kZInit;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned int lo = __shfl_xor((unsigned int)m, o, 64), hi = __shfl_xor((unsigned int)(m >> 32), o, 64);
+    const unsigned long long other = ((unsigned long long)hi << 32) | lo;
+    m = other < m ? other : m;
+  }
+  if (cell >= 0 && !same) {
+    if (strategy == 0 || *gview(pd.zbuf + cell) > key)
+      __hip_atomic_fetch_min(gmut(pd.zbuf + cell), key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else if (lane == lead) {
+    if (strategy == 0 || *gview(pd.zbuf + cell) > m)
+      __hip_atomic_fetch_min(gmut(pd.zbuf + cell), m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // which = 0: the reference cloud with the pair's current KRt; 1: the current cloud
 // strategy 0: every merged atomic is issued; 1: the cell is read first and the atomic skipped
 // when the stored key is already no larger (the read may be stale, but only ever larger than
 // the cell's current value, so a skip never loses a smaller key). The z-buffer is the same.
-template <int M>
+// twin > 0 (reference cloud only): pair q + twin has the same frames as pair q, another
+// transform and its own z-buffer; each point is read once and projected for both pairs.
+template <int M, bool Twin>
 __device__ __forceinline__ void zproject_body(const PairDev *__restrict__ Pp, const FrameDev *__restrict__ F,
-                                              const int *__restrict__ blk_pair, int which, int strategy) {
+                                              const int *__restrict__ blk_pair, int which, int strategy, int twin) {
   const int g = blockIdx.x;
-  const PairDev &pd = Pp[blk_pair[g]];
+  const int q = blk_pair[g];
+  const PairDev &pd = Pp[q];
   const FrameDev &fd = F[which ? pd.cur : pd.ref];
   const int n = *gview(fd.npoints);
   const int base = (g - (which ? pd.ctile_base : pd.rtile_base)) * kTile;
-  const float *S = pd.state + (which ? kSlotKRtCur : kSlotKRt);
-  float KRt[16];
+  constexpr int J = Twin ? 2 : 1;
+  float KRt[J][16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) KRt[k] = *gview(S + k);
-  const int cols = pd.cols, rows = pd.rows;
+  for (int j = 0; j < J; ++j) {
+    const float *S = Pp[q + j * twin].state + (which ? kSlotKRtCur : kSlotKRt);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) KRt[j][k] = *gview(S + k);
+  }
   const int lane = threadIdx.x & 63;
   // a subsampled pair (the start probe) reads only the cells of every sub-th row and column:
   // points landing elsewhere are dropped before their atomics
@@ -942,54 +1004,15 @@ __device__ __forceinline__ void zproject_body(const PairDev *__restrict__ Pp, co
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int i = base + k * kThreads + threadIdx.x;
-    int cell = -1;
-    unsigned long long key = kZInit;
-    if (i < n) {
-      const float4 p = gload4(fd.pts + i);
-      if (M == kSpherical) {
-        int x, y;
-        float d;
-        if (sph_project(pd.sph, KRt, p, pd.min_d, pd.max_d, x, y, d) && x >= 0 && x < cols && y >= 0 && y < rows &&
-            (sub == 1 || (x % sub == 0 && y % sub == 0))) {
-          cell = y * cols + x;
-          key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
-        }
-      } else {
-        float ip[4];
-        mul4_packet(KRt, p.x, p.y, p.z, p.w, ip);
-        const float d = ip[2];
-        if (!(d < pd.min_d || d > pd.max_d)) {
-          const float inv = 1.0f / d;
-          const int x = (int)roundf(ip[0] * inv), y = (int)roundf(ip[1] * inv);
-          if (x >= 0 && x < cols && y >= 0 && y < rows && (sub == 1 || (x % sub == 0 && y % sub == 0))) {
-            cell = y * cols + x;
-            key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
-          }
-        }
-      }
-    }
-    // The lanes that hit the cell of the wavefront's first active lane send one atomicMin
-    // with their smallest key. Device-scope atomics run one by one on the memory side, so
-    // this keeps a pair whose points pile onto a few pixels (a diverging transform) from
-    // stalling the whole launch; the z-buffer is the same.
-    const unsigned long long active = __ballot(cell >= 0);
-    if (active == 0) continue;
-    const int lead = __ffsll((long long)active) - 1;
-    const int lead_cell = __shfl(cell, lead, 64);
-    const bool same = cell >= 0 && cell == lead_cell;
-    unsigned long long m = same ? key : kZInit;
+    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < n) p = gload4(fd.pts + i);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const unsigned int lo = __shfl_xor((unsigned int)m, o, 64), hi = __shfl_xor((unsigned int)(m >> 32), o, 64);
-      const unsigned long long other = ((unsigned long long)hi << 32) | lo;
-      m = other < m ? other : m;
-    }
-    if (cell >= 0 && !same) {
-      if (strategy == 0 || *gview(pd.zbuf + cell) > key)
-        __hip_atomic_fetch_min(gmut(pd.zbuf + cell), key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else if (lane == lead) {
-      if (strategy == 0 || *gview(pd.zbuf + cell) > m)
-        __hip_atomic_fetch_min(gmut(pd.zbuf + cell), m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int j = 0; j < J; ++j) {
+      const PairDev &pj = Pp[q + j * twin];
+      int cell = -1;
+      unsigned long long key = kZInit;
+      if (i < n) zproject_point<M>(pj, KRt[j], p, i, sub, cell, key);
+      zmerge(pj, cell, key, lane, strategy);
     }
   }
 }
@@ -997,13 +1020,19 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_zproject(const PairDev 
                                                                  const FrameDev *__restrict__ F,
                                                                  const int *__restrict__ blk_pair, int which,
                                                                  int strategy) {
-  zproject_body<kPinhole>(Pp, F, blk_pair, which, strategy);
+  zproject_body<kPinhole, false>(Pp, F, blk_pair, which, strategy, 0);
 }
 extern "C" __global__ __launch_bounds__(kThreads) void k_zproject_sph(const PairDev *__restrict__ Pp,
                                                                      const FrameDev *__restrict__ F,
                                                                      const int *__restrict__ blk_pair, int which,
                                                                      int strategy) {
-  zproject_body<kSpherical>(Pp, F, blk_pair, which, strategy);
+  zproject_body<kSpherical, false>(Pp, F, blk_pair, which, strategy, 0);
+}
+// the reference clouds of pairs q and q + twin (same frames) in one pass
+extern "C" __global__ __launch_bounds__(kThreads) void k_zproject_twin(const PairDev *__restrict__ Pp,
+                                                                      const FrameDev *__restrict__ F,
+                                                                      const int *__restrict__ blk_pair, int twin) {
+  zproject_body<kPinhole, true>(Pp, F, blk_pair, 0, 0, twin);
 }
 
 // z-buffer -> index image (and depth image); resets the z-buffer. Pairs that use the
