@@ -8,8 +8,10 @@ One step = the registration path over one batch of pairs resident in HBM: both
 frames of every pair converted to clouds (unprojection, integral-image normals,
 information matrices), the network input and forward pass of the learned initial
 guess, one linearisation at the predicted and at the identity start to choose the
-start, and 10 Gauss-Newton iterations of projective NICP from the chosen start --
-one launch per stage for the whole batch. A pair counts as aligned when its
+start, 10 Gauss-Newton iterations of projective NICP from the chosen start, and
+Aligner::_computeStatistics (the statistics linearisation and the FP64 unscented
+solution statistics, on the device) -- one launch per stage for the whole batch, with
+every pair's transform and result record copied back. A pair counts as aligned when its
 rotation error is below 0.1 deg and its translation error below 2 mm against the
 sampled ground truth.
 
@@ -98,34 +100,123 @@ def aligned(T, Tgt):
     return (e[:, 0] < 0.1) & (e[:, 1] < 0.002), e
 
 
-def cpu_baseline(seeds, max_rot, max_trans, budget_s=20.0):
-    """The CPU oracle (the reference's FP32 arithmetic, OpenMP as in the reference) on a
-    bounded sample: both conversions and AlignerProjective::align from the identity."""
+def host_cpu():
+    """(model name, CPUs this process may run on, CPUs of the machine)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, usable, os.cpu_count() or usable
+
+
+def cpu_leg(rows, cols, threads, seeds, warmup):
+    """Median seconds per pair of the CPU oracle: both DepthImageConverterIntegralImage::compute
+    calls and a 10-iteration AlignerProjective::align from the identity (scene rendering excluded)."""
     import oracle as O
     from nicp_amd import synth
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-    fx, fy, cx, cy = synth.camera_for(ROWS, COLS)
+    fx, fy, cx, cy = synth.camera_for(rows, cols)
     cam = O.camera(fx, fy, cx, cy)
     sp = O.stats_params()
     ap = O.align_params()
     I = np.eye(4, dtype=np.float32).flatten(order="F")
-    pairs = [synth.pair(int(s), ROWS, COLS, max_rot_deg=max_rot, max_trans_m=max_trans) for s in seeds]
-    dr, dc, _ = pairs[0]
-    fr, fc = O.Frame(dr, 0.001, cam, sp), O.Frame(dc, 0.001, cam, sp)  # the clouds point into these
-    O.align(fr.cloud(), fc.cloud(), ROWS, COLS, cam, I, ap, threads)
-    t0 = time.perf_counter()
-    n = 0
-    for dr, dc, _ in pairs:
+    O.set_num_threads(threads)
+    times = []
+    for k, s in enumerate(seeds):
+        dr, dc, _ = synth.pair(int(s), rows, cols, max_rot_deg=5.0, max_trans_m=0.05)
+        t0 = time.perf_counter()
         fr = O.Frame(dr, 0.001, cam, sp)
         fc = O.Frame(dc, 0.001, cam, sp)
-        O.align(fr.cloud(), fc.cloud(), ROWS, COLS, cam, I, ap, threads)
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": round(n / dt, 3), "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": "%d of the benchmark's pairs at %dx%d (2 conversions + 10-iteration align from the identity "
-                      "each), OpenMP threads %d" % (n, COLS, ROWS, threads)}
+        O.align(fr.cloud(), fc.cloud(), rows, cols, cam, I, ap, threads)
+        dt = time.perf_counter() - t0
+        if k >= warmup:
+            times.append(dt)
+    return float(np.median(times))
+
+
+def cpu_baseline(n_pairs=32, warmup=3):
+    """The CPU oracle (the reference's FP32 arithmetic and OpenMP structure, built with the
+    reference's Release flags) on the host: 320x240 (fx=fy=262.5) and 640x480 (fx=fy=525) pairs
+    with motions <= 5 deg / 5 cm, the median of n_pairs seeded pairs after `warmup` pairs, with all
+    of this process's CPUs (OMP_NUM_THREADS when set) and with one thread. `value` is the 640x480
+    leg on all threads; the iteration count is fixed (10, no early exit), so the work per pair does
+    not depend on the motion range."""
+    model, usable, machine = host_cpu()
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else usable
+    seeds = np.arange(EVAL_SEED_BASE, EVAL_SEED_BASE + warmup + n_pairs)
+    legs = []
+    for rows, cols in ((240, 320), (480, 640)):
+        for th in (threads, 1):
+            sec = cpu_leg(rows, cols, th, seeds, warmup)
+            legs.append({"resolution": "%dx%d" % (cols, rows), "threads": th, "median_ms_per_pair": round(1000 * sec, 2),
+                         "pairs_per_s": round(1.0 / sec, 3)})
+            log("cpu oracle %dx%d, %d threads: %.1f ms per pair" % (cols, rows, th, 1000 * sec))
+    main = next(l for l in legs if l["resolution"] == "640x480" and l["threads"] == threads)
+    return {"value": main["pairs_per_s"], "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": "median of %d seeded pairs after %d warm-up pairs per leg (rot<=5deg, trans<=5cm): 2 "
+                      "conversions + 10-iteration align from the identity each; legs 320x240 and 640x480, "
+                      "%d threads and 1 thread" % (n_pairs, warmup, threads),
+            "legs": legs, "cpu_model": model, "cpus_usable": usable, "cpus_machine": machine}
+
+
+def parity_sample(ctx, seeds, max_rot, max_trans):
+    """Per-run parity counts on pairs of the headline workload (640x480): normals of the device
+    conversion beyond 1e-5 of the CPU oracle's, points whose normal is present on one side only
+    (curvature at the stats threshold), and correspondences that differ from the oracle's on
+    identical clouds at the ground-truth transform (gate-boundary mismatches)."""
+    import oracle as O
+    from nicp_amd import capi, synth
+    fx, fy, cx, cy = synth.camera_for(ROWS, COLS)
+    ocam, dcam = O.camera(fx, fy, cx, cy), capi.camera(fx, fy, cx, cy)
+    osp, dsp = O.stats_params(), capi.stats_params()
+    oap, dap = O.align_params(), capi.align_params()
+    I = np.eye(4, dtype=np.float32).flatten(order="F")
+    out = {"pairs": len(seeds), "points": 0, "normals_beyond_1e-5": 0, "normal_presence_flips": 0,
+           "max_normal_err": 0.0, "correspondences": 0, "correspondence_mismatches": 0}
+    for s in seeds:
+        dr, dc, Tgt = synth.pair(int(s), ROWS, COLS, max_rot_deg=max_rot, max_trans_m=max_trans)
+        fo = []
+        for d in (dr, dc):
+            f = O.Frame(d, 0.001, ocam, osp)
+            cl = ctx.cloud_from_depth(d, dcam, dsp)
+            _, nrm, _, _, _ = cl.download()
+            cl.close()
+            has_o = np.abs(f.normals[:, :3]).sum(1) > 0
+            has_d = np.abs(nrm[:, :3]).sum(1) > 0
+            both = has_o & has_d
+            err = np.abs(nrm[both, :3] - f.normals[both, :3]).max(axis=1) if both.any() else np.zeros(1)
+            out["points"] += int(f.n)
+            out["normals_beyond_1e-5"] += int((err > 1e-5).sum())
+            out["normal_presence_flips"] += int((has_o != has_d).sum())
+            out["max_normal_err"] = max(out["max_normal_err"], float(err.max()))
+            fo.append(f)
+        fr, fc = fo
+        KRt, _ = O.projector_matrices(np.array(ocam.K, np.float32), Tgt)
+        ri, _ = O.project(fr.points, KRt, ROWS, COLS, ocam.min_distance, ocam.max_distance)
+        KRt0, _ = O.projector_matrices(np.array(ocam.K, np.float32), I)
+        ci, _ = O.project(fc.points, KRt0, ROWS, COLS, ocam.min_distance, ocam.max_distance)
+        Tinv = np.linalg.inv(Tgt.reshape(4, 4, order="F").astype(np.float64)).astype(np.float32).flatten(order="F")
+        op = O.correspondences(fr.cloud(), fc.cloud(), ri, ci, Tinv, oap, 1)
+        ur = ctx.cloud_from_host(fr.points, fr.normals, fr.curvature, fr.omega_p, fr.omega_n)
+        uc = ctx.cloud_from_host(fc.points, fc.normals, fc.curvature, fc.omega_p, fc.omega_n)
+        dp = ctx.correspondences(ur, uc, ri, ci, Tinv, dap)
+        ur.close()
+        uc.close()
+        so = set(map(tuple, op.tolist()))
+        sd = set(map(tuple, dp.tolist()))
+        out["correspondences"] += len(so)
+        out["correspondence_mismatches"] += len(so ^ sd)
+    out["max_normal_err"] = float("%.3g" % out["max_normal_err"])
+    return out
 
 
 def uneven_batch(ctx, n_pairs, iters, steps):
@@ -160,7 +251,7 @@ def uneven_batch(ctx, n_pairs, iters, steps):
     t0 = time.perf_counter()
     for _ in range(steps):
         batch.convert()
-        Tout, _ = batch.align(ref, cur, None, ap, use_graph=True, want_results=False)
+        Tout, _ = batch.align(ref, cur, None, ap, use_graph=True, want_results=True)
     ctx.synchronize()
     dt = (time.perf_counter() - t0) / steps
     ok, _ = aligned(Tout, T)
@@ -285,14 +376,17 @@ def main():
     def seeds_of(base, k):
         return np.arange(base + k * B, base + (k + 1) * B, dtype=np.uint64)
 
+    # a step is the whole AlignerProjective::align of every pair, results included: the
+    # statistics linearisation and Aligner::_computeStatistics run on the device, and every
+    # pair's transform, H, b, error, inliers, Omega and validity are copied back
     def learned_step(use_graph):
         batch.convert()
-        T, _, _, used = batch.align_learned(net, ref, cur, ap, use_graph=use_graph, want_results=False)
+        T, res, _, used = batch.align_learned(net, ref, cur, ap, use_graph=use_graph, want_results=True)
         return T, used
 
     def plain_step(use_graph):
         batch.convert()
-        T, _ = batch.align(ref, cur, None, ap, use_graph=use_graph, want_results=False)
+        T, _ = batch.align(ref, cur, None, ap, use_graph=use_graph, want_results=True)
         return T
 
     # warm-up (separate seeds)
@@ -424,7 +518,7 @@ def main():
             ctx.synchronize()
             s0 = time.perf_counter()
             pb.convert()
-            pb.align(one[0], one[1], None, ap, use_graph=False, want_results=False)
+            pb.align(one[0], one[1], None, ap, use_graph=False, want_results=True)
             ctx.synchronize()
             if i > 0:
                 times.append(time.perf_counter() - s0)
@@ -447,9 +541,11 @@ def main():
         nn = nn_batch(ctx, args.nn_pairs, args.nn_points, 20, args.nn_steps)
         log("kNN batch: %s" % nn)
 
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu:
-        cpu = cpu_baseline(np.arange(EVAL_SEED_BASE, EVAL_SEED_BASE + 16), max_rot, max_trans)
+        parity = parity_sample(ctx, np.arange(EVAL_SEED_BASE, EVAL_SEED_BASE + 2), max_rot, max_trans)
+        log("parity sample: %s" % parity)
+        cpu = cpu_baseline()
         log("cpu baseline: %s" % cpu)
 
     out = {
@@ -494,6 +590,7 @@ def main():
         "autotune": [{"zproject_strategy": c, "ms_strategy0": round(m0, 4), "ms_strategy1": round(m1, 4)}
                      for c, m0, m1 in tuned_main],
         "kernel_timing_error": prof_error,
+        "parity_sample": parity,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

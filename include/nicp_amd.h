@@ -222,6 +222,13 @@ int nicp_linearize(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, 
 int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const nicp_camera *cam, int rows,
                int cols, const float init_guess[16], const nicp_align_params *ap, float T_out[16],
                nicp_align_result *result);
+/* Replaces Aligner::_computeStatistics (nicp/nicp/aligner.cpp:82-135, nicp/nicp/unscented.h:11-65)
+ * for a given statistics linearisation H (column-major 6x6) at the final transform T: fills
+ * result->mean, omega, translational_ratio, rotational_ratio and solution_valid (the other fields
+ * are left unchanged). It runs the device computation every align call runs after its statistics
+ * linearisation (FP64, pseudo-inverse of H, unscented sigma points through t2v(T v2t(s)^-1)). */
+int nicp_solution_statistics(nicp_ctx *ctx, const double H[36], const float T[16], const nicp_align_params *ap,
+                             nicp_align_result *result);
 /* CorrespondenceFinder::correspondences() of the last nicp_align on ctx (capacity in pairs) */
 int nicp_last_correspondences(nicp_ctx *ctx, int *pairs, int capacity, int *num_pairs);
 

@@ -78,7 +78,7 @@ EXPORTS = [
     "nicp_batch_align_learned", "nicp_device_alloc", "nicp_device_free", "nicp_device_copy",
     "nicp_initnet_train_step_rows", "nicp_initnet_set_input_mode", "nicp_cloud_gaussians", "nicp_cloud_set_gaussians",
     "nicp_cloud_download_gaussians", "nicp_cloud_transform", "nicp_cloud_add", "nicp_merge",
-    "nicp_batch_set_autotune", "nicp_batch_autotune_read", "nicp_align_nn_batch",
+    "nicp_batch_set_autotune", "nicp_batch_autotune_read", "nicp_align_nn_batch", "nicp_solution_statistics",
 ]
 
 _lib = None
@@ -111,6 +111,7 @@ def lib():
             "nicp_linearize": [vp, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp],
             "nicp_align": [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp],
             "nicp_last_correspondences": [vp, vp, C.c_int, vp],
+            "nicp_solution_statistics": [vp, vp, vp, vp, vp],
             "nicp_batch_create": [vp, C.c_int, vp, vp, vp, vp, vp],
             "nicp_batch_destroy": [vp], "nicp_batch_upload_depth": [vp, C.c_int, vp, C.c_float],
             "nicp_batch_convert": [vp],
@@ -324,6 +325,15 @@ class Context:
                                     _p(np.ascontiguousarray(T16, np.float32)), C.byref(ap), _p(H), _p(b), C.byref(e),
                                     C.byref(i)))
         return H.reshape(6, 6, order="F"), b, e.value, i.value
+
+    def solution_statistics(self, H, T16, ap):
+        """Aligner::_computeStatistics on the device for a given H (6x6) at T: an AlignResult whose
+        mean, omega, ratios and solution_valid are set."""
+        res = AlignResult()
+        Hc = np.ascontiguousarray(np.asarray(H, np.float64).reshape(6, 6).flatten(order="F"))
+        _check(lib().nicp_solution_statistics(self.h, _p(Hc), _p(np.ascontiguousarray(T16, np.float32)),
+                                              C.byref(ap), C.byref(res)))
+        return res
 
     def align(self, ref, cur, cam, rows, cols, init16, ap):
         T = np.empty(16, np.float32)

@@ -815,7 +815,7 @@ extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_
   //    least half the prediction's correspondences, or the prediction has none.
   std::vector<float> chosen((size_t)num_pairs * 16);
   for (int i = 0; i < num_pairs; ++i) {
-    const double *a = &raw[(size_t)i * 2 * kPartial], *f = &raw[((size_t)num_pairs + i) * 2 * kPartial];
+    const double *a = &raw[(size_t)i * kResultDoubles], *f = &raw[((size_t)num_pairs + i) * kResultDoubles];
     const bool net = !fallback_wins(a[27], a[28], f[27], f[28]);
     memcpy(&chosen[16 * (size_t)i], &starts[16 * (size_t)(net ? i : num_pairs + i)], 16 * sizeof(float));
     if (used_network) used_network[i] = net ? 1 : 0;

@@ -102,7 +102,7 @@ struct Carver {
 enum KernelId {
   KID_COUNT, KID_SCAN, KID_UNPROJECT, KID_INTEGRAL_X, KID_INTEGRAL_Y, KID_STATS, KID_ZFILL, KID_ZPROJECT_CUR,
   KID_ZRESOLVE, KID_ZPROJECT_REF, KID_CORR_LIN, KID_SOLVE, KID_LIN_PAIRS, KID_SOLVE_STATS, KID_NN_QUERY,
-  KID_NET_INPUT, KID_NET_FORWARD, KID_START_PROBE, KID_INTEGRAL_FRAME, KID_N
+  KID_NET_INPUT, KID_NET_FORWARD, KID_START_PROBE, KID_INTEGRAL_FRAME, KID_SOLUTION_STATS, KID_N
 };
 // the last three are stages: the network input kernel, the network's forward pass (its six
 // GEMM launches) and the one-iteration probe of both starts of the learned-start aligner
@@ -110,7 +110,7 @@ static const char *kKernelNames[KID_N] = {"k_count", "k_scan", "k_unproject", "k
                                           "k_stats", "k_zfill", "k_zproject(current)", "k_zresolve",
                                           "k_zproject(reference)", "k_corr_lin", "k_solve", "k_lin_pairs",
                                           "k_solve(statistics)", "k_nn_query", "k_net_input", "initnet_forward",
-                                          "start_probe", "k_integral_frame"};
+                                          "start_probe", "k_integral_frame", "k_solution_stats"};
 
 namespace {
 struct ProfEntry {
@@ -548,7 +548,7 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
   CHK(B.pairs_buf->reserve(plan.total_pix * sizeof(int2)));
   CHK(B.state->reserve((size_t)np * kStateFloats * sizeof(float)));
   CHK(B.partials->reserve((size_t)std::max(plan.total_pix_tiles, 1) * kPartial * sizeof(double)));
-  CHK(B.result->reserve((size_t)np * 2 * kPartial * sizeof(double)));
+  CHK(B.result->reserve((size_t)np * kResultDoubles * sizeof(double)));
   size_t tile_part = 0;
   std::vector<float> st((size_t)np * kStateFloats, 0.0f);
   for (int i = 0; i < np; ++i) {
@@ -560,7 +560,7 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
     pd.state = B.state->as<float>() + (size_t)i * kStateFloats;
     pd.partials = B.partials->as<double>() + tile_part * kPartial;
     tile_part += pd.n_tiles;
-    pd.result = B.result->as<double>() + (size_t)i * 2 * kPartial;
+    pd.result = B.result->as<double>() + (size_t)i * kResultDoubles;
     // initial state: AlignerProjective::align, alignerprojective.cpp:36-54
     float *S = &st[(size_t)i * kStateFloats];
     float T[16];
@@ -663,178 +663,16 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
            kSlotTinv, 0);
     LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, 0.0f, 2, 0);
+    LAUNCH(pf, KID_SOLUTION_STATS, k_solution_stats, dim3((np + 63) / 64), dim3(64), s, d_pairs, np,
+           ap->translational_min_eigen_ratio, ap->rotational_min_eigen_ratio);
   }
   HIPCHK(hipGetLastError());
   return NICP_OK;
 }
 
-// ---------------------------------------------------------------------------------------
-// FP64 solution statistics (Aligner::_computeStatistics + unscented.h:11-65)
-// ---------------------------------------------------------------------------------------
-// symmetric eigen-decomposition by cyclic Jacobi rotations (ascending eigenvalues)
-void sym_eigen(int n, const double *Ain, double *w, double *V) {
-  double a[36];
-  for (int i = 0; i < n * n; ++i) a[i] = Ain[i];
-  for (int i = 0; i < n * n; ++i) V[i] = 0.0;
-  for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
-  for (int sweep = 0; sweep < 64; ++sweep) {
-    double off = 0.0, diag = 0.0;
-    for (int c = 0; c < n; ++c)
-      for (int r = 0; r < n; ++r) (r == c ? diag : off) += a[c * n + r] * a[c * n + r];
-    if (off <= 1e-32 * diag || off == 0.0) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = a[q * n + p];
-        if (apq == 0.0) continue;
-        const double th = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
-        const double t = (th >= 0.0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
-        for (int k = 0; k < n; ++k) {
-          const double x = a[p * n + k], y = a[q * n + k];
-          a[p * n + k] = c * x - sn * y;
-          a[q * n + k] = sn * x + c * y;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double x = a[k * n + p], y = a[k * n + q];
-          a[k * n + p] = c * x - sn * y;
-          a[k * n + q] = sn * x + c * y;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double x = V[p * n + k], y = V[q * n + k];
-          V[p * n + k] = c * x - sn * y;
-          V[q * n + k] = sn * x + c * y;
-        }
-      }
-  }
-  for (int i = 0; i < n; ++i) w[i] = a[i * n + i];
-  // selection sort, ascending
-  for (int i = 0; i < n; ++i) {
-    int m = i;
-    for (int j = i + 1; j < n; ++j)
-      if (w[j] < w[m]) m = j;
-    if (m != i) {
-      std::swap(w[i], w[m]);
-      for (int k = 0; k < n; ++k) std::swap(V[i * n + k], V[m * n + k]);
-    }
-  }
-}
-
-bool cholesky6(const double *A, double *L) {
-  for (int i = 0; i < 36; ++i) L[i] = 0.0;
-  for (int j = 0; j < 6; ++j) {
-    double d = A[j * 6 + j];
-    for (int k = 0; k < j; ++k) d -= L[k * 6 + j] * L[k * 6 + j];
-    if (!(d > 0.0)) return false;
-    d = std::sqrt(d);
-    L[j * 6 + j] = d;
-    for (int i = j + 1; i < 6; ++i) {
-      double v = A[j * 6 + i];
-      for (int k = 0; k < j; ++k) v -= L[k * 6 + i] * L[k * 6 + j];
-      L[j * 6 + i] = v / d;
-    }
-  }
-  return true;
-}
-
-// inverse of a symmetric positive definite 6x6 through its eigen-decomposition
-bool spd_inverse6(const double *A, double *Ai) {
-  double w[6], V[36];
-  sym_eigen(6, A, w, V);
-  if (!(w[0] > 0.0)) return false;
-  for (int c = 0; c < 6; ++c)
-    for (int r = 0; r < 6; ++r) {
-      double s = 0.0;
-      for (int k = 0; k < 6; ++k) s += V[k * 6 + r] * V[k * 6 + c] / w[k];
-      Ai[c * 6 + r] = s;
-    }
-  return true;
-}
-
-double block_ratio(const double *O, int off) {
-  double a[9], w[3], V[9];
-  for (int c = 0; c < 3; ++c)
-    for (int r = 0; r < 3; ++r) a[c * 3 + r] = O[(off + c) * 6 + off + r];
-  sym_eigen(3, a, w, V);
-  double s[3] = {std::fabs(w[0]), std::fabs(w[1]), std::fabs(w[2])};
-  std::sort(s, s + 3);
-  return s[2] / s[0];
-}
-
-void solution_statistics(const double *H, const float *Tf, const nicp_align_params *ap, nicp_align_result *res) {
-  // localSigma = pinv(H) with JacobiSVD's default threshold for Matrix6f (6 * FLT_EPSILON * s_max)
-  double w[6], V[36], Sigma[36];
-  sym_eigen(6, H, w, V);
-  double smax = 0.0;
-  for (int i = 0; i < 6; ++i) smax = std::max(smax, std::fabs(w[i]));
-  const double thr = std::max(smax * 6.0 * (double)FLT_EPSILON, 2.2250738585072014e-308);
-  for (int i = 0; i < 36; ++i) Sigma[i] = 0.0;
-  bool zero = true;
-  for (int i = 0; i < 36; ++i) zero = zero && H[i] == 0.0;
-  for (int k = 0; k < 6; ++k) {
-    if (std::fabs(w[k]) < thr) continue;
-    for (int c = 0; c < 6; ++c)
-      for (int r = 0; r < 6; ++r) Sigma[c * 6 + r] += V[k * 6 + r] * V[k * 6 + c] / w[k];
-  }
-  // unscented transform: 2 dim + 1 sigma points (alpha 1e-3, beta 2)
-  const int dim = 6;
-  const double alpha = 1e-3, beta = 2.0, lam = alpha * alpha * dim;
-  double C[36], L[36];
-  for (int i = 0; i < 36; ++i) C[i] = Sigma[i] * (dim + lam);
-  const bool chol = cholesky6(C, L);
-  double smp[13][6], wi[13], wp[13];
-  for (int k = 0; k < 6; ++k) smp[0][k] = 0.0;
-  wi[0] = lam / (dim + lam);
-  wp[0] = lam / (dim + lam) + (1.0 - alpha * alpha + beta);
-  for (int i = 0; i < dim; ++i)
-    for (int sgn = 0; sgn < 2; ++sgn) {
-      const int j = 1 + 2 * i + sgn;
-      for (int k = 0; k < 6; ++k) smp[j][k] = sgn ? -L[i * 6 + k] : L[i * 6 + k];
-      wi[j] = wp[j] = 1.0 / (2.0 * (dim + lam));
-    }
-  double T[16];
-  for (int i = 0; i < 16; ++i) T[i] = Tf[i];
-  for (int j = 0; j < 13; ++j) {
-    double X[16], Y[16];
-    v2t<double>(smp[j], X);
-    // Y = T * X^-1
-    double Ri[9], ti[3];
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) Ri[c * 3 + r] = X[r * 4 + c];
-    for (int r = 0; r < 3; ++r) ti[r] = -(Ri[r] * X[12] + Ri[3 + r] * X[13] + Ri[6 + r] * X[14]);
-    for (int i = 0; i < 16; ++i) Y[i] = 0.0;
-    Y[15] = 1.0;
-    for (int r = 0; r < 3; ++r) {
-      for (int c = 0; c < 3; ++c) Y[c * 4 + r] = T[r] * Ri[c * 3] + T[4 + r] * Ri[c * 3 + 1] + T[8 + r] * Ri[c * 3 + 2];
-      Y[12 + r] = T[r] * ti[0] + T[4 + r] * ti[1] + T[8 + r] * ti[2] + T[12 + r];
-    }
-    t2v<double>(Y, smp[j]);
-  }
-  double mean[6] = {0, 0, 0, 0, 0, 0}, cov[36];
-  for (int j = 0; j < 13; ++j)
-    for (int k = 0; k < 6; ++k) mean[k] += wi[j] * smp[j][k];
-  for (int i = 0; i < 36; ++i) cov[i] = 0.0;
-  for (int j = 0; j < 13; ++j) {
-    double d[6];
-    for (int k = 0; k < 6; ++k) d[k] = smp[j][k] - mean[k];
-    for (int c = 0; c < 6; ++c)
-      for (int r = 0; r < 6; ++r) cov[c * 6 + r] += wp[j] * d[r] * d[c];
-  }
-  double Om[36];
-  const bool inv = spd_inverse6(cov, Om);
-  for (int c = 0; c < 6; ++c)
-    for (int r = 0; r < 6; ++r) res->omega[c * 6 + r] = inv ? 0.5 * (Om[c * 6 + r] + Om[r * 6 + c]) : 0.0;
-  for (int k = 0; k < 6; ++k) res->mean[k] = mean[k];
-  res->translational_ratio = inv ? block_ratio(res->omega, 0) : INFINITY;
-  res->rotational_ratio = inv ? block_ratio(res->omega, 3) : INFINITY;
-  const bool finite = std::isfinite(res->translational_ratio) && std::isfinite(res->rotational_ratio);
-  res->solution_valid = !zero && chol && inv && finite &&
-                        !(res->rotational_ratio > ap->rotational_min_eigen_ratio) &&
-                        !(res->translational_ratio > ap->translational_min_eigen_ratio);
-}
-
-void fill_result(const double *r, const float *T, const nicp_align_params *ap, nicp_align_result *out,
-                 float ms) {
-  // r[0..31]: last in-loop linearisation; r[32..63]: statistics linearisation
+// r: a pair's result record (kResultDoubles): the last in-loop linearisation, the statistics
+// linearisation and the solution statistics k_solution_stats computed on the device
+void fill_result(const double *r, const nicp_align_params *ap, nicp_align_result *out, float ms) {
   double H[36];
   int e = 0;
   for (int i = 0; i < 6; ++i)
@@ -848,16 +686,14 @@ void fill_result(const double *r, const float *T, const nicp_align_params *ap, n
   out->inliers = (int)llround(r[28]);
   out->num_correspondences = (int)llround(r[29]);
   out->overlap_pixels = (int)llround(r[30]);
-  double Hs[36];
-  e = 0;
-  for (int i = 0; i < 6; ++i)
-    for (int j = i; j < 6; ++j) {
-      Hs[j * 6 + i] = Hs[i * 6 + j] = r[kPartial + e];
-      ++e;
-    }
   out->total_time_ms = ms;
   if (ap->outer_iterations > 0) {
-    solution_statistics(Hs, T, ap, out);
+    const double *st = r + 2 * kPartial;
+    memcpy(out->omega, st + nicp_stats::kSolOmega, sizeof(out->omega));
+    memcpy(out->mean, st + nicp_stats::kSolMean, sizeof(out->mean));
+    out->translational_ratio = st[nicp_stats::kSolTrans];
+    out->rotational_ratio = st[nicp_stats::kSolRot];
+    out->solution_valid = st[nicp_stats::kSolValid] != 0.0;
   } else {
     memset(out->mean, 0, sizeof(out->mean));
     memset(out->omega, 0, sizeof(out->omega));
@@ -1468,7 +1304,7 @@ extern "C" int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud
   CHK(launch_align(&ctx->prof, ctx->stream, plan, d_pairs, d_frames, d_bp, d_br, d_bc, ap));
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   float T[16];
-  double r[2 * kPartial];
+  double r[kResultDoubles];
   HIPCHK(hipMemcpyAsync(T, ctx->state.p, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipMemcpyAsync(r, ctx->result.p, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1476,10 +1312,42 @@ extern "C" int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud
   HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   if (ap->outer_iterations == 0) memset(r, 0, sizeof(r));
   if (T_out) memcpy(T_out, T, sizeof(T));
-  if (result) fill_result(r, T, ap, result, ms);
+  if (result) fill_result(r, ap, result, ms);
   ctx->last_rows = rows;
   ctx->last_cols = cols;
   ctx->last_valid = ap->outer_iterations > 0;
+  return NICP_OK;
+}
+
+extern "C" int nicp_solution_statistics(nicp_ctx *ctx, const double H[36], const float T[16],
+                                        const nicp_align_params *ap, nicp_align_result *result) {
+  if (!ctx || !H || !T || !ap || !result) return set_err(NICP_ERR_INVALID, "null argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  CHK(ctx->pairs_tbl.reserve(sizeof(PairDev)));
+  CHK(ctx->state.reserve(kStateFloats * sizeof(float)));
+  CHK(ctx->result.reserve(kResultDoubles * sizeof(double)));
+  PairDev pd{};
+  pd.state = ctx->state.as<float>();
+  pd.result = ctx->result.as<double>();
+  double r[kResultDoubles] = {0};
+  int e = 0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) r[kPartial + e++] = H[j * 6 + i];
+  HIPCHK(hipMemcpyAsync(ctx->pairs_tbl.p, &pd, sizeof(pd), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(pd.state + kSlotT, T, 16 * sizeof(float), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(pd.result, r, sizeof(r), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_solution_stats, dim3(1), dim3(64), 0, s, ctx->pairs_tbl.as<PairDev>(), 1,
+                     ap->translational_min_eigen_ratio, ap->rotational_min_eigen_ratio);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(r, pd.result, sizeof(r), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const double *st = r + 2 * kPartial;
+  memcpy(result->omega, st + nicp_stats::kSolOmega, sizeof(result->omega));
+  memcpy(result->mean, st + nicp_stats::kSolMean, sizeof(result->mean));
+  result->translational_ratio = st[nicp_stats::kSolTrans];
+  result->rotational_ratio = st[nicp_stats::kSolRot];
+  result->solution_valid = st[nicp_stats::kSolValid] != 0.0;
   return NICP_OK;
 }
 
@@ -1621,7 +1489,7 @@ extern "C" int nicp_batch_convert(nicp_batch *b) {
 namespace {
 // AlignerProjective::align for a list of pairs of a batch. with_stats = false skips the
 // statistics linearisation (results then hold the in-loop linearisation only and no
-// solution statistics); raw, when not null, receives the 2 x kPartial result slots per pair;
+// solution statistics); raw, when not null, receives the kResultDoubles result slots per pair;
 // sub > 1 linearises over every sub-th row and column only (the start probe).
 int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *cur_frame, const float *init_guesses16,
                 const nicp_align_params *ap, int use_graph, float *T_out16, nicp_align_result *results,
@@ -1728,9 +1596,13 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
     std::vector<int> key = {num_pairs, plan.total_pix_tiles, plan.total_rtiles, plan.total_ctiles};
     const int *apw = (const int *)ap;
     key.insert(key.end(), apw, apw + sizeof(nicp_align_params) / sizeof(int));
-    key.push_back((int)(((uintptr_t)d_pairs) & 0x7fffffff));
-    key.push_back((int)(((uintptr_t)b->partials.p) & 0x7fffffff));
-    key.push_back((int)(((uintptr_t)b->zbuf.p) & 0x7fffffff));
+    // every device address a captured launch takes as an argument (the tables' contents are
+    // uploaded before each launch, so only the argument addresses are baked into the graph)
+    for (const void *ptr : {(const void *)d_pairs, (const void *)d_bp, (const void *)d_br, (const void *)d_bc,
+                            (const void *)b->d_frames}) {
+      key.push_back((int)(uint32_t)((uintptr_t)ptr));
+      key.push_back((int)(uint32_t)((uint64_t)(uintptr_t)ptr >> 32));
+    }
     key.push_back(ctx->prof.on ? 1 : 0);
     key.push_back(with_stats ? 1 : 0);
     key.push_back(sub);
@@ -1769,7 +1641,7 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
   std::vector<float> st((size_t)num_pairs * kStateFloats);
   std::vector<double> local;
   std::vector<double> &rr = raw ? *raw : local;
-  rr.assign((size_t)num_pairs * 2 * kPartial, 0.0);
+  rr.assign((size_t)num_pairs * kResultDoubles, 0.0);
   HIPCHK(hipMemcpyAsync(st.data(), b->state.p, st.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   if ((results || raw) && ap->outer_iterations > 0)
     HIPCHK(hipMemcpyAsync(rr.data(), b->result.p, rr.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1786,7 +1658,7 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
   for (int i = 0; i < num_pairs; ++i) {
     const float *T = &st[(size_t)i * kStateFloats];
     if (T_out16) memcpy(T_out16 + 16 * (size_t)i, T, 16 * sizeof(float));
-    if (results) fill_result(&rr[(size_t)i * 2 * kPartial], T, &ap_res, &results[i], ms);
+    if (results) fill_result(&rr[(size_t)i * kResultDoubles], &ap_res, &results[i], ms);
   }
   return NICP_OK;
 }
@@ -2140,11 +2012,13 @@ int align_nn_pairs(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const n
   if (ap->outer_iterations > 0) {
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, ntiles, tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
     LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, 0.0f, 2, 0);
+    LAUNCH(pf, KID_SOLUTION_STATS, k_solution_stats, dim3((np + 63) / 64), dim3(64), s, d_pairs, np,
+           ap->translational_min_eigen_ratio, ap->rotational_min_eigen_ratio);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev1, s));
   std::vector<float> st((size_t)np * kStateFloats);
-  std::vector<double> r((size_t)np * 2 * kPartial);
+  std::vector<double> r((size_t)np * kResultDoubles);
   HIPCHK(hipMemcpyAsync(st.data(), ctx->state.p, st.size() * sizeof(float), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(r.data(), ctx->result.p, r.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -2154,7 +2028,7 @@ int align_nn_pairs(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const n
   for (int i = 0; i < np; ++i) {
     const float *T = &st[(size_t)i * kStateFloats + kSlotT];
     if (T_out) memcpy(T_out + (size_t)i * 16, T, 16 * sizeof(float));
-    if (results) fill_result(&r[(size_t)i * 2 * kPartial], T, ap, &results[i], ms);
+    if (results) fill_result(&r[(size_t)i * kResultDoubles], ap, &results[i], ms);
   }
   return NICP_OK;
 }

@@ -12,6 +12,9 @@ constexpr int kVals = 31;            // used partial slots: H (21), b (6), chi2,
 constexpr int kSolveThreads = 256;   // threads per pair of k_solve (kSolveLanes per partial slot)
 constexpr int kSolveLanes = 8;
 constexpr uint64_t kZInit = ~0ull;   // empty z-buffer cell
+// doubles of a pair's result record: the last in-loop linearisation's totals (kPartial), the
+// statistics linearisation's totals (kPartial) and the solution statistics (48)
+constexpr int kResultDoubles = 2 * kPartial + 48;
 
 // per-pixel blocks of a pair: 1024-pixel tiles of its projector image
 __host__ __device__ inline int pair_tiles(int rows, int cols) { return (rows * cols + kTile - 1) / kTile; }
@@ -71,7 +74,7 @@ struct PairDev {
   int2 *pairs;                     // rows*cols stored correspondences (or (-1,-1))
   float *state;                    // kStateFloats: T, Tinv, KRt (reference), KRt (current), invT (inner)
   double *partials;                // n_tiles * kPartial
-  double *result;                  // 2 * kPartial: last linearisation, statistics linearisation
+  double *result;                  // kResultDoubles: last linearisation, statistics linearisation, solution statistics
 };
 
 constexpr int kStateFloats = 80;

@@ -25,6 +25,7 @@
 
 #include "nicp_kernels.h"
 #include "nicp_math.h"
+#include "solstats.h"
 
 using namespace nicp_dev;
 using namespace nicp_math;
@@ -965,11 +966,12 @@ __device__ __forceinline__ void zmerge(const PairDev &pd, int cell, unsigned lon
     const unsigned long long other = ((unsigned long long)hi << 32) | lo;
     m = other < m ? other : m;
   }
+  // strategy 1 reads the cell with a relaxed atomic load (one untorn 64-bit read)
   if (cell >= 0 && !same) {
-    if (strategy == 0 || *gview(pd.zbuf + cell) > key)
+    if (strategy == 0 || __hip_atomic_load(gmut(pd.zbuf + cell), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > key)
       __hip_atomic_fetch_min(gmut(pd.zbuf + cell), key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else if (lane == lead) {
-    if (strategy == 0 || *gview(pd.zbuf + cell) > m)
+    if (strategy == 0 || __hip_atomic_load(gmut(pd.zbuf + cell), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > m)
       __hip_atomic_fetch_min(gmut(pd.zbuf + cell), m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
@@ -1549,6 +1551,28 @@ extern "C" __global__ __launch_bounds__(kSolveThreads) void k_solve(const PairDe
     S[kSlotTinv + k] = Ti[k];
     S[kSlotKRt + k] = KRt[k];
   }
+}
+
+// Aligner::_computeStatistics (aligner.cpp:82-135) of every pair, one thread per pair: the
+// FP64 unscented statistics from the statistics linearisation's totals (result slots
+// kPartial..kPartial + 30) at the final transform, into result slots 2 kPartial ...
+static_assert(kResultDoubles == 2 * kPartial + nicp_stats::kSolStats, "result record layout");
+extern "C" __global__ __launch_bounds__(64) void k_solution_stats(const PairDev *__restrict__ Pp, int np,
+                                                                 float trans_max_ratio, float rot_max_ratio) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= np) return;
+  const PairDev &pd = Pp[i];
+  const double *t = pd.result + kPartial;
+  double H[36];
+  int e = 0;
+  for (int a = 0; a < 6; ++a)
+    for (int b = a; b < 6; ++b) {
+      H[b * 6 + a] = H[a * 6 + b] = t[e];
+      ++e;
+    }
+  float T[16];
+  for (int k = 0; k < 16; ++k) T[k] = pd.state[kSlotT + k];
+  nicp_stats::solution_statistics(H, T, trans_max_ratio, rot_max_ratio, pd.result + 2 * kPartial);
 }
 
 // ---------------------------------------------------------------------------------------

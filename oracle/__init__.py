@@ -297,6 +297,17 @@ def intervals(depth, K9, min_d, max_d, world_radius):
     return out.reshape(rows, cols, 2)
 
 
+def set_num_threads(n):
+    """OpenMP threads of the conversion's loops (the aligner's count is an argument of align)."""
+    lib().orc_set_num_threads(int(n))
+
+
+def max_threads():
+    f = lib().orc_get_max_threads
+    f.restype = C.c_int
+    return int(f())
+
+
 def v2t(x):
     T = np.empty(16, np.float32)
     lib().orc_v2t(_p(np.ascontiguousarray(x, np.float32)), _p(T))

@@ -44,6 +44,8 @@
 #include <limits>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 inline float mc(const float *m, int r, int c) { return m[c * 4 + r]; }
@@ -1261,6 +1263,10 @@ void orc_linearize_f64(const orc_cloud *ref, const orc_cloud *cur, const int *pa
   *error = e;
   *inliers = in;
 }
+
+/* OpenMP threads of the conversion's parallel loops (the aligner takes its count per call) */
+void orc_set_num_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_max_threads(void) { return omp_get_max_threads(); }
 
 void orc_v2t(const float *x, float *T) { v2t_t<float>(x, T); }
 void orc_t2v(const float *T, float *x) { t2v_t<float>(T, x); }

@@ -134,6 +134,9 @@ void orc_linearize(const orc_cloud *ref, const orc_cloud *cur, const int *pairs,
 void orc_linearize_f64(const orc_cloud *ref, const orc_cloud *cur, const int *pairs, int num_pairs,
                        const float T[16], const orc_align_params *ap,
                        double H[36], double b[6], double *error, int *inliers);
+/* OpenMP threads of the conversion (omp_set_num_threads) and the current maximum */
+void orc_set_num_threads(int n);
+int orc_get_max_threads(void);
 /* A19 */
 void orc_v2t(const float x[6], float T[16]);
 void orc_t2v(const float T[16], float x[6]);

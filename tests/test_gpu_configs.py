@@ -1,0 +1,197 @@
+"""The batched and learned configurations at their own sizes, through the C ABI.
+
+- Learned initial guess (640x480, motions up to 30 deg / 0.5 m): the trained network
+  (nicp_amd/data/initnet_640x480.npy) against the identity start on 256 held-out seeded
+  pairs; a sample of the refined transforms against the CPU oracle's aligner run from the
+  same start on the same clouds.
+- Uneven batch: 512 pairs uniform over 160x120, 320x240, 640x480 and 1280x960 (intrinsics
+  scaled with the width), motions up to 8 deg / 10 cm, 15 iterations, autotuned and replayed
+  from a hipGraph; pairs of every resolution against single-pair calls (bit for bit) and
+  against the oracle (1e-5 rad / 1e-5 m).
+- A captured graph replayed after the batch's block maps were reallocated by a larger call.
+
+Reference: AlignerProjective::align, nicp/nicp/alignerprojective.cpp:21-136, and the
+initial guess it starts from, nicp/nicp/aligner.h:123-141.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from nicp_amd import capi, initnet, synth
+from tests.helpers import aparams, cams, identity16, params
+
+pytestmark = pytest.mark.gpu
+
+RES = [(120, 160), (240, 320), (480, 640), (960, 1280)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _errors(T, Tgt):
+    return np.array([synth.pose_error(T[i], Tgt[i]) if np.all(np.isfinite(T[i])) else (np.inf, np.inf)
+                     for i in range(len(T))])
+
+
+def _aligned(e):
+    return (e[:, 0] < 0.1) & (e[:, 1] < 0.002)
+
+
+def _oracle_cloud(cl):
+    pts, nrm, curv, op, on = cl.download()
+    nrm = nrm.copy()
+    nrm[:, 3] = 0.0
+    return O.cloud_from_arrays(pts, nrm, curv, op, on)
+
+
+def _oracle_align_on_device_clouds(b, f_ref, f_cur, shape, init16, ap_oracle, cam):
+    rows, cols = shape
+    cr = b.frame_cloud(f_ref, shape)
+    cc = b.frame_cloud(f_cur, shape)
+    # one thread: the reference's OpenMP partition drops rows and correspondences with more
+    # (correspondencefinderprojective.cpp:41-53, linearizer.cpp:36-43); the device drops none
+    T, res = O.align(_oracle_cloud(cr), _oracle_cloud(cc), rows, cols, cam, init16, ap_oracle, 1, fp64_solve=True)
+    cr.close()
+    cc.close()
+    return T, res
+
+
+def test_learned_gain_640x480(ctx):
+    """What the trained network gains over the identity start on held-out pairs (seeds from 1000;
+    training used seeds from 10,000,000)."""
+    w = initnet.load_weights()
+    assert w is not None, "trained weights missing"
+    n_pairs, B = 256, 128
+    say("case learned_gain 640x480 pairs=%d" % n_pairs)
+    net = capi.InitNet(ctx, seed=1)
+    net.set_params(w)
+    _, ap = aparams()
+    oap, _ = aparams()
+    ocam, _ = cams(480, 640)
+    b = initnet.make_batch(ctx, B)
+    ref = np.arange(0, 2 * B, 2, dtype=np.int32)
+    cur = ref + 1
+    e_net, e_id, used_all = [], [], []
+    sample = None
+    for k in range(n_pairs // B):
+        seeds = np.arange(1000 + k * B, 1000 + (k + 1) * B, dtype=np.uint64)
+        Tgt = initnet.render_pairs(b, seeds, 30.0, 0.5)
+        b.convert()
+        Tn, rn, starts, used = b.align_learned(net, ref, cur, ap, use_graph=True)
+        Ti, _ = b.align(ref, cur, None, ap, use_graph=True)
+        en, ei = _errors(Tn, Tgt), _errors(Ti, Tgt)
+        e_net.append(en)
+        e_id.append(ei)
+        used_all.append(used)
+        if k == 0:
+            # oracle check: four pairs the learned path aligned, from the start it chose
+            idx = np.nonzero(_aligned(en))[0][:4]
+            sample = []
+            for i in idx:
+                To, _ = _oracle_align_on_device_clouds(b, int(ref[i]), int(cur[i]), (480, 640), starts[i], oap, ocam)
+                sample.append(synth.pose_error(Tn[i], To))
+    e_net, e_id, used = np.concatenate(e_net), np.concatenate(e_id), np.concatenate(used_all)
+    a_net, a_id = int(_aligned(e_net).sum()), int(_aligned(e_id).sum())
+    m_net, m_id = float(np.median(e_net[:, 0])), float(np.median(e_id[:, 0]))
+    say("  aligned: learned %d, identity %d of %d; median rotation error learned %.3f deg, identity %.3f deg; "
+        "network start kept for %d pairs (fallback rate %.3f)" % (a_net, a_id, n_pairs, m_net, m_id, int(used.sum()),
+                                                                   1.0 - used.mean()))
+    say("  oracle from the same start (aligned sample): " + ", ".join("%.2e rad %.2e m" % (np.radians(r), t)
+                                                                       for r, t in sample))
+    assert a_net >= a_id
+    assert m_net < m_id
+    assert len(sample) >= 2
+    for r, t in sample:
+        assert np.radians(r) < 1e-5 and t < 1e-5
+    b.close()
+    net.close()
+
+
+def test_uneven_batch_512(ctx):
+    """512 pairs of the four resolutions in one batch, 15 iterations, autotuned, graph replay."""
+    n_pairs = 512
+    say("case uneven_batch pairs=%d" % n_pairs)
+    rng = np.random.default_rng(7)
+    pick = rng.integers(0, len(RES), n_pairs)
+    shapes, cam_list = [], []
+    for k in pick:
+        r, c = RES[k]
+        cam = cams(r, c)[1]
+        shapes += [(r, c), (r, c)]
+        cam_list += [cam, cam]
+    _, dsp = params()
+    oap, dap = aparams(outer_iterations=15)
+    b = capi.Batch(ctx, shapes, cam_list, dsp)
+    seeds = np.arange(2_000_000, 2_000_000 + n_pairs, dtype=np.uint64)
+    Tgt = np.stack([synth.motion(int(s), 8.0, 0.1) for s in seeds])
+    poses = np.empty((2 * n_pairs, 16), np.float32)
+    poses[0::2] = identity16()
+    poses[1::2] = Tgt
+    b.render_synthetic(np.arange(2 * n_pairs, dtype=np.int32), np.repeat(seeds, 2),
+                       np.tile(np.array([10, 11], np.uint64), n_pairs), poses)
+    ref = np.arange(0, 2 * n_pairs, 2, dtype=np.int32)
+    cur = ref + 1
+    b.convert()
+    b.set_autotune(True)
+    Te, re = b.align(ref, cur, None, dap, use_graph=True)   # tuning call (eager)
+    Tg, rg = b.align(ref, cur, None, dap, use_graph=True)   # captured
+    Tg2, _ = b.align(ref, cur, None, dap, use_graph=True)   # replayed
+    np.testing.assert_array_equal(Tg.view(np.uint32), Te.view(np.uint32))
+    np.testing.assert_array_equal(Tg2.view(np.uint32), Te.view(np.uint32))
+    e = _errors(Tg, Tgt)
+    ok = _aligned(e)
+    for k, (r, c) in enumerate(RES):
+        sel = pick == k
+        say("  %dx%d: %d pairs, aligned %.3f, median rotation error %.4f deg" % (
+            c, r, int(sel.sum()), float(ok[sel].mean()), float(np.median(e[sel, 0]))))
+    # per resolution: two pairs against single-pair calls, one against the oracle
+    for k, (r, c) in enumerate(RES):
+        idx = np.nonzero(pick == k)[0]
+        ocam, dcam = cams(r, c)
+        for i in idx[:2]:
+            cr = b.frame_cloud(int(ref[i]), (r, c))
+            cc = b.frame_cloud(int(cur[i]), (r, c))
+            Ts, rs = ctx.align(cr, cc, dcam, r, c, identity16(), dap)
+            np.testing.assert_array_equal(Tg[i].view(np.uint32), Ts.view(np.uint32))
+            assert rg[i].inliers == rs.inliers
+            cr.close()
+            cc.close()
+        # the oracle on the same clouds: an aligned pair of this resolution when there is one
+        al = idx[ok[idx]]
+        i = int(al[0]) if len(al) else int(idx[0])
+        To, ro = _oracle_align_on_device_clouds(b, int(ref[i]), int(cur[i]), (r, c), identity16(), oap, ocam)
+        rot, tr = synth.pose_error(Tg[i], To)
+        say("  %dx%d pair %d vs oracle: %.2e rad %.2e m (aligned %s)" % (c, r, i, np.radians(rot), tr, bool(ok[i])))
+        if ok[i]:
+            assert np.radians(rot) < 1e-5 and tr < 1e-5
+    for k in (1, 2, 3):
+        assert ok[pick == k].sum() >= 1
+    b.close()
+
+
+def test_graph_replay_after_map_growth(ctx):
+    """A graph captured for one plan, then an eager call whose larger reference frame grows (and so
+    reallocates) the batch's block maps, then the first plan again: the replay must use the new
+    maps (it is captured again) and give the eager result."""
+    say("case graph_replay_after_map_growth")
+    _, dsp = params()
+    _, dap = aparams()
+    shapes = [(240, 320), (240, 320), (960, 1280)]
+    cam_list = [cams(r, c)[1] for r, c in shapes]
+    b = capi.Batch(ctx, shapes, cam_list, dsp)
+    dr, dc, _ = synth.pair(41, 240, 320)
+    big = synth.pair(42, 960, 1280)[0]
+    for f, d in enumerate((dr, dc, big)):
+        b.upload(f, d)
+    b.convert()
+    Te, _ = b.align([0], [1], None, dap, use_graph=False)
+    Tg1, _ = b.align([0], [1], None, dap, use_graph=True)
+    b.align([2], [1], None, dap, use_graph=False)           # 1200 reference tiles: maps grow
+    Tg2, _ = b.align([0], [1], None, dap, use_graph=True)
+    np.testing.assert_array_equal(Tg1.view(np.uint32), Te.view(np.uint32))
+    np.testing.assert_array_equal(Tg2.view(np.uint32), Te.view(np.uint32))
+    b.close()

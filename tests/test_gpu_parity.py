@@ -146,7 +146,7 @@ def test_linearize_vs_fp64(ctx):
 @pytest.mark.parametrize("seed", [1, 4])
 def test_align_vs_oracle(ctx, seed):
     """AlignerProjective::align on identical clouds. The device computes H and b in
-    compensated FP32/FP64 and solves in FP64: it must match the oracle loop run with the same
+    FP32 per thread and FP64 beyond, and solves in FP64: it must match the oracle loop run with the same
     design within 1e-5 rad / 1e-5 m, and the reference's FP32 loop within the larger of 1e-5 and
     twice the spread of that FP32 loop itself across OpenMP thread counts (1 vs 8 threads
     group its partial sums differently)."""
@@ -170,9 +170,58 @@ def test_align_vs_oracle(ctx, seed):
     assert np.radians(r32d) < max(1e-5, 2 * np.radians(rs)) and t32d < max(1e-5, 2 * ts)
     assert abs(rd.inliers - r64.inliers) <= max(5, r64.inliers // 5000)
     assert rd.solution_valid == ro.solution_valid
+    # solution statistics (computed on the device in every align call) against the FP64-solve
+    # oracle's: H agrees to ~1e-5 and T to ~1e-7, which bounds how far Omega and the ratios move
+    Od = np.array(rd.omega).reshape(6, 6)
+    Oo = np.array(r64.omega).reshape(6, 6)
+    relO = np.abs(Od - Oo).max() / np.abs(Oo).max()
+    say("  statistics: |dOmega|/|Omega| %.2e ratios %.4g/%.4g vs %.4g/%.4g" % (
+        relO, rd.translational_ratio, rd.rotational_ratio, r64.translational_ratio, r64.rotational_ratio))
+    assert relO < 1e-3
+    assert abs(rd.translational_ratio - r64.translational_ratio) <= 1e-3 * r64.translational_ratio
+    assert abs(rd.rotational_ratio - r64.rotational_ratio) <= 1e-3 * r64.rotational_ratio
+    assert np.abs(np.array(rd.mean) - np.array(r64.mean)).max() < 1e-5
     # the device's last correspondences agree with the oracle's up to gate-boundary flips
     dp = ctx.last_correspondences()
     assert abs(len(dp) - r64.num_correspondences) <= max(5, r64.num_correspondences // 5000)
+
+
+def test_solution_statistics_vs_oracle(ctx):
+    """Aligner::_computeStatistics on the device (nicp_solution_statistics, the kernel every align
+    call runs) against the oracle's FP64 restatement on the same H and T: both are FP64 with
+    different factorisations (Jacobi eigen-decompositions here, SVD/LLT/inverse there)."""
+    say("case solution_statistics_vs_oracle")
+    rng = np.random.default_rng(3)
+    oap, dap = aparams()
+    cases = []
+    for k in range(12):
+        U, _ = np.linalg.qr(rng.normal(size=(6, 6)))
+        H = U @ np.diag(10.0 ** rng.uniform(4, 4.0 + k / 4.0, 6)) @ U.T
+        q = rng.normal(size=3)
+        q *= np.sin(np.radians(rng.uniform(0, 30)) / 2) / np.linalg.norm(q)
+        T = O.v2t(np.concatenate([rng.uniform(-0.5, 0.5, 3), q]).astype(np.float32))
+        cases.append((H, T))
+    # degenerate: zero information, and a single plane (large eigen ratio)
+    cases.append((np.zeros((6, 6)), identity16()))
+    Hp = np.diag([1e-3, 1e-3, 1e6, 1e5, 1e5, 1e-3])
+    cases.append((Hp, identity16()))
+    n_valid = n_cmp = 0
+    for H, T in cases:
+        ro = O.compute_statistics(H, T, oap)
+        rd = ctx.solution_statistics(H, T, dap)
+        assert rd.solution_valid == ro.solution_valid
+        n_valid += ro.solution_valid
+        if not np.isfinite(ro.translational_ratio):
+            continue
+        n_cmp += 1
+        Oo = np.array(ro.omega).reshape(6, 6)
+        Od = np.array(rd.omega).reshape(6, 6)
+        assert np.abs(Od - Oo).max() <= 1e-8 * np.abs(Oo).max()
+        assert np.abs(np.array(rd.mean) - np.array(ro.mean)).max() < 1e-12
+        assert abs(rd.translational_ratio - ro.translational_ratio) <= 1e-8 * ro.translational_ratio
+        assert abs(rd.rotational_ratio - ro.rotational_ratio) <= 1e-8 * ro.rotational_ratio
+    say("  %d cases, %d valid, %d compared" % (len(cases), n_valid, n_cmp))
+    assert n_valid >= 3 and n_cmp >= 12
 
 
 def test_pipeline_accuracy(ctx):

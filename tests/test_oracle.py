@@ -384,6 +384,92 @@ def test_align_recovers_motion_and_statistics():
     assert rp.solution_valid == 0 and max(rp.translational_ratio, rp.rotational_ratio) > 50
 
 
+# A20 against the mathematics: for a well-conditioned H (small Sigma = H^-1) the unscented
+# transform through f(s) = t2v(T v2t(s)^-1) (N/aligner.cpp:111-120) is exact to first order, so
+# Omega must equal J^-T H J^-1 with J the Jacobian of f at s = 0, and the mean f(0) = t2v(T).
+# f is written here with scipy's rotations (v2t: quaternion (x, y, z, w >= 0) -> matrix), not with
+# the oracle's own v2t/t2v.
+def _f64_v2t(x):
+    from scipy.spatial.transform import Rotation
+    x = np.atleast_2d(np.asarray(x, np.float64))
+    q = x[:, 3:6]
+    w = np.sqrt(1.0 - (q * q).sum(1))
+    T = np.tile(np.eye(4), (len(x), 1, 1))
+    T[:, :3, :3] = Rotation.from_quat(np.column_stack([q, w])).as_matrix()
+    T[:, :3, 3] = x[:, :3]
+    return T
+
+
+def _f64_t2v(T):
+    from scipy.spatial.transform import Rotation
+    q = Rotation.from_matrix(T[:, :3, :3]).as_quat()
+    q = np.where(q[:, 3:4] < 0, -q, q)
+    return np.column_stack([T[:, :3, 3], q[:, :3]])
+
+
+def _stat_map(T44, s):
+    """f(s) = t2v(T v2t(s)^-1) for rows s."""
+    return _f64_t2v(T44[None] @ np.linalg.inv(_f64_v2t(s)))
+
+
+def _stat_cases():
+    rng = np.random.default_rng(11)
+    cases = []
+    for deg in (0.0, 20.0):
+        ax = rng.normal(size=3)
+        ax /= np.linalg.norm(ax)
+        q = np.sin(np.radians(deg) / 2) * ax
+        x = np.concatenate([rng.uniform(-0.3, 0.3, 3), q])
+        A = rng.normal(size=(6, 6))
+        U, _ = np.linalg.qr(A)
+        H = U @ np.diag(10.0 ** rng.uniform(6, 8, 6)) @ U.T
+        cases.append((deg, H, _f64_v2t(x)[0]))
+    return cases
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_statistics_against_linearised_covariance(k):
+    deg, H, T44 = _stat_cases()[k]
+    T16 = T44.astype(np.float32).flatten(order="F")
+    T44 = T16.reshape(4, 4, order="F").astype(np.float64)   # the transform the oracle sees
+    res = O.compute_statistics(H, T16, O.align_params())
+    Om = np.array(res.omega).reshape(6, 6, order="F")
+    h = 1e-6
+    J = np.column_stack([(_stat_map(T44, h * e[None])[0] - _stat_map(T44, -h * e[None])[0]) / (2 * h)
+                         for e in np.eye(6)])
+    Ji = np.linalg.inv(J)
+    expect = Ji.T @ H @ Ji
+    rel = np.abs(Om - expect).max() / np.abs(expect).max()
+    assert rel < 1e-4, (deg, rel)
+    # the mean is f(0) = t2v(T) up to O(Sigma)
+    assert np.abs(np.array(res.mean) - _stat_map(T44, np.zeros((1, 6)))[0]).max() < 1e-6
+    # ratios: largest over smallest singular value of each 3x3 block of Omega
+    for blk, ratio in ((slice(0, 3), res.translational_ratio), (slice(3, 6), res.rotational_ratio)):
+        sv = np.abs(np.linalg.eigvalsh(expect[blk, blk]))
+        assert abs(ratio - sv.max() / sv.min()) <= 1e-3 * sv.max() / sv.min()
+    # sigma points from the rows of the Cholesky factor (a transposed factor) would carry the
+    # covariance L^T L instead of L L^T = Sigma: for this H that moves Omega far beyond the bar
+    L = np.linalg.cholesky(np.linalg.inv(H))
+    assert np.abs(L.T @ L - L @ L.T).max() > 1e-2 * np.abs(L @ L.T).max()
+
+
+def test_statistics_monte_carlo():
+    """Omega against the inverse of the empirical covariance of f(s), s ~ N(0, H^-1)."""
+    deg, H, T44 = _stat_cases()[1]
+    T16 = T44.astype(np.float32).flatten(order="F")
+    T44 = T16.reshape(4, 4, order="F").astype(np.float64)
+    res = O.compute_statistics(H, T16, O.align_params())
+    Om = np.array(res.omega).reshape(6, 6, order="F")
+    rng = np.random.default_rng(5)
+    s = rng.multivariate_normal(np.zeros(6), np.linalg.inv(H), size=400_000)
+    y = _stat_map(T44, s)
+    Cy = np.cov(y.T)
+    # whitened: Cy Omega must be the identity up to sampling error (~ sqrt(2/n) = 0.002 per entry)
+    M = Cy @ Om
+    assert np.abs(M - np.eye(6)).max() < 0.03
+    assert np.abs(np.array(res.mean) - y.mean(0)).max() < 1e-5
+
+
 # ---- DepthImage_scale (N/imageutils.cpp:6-37) --------------------------------------------
 def test_depth_scale_closed_form():
     dr, _, _ = _frame(4, 120, 160)
