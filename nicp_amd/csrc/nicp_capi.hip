@@ -94,7 +94,6 @@ struct Carver {
 
 }  // namespace
 
-
 // ---------------------------------------------------------------------------------------
 // per-kernel device timing with HIP events (recorded around each launch on the
 // launching stream; inside a captured graph the records become event nodes)
@@ -663,7 +662,7 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
            kSlotTinv, 0);
     LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, 0.0f, 2, 0);
-    LAUNCH(pf, KID_SOLUTION_STATS, k_solution_stats, dim3((np + 63) / 64), dim3(64), s, d_pairs, np,
+    LAUNCH(pf, KID_SOLUTION_STATS, k_solution_stats, dim3(np), dim3(64), s, d_pairs,
            ap->translational_min_eigen_ratio, ap->rotational_min_eigen_ratio);
   }
   HIPCHK(hipGetLastError());
@@ -1267,24 +1266,28 @@ extern "C" int nicp_linearize(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_c
   return NICP_OK;
 }
 
-extern "C" int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const nicp_camera *cam,
-                          int rows, int cols, const float init_guess[16], const nicp_align_params *ap, float T_out[16],
-                          nicp_align_result *result) {
+namespace {
+// AlignerProjective::align of one pair with the given workspace (the context's growable
+// buffers, or fixed views of the no-allocation workspace)
+int align_single(nicp_ctx *ctx, DevBuf &frames_tbl, AlignBuffers B, const nicp_cloud *ref, const nicp_cloud *cur,
+                 const nicp_camera *cam, int rows, int cols, const float init_guess[16], const nicp_align_params *ap,
+                 float T_out[16], nicp_align_result *result) {
   if (!ctx || !ref || !cur) return set_err(NICP_ERR_INVALID, "null argument");
   if (rows <= 0 || cols <= 0) return set_err(NICP_ERR_INVALID, "bad image size");
   CHK(check_camera(cam));
   CHK(check_align_params(ap));
   HIPCHK(hipSetDevice(ctx->device));
-  std::vector<FrameDev> frames = {ref->fd, cur->fd};
+  FrameDev frames[2] = {ref->fd, cur->fd};
   // projections cover the clouds' capacities and use the projector image rows x cols
   frames[0].rows = 1; frames[0].cols = std::max(1, ref->n);
   frames[1].rows = 1; frames[1].cols = std::max(1, cur->n);
-  FrameDev *d_frames;
-  CHK(upload_frames(ctx, frames, &d_frames));
+  CHK(frames_tbl.reserve(sizeof(frames)));
+  FrameDev *d_frames = frames_tbl.as<FrameDev>();
+  HIPCHK(hipMemcpyAsync(d_frames, frames, sizeof(frames), hipMemcpyHostToDevice, ctx->stream));
   const nicp_camera *cams[1] = {cam};
   int r0 = 0, c1 = 1;
   PairPlan plan;
-  make_plan(frames, 1, &r0, &c1, cams, plan);
+  make_plan(std::vector<FrameDev>(frames, frames + 2), 1, &r0, &c1, cams, plan);
   PairDev &pd = plan.pairs[0];
   pd.rows = rows;
   pd.cols = cols;
@@ -1293,7 +1296,6 @@ extern "C" int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud
   plan.blk_pair.assign(pd.n_tiles, 0);
   plan.total_pix_tiles = pd.n_tiles;
   plan.total_pix = align_up((size_t)rows * cols, 64);
-  AlignBuffers B = ctx_buffers(ctx);
   int *d_bp, *d_br, *d_bc;
   PairDev *d_pairs;
   float I4[16];
@@ -1305,14 +1307,24 @@ extern "C" int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   float T[16];
   double r[kResultDoubles];
-  HIPCHK(hipMemcpyAsync(T, ctx->state.p, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipMemcpyAsync(r, ctx->result.p, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(T, B.state->p, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(r, B.result->p, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   if (ap->outer_iterations == 0) memset(r, 0, sizeof(r));
   if (T_out) memcpy(T_out, T, sizeof(T));
   if (result) fill_result(r, ap, result, ms);
+  return NICP_OK;
+}
+}  // namespace
+
+extern "C" int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const nicp_camera *cam,
+                          int rows, int cols, const float init_guess[16], const nicp_align_params *ap, float T_out[16],
+                          nicp_align_result *result) {
+  if (!ctx) return set_err(NICP_ERR_INVALID, "null argument");
+  ctx->last_valid = false;
+  CHK(align_single(ctx, ctx->frames_tbl, ctx_buffers(ctx), ref, cur, cam, rows, cols, init_guess, ap, T_out, result));
   ctx->last_rows = rows;
   ctx->last_cols = cols;
   ctx->last_valid = ap->outer_iterations > 0;
@@ -1337,7 +1349,7 @@ extern "C" int nicp_solution_statistics(nicp_ctx *ctx, const double H[36], const
   HIPCHK(hipMemcpyAsync(ctx->pairs_tbl.p, &pd, sizeof(pd), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(pd.state + kSlotT, T, 16 * sizeof(float), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(pd.result, r, sizeof(r), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_solution_stats, dim3(1), dim3(64), 0, s, ctx->pairs_tbl.as<PairDev>(), 1,
+  hipLaunchKernelGGL(k_solution_stats, dim3(1), dim3(64), 0, s, ctx->pairs_tbl.as<PairDev>(),
                      ap->translational_min_eigen_ratio, ap->rotational_min_eigen_ratio);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(r, pd.result, sizeof(r), hipMemcpyDeviceToHost, s));
@@ -2012,7 +2024,7 @@ int align_nn_pairs(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const n
   if (ap->outer_iterations > 0) {
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, ntiles, tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
     LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, 0.0f, 2, 0);
-    LAUNCH(pf, KID_SOLUTION_STATS, k_solution_stats, dim3((np + 63) / 64), dim3(64), s, d_pairs, np,
+    LAUNCH(pf, KID_SOLUTION_STATS, k_solution_stats, dim3(np), dim3(64), s, d_pairs,
            ap->translational_min_eigen_ratio, ap->rotational_min_eigen_ratio);
   }
   HIPCHK(hipGetLastError());

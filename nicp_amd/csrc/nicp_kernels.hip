@@ -1553,26 +1553,25 @@ extern "C" __global__ __launch_bounds__(kSolveThreads) void k_solve(const PairDe
   }
 }
 
-// Aligner::_computeStatistics (aligner.cpp:82-135) of every pair, one thread per pair: the
+// Aligner::_computeStatistics (aligner.cpp:82-135) of every pair, one wavefront per pair: the
 // FP64 unscented statistics from the statistics linearisation's totals (result slots
-// kPartial..kPartial + 30) at the final transform, into result slots 2 kPartial ...
+// kPartial..kPartial + 20, H's row-major upper triangle) at the final transform, into result
+// slots 2 kPartial ...
 static_assert(kResultDoubles == 2 * kPartial + nicp_stats::kSolStats, "result record layout");
-extern "C" __global__ __launch_bounds__(64) void k_solution_stats(const PairDev *__restrict__ Pp, int np,
+extern "C" __global__ __launch_bounds__(64) void k_solution_stats(const PairDev *__restrict__ Pp,
                                                                  float trans_max_ratio, float rot_max_ratio) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= np) return;
-  const PairDev &pd = Pp[i];
-  const double *t = pd.result + kPartial;
-  double H[36];
-  int e = 0;
-  for (int a = 0; a < 6; ++a)
-    for (int b = a; b < 6; ++b) {
-      H[b * 6 + a] = H[a * 6 + b] = t[e];
-      ++e;
-    }
+  const PairDev &pd = Pp[blockIdx.x];
+  __shared__ double H[36];
+  const int lane = threadIdx.x;
+  if (lane < 36) {
+    const int r = lane % 6, c = lane / 6, i = min(r, c), j = max(r, c);
+    H[lane] = pd.result[kPartial + 6 * i - i * (i - 1) / 2 + (j - i)];
+  }
+  __syncthreads();
   float T[16];
+#pragma unroll
   for (int k = 0; k < 16; ++k) T[k] = pd.state[kSlotT + k];
-  nicp_stats::solution_statistics(H, T, trans_max_ratio, rot_max_ratio, pd.result + 2 * kPartial);
+  nicp_stats::solution_statistics_wave(H, T, trans_max_ratio, rot_max_ratio, pd.result + 2 * kPartial);
 }
 
 // ---------------------------------------------------------------------------------------
