@@ -229,6 +229,18 @@ int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, cons
  * linearisation (FP64, pseudo-inverse of H, unscented sigma points through t2v(T v2t(s)^-1)). */
 int nicp_solution_statistics(nicp_ctx *ctx, const double H[36], const float T[16], const nicp_align_params *ap,
                              nicp_align_result *result);
+/* No counterpart in the reference: nicp_align for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It
+ * calls no hipMalloc/hipFree: its z-buffer, index image, correspondences, partial sums, loop
+ * state and tables live in one fixed workspace, a 7 MiB array in device memory that each GPU
+ * the library runs on holds until the process ends. Limits: rows * cols <= 307200 (640x480)
+ * and at most 307200 points in each cloud; larger inputs return NICP_ERR_INVALID. All
+ * contexts on one GPU share that workspace, so calls that overlap on one GPU (other threads or
+ * streams) race; each call rewrites what it reads. Results equal nicp_align's bit for bit;
+ * nicp_last_correspondences has nothing to return after it. */
+int nicp_align_noalloc(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const nicp_camera *cam, int rows,
+                       int cols, const float init_guess[16], const nicp_align_params *ap, float T_out[16],
+                       nicp_align_result *result);
 /* CorrespondenceFinder::correspondences() of the last nicp_align on ctx (capacity in pairs) */
 int nicp_last_correspondences(nicp_ctx *ctx, int *pairs, int capacity, int *num_pairs);
 

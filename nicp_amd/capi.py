@@ -79,6 +79,7 @@ EXPORTS = [
     "nicp_initnet_train_step_rows", "nicp_initnet_set_input_mode", "nicp_cloud_gaussians", "nicp_cloud_set_gaussians",
     "nicp_cloud_download_gaussians", "nicp_cloud_transform", "nicp_cloud_add", "nicp_merge",
     "nicp_batch_set_autotune", "nicp_batch_autotune_read", "nicp_align_nn_batch", "nicp_solution_statistics",
+    "nicp_align_noalloc",
 ]
 
 _lib = None
@@ -112,6 +113,7 @@ def lib():
             "nicp_align": [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp],
             "nicp_last_correspondences": [vp, vp, C.c_int, vp],
             "nicp_solution_statistics": [vp, vp, vp, vp, vp],
+            "nicp_align_noalloc": [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp],
             "nicp_batch_create": [vp, C.c_int, vp, vp, vp, vp, vp],
             "nicp_batch_destroy": [vp], "nicp_batch_upload_depth": [vp, C.c_int, vp, C.c_float],
             "nicp_batch_convert": [vp],
@@ -340,6 +342,15 @@ class Context:
         res = AlignResult()
         _check(lib().nicp_align(self.h, ref.h, cur.h, C.byref(cam), rows, cols,
                                 _p(np.ascontiguousarray(init16, np.float32)), C.byref(ap), _p(T), C.byref(res)))
+        return T, res
+
+    def align_noalloc(self, ref, cur, cam, rows, cols, init16, ap):
+        """nicp_align in the library's fixed device workspace (no device allocation)."""
+        T = np.empty(16, np.float32)
+        res = AlignResult()
+        _check(lib().nicp_align_noalloc(self.h, ref.h, cur.h, C.byref(cam), rows, cols,
+                                        _p(np.ascontiguousarray(init16, np.float32)), C.byref(ap), _p(T),
+                                        C.byref(res)))
         return T, res
 
     def nn_correspondences(self, ref, cur, T16, ap, normal_scaling=1.0, cell_size=0.0):

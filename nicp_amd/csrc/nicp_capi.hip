@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -54,12 +55,14 @@ namespace {
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-// a growable device buffer
+// a growable device buffer, or (fixed) a view of preallocated memory that never grows
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  bool fixed = false;
   int reserve(size_t n) {
     if (n <= bytes) return NICP_OK;
+    if (fixed) return set_err(NICP_ERR_INVALID, "%zu bytes needed, %zu available in the fixed workspace", n, bytes);
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
@@ -73,7 +76,7 @@ struct DevBuf {
     return NICP_OK;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !fixed) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
   }
@@ -93,6 +96,38 @@ struct Carver {
 };
 
 }  // namespace
+
+// ---------------------------------------------------------------------------------------
+// the fixed workspace of nicp_align_noalloc: one statically sized array in device memory,
+// which every GPU the library runs on holds until the process ends. Host code never passes
+// the array's name to a kernel or a copy (on the host it is only a stand-in): it uses the
+// device address hipGetSymbolAddress returns for the current device, kept per device.
+// ---------------------------------------------------------------------------------------
+constexpr int kNoAllocMaxPixels = 640 * 480;            // projector pixels and points per cloud
+constexpr size_t kNoAllocScratchBytes = 7u << 20;        // 7 MiB
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// the device address of `scratch` on `device` (which must be current), looked up once per device
+int scratch_address(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices) return set_err(NICP_ERR_INVALID, "device %d beyond the workspace table", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_err(NICP_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return NICP_OK;
+}
+}  // namespace
+
 
 // ---------------------------------------------------------------------------------------
 // per-kernel device timing with HIP events (recorded around each launch on the
@@ -213,6 +248,7 @@ struct Prof {
 // ---------------------------------------------------------------------------------------
 struct nicp_ctx {
   int device = 0;
+  void *noalloc_ws = nullptr;  // device address of the fixed workspace on this device
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // workspace of single-pair calls
@@ -715,6 +751,13 @@ extern "C" int nicp_ctx_create(int device, nicp_ctx **out) {
   nicp_ctx *c = new (std::nothrow) nicp_ctx();
   if (!c) return set_err(NICP_ERR_OOM, "out of host memory");
   c->device = device;
+  {
+    const int rc = scratch_address(device, &c->noalloc_ws);
+    if (rc != NICP_OK) {
+      delete c;
+      return rc;
+    }
+  }
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
@@ -1329,6 +1372,50 @@ extern "C" int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud
   ctx->last_cols = cols;
   ctx->last_valid = ap->outer_iterations > 0;
   return NICP_OK;
+}
+
+extern "C" int nicp_align_noalloc(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const nicp_camera *cam,
+                                  int rows, int cols, const float init_guess[16], const nicp_align_params *ap,
+                                  float T_out[16], nicp_align_result *result) {
+  if (!ctx || !ref || !cur) return set_err(NICP_ERR_INVALID, "null argument");
+  if ((long long)rows * cols > kNoAllocMaxPixels || ref->n > kNoAllocMaxPixels || cur->n > kNoAllocMaxPixels)
+    return set_err(NICP_ERR_INVALID,
+                   "nicp_align_noalloc handles at most %d projector pixels and %d points per cloud "
+                   "(got %dx%d, %d and %d points)",
+                   kNoAllocMaxPixels, kNoAllocMaxPixels, rows, cols, ref->n, cur->n);
+  if (!ctx->noalloc_ws) return set_err(NICP_ERR_STATE, "the context has no fixed workspace");
+  // views of the fixed workspace, sized for the largest pair the entry point accepts; every
+  // region is rewritten or re-initialised by each call's launches (k_zfill clears the z-buffer)
+  const size_t P = align_up((size_t)kNoAllocMaxPixels, 64);
+  const size_t tiles = (size_t)(kNoAllocMaxPixels + kTile - 1) / kTile;
+  Carver cv;
+  const size_t o_frames = cv.take<FrameDev>(2), o_pairs = cv.take<PairDev>(1), o_maps = cv.take<int>(3 * tiles),
+               o_z = cv.take<unsigned long long>(P), o_ci = cv.take<int>(P), o_pp = cv.take<int2>(P),
+               o_state = cv.take<float>(kStateFloats), o_part = cv.take<double>(tiles * kPartial),
+               o_res = cv.take<double>(kResultDoubles);
+  if (cv.off > kNoAllocScratchBytes) return set_err(NICP_ERR_STATE, "fixed workspace layout exceeds its size");
+  char *base = (char *)ctx->noalloc_ws;
+  DevBuf frames_tbl, pairs_tbl, maps, zbuf, cur_index, pairs_buf, state, partials, res;
+  const struct {
+    DevBuf *b;
+    size_t off, bytes;
+  } views[] = {{&frames_tbl, o_frames, 2 * sizeof(FrameDev)},
+               {&pairs_tbl, o_pairs, sizeof(PairDev)},
+               {&maps, o_maps, 3 * tiles * sizeof(int)},
+               {&zbuf, o_z, P * sizeof(unsigned long long)},
+               {&cur_index, o_ci, P * sizeof(int)},
+               {&pairs_buf, o_pp, P * sizeof(int2)},
+               {&state, o_state, kStateFloats * sizeof(float)},
+               {&partials, o_part, tiles * kPartial * sizeof(double)},
+               {&res, o_res, kResultDoubles * sizeof(double)}};
+  for (const auto &v : views) {
+    v.b->p = base + v.off;
+    v.b->bytes = v.bytes;
+    v.b->fixed = true;
+  }
+  AlignBuffers B{&pairs_tbl, &maps, &zbuf, &cur_index, &pairs_buf, &state, &partials, &res};
+  ctx->last_valid = false;  // the correspondences of this call stay in the fixed workspace
+  return align_single(ctx, frames_tbl, B, ref, cur, cam, rows, cols, init_guess, ap, T_out, result);
 }
 
 extern "C" int nicp_solution_statistics(nicp_ctx *ctx, const double H[36], const float T[16],
