@@ -69,7 +69,13 @@ def kernel_bytes(name, pairs_info, frames_info):
         # z-buffer key read + reset (16 B) and current index (4 B) per pixel; per overlap pixel the
         # reference point + normal/curvature (28 B), the current point + normal/curvature (28 B)
         # and both information matrices (48 B)
-        return sum(20 * p + 104 * c2 for p, _, c2 in pairs_info)
+        return sum(20 * p + 104 * c2 for p, _, c2, _ in pairs_info)
+    if name == "k_proj_corr_lin":
+        # the projection's inputs: the reference index image (4 B per pixel) and points (16 B per
+        # point); the current index (4 B per pixel); per overlap pixel the reference point +
+        # normal/curvature (28 B), the current point + normal/curvature (28 B) and both
+        # information matrices (48 B); the z-buffer stays in LDS
+        return sum(4 * pr + 16 * n + 4 * p + 104 * c2 for p, n, c2, pr in pairs_info)
     if name == "k_stats":
         # depth (2 B) + index (4 B) + one pass over the 40 B integral image per pixel;
         # normal + curvature (16 B) and two 6-float information matrices (48 B) per point
@@ -87,7 +93,7 @@ def kernel_bytes(name, pairs_info, frames_info):
         return 2 * P
     if name == "k_zproject(reference)":
         # reference points (16 B) and one 8 B atomic per point
-        return sum(24 * n for _, n, _ in pairs_info)
+        return sum(24 * n for _, n, _, _ in pairs_info)
     return None
 
 
@@ -259,7 +265,7 @@ def uneven_batch(ctx, n_pairs, iters, steps):
     tuned = batch.autotune_state()
     batch.close()
     return {"pairs": n_pairs, "iterations": iters, "ms_per_step": round(1000 * dt, 3),
-            "zproject_strategy": [c for c, _, _ in tuned],
+            "projection_strategy": [t[0] for t in tuned],
             "pairs_per_s": round(n_pairs / dt, 1), "aligned_fraction": round(float(ok.mean()), 4),
             "depth_pixels_per_step": px,
             "pairs_per_resolution": {"%dx%d" % (c, r): int((pick == k).sum()) for k, (r, c) in enumerate(res)},
@@ -475,7 +481,8 @@ def main():
         c = batch.frame_cloud(f, (ROWS, COLS))
         frames_info.append((ROWS * COLS, c.n))
         c.close()
-    pairs_info = [(ROWS * COLS, frames_info[2 * i][1], res[i].overlap_pixels) for i in range(B)]
+    # per pair: projector pixels, reference points, both-valid pixels, reference frame pixels
+    pairs_info = [(ROWS * COLS, frames_info[2 * i][1], res[i].overlap_pixels, ROWS * COLS) for i in range(B)]
     kernels = {}
     total_ms = sum(v[0] for v in prof.values()) if prof else 0.0
     for name, (ms, cnt) in prof.items():
@@ -587,8 +594,8 @@ def main():
         "roofline": roofline,
         "net_roofline": net_roofline,
         "kernels": kernels,
-        "autotune": [{"zproject_strategy": c, "ms_strategy0": round(m0, 4), "ms_strategy1": round(m1, 4)}
-                     for c, m0, m1 in tuned_main],
+        "autotune": [{"projection_strategy": c, "ms_strategy0": round(m0, 4), "ms_strategy1": round(m1, 4),
+                      "ms_strategy2": round(m2, 4)} for c, m0, m1, m2 in tuned_main],
         "kernel_timing_error": prof_error,
         "parity_sample": parity,
         "cpu_baseline": cpu,

@@ -233,8 +233,9 @@ int nicp_solution_statistics(nicp_ctx *ctx, const double H[36], const float T[16
  * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It
  * calls no hipMalloc/hipFree: its z-buffer, index image, correspondences, partial sums, loop
  * state and tables live in one fixed workspace, a 7 MiB array in device memory that each GPU
- * the library runs on holds until the process ends. Limits: rows * cols <= 307200 (640x480)
- * and at most 307200 points in each cloud; larger inputs return NICP_ERR_INVALID. All
+ * the library runs on holds until the process ends. Limits: rows * cols <= 307200 (640x480),
+ * rows <= 960, cols <= 1280 and at most 307200 points in each cloud; larger inputs return
+ * NICP_ERR_INVALID. All
  * contexts on one GPU share that workspace, so calls that overlap on one GPU (other threads or
  * streams) race; each call rewrites what it reads. Results equal nicp_align's bit for bit;
  * nicp_last_correspondences has nothing to return after it. */
@@ -284,15 +285,30 @@ int nicp_batch_align(nicp_batch *batch, int num_pairs, const int *ref_frame, con
                      nicp_align_result *results);
 /* number of points of a batch frame, and a cloud view of it (downloads only) */
 int nicp_batch_frame_cloud(nicp_batch *batch, int frame, nicp_cloud **out);
-/* Autotuner (no reference counterpart): with enable, the first nicp_batch_align call of each plan
- * shape (pairs, image tiles, cloud tiles, subsampling, statistics) runs eagerly and projects the
- * reference clouds with two strategies on alternate iterations -- every wave-merged atomicMin
- * issued (0), or the z-buffer cell read first and the atomic skipped when it cannot lower the
- * cell (1); both give the same z-buffer -- and later calls of the shape use the faster. Enabling
- * or disabling forgets earlier choices. */
+/* Projection strategies of a batch iteration (no reference counterpart; they replace the serial
+ * z-buffer loop of PinholePointProjector::project, nicp/nicp/pinholepointprojector.cpp:33-66, and
+ * give the same z-buffer, correspondences and sums): 0 wave-merged 64-bit atomicMin into a global
+ * z-buffer, then the correspondence kernel; 1 the same, reading each cell first and skipping atomics
+ * that cannot lower it; 2 tile-binned: the reference frame's 16x32 pixel tiles are listed for the
+ * 64x64 projector tiles their frustums may reach, and one workgroup per projector tile z-buffers its
+ * listed points in LDS and runs the correspondences and the linearisation on them (pinhole batches
+ * without window search; elsewhere 0 is used instead). */
+/* Autotuner: with enable, the first nicp_batch_align call of each plan shape (pairs, image tiles,
+ * cloud tiles, subsampling, statistics, tile-binned applicable) runs eagerly with the applicable
+ * strategies on alternate iterations, and later calls of the shape use the fastest. Enabling or
+ * disabling forgets earlier choices. Without the autotuner a batch uses strategy 2 where it
+ * applies and the call has at least 1024 projector tiles of 64x64 pixels (13 pairs of 640x480),
+ * else 0. */
 int nicp_batch_set_autotune(nicp_batch *batch, int enable);
-/* the tuned shapes: choice per shape, ms = 2 per shape (average ms per launch of strategies 0 and 1,
- * -1 where unmeasured) */
+/* forces strategy 0, 1 or 2 on every later align call of the batch (2 where it applies, else 0);
+ * -1 returns to the autotuner's or the default choice */
+int nicp_batch_set_projection(nicp_batch *batch, int strategy);
+/* list capacity of the tile-binned projection, 0..128 source tiles per projector tile (default
+ * 128); a projector tile reached by more scans every source tile of its reference frame (same
+ * results, slower). */
+int nicp_batch_set_bin_capacity(nicp_batch *batch, int capacity);
+/* the tuned shapes: choice per shape, ms = 3 per shape (average ms per iteration of the projection
+ * and correspondence launches of strategies 0, 1, 2; -1 where unmeasured) */
 int nicp_batch_autotune_read(nicp_batch *batch, int capacity, int *num_shapes, int *choice, float *ms);
 
 /* ---- Synthetic scenes (no reference counterpart: the training-set and benchmark input

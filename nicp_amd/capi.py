@@ -79,7 +79,7 @@ EXPORTS = [
     "nicp_initnet_train_step_rows", "nicp_initnet_set_input_mode", "nicp_cloud_gaussians", "nicp_cloud_set_gaussians",
     "nicp_cloud_download_gaussians", "nicp_cloud_transform", "nicp_cloud_add", "nicp_merge",
     "nicp_batch_set_autotune", "nicp_batch_autotune_read", "nicp_align_nn_batch", "nicp_solution_statistics",
-    "nicp_align_noalloc",
+    "nicp_align_noalloc", "nicp_batch_set_projection", "nicp_batch_set_bin_capacity",
 ]
 
 _lib = None
@@ -114,6 +114,7 @@ def lib():
             "nicp_last_correspondences": [vp, vp, C.c_int, vp],
             "nicp_solution_statistics": [vp, vp, vp, vp, vp],
             "nicp_align_noalloc": [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp],
+            "nicp_batch_set_projection": [vp, C.c_int], "nicp_batch_set_bin_capacity": [vp, C.c_int],
             "nicp_batch_create": [vp, C.c_int, vp, vp, vp, vp, vp],
             "nicp_batch_destroy": [vp], "nicp_batch_upload_depth": [vp, C.c_int, vp, C.c_float],
             "nicp_batch_convert": [vp],
@@ -527,14 +528,22 @@ class Batch:
         uses the faster one afterwards."""
         _check(lib().nicp_batch_set_autotune(self.h, int(enable)))
 
+    def set_projection(self, strategy):
+        """Forces projection strategy 0, 1 or 2 (tile-binned) on later align calls; -1: default."""
+        _check(lib().nicp_batch_set_projection(self.h, int(strategy)))
+
+    def set_bin_capacity(self, capacity):
+        """Source tiles a projector tile of the tile-binned projection lists (0..128)."""
+        _check(lib().nicp_batch_set_bin_capacity(self.h, int(capacity)))
+
     def autotune_state(self):
-        """[(choice, ms strategy 0, ms strategy 1)] per tuned shape."""
+        """[(choice, ms strategy 0, ms strategy 1, ms strategy 2)] per tuned shape."""
         n = C.c_int()
         _check(lib().nicp_batch_autotune_read(self.h, 0, C.byref(n), None, None))
         ch = np.empty(max(n.value, 1), np.int32)
-        ms = np.empty(2 * max(n.value, 1), np.float32)
+        ms = np.empty(3 * max(n.value, 1), np.float32)
         _check(lib().nicp_batch_autotune_read(self.h, n.value, C.byref(n), _p(ch), _p(ms)))
-        return [(int(ch[i]), float(ms[2 * i]), float(ms[2 * i + 1])) for i in range(n.value)]
+        return [(int(ch[i]), float(ms[3 * i]), float(ms[3 * i + 1]), float(ms[3 * i + 2])) for i in range(n.value)]
 
     def frame_cloud(self, frame, shape):
         h = C.c_void_p()

@@ -104,6 +104,7 @@ struct Carver {
 // device address hipGetSymbolAddress returns for the current device, kept per device.
 // ---------------------------------------------------------------------------------------
 constexpr int kNoAllocMaxPixels = 640 * 480;            // projector pixels and points per cloud
+constexpr int kNoAllocMaxRows = 960, kNoAllocMaxCols = 1280;  // projector image sides
 constexpr size_t kNoAllocScratchBytes = 7u << 20;        // 7 MiB
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
@@ -136,7 +137,8 @@ int scratch_address(int device, void **out) {
 enum KernelId {
   KID_COUNT, KID_SCAN, KID_UNPROJECT, KID_INTEGRAL_X, KID_INTEGRAL_Y, KID_STATS, KID_ZFILL, KID_ZPROJECT_CUR,
   KID_ZRESOLVE, KID_ZPROJECT_REF, KID_CORR_LIN, KID_SOLVE, KID_LIN_PAIRS, KID_SOLVE_STATS, KID_NN_QUERY,
-  KID_NET_INPUT, KID_NET_FORWARD, KID_START_PROBE, KID_INTEGRAL_FRAME, KID_SOLUTION_STATS, KID_N
+  KID_NET_INPUT, KID_NET_FORWARD, KID_START_PROBE, KID_INTEGRAL_FRAME, KID_SOLUTION_STATS, KID_SRC_TILES, KID_BIN,
+  KID_PROJ_CORR_LIN, KID_N
 };
 // the last three are stages: the network input kernel, the network's forward pass (its six
 // GEMM launches) and the one-iteration probe of both starts of the learned-start aligner
@@ -144,7 +146,8 @@ static const char *kKernelNames[KID_N] = {"k_count", "k_scan", "k_unproject", "k
                                           "k_stats", "k_zfill", "k_zproject(current)", "k_zresolve",
                                           "k_zproject(reference)", "k_corr_lin", "k_solve", "k_lin_pairs",
                                           "k_solve(statistics)", "k_nn_query", "k_net_input", "initnet_forward",
-                                          "start_probe", "k_integral_frame", "k_solution_stats"};
+                                          "start_probe", "k_integral_frame", "k_solution_stats", "k_src_tiles",
+                                          "k_bin", "k_proj_corr_lin"};
 
 namespace {
 struct ProfEntry {
@@ -305,16 +308,20 @@ struct nicp_batch {
   int *d_blk_frame = nullptr, *d_xblk_frame = nullptr, *d_yblk_frame = nullptr;
   int total_tiles = 0, total_xblk = 0, total_yblk = 0;
   // align workspace
-  DevBuf pairs_tbl, maps, zbuf, cur_index, pairs_buf, state, partials, result;
+  DevBuf pairs_tbl, maps, zbuf, cur_index, pairs_buf, state, partials, result, bins;
   GraphCache graph;
   GraphCache conv_graph;
   DevBuf synth_buf;  // render job tables of nicp_batch_render_synthetic
+  // projection strategy (nicp_batch_set_projection): -1 the autotuner's choice when it is on,
+  // else the tile-binned projection where it applies; 0, 1, 2 forced
+  int projection = -1;
+  int bin_cap = kBinCap;  // list capacity of the tile-binned projection (nicp_batch_set_bin_capacity)
   // autotuner of the reference projection (nicp_batch_set_autotune): per plan shape, the
-  // strategy of k_zproject chosen from the timed iterations of the shape's first call
+  // strategy chosen from the timed iterations of the shape's first call
   int autotune = 0;
   std::vector<std::vector<int>> tuned_keys;
   std::vector<int> tuned_choice;
-  std::vector<float> tuned_ms;  // 2 per shape: average ms of strategies 0 and 1 (-1 unmeasured)
+  std::vector<float> tuned_ms;  // kNumStrategies per shape: average ms per iteration (-1 unmeasured)
 };
 
 // ---------------------------------------------------------------------------------------
@@ -423,7 +430,7 @@ int yblocks(int cols) { return (cols * 10 + kThreads - 1) / kThreads; }
 
 // Layout of the per-frame device arrays inside one allocation.
 struct FrameLayout {
-  size_t depth, depthf, index, ii, pts, nrm, omg, stats, npoints, tcount, toff, total;
+  size_t depth, depthf, index, ii, pts, nrm, omg, stats, stile, npoints, tcount, toff, total;
 };
 FrameLayout frame_layout(int rows, int cols, bool with_stats, bool with_ii, size_t base, bool with_f32 = false) {
   Carver cv;
@@ -439,6 +446,7 @@ FrameLayout frame_layout(int rows, int cols, bool with_stats, bool with_ii, size
   L.nrm = cv.take<float4>(P);
   L.omg = cv.take<float4>(P * 3);
   L.stats = with_stats ? cv.take<float4>(P * 4) : 0;
+  L.stile = with_ii ? cv.take<float2>((size_t)stiles(rows, cols)) : 0;
   L.npoints = cv.take<int>(1);
   L.tcount = cv.take<int>(T);
   L.toff = cv.take<int>(T);
@@ -456,6 +464,8 @@ void frame_pointers(FrameDev &f, char *base, const FrameLayout &L, bool with_sta
   f.nrm = (float4 *)(base + L.nrm);
   f.omg = (float4 *)(base + L.omg);
   f.stats = with_stats ? (float4 *)(base + L.stats) : nullptr;
+  // source-tile depth ranges of the tile-binned projection (pinhole frames)
+  f.stile_d = with_ii && f.model == NICP_PINHOLE ? (float2 *)(base + L.stile) : nullptr;
   f.npoints = (int *)(base + L.npoints);
   f.tile_count = (int *)(base + L.tcount);
   f.tile_off = (int *)(base + L.toff);
@@ -478,6 +488,7 @@ int launch_convert(Prof *pf, hipStream_t s, const FrameDev *d_frames, int num_fr
   LAUNCH(pf, KID_SCAN, k_scan, dim3(num_frames), dim3(1024), s, d_frames);
   if (sph) LAUNCH(pf, KID_UNPROJECT, k_unproject_sph, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
   else LAUNCH(pf, KID_UNPROJECT, k_unproject, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
+  if (!sph) LAUNCH(pf, KID_SRC_TILES, k_src_tiles, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
   if (num_frames >= kFusedIntegralFrames) {
     if (sph) LAUNCH(pf, KID_INTEGRAL_FRAME, k_integral_frame_sph, dim3(num_frames), dim3(kIFrameThreads), s, d_frames);
     else LAUNCH(pf, KID_INTEGRAL_FRAME, k_integral_frame, dim3(num_frames), dim3(kIFrameThreads), s, d_frames);
@@ -497,13 +508,51 @@ int launch_convert(Prof *pf, hipStream_t s, const FrameDev *d_frames, int num_fr
 // ---------------------------------------------------------------------------------------
 struct PairPlan {
   std::vector<PairDev> pairs;
-  std::vector<int> blk_pair, blk_ref, blk_cur;  // block maps
-  int total_pix_tiles = 0, total_rtiles = 0, total_ctiles = 0;
+  std::vector<int> blk_pair, blk_ref, blk_cur, blk_dpair, blk_spair, blk_fpair;  // block maps
+  int total_pix_tiles = 0, total_rtiles = 0, total_ctiles = 0, total_dtiles = 0, total_sblk = 0, total_ftiles = 0;
   size_t total_pix = 0;
+  size_t total_parts = 0;       // partial slots: max(n_tiles, n_dtiles) per pair
+  size_t total_bins = 0;        // ints of the fused tiles' lists: n_ftiles (1 + kBinCap) per pair
+  // device copies of blk_dpair, blk_spair, blk_fpair (set by prepare_align)
+  int *d_dpair = nullptr, *d_spair = nullptr, *d_fpair = nullptr;
+  // the tile-binned projection applies: pinhole pairs, every current index image the frame's own,
+  // every reference frame with source-tile depth ranges, no subsampling (set by batch_align)
+  bool fused_ok = false;
+  int bin_cap = kBinCap;        // source tiles a fused tile lists before it scans them all
   // > 0: pairs q and q + twin share both frames (the two starts of the start probe), and
   // one k_zproject_twin pass over the first half's reference tiles projects both
   int twin = 0;
 };
+
+// the per-pixel block maps of every pair from its projector image and sampling: linear
+// 1024-pixel tiles (k_zfill, k_zresolve, k_lin_pairs) and kDTH x kDTW tiles (k_corr_lin) of
+// the sampled grid
+void plan_pixel_tiles(PairPlan &plan) {
+  plan.blk_pair.clear();
+  plan.blk_dpair.clear();
+  plan.blk_fpair.clear();
+  plan.total_parts = 0;
+  plan.total_bins = 0;
+  for (size_t i = 0; i < plan.pairs.size(); ++i) {
+    PairDev &pd = plan.pairs[i];
+    const int GR = (pd.rows + pd.sub - 1) / pd.sub, GC = (pd.cols + pd.sub - 1) / pd.sub;
+    pd.n_tiles = pair_tiles(GR, GC);
+    pd.tile_base = (int)plan.blk_pair.size();
+    plan.blk_pair.insert(plan.blk_pair.end(), pd.n_tiles, (int)i);
+    pd.dtiles_x = dtiles_x(GC);
+    pd.n_dtiles = dtiles_y(GR) * pd.dtiles_x;
+    pd.dtile_base = (int)plan.blk_dpair.size();
+    plan.blk_dpair.insert(plan.blk_dpair.end(), pd.n_dtiles, (int)i);
+    plan.total_parts += (size_t)std::max(pd.n_tiles, pd.n_dtiles);
+    pd.n_ftiles = ftiles_y(GR) * pd.dtiles_x;
+    pd.ftile_base = (int)plan.blk_fpair.size();
+    plan.blk_fpair.insert(plan.blk_fpair.end(), pd.n_ftiles, (int)i);
+    plan.total_bins += (size_t)pd.n_ftiles * (1 + kBinCap);
+  }
+  plan.total_pix_tiles = (int)plan.blk_pair.size();
+  plan.total_dtiles = (int)plan.blk_dpair.size();
+  plan.total_ftiles = (int)plan.blk_fpair.size();
+}
 
 // hosts frames[] (host copies) and pair list -> plan with block maps and buffer offsets
 void make_plan(const std::vector<FrameDev> &frames, int num_pairs, const int *ref, const int *cur,
@@ -512,6 +561,7 @@ void make_plan(const std::vector<FrameDev> &frames, int num_pairs, const int *re
   plan.blk_pair.clear();
   plan.blk_ref.clear();
   plan.blk_cur.clear();
+  plan.blk_spair.clear();
   size_t pix = 0;
   for (int i = 0; i < num_pairs; ++i) {
     PairDev &pd = plan.pairs[i];
@@ -521,15 +571,16 @@ void make_plan(const std::vector<FrameDev> &frames, int num_pairs, const int *re
     pd.rows = fc.rows;
     pd.cols = fc.cols;
     const int P = pd.rows * pd.cols;
-    pd.n_tiles = pair_tiles(pd.rows, pd.cols);
-    pd.tile_base = (int)plan.blk_pair.size();
-    for (int t = 0; t < pd.n_tiles; ++t) plan.blk_pair.push_back(i);
     pd.n_rtiles = (fr.rows * fr.cols + kTile - 1) / kTile;
     pd.rtile_base = (int)plan.blk_ref.size();
     for (int t = 0; t < pd.n_rtiles; ++t) plan.blk_ref.push_back(i);
     pd.n_ctiles = (fc.rows * fc.cols + kTile - 1) / kTile;
     pd.ctile_base = (int)plan.blk_cur.size();
     for (int t = 0; t < pd.n_ctiles; ++t) plan.blk_cur.push_back(i);
+    pd.n_sblk = (stiles(fr.rows, fr.cols) + kThreads - 1) / kThreads;
+    pd.sblk_base = (int)plan.blk_spair.size();
+    plan.blk_spair.insert(plan.blk_spair.end(), pd.n_sblk, i);
+    pd.bins = nullptr;
     for (int k = 0; k < 9; ++k) pd.K[k] = cams[i]->K[k];
     pd.min_d = cams[i]->min_distance;
     pd.max_d = cams[i]->max_distance;
@@ -541,7 +592,8 @@ void make_plan(const std::vector<FrameDev> &frames, int num_pairs, const int *re
     pd.zbuf = (unsigned long long *)(uintptr_t)pix;
     pix += align_up((size_t)P, 64);
   }
-  plan.total_pix_tiles = (int)plan.blk_pair.size();
+  plan_pixel_tiles(plan);
+  plan.total_sblk = (int)plan.blk_spair.size();
   plan.total_rtiles = (int)plan.blk_ref.size();
   plan.total_ctiles = (int)plan.blk_cur.size();
   plan.total_pix = pix;
@@ -572,6 +624,7 @@ void plan_batch_frames(const std::vector<FrameDev> &frames, PairPlan &plan) {
 
 struct AlignBuffers {
   DevBuf *pairs_tbl, *maps, *zbuf, *cur_index, *pairs_buf, *state, *partials, *result;
+  DevBuf *bins = nullptr;  // fused tiles' source lists (batches: the tile-binned projection)
 };
 
 // allocates/assigns the per-pair buffers and uploads tables and the initial state
@@ -582,9 +635,10 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
   CHK(B.cur_index->reserve(plan.total_pix * sizeof(int)));
   CHK(B.pairs_buf->reserve(plan.total_pix * sizeof(int2)));
   CHK(B.state->reserve((size_t)np * kStateFloats * sizeof(float)));
-  CHK(B.partials->reserve((size_t)std::max(plan.total_pix_tiles, 1) * kPartial * sizeof(double)));
+  CHK(B.partials->reserve(std::max<size_t>(plan.total_parts, 1) * kPartial * sizeof(double)));
   CHK(B.result->reserve((size_t)np * kResultDoubles * sizeof(double)));
-  size_t tile_part = 0;
+  if (B.bins && plan.fused_ok) CHK(B.bins->reserve(std::max<size_t>(plan.total_bins, 1) * sizeof(int)));
+  size_t tile_part = 0, bin_off = 0;
   std::vector<float> st((size_t)np * kStateFloats, 0.0f);
   for (int i = 0; i < np; ++i) {
     PairDev &pd = plan.pairs[i];
@@ -594,8 +648,10 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
     pd.pairs = B.pairs_buf->as<int2>() + off;
     pd.state = B.state->as<float>() + (size_t)i * kStateFloats;
     pd.partials = B.partials->as<double>() + tile_part * kPartial;
-    tile_part += pd.n_tiles;
+    tile_part += (size_t)std::max(pd.n_tiles, pd.n_dtiles);
     pd.result = B.result->as<double>() + (size_t)i * kResultDoubles;
+    pd.bins = B.bins && plan.fused_ok ? B.bins->as<int>() + bin_off : nullptr;
+    bin_off += (size_t)pd.n_ftiles * (1 + kBinCap);
     // initial state: AlignerProjective::align, alignerprojective.cpp:36-54
     float *S = &st[(size_t)i * kStateFloats];
     float T[16];
@@ -615,7 +671,8 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
     (void)iKRt;
     for (int k = 0; k < 16; ++k) S[kSlotKRtCur + k] = KRt[k];
   }
-  const size_t nb = plan.blk_pair.size() + plan.blk_ref.size() + plan.blk_cur.size();
+  const size_t nb = plan.blk_pair.size() + plan.blk_ref.size() + plan.blk_cur.size() + plan.blk_dpair.size() +
+                    plan.blk_spair.size() + plan.blk_fpair.size();
   CHK(B.pairs_tbl->reserve((size_t)np * sizeof(PairDev)));
   CHK(B.maps->reserve(std::max<size_t>(nb, 1) * sizeof(int)));
   std::vector<int> maps;
@@ -623,9 +680,15 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
   maps.insert(maps.end(), plan.blk_pair.begin(), plan.blk_pair.end());
   maps.insert(maps.end(), plan.blk_ref.begin(), plan.blk_ref.end());
   maps.insert(maps.end(), plan.blk_cur.begin(), plan.blk_cur.end());
+  maps.insert(maps.end(), plan.blk_dpair.begin(), plan.blk_dpair.end());
+  maps.insert(maps.end(), plan.blk_spair.begin(), plan.blk_spair.end());
+  maps.insert(maps.end(), plan.blk_fpair.begin(), plan.blk_fpair.end());
   *d_blk_pair = B.maps->as<int>();
   *d_blk_ref = *d_blk_pair + plan.blk_pair.size();
   *d_blk_cur = *d_blk_ref + plan.blk_ref.size();
+  plan.d_dpair = *d_blk_cur + plan.blk_cur.size();
+  plan.d_spair = plan.d_dpair + plan.blk_dpair.size();
+  plan.d_fpair = plan.d_spair + plan.blk_spair.size();
   *d_pairs = B.pairs_tbl->as<PairDev>();
   HIPCHK(hipMemcpyAsync(*d_pairs, plan.pairs.data(), (size_t)np * sizeof(PairDev), hipMemcpyHostToDevice, s));
   if (nb) HIPCHK(hipMemcpyAsync(*d_blk_pair, maps.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
@@ -633,9 +696,16 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
   return NICP_OK;
 }
 
-// the launch sequence of AlignerProjective::align for all pairs of a plan
-// Timing of the reference projections of a tuning call: iteration i uses strategy i % 2 and
-// records its launch between two events.
+// Strategies of the reference projection and correspondence search of an iteration:
+//   0  k_zproject (merged 64-bit atomics into the global z-buffer) + k_corr_lin;
+//   1  the same, reading each cell before its atomic;
+//   2  tile-binned: k_bin + k_proj_corr_lin (the z-buffer of each fused tile in LDS).
+// All three give the same z-buffer, correspondences and partial sums.
+constexpr int kNumStrategies = 3;
+constexpr int kFusedMinTiles = 1024;  // fused tiles from which strategy 2 is the default
+
+// Timing of a tuning call: iteration i uses strategy i % (number of applicable strategies) and
+// records its projection and correspondence launches between two events.
 struct ZTune {
   std::vector<hipEvent_t> ev;  // 2 per outer iteration
   std::vector<int> strategy;
@@ -647,6 +717,7 @@ struct ZTune {
   }
 };
 
+// the launch sequence of AlignerProjective::align for all pairs of a plan
 int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d_pairs, const FrameDev *d_frames,
                  const int *d_blk_pair, const int *d_blk_ref, const int *d_blk_cur, const nicp_align_params *ap,
                  bool with_stats = true, int zstrategy = 0, ZTune *tune = nullptr) {
@@ -656,8 +727,13 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
   if (plan.total_pix_tiles == 0) return NICP_OK;
   // one projector model per plan (a batch's frames share it)
   const bool sph = plan.pairs[0].model == NICP_SPHERICAL;
+  const bool fused_ok = plan.fused_ok && !sph && a.row_region == 0 && a.col_region == 0;
+  const int nstrat = fused_ok ? 3 : 2;
+  if (!fused_ok && zstrategy == 2) zstrategy = 0;
+  const bool only_fused = fused_ok && !tune && zstrategy == 2;
   (void)hipGetLastError();
-  LAUNCH(pf, KID_ZFILL, k_zfill, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair);
+  if (!only_fused || plan.total_ctiles) LAUNCH(pf, KID_ZFILL, k_zfill, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair);
+  if (fused_ok) LAUNCH(pf, KID_BIN, k_bin_clear, dim3(1, np), tb, s, d_pairs, np);
   if (plan.total_ctiles) {
     if (sph) LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject_sph, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1, zstrategy);
     else LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1, zstrategy);
@@ -666,38 +742,47 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
   const int inner = ap->inner_iterations;
   for (int it = 0; it < ap->outer_iterations; ++it) {
     const bool last = it == ap->outer_iterations - 1;
-    if (plan.total_rtiles) {
-      const int zs = tune ? (it & 1) : zstrategy;
-      if (tune) {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        HIPCHK(hipEventCreate(&e0));
-        tune->ev.push_back(e0);
-        HIPCHK(hipEventCreate(&e1));
-        tune->ev.push_back(e1);
-        tune->strategy.push_back(zs);
-        HIPCHK(hipEventRecord(e0, s));
-      }
-      if (sph) LAUNCH(pf, KID_ZPROJECT_REF, k_zproject_sph, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0, zs);
-      else if (plan.twin > 0)
-        LAUNCH(pf, KID_ZPROJECT_REF, k_zproject_twin, dim3(plan.pairs[plan.twin].rtile_base), tb, s, d_pairs, d_frames,
-               d_blk_ref, plan.twin);
-      else LAUNCH(pf, KID_ZPROJECT_REF, k_zproject, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0, zs);
-      if (tune) HIPCHK(hipEventRecord(tune->ev.back(), s));
+    const int store = (int)(last || inner > 1);
+    const int zs = tune ? it % nstrat : zstrategy;
+    if (tune) {
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      HIPCHK(hipEventCreate(&e0));
+      tune->ev.push_back(e0);
+      HIPCHK(hipEventCreate(&e1));
+      tune->ev.push_back(e1);
+      tune->strategy.push_back(zs);
+      HIPCHK(hipEventRecord(e0, s));
     }
-    LAUNCH(pf, KID_CORR_LIN, k_corr_lin, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
-           (int)(last || inner > 1));
-    LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, ap->lambda, inner > 1 ? 1 : 0, 0);
+    if (zs == 2) {
+      if (plan.total_sblk)
+        LAUNCH(pf, KID_BIN, k_bin, dim3(plan.total_sblk), tb, s, d_pairs, d_frames, plan.d_spair, plan.bin_cap);
+      LAUNCH(pf, KID_PROJ_CORR_LIN, k_proj_corr_lin, dim3(plan.total_ftiles), tb, s, d_pairs, d_frames, plan.d_fpair, a,
+             store, plan.bin_cap);
+    } else {
+      if (plan.total_rtiles) {
+        if (sph)
+          LAUNCH(pf, KID_ZPROJECT_REF, k_zproject_sph, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0, zs);
+        else if (plan.twin > 0)
+          LAUNCH(pf, KID_ZPROJECT_REF, k_zproject_twin, dim3(plan.pairs[plan.twin].rtile_base), tb, s, d_pairs, d_frames,
+                 d_blk_ref, plan.twin);
+        else LAUNCH(pf, KID_ZPROJECT_REF, k_zproject, dim3(plan.total_rtiles), tb, s, d_pairs, d_frames, d_blk_ref, 0, zs);
+      }
+      LAUNCH(pf, KID_CORR_LIN, k_corr_lin, dim3(plan.total_dtiles), tb, s, d_pairs, d_frames, plan.d_dpair, a, store);
+    }
+    if (tune) HIPCHK(hipEventRecord(tune->ev.back(), s));
+    LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, ap->lambda, inner > 1 ? 1 : 0, 0, 1);
     for (int k = 1; k < inner; ++k) {
       LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
              kSlotInvTInner, 0);
-      LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, ap->lambda, k == inner - 1 ? 0 : 1, 1);
+      LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, ap->lambda, k == inner - 1 ? 0 : 1, 1,
+             0);
     }
   }
   // Aligner::_computeStatistics: one more linearisation at the final T over the last correspondences
   if (ap->outer_iterations > 0 && with_stats) {
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_frames, d_blk_pair, a,
            kSlotTinv, 0);
-    LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, 0.0f, 2, 0);
+    LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, 0.0f, 2, 0, 0);
     LAUNCH(pf, KID_SOLUTION_STATS, k_solution_stats, dim3(np), dim3(64), s, d_pairs,
            ap->translational_min_eigen_ratio, ap->rotational_min_eigen_ratio);
   }
@@ -1155,9 +1240,7 @@ int project_device(nicp_ctx *ctx, const nicp_cloud *cloud, const float transform
   pd.cols = cols;
   pd.sph = sph_params(cam, rows, cols);
   const int P = rows * cols;
-  pd.n_tiles = pair_tiles(rows, cols);
-  plan.blk_pair.assign(pd.n_tiles, 0);
-  plan.total_pix_tiles = pd.n_tiles;
+  plan_pixel_tiles(plan);
   plan.total_pix = align_up((size_t)P, 64);
   plan.blk_ref.clear();
   plan.total_rtiles = 0;
@@ -1240,8 +1323,8 @@ extern "C" int nicp_correspondences(nicp_ctx *ctx, const nicp_cloud *ref, const 
   for (int i = 0; i < P; ++i) keys[i] = ref_index[i] < 0 ? kZInit : (unsigned long long)(unsigned int)ref_index[i];
   HIPCHK(hipMemcpyAsync(ctx->zbuf.p, keys.data(), (size_t)P * sizeof(unsigned long long), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(ctx->cur_index.p, cur_index, (size_t)P * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_corr_lin, dim3(plan.total_pix_tiles), dim3(kThreads), 0, ctx->stream, d_pairs, d_frames, d_bp,
-                     to_dev(ap), 1);
+  hipLaunchKernelGGL(k_corr_lin, dim3(plan.total_dtiles), dim3(kThreads), 0, ctx->stream, d_pairs, d_frames,
+                     (const int *)plan.d_dpair, to_dev(ap), 1);
   HIPCHK(hipGetLastError());
   std::vector<int2> pp(P);
   HIPCHK(hipMemcpyAsync(pp.data(), ctx->pairs_buf.p, (size_t)P * sizeof(int2), hipMemcpyDeviceToHost, ctx->stream));
@@ -1290,7 +1373,7 @@ extern "C" int nicp_linearize(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_c
   HIPCHK(hipMemcpyAsync(ctx->pairs_buf.p, pp.data(), (size_t)npx * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_lin_pairs, dim3(plan.total_pix_tiles), dim3(kThreads), 0, ctx->stream, d_pairs, d_frames, d_bp,
                      to_dev(ap), kSlotTinv, npx);
-  hipLaunchKernelGGL(k_solve, dim3(1), dim3(kSolveThreads), 0, ctx->stream, d_pairs, 0.0f, 2, 0);
+  hipLaunchKernelGGL(k_solve, dim3(1), dim3(kSolveThreads), 0, ctx->stream, d_pairs, 0.0f, 2, 0, 0);
   HIPCHK(hipGetLastError());
   double r[2 * kPartial];
   HIPCHK(hipMemcpyAsync(r, ctx->result.p, sizeof(r), hipMemcpyDeviceToHost, ctx->stream));
@@ -1335,9 +1418,7 @@ int align_single(nicp_ctx *ctx, DevBuf &frames_tbl, AlignBuffers B, const nicp_c
   pd.rows = rows;
   pd.cols = cols;
   pd.sph = sph_params(cam, rows, cols);
-  pd.n_tiles = pair_tiles(rows, cols);
-  plan.blk_pair.assign(pd.n_tiles, 0);
-  plan.total_pix_tiles = pd.n_tiles;
+  plan_pixel_tiles(plan);
   plan.total_pix = align_up((size_t)rows * cols, 64);
   int *d_bp, *d_br, *d_bc;
   PairDev *d_pairs;
@@ -1378,21 +1459,25 @@ extern "C" int nicp_align_noalloc(nicp_ctx *ctx, const nicp_cloud *ref, const ni
                                   int rows, int cols, const float init_guess[16], const nicp_align_params *ap,
                                   float T_out[16], nicp_align_result *result) {
   if (!ctx || !ref || !cur) return set_err(NICP_ERR_INVALID, "null argument");
-  if ((long long)rows * cols > kNoAllocMaxPixels || ref->n > kNoAllocMaxPixels || cur->n > kNoAllocMaxPixels)
+  if ((long long)rows * cols > kNoAllocMaxPixels || rows > kNoAllocMaxRows || cols > kNoAllocMaxCols ||
+      ref->n > kNoAllocMaxPixels || cur->n > kNoAllocMaxPixels)
     return set_err(NICP_ERR_INVALID,
-                   "nicp_align_noalloc handles at most %d projector pixels and %d points per cloud "
-                   "(got %dx%d, %d and %d points)",
-                   kNoAllocMaxPixels, kNoAllocMaxPixels, rows, cols, ref->n, cur->n);
+                   "nicp_align_noalloc handles at most %d projector pixels (%d rows, %d columns) and %d points "
+                   "per cloud (got %dx%d, %d and %d points)",
+                   kNoAllocMaxPixels, kNoAllocMaxRows, kNoAllocMaxCols, kNoAllocMaxPixels, rows, cols, ref->n, cur->n);
   if (!ctx->noalloc_ws) return set_err(NICP_ERR_STATE, "the context has no fixed workspace");
   // views of the fixed workspace, sized for the largest pair the entry point accepts; every
   // region is rewritten or re-initialised by each call's launches (k_zfill clears the z-buffer)
+  // bounds within these limits: 1024-pixel tiles <= 300 per image and cloud; kDTH x kDTW tiles
+  // <= P / 1024 + rows / 16 + cols / 64 + 1 <= 381; source-tile blocks <= 38; fused tiles <= 111
   const size_t P = align_up((size_t)kNoAllocMaxPixels, 64);
-  const size_t tiles = (size_t)(kNoAllocMaxPixels + kTile - 1) / kTile;
+  const size_t parts = 400, nmaps = 2048;
   Carver cv;
-  const size_t o_frames = cv.take<FrameDev>(2), o_pairs = cv.take<PairDev>(1), o_maps = cv.take<int>(3 * tiles),
+  const size_t o_frames = cv.take<FrameDev>(2), o_pairs = cv.take<PairDev>(1), o_maps = cv.take<int>(nmaps),
                o_z = cv.take<unsigned long long>(P), o_ci = cv.take<int>(P), o_pp = cv.take<int2>(P),
-               o_state = cv.take<float>(kStateFloats), o_part = cv.take<double>(tiles * kPartial),
+               o_state = cv.take<float>(kStateFloats), o_part = cv.take<double>(parts * kPartial),
                o_res = cv.take<double>(kResultDoubles);
+  static_assert(3 * (kNoAllocMaxPixels + kTile - 1) / kTile + 400 + 40 + 120 <= 2048, "no-allocation maps");
   if (cv.off > kNoAllocScratchBytes) return set_err(NICP_ERR_STATE, "fixed workspace layout exceeds its size");
   char *base = (char *)ctx->noalloc_ws;
   DevBuf frames_tbl, pairs_tbl, maps, zbuf, cur_index, pairs_buf, state, partials, res;
@@ -1401,12 +1486,12 @@ extern "C" int nicp_align_noalloc(nicp_ctx *ctx, const nicp_cloud *ref, const ni
     size_t off, bytes;
   } views[] = {{&frames_tbl, o_frames, 2 * sizeof(FrameDev)},
                {&pairs_tbl, o_pairs, sizeof(PairDev)},
-               {&maps, o_maps, 3 * tiles * sizeof(int)},
+               {&maps, o_maps, nmaps * sizeof(int)},
                {&zbuf, o_z, P * sizeof(unsigned long long)},
                {&cur_index, o_ci, P * sizeof(int)},
                {&pairs_buf, o_pp, P * sizeof(int2)},
                {&state, o_state, kStateFloats * sizeof(float)},
-               {&partials, o_part, tiles * kPartial * sizeof(double)},
+               {&partials, o_part, parts * kPartial * sizeof(double)},
                {&res, o_res, kResultDoubles * sizeof(double)}};
   for (const auto &v : views) {
     v.b->p = base + v.off;
@@ -1554,8 +1639,8 @@ extern "C" int nicp_batch_destroy(nicp_batch *b) {
   if (b->ctx->pending_graph == &b->graph.events) b->ctx->pending_graph = nullptr;
   b->graph.reset();
   b->conv_graph.reset();
-  DevBuf *bufs[] = {&b->pairs_tbl, &b->maps,  &b->zbuf,   &b->cur_index, &b->pairs_buf,
-                    &b->state,     &b->partials, &b->result, &b->synth_buf};
+  DevBuf *bufs[] = {&b->pairs_tbl, &b->maps,     &b->zbuf,   &b->cur_index, &b->pairs_buf,
+                    &b->state,     &b->partials, &b->result, &b->bins,      &b->synth_buf};
   for (DevBuf *x : bufs) x->release();
   (void)hipFree(b->arena);
   delete b;
@@ -1622,42 +1707,49 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
   make_plan(b->frames, num_pairs, ref_frame, cur_frame, cams.data(), plan);
   plan_batch_frames(b->frames, plan);
   if (sub > 1) {
-    // per-pixel blocks cover the subsampled grid only; k_zfill still clears whole z-buffers
-    plan.blk_pair.clear();
-    for (int i = 0; i < num_pairs; ++i) {
-      PairDev &pd = plan.pairs[i];
-      pd.sub = sub;
-      pd.n_tiles = pair_tiles((pd.rows + sub - 1) / sub, (pd.cols + sub - 1) / sub);
-      pd.tile_base = (int)plan.blk_pair.size();
-      for (int t = 0; t < pd.n_tiles; ++t) plan.blk_pair.push_back(i);
-    }
-    plan.total_pix_tiles = (int)plan.blk_pair.size();
+    // per-pixel blocks cover the subsampled grid only (k_zfill clears its cells only)
+    for (int i = 0; i < num_pairs; ++i) plan.pairs[i].sub = sub;
+    plan_pixel_tiles(plan);
     // the twin pass needs pairs q and q + twin on the same frames (pinhole)
     bool twins = twin > 0 && num_pairs == 2 * twin && plan.pairs[0].model != NICP_SPHERICAL;
     for (int i = 0; twins && i < twin; ++i)
       twins = ref_frame[i] == ref_frame[i + twin] && cur_frame[i] == cur_frame[i + twin];
     plan.twin = twins ? twin : 0;
   }
-  AlignBuffers B{&b->pairs_tbl, &b->maps, &b->zbuf, &b->cur_index, &b->pairs_buf, &b->state, &b->partials, &b->result};
+  // the tile-binned projection needs pinhole pairs whose current index image is the frame's own,
+  // reference frames with source-tile depth ranges and the full projector image
+  plan.fused_ok = sub == 1;
+  for (int i = 0; plan.fused_ok && i < num_pairs; ++i) {
+    const PairDev &pd = plan.pairs[i];
+    plan.fused_ok = pd.model == NICP_PINHOLE && pd.cur_own && b->frames[pd.ref].stile_d && b->frames[pd.ref].index;
+  }
+  const bool fused = plan.fused_ok && ap->row_search_region == 0 && ap->col_search_region == 0;
+  plan.bin_cap = b->bin_cap;
+  AlignBuffers B{&b->pairs_tbl, &b->maps, &b->zbuf, &b->cur_index, &b->pairs_buf, &b->state, &b->partials, &b->result,
+                 &b->bins};
   int *d_bp, *d_br, *d_bc;
   PairDev *d_pairs;
   CHK(prepare_align(ctx->stream, plan, B, init_guesses16, &d_bp, &d_br, &d_bc, &d_pairs));
-  // autotuner: the first call of a shape times both projection strategies on alternate
-  // iterations (eagerly), later calls use the faster one
-  int zstrategy = 0;
+  // strategy: forced, or the autotuner's (the first call of a shape times the applicable
+  // strategies on alternate iterations, eagerly; later calls use the fastest), or by default
+  // the tile-binned projection where it applies and its fused tiles fill the device (one
+  // workgroup per 4096 pixels: a single 640x480 pair has 80, and then the per-launch latency
+  // of the global z-buffer kernels is lower)
+  int zstrategy = fused && plan.total_ftiles >= kFusedMinTiles ? 2 : 0;
   bool tuning = false;
-  size_t shape = 0;
-  if (b->autotune) {
+  if (b->projection >= 0) {
+    zstrategy = b->projection == 2 && !fused ? 0 : b->projection;
+  } else if (b->autotune) {
     const std::vector<int> skey = {num_pairs, plan.total_pix_tiles, plan.total_rtiles, plan.total_ctiles, sub,
-                                   with_stats ? 1 : 0};
+                                   with_stats ? 1 : 0, fused ? 1 : 0};
+    size_t shape = 0;
     while (shape < b->tuned_keys.size() && b->tuned_keys[shape] != skey) ++shape;
     if (shape == b->tuned_keys.size()) {
       tuning = ap->outer_iterations >= 2;
       if (!tuning) {  // nothing to compare: keep the default
         b->tuned_keys.push_back(skey);
-        b->tuned_choice.push_back(0);
-        b->tuned_ms.push_back(-1.0f);
-        b->tuned_ms.push_back(-1.0f);
+        b->tuned_choice.push_back(zstrategy);
+        for (int k = 0; k < kNumStrategies; ++k) b->tuned_ms.push_back(-1.0f);
       }
     } else {
       zstrategy = b->tuned_choice[shape];
@@ -1670,8 +1762,8 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
     int rc = launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats, 0,
                           &tune);
     hipError_t e = rc == NICP_OK ? hipStreamSynchronize(ctx->stream) : hipSuccess;
-    double sum[2] = {0.0, 0.0};
-    int cnt[2] = {0, 0};
+    double sum[kNumStrategies] = {0.0, 0.0, 0.0};
+    int cnt[kNumStrategies] = {0, 0, 0};
     for (size_t i = 0; rc == NICP_OK && e == hipSuccess && i < tune.strategy.size(); ++i) {
       float ms = 0.0f;
       e = hipEventElapsedTime(&ms, tune.ev[2 * i], tune.ev[2 * i + 1]);
@@ -1684,20 +1776,31 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
     tune.release();
     if (rc != NICP_OK) return rc;
     if (e != hipSuccess) return set_err(NICP_ERR_HIP, "autotune timing: %s", hipGetErrorString(e));
-    const float m0 = cnt[0] ? (float)(sum[0] / cnt[0]) : -1.0f, m1 = cnt[1] ? (float)(sum[1] / cnt[1]) : -1.0f;
     b->tuned_keys.push_back({num_pairs, plan.total_pix_tiles, plan.total_rtiles, plan.total_ctiles, sub,
-                             with_stats ? 1 : 0});
-    b->tuned_choice.push_back(m0 >= 0.0f && m1 >= 0.0f && m1 < m0 ? 1 : 0);
-    b->tuned_ms.push_back(m0);
-    b->tuned_ms.push_back(m1);
+                             with_stats ? 1 : 0, fused ? 1 : 0});
+    int best = -1;
+    float best_ms = 0.0f;
+    for (int k = 0; k < kNumStrategies; ++k) {
+      const float m = cnt[k] ? (float)(sum[k] / cnt[k]) : -1.0f;
+      b->tuned_ms.push_back(m);
+      if (m >= 0.0f && (best < 0 || m < best_ms)) {
+        best = k;
+        best_ms = m;
+      }
+    }
+    b->tuned_choice.push_back(best >= 0 ? best : zstrategy);
   } else if (use_graph) {
     // graph key: the launch geometry and the parameters baked into the kernel arguments
-    std::vector<int> key = {num_pairs, plan.total_pix_tiles, plan.total_rtiles, plan.total_ctiles};
+    std::vector<int> key = {num_pairs, plan.total_pix_tiles, plan.total_rtiles, plan.total_ctiles, plan.total_dtiles};
     const int *apw = (const int *)ap;
     key.insert(key.end(), apw, apw + sizeof(nicp_align_params) / sizeof(int));
     // every device address a captured launch takes as an argument (the tables' contents are
     // uploaded before each launch, so only the argument addresses are baked into the graph)
+    key.push_back(plan.total_sblk);
+    key.push_back(plan.total_ftiles);
+    key.push_back(plan.bin_cap);
     for (const void *ptr : {(const void *)d_pairs, (const void *)d_bp, (const void *)d_br, (const void *)d_bc,
+                            (const void *)plan.d_dpair, (const void *)plan.d_spair, (const void *)plan.d_fpair,
                             (const void *)b->d_frames}) {
       key.push_back((int)(uint32_t)((uintptr_t)ptr));
       key.push_back((int)(uint32_t)((uint64_t)(uintptr_t)ptr >> 32));
@@ -1785,11 +1888,23 @@ extern "C" int nicp_batch_autotune_read(nicp_batch *b, int capacity, int *num_sh
   *num_shapes = n;
   for (int i = 0; i < n && i < capacity; ++i) {
     if (choice) choice[i] = b->tuned_choice[i];
-    if (ms) {
-      ms[2 * i] = b->tuned_ms[2 * i];
-      ms[2 * i + 1] = b->tuned_ms[2 * i + 1];
-    }
+    if (ms)
+      for (int k = 0; k < kNumStrategies; ++k) ms[kNumStrategies * i + k] = b->tuned_ms[kNumStrategies * i + k];
   }
+  return NICP_OK;
+}
+
+extern "C" int nicp_batch_set_bin_capacity(nicp_batch *b, int capacity) {
+  if (!b) return set_err(NICP_ERR_INVALID, "batch is null");
+  if (capacity < 0 || capacity > kBinCap) return set_err(NICP_ERR_INVALID, "capacity must lie in [0, %d]", kBinCap);
+  b->bin_cap = capacity;
+  return NICP_OK;
+}
+
+extern "C" int nicp_batch_set_projection(nicp_batch *b, int strategy) {
+  if (!b) return set_err(NICP_ERR_INVALID, "batch is null");
+  if (strategy < -1 || strategy >= kNumStrategies) return set_err(NICP_ERR_INVALID, "unknown strategy %d", strategy);
+  b->projection = strategy;
   return NICP_OK;
 }
 
@@ -2106,11 +2221,11 @@ int align_nn_pairs(nicp_ctx *ctx, int np, const nicp_cloud *const *refs, const n
            coarse ? G.cells_coarse : G.cells_fine, coarse ? G.sorted_coarse : G.sorted_fine,
            (const int *)ctx->nn_order.as<int>(), a, normal_scaling, (int *)nullptr);
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, ntiles, tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
-    LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, ap->lambda, 0, 0);
+    LAUNCH(pf, KID_SOLVE, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, ap->lambda, 0, 0, 0);
   }
   if (ap->outer_iterations > 0) {
     LAUNCH(pf, KID_LIN_PAIRS, k_lin_pairs, ntiles, tb, s, d_pairs, d_frames, d_bp, a, kSlotTinv, 0);
-    LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, 0.0f, 2, 0);
+    LAUNCH(pf, KID_SOLVE_STATS, k_solve, dim3(np), dim3(kSolveThreads), s, d_pairs, 0.0f, 2, 0, 0);
     LAUNCH(pf, KID_SOLUTION_STATS, k_solution_stats, dim3(np), dim3(64), s, d_pairs,
            ap->translational_min_eigen_ratio, ap->rotational_min_eigen_ratio);
   }

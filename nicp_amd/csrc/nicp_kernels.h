@@ -19,6 +19,25 @@ constexpr int kResultDoubles = 2 * kPartial + 48;
 // per-pixel blocks of a pair: 1024-pixel tiles of its projector image
 __host__ __device__ inline int pair_tiles(int rows, int cols) { return (rows * cols + kTile - 1) / kTile; }
 
+// correspondence + linearisation blocks: kDTH x kDTW pixel tiles of a pair's projector image
+// (of its sampled grid when the pair is subsampled); a wavefront covers one 64-pixel tile row
+constexpr int kDTH = 16, kDTW = 64;
+__host__ __device__ inline int dtiles_x(int cols) { return (cols + kDTW - 1) / kDTW; }
+__host__ __device__ inline int dtiles_y(int rows) { return (rows + kDTH - 1) / kDTH; }
+
+// Tile-binned projection (k_bin, k_proj_corr_lin). Source tiles: kSTH x kSTW pixel blocks of a
+// reference frame, each with the depth range of its valid pixels. Fused tiles: kFH x kDTW pixels
+// of the projector image, kFH / kDTH correspondence tiles each, z-buffered in LDS; a fused tile
+// lists up to a capacity of at most kBinCap source tiles whose projection may reach it (more:
+// every source tile of the reference frame is scanned).
+constexpr int kSTH = 16, kSTW = 32;
+constexpr int kFH = 64;
+constexpr int kBinCap = 128;
+static_assert(kFH % kDTH == 0, "a fused tile holds whole correspondence tiles");
+__host__ __device__ inline int stiles_x(int cols) { return (cols + kSTW - 1) / kSTW; }
+__host__ __device__ inline int stiles(int rows, int cols) { return ((rows + kSTH - 1) / kSTH) * stiles_x(cols); }
+__host__ __device__ inline int ftiles_y(int rows) { return (rows + kFH - 1) / kFH; }
+
 // SphericalPointProjector parameters (sphericalpointprojector.h:304-313): half fields of view,
 // image centre, angular resolutions in pixels per radian and their inverses.
 struct SphDev {
@@ -50,6 +69,7 @@ struct FrameDev {
   float4 *nrm;          // (nx, ny, nz, curvature)
   float4 *omg;          // 3 float4 per point: Omega_p (00 01 02 11 12 22), Omega_n (same)
   float4 *stats;        // 4 float4 per point or null: eigvecs (9), eigvals (3), mean (3), n
+  float2 *stile_d;      // per source tile: (min, max) depth of its valid pixels, or null
   int *npoints;         // device scalar
   int *tile_count;      // per tile
   int *tile_off;        // per tile
@@ -61,6 +81,10 @@ struct PairDev {
   int ref, cur;
   int rows, cols;       // projector image size
   int tile_base, n_tiles;          // per-pixel blocks over the projector image
+  int dtile_base, n_dtiles, dtiles_x;  // kDTH x kDTW tiles of the (sampled) projector image
+  int ftile_base, n_ftiles;        // kFH x kDTW fused tiles of the projector image
+  int sblk_base, n_sblk;           // k_bin blocks over the reference frame's source tiles
+  int *bins;                       // n_ftiles counts, then kBinCap source tiles per fused tile
   int rtile_base, n_rtiles;        // blocks over the reference cloud capacity
   int ctile_base, n_ctiles;        // blocks over the current cloud capacity
   int cur_own;                     // 1: cur_index is the current frame's own index image
@@ -73,7 +97,7 @@ struct PairDev {
   int *cur_index;                  // rows*cols
   int2 *pairs;                     // rows*cols stored correspondences (or (-1,-1))
   float *state;                    // kStateFloats: T, Tinv, KRt (reference), KRt (current), invT (inner)
-  double *partials;                // n_tiles * kPartial
+  double *partials;                // max(n_tiles, n_dtiles) * kPartial
   double *result;                  // kResultDoubles: last linearisation, statistics linearisation, solution statistics
 };
 

@@ -257,6 +257,43 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_unproject_sph(const Fra
   unproject_body<kSpherical>(F, blk_frame);
 }
 
+// The depth range of the valid pixels of every kSTH x kSTW source tile of a pinhole frame (for
+// the tile-binned projection): one wavefront per tile, lane l covering 8 pixels of tile row
+// l / 4; an empty tile stores (+inf, -inf). Block t of a frame (its 1024-pixel tile map) takes
+// source tiles 4 t + wave, 4 t + wave + 4 n_tiles, ...
+extern "C" __global__ __launch_bounds__(kThreads) void k_src_tiles(const FrameDev *__restrict__ F,
+                                                                  const int *__restrict__ blk_frame) {
+  const int g = blockIdx.x;
+  const FrameDev &fd = F[blk_frame[g]];
+  if (!fd.stile_d) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sx = stiles_x(fd.cols), ns = stiles(fd.rows, fd.cols);
+  for (int s = 4 * (g - fd.tile_base) + wave; s < ns; s += 4 * fd.n_tiles) {
+    const int ty = s / sx, tx = s - ty * sx;
+    const int r = ty * kSTH + (lane >> 2), c0 = tx * kSTW + (lane & 3) * 8;
+    float lo = INFINITY, hi = -INFINITY;
+    if (r < fd.rows) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = c0 + u;
+        if (c < fd.cols) {
+          const float d = frame_depth(fd, r * fd.cols + c);
+          if (depth_valid(d, fd.min_d, fd.max_d)) {
+            lo = fminf(lo, d);
+            hi = fmaxf(hi, d);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo = fminf(lo, __shfl_xor(lo, o, 64));
+      hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+    }
+    if (lane == 0) fd.stile_d[s] = make_float2(lo, hi);
+  }
+}
+
 // x-prefix: one thread per (row, channel), sequential along the row as the
 // reference's pass over image columns. Channel ch = p[i] * p[j] with p = (x, y, z, 1).
 __constant__ int c_ch_i[10] = {3, 0, 1, 2, 0, 0, 0, 1, 1, 2};
@@ -919,6 +956,23 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_zfill(const PairDev *__
   }
 }
 
+// PinholePointProjector::_project (pinholepointprojector.h:253-262) of cloud point i (p) under
+// KRt: false when the point falls outside the pair's depth range or image; else its pixel
+// (x = column, y = row) and z-buffer key (depth bits << 32 | index)
+__device__ __forceinline__ bool pinhole_project(const PairDev &pd, const float *KRt, float4 p, int i, int &x, int &y,
+                                                unsigned long long &key) {
+  float ip[4];
+  mul4_packet(KRt, p.x, p.y, p.z, p.w, ip);
+  const float d = ip[2];
+  if (d < pd.min_d || d > pd.max_d) return false;
+  const float inv = 1.0f / d;
+  x = (int)roundf(ip[0] * inv);
+  y = (int)roundf(ip[1] * inv);
+  if (!(x >= 0 && x < pd.cols && y >= 0 && y < pd.rows)) return false;
+  key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
+  return true;
+}
+
 // the z-buffer cell and key of cloud point i (p) under KRt, or cell -1 when the point
 // falls outside the pair's depth range or image (or off the sampled cells, sub > 1)
 template <int M>
@@ -934,16 +988,11 @@ __device__ __forceinline__ void zproject_point(const PairDev &pd, const float *K
       key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
     }
   } else {
-    float ip[4];
-    mul4_packet(KRt, p.x, p.y, p.z, p.w, ip);
-    const float d = ip[2];
-    if (!(d < pd.min_d || d > pd.max_d)) {
-      const float inv = 1.0f / d;
-      const int x = (int)roundf(ip[0] * inv), y = (int)roundf(ip[1] * inv);
-      if (x >= 0 && x < cols && y >= 0 && y < rows && (sub == 1 || (x % sub == 0 && y % sub == 0))) {
-        cell = y * cols + x;
-        key = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)i;
-      }
+    int x, y;
+    unsigned long long k;
+    if (pinhole_project(pd, KRt, p, i, x, y, k) && (sub == 1 || (x % sub == 0 && y % sub == 0))) {
+      cell = y * cols + x;
+      key = k;
     }
   }
 }
@@ -1309,47 +1358,40 @@ __device__ int window_search(const FrameDev &fr, const FrameDev &fc, const int *
 
 }  // namespace
 
-// Fused correspondence + linearisation over the projector image of each pair.
-// Reads and resets the z-buffer of the reference projection.
-extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_corr_lin(
-    const PairDev *__restrict__ Pp, const FrameDev *__restrict__ F, const int *__restrict__ blk_pair, AlignParamsDev ap,
-    int store_pairs) {
-  const int g = xcd_tile(blockIdx.x, gridDim.x);
-  const PairDev &pd = Pp[blk_pair[g]];
-  const FrameDev &fr = F[pd.ref];
-  const FrameDev &fc = F[pd.cur];
-  // pd.sub > 1 (the start probe of the learned-start aligner) evaluates only the pixels of
-  // every sub-th row and column: linear index p runs over that grid
-  const int sub = pd.sub, gcols = (pd.cols + sub - 1) / sub;
-  const int P = ((pd.rows + sub - 1) / sub) * gcols;
-  const int t = g - pd.tile_base;
-  const int base = t * kTile;
-  float T[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) T[k] = *gview(pd.state + kSlotTinv + k);
-  const bool multi = ap.row_region != 0 || ap.col_region != 0;
-  Acc2 acc[kVals];
-#pragma unroll
-  for (int v = 0; v < kVals; ++v) acc[v] = 0.0f;
-  if (!multi) {
-    // all loads of the thread's four pixels are issued before any of them is used
+// The correspondences and linearisation terms of one kDTH x kDTW tile t of a pair's (sampled)
+// projector image, accumulated into the thread's slots. Thread x of the block takes tile column
+// x & 63 and the tile rows (x >> 6) + 4 k: a wavefront reads 64 consecutive pixels of a row. The
+// reference index of a pixel comes from the z-buffer key: in global memory (Lds = false; the cell
+// is reset for the next projection) or in the block's LDS tile (Lds = true, cell = tile offset).
+// Both variants run the same arithmetic in the same order, so they give the same partial sums.
+template <bool Lds>
+__device__ __forceinline__ void corr_lin_tile(const PairDev &pd, const FrameDev &fr, const FrameDev &fc, const float *T,
+                                              const AlignParamsDev &ap, int store_pairs, int t,
+                                              const unsigned long long *zl, Acc2 *acc) {
+  const int sub = pd.sub;
+  const int GR = (pd.rows + sub - 1) / sub, GC = (pd.cols + sub - 1) / sub;
+  const int ty = t / pd.dtiles_x, tx = t - ty * pd.dtiles_x;
+  const int lc = threadIdx.x & 63, lr = threadIdx.x >> 6;
+  const int gc = tx * kDTW + lc;
+#pragma unroll 1
+  for (int grp = 0; grp < kDTH / 16; ++grp) {
+    // all loads of the thread's four pixels of this group are issued before any of them is used
     int ri[4], ci[4], pix[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int p = base + k * kThreads + threadIdx.x;
-      if (sub == 1) {
-        pix[k] = p < P ? p : -1;
-      } else {
-        const int gr = p / gcols;
-        pix[k] = p < P ? gr * sub * pd.cols + (p - gr * gcols) * sub : -1;
-      }
-      const unsigned long long key = pix[k] >= 0 ? *gview(pd.zbuf + pix[k]) : kZInit;
+      const int lrow = grp * 16 + k * 4 + lr;
+      const int gr = ty * kDTH + lrow;
+      pix[k] = gr < GR && gc < GC ? (gr * sub) * pd.cols + gc * sub : -1;
+      unsigned long long key = kZInit;
+      if (pix[k] >= 0) key = Lds ? zl[lrow * kDTW + lc] : *gview(pd.zbuf + pix[k]);
       ri[k] = key == kZInit ? -1 : (int)(unsigned int)(key & 0xffffffffull);
       ci[k] = pix[k] >= 0 ? *gview(pd.cur_index + pix[k]) : -1;
     }
+    if (!Lds) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (pix[k] >= 0) *gmut(pd.zbuf + pix[k]) = kZInit;
+      for (int k = 0; k < 4; ++k)
+        if (pix[k] >= 0) *gmut(pd.zbuf + pix[k]) = kZInit;
+    }
     float4 cn[4], rn0[4], cp[4], rp0[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -1386,11 +1428,38 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
             ok[k] ? ((unsigned long long)(unsigned int)ci[k] << 32) | (unsigned int)ri[k] : ~0ull;
       if (ok[k]) lin_terms(rp[k], rn[k], cp[k], cn[k], om[k], ap, acc);
     }
+  }
+}
+
+// Fused correspondence + linearisation over the projector image of each pair, one block per
+// kDTH x kDTW tile (blk_dpair maps blocks to pairs). Reads and resets the z-buffer of the
+// reference projection.
+extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_corr_lin(
+    const PairDev *__restrict__ Pp, const FrameDev *__restrict__ F, const int *__restrict__ blk_dpair,
+    AlignParamsDev ap, int store_pairs) {
+  const int g = xcd_tile(blockIdx.x, gridDim.x);
+  const PairDev &pd = Pp[blk_dpair[g]];
+  const FrameDev &fr = F[pd.ref];
+  const FrameDev &fc = F[pd.cur];
+  const int t = g - pd.dtile_base;
+  float T[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) T[k] = *gview(pd.state + kSlotTinv + k);
+  const bool multi = ap.row_region != 0 || ap.col_region != 0;
+  Acc2 acc[kVals];
+#pragma unroll
+  for (int v = 0; v < kVals; ++v) acc[v] = 0.0f;
+  if (!multi) {
+    corr_lin_tile<false>(pd, fr, fc, T, ap, store_pairs, t, nullptr, acc);
   } else {
-    for (int k = 0; k < 4; ++k) {
-      const int q = base + k * kThreads + threadIdx.x;
-      if (q >= P) break;
-      const int r = (q / gcols) * sub, c = (q - (q / gcols) * gcols) * sub;
+    const int sub = pd.sub;
+    const int GR = (pd.rows + sub - 1) / sub, GC = (pd.cols + sub - 1) / sub;
+    const int ty = t / pd.dtiles_x, tx = t - ty * pd.dtiles_x;
+    const int gc = tx * kDTW + (threadIdx.x & 63);
+    for (int lrow = threadIdx.x >> 6; lrow < kDTH; lrow += 4) {
+      const int gr = ty * kDTH + lrow;
+      if (gr >= GR || gc >= GC) continue;
+      const int r = gr * sub, c = gc * sub;
       const int p = r * pd.cols + c;
       const unsigned long long key = pd.zbuf[p];
       pd.zbuf[p] = kZInit;
@@ -1412,6 +1481,162 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
     }
   }
   block_partial(acc, pd.partials + (size_t)t * kPartial);
+}
+
+// ---- tile-binned projection fused with the correspondences and the linearisation ----------
+// The reference cloud's z-buffer of a pair (PinholePointProjector::project,
+// pinholepointprojector.cpp:33-66) is built per fused tile in LDS: k_bin lists for every fused
+// tile the source tiles of the reference frame whose projection may reach it, and one
+// k_proj_corr_lin block per fused tile projects exactly those points into its LDS z-buffer with
+// the 64-bit minimum on (depth bits << 32 | index), then runs k_corr_lin's correspondence and
+// linearisation code on it. Every point that lands in the tile comes from a listed source tile
+// and the minimum does not depend on the order, so the z-buffer, the correspondences and the
+// partial sums are those of k_zproject + k_corr_lin, bit for bit.
+
+// clears the fused tiles' list counts of all pairs (blk_fpair: one block per 256 fused tiles)
+extern "C" __global__ __launch_bounds__(kThreads) void k_bin_clear(const PairDev *__restrict__ Pp, int num_pairs) {
+  const int i = blockIdx.y;
+  if (i >= num_pairs) return;
+  const PairDev &pd = Pp[i];
+  for (int t = blockIdx.x * kThreads + threadIdx.x; t < pd.n_ftiles; t += gridDim.x * kThreads) pd.bins[t] = 0;
+}
+
+// One thread per source tile of each pair's reference frame: the bounding box of the tile's
+// projection under the pair's KRt, from the eight corners of its frustum (the pixel corners of
+// the tile at its minimum and maximum depth: the tile's points lie in their convex hull, and a
+// perspective projection maps the hull into the hull of the projected corners when every corner
+// lies in front of the camera), widened by 2 pixels; the tile is appended to every fused tile
+// the box touches. A tile whose frustum may reach the camera plane goes to every fused tile.
+extern "C" __global__ __launch_bounds__(kThreads) void k_bin(const PairDev *__restrict__ Pp,
+                                                            const FrameDev *__restrict__ F,
+                                                            const int *__restrict__ blk_spair, int bin_cap) {
+  const int g = blockIdx.x;
+  const PairDev &pd = Pp[blk_spair[g]];
+  const FrameDev &fr = F[pd.ref];
+  const int sx = stiles_x(fr.cols);
+  const int s = (g - pd.sblk_base) * kThreads + threadIdx.x;
+  if (s >= stiles(fr.rows, fr.cols)) return;
+  const gvec2 dr = *gview((const gvec2 *)(fr.stile_d + s));
+  if (!(dr.x <= dr.y)) return;  // no valid pixel
+  float KRt[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) KRt[k] = *gview(pd.state + kSlotKRt + k);
+  const int ty = s / sx, tx = s - ty * sx;
+  const int c0 = tx * kSTW, c1 = min(c0 + kSTW, fr.cols) - 1, r0 = ty * kSTH, r1 = min(r0 + kSTH, fr.rows) - 1;
+  float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY, zmin = INFINITY, zmax = -INFINITY;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const float4 p = unproject_point(fr.iKRt, (k & 1) ? c1 : c0, (k & 2) ? r1 : r0, (k & 4) ? dr.y : dr.x);
+    float ip[4];
+    mul4_packet(KRt, p.x, p.y, p.z, 1.0f, ip);
+    zmin = fminf(zmin, ip[2]);
+    zmax = fmaxf(zmax, ip[2]);
+    const float u = ip[0] / ip[2], v = ip[1] / ip[2];
+    umin = fminf(umin, u);
+    umax = fmaxf(umax, u);
+    vmin = fminf(vmin, v);
+    vmax = fmaxf(vmax, v);
+  }
+  // depth range with a relative margin for the rounding of the points' own depths
+  if (zmax < pd.min_d * (1.0f - 1e-4f) || zmin > pd.max_d * (1.0f + 1e-4f)) return;
+  int x0 = 0, x1 = pd.cols - 1, y0 = 0, y1 = pd.rows - 1;
+  if (zmin > 0.1f * pd.min_d && umin >= -1e6f && umax <= 1e6f && vmin >= -1e6f && vmax <= 1e6f) {
+    x0 = max(x0, (int)floorf(umin) - 2);
+    x1 = min(x1, (int)ceilf(umax) + 2);
+    y0 = max(y0, (int)floorf(vmin) - 2);
+    y1 = min(y1, (int)ceilf(vmax) + 2);
+    if (x0 > x1 || y0 > y1) return;
+  }
+  int *cnt = pd.bins;
+  __attribute__((address_space(1))) int *lst = gmut(pd.bins + pd.n_ftiles);
+  for (int fy = y0 / kFH; fy <= y1 / kFH; ++fy)
+    for (int fx = x0 / kDTW; fx <= x1 / kDTW; ++fx) {
+      const int ft = fy * pd.dtiles_x + fx;
+      const int slot = atomicAdd(cnt + ft, 1);
+      if (slot < bin_cap) lst[ft * kBinCap + slot] = s;
+    }
+}
+
+// One block per fused tile (blk_fpair, in the XCD-aware order of k_corr_lin): the LDS z-buffer
+// of the tile's kFH x kDTW pixels from the listed source tiles (all of them when the list
+// overflowed), then the kFH / kDTH correspondence tiles' correspondences and linearisation terms,
+// one block partial each. Resets the tile's list count for the next iteration.
+extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_proj_corr_lin(
+    const PairDev *__restrict__ Pp, const FrameDev *__restrict__ F, const int *__restrict__ blk_fpair,
+    AlignParamsDev ap, int store_pairs, int bin_cap) {
+  __shared__ unsigned long long zl[kFH * kDTW];
+  __shared__ int sl[kBinCap];
+  __shared__ int scount;
+  const int g = xcd_tile(blockIdx.x, gridDim.x);
+  const PairDev &pd = Pp[blk_fpair[g]];
+  const FrameDev &fr = F[pd.ref];
+  const FrameDev &fc = F[pd.cur];
+  const int ft = g - pd.ftile_base;
+  const int fy = ft / pd.dtiles_x, fx = ft - fy * pd.dtiles_x;
+  for (int i = threadIdx.x; i < kFH * kDTW; i += kThreads) zl[i] = kZInit;
+  if (threadIdx.x == 0) {
+    scount = *gview(pd.bins + ft);
+    *gmut(pd.bins + ft) = 0;
+  }
+  __syncthreads();
+  const int cnt = scount;
+  const bool all = cnt > bin_cap;
+  for (int i = threadIdx.x; i < (all ? 0 : cnt); i += kThreads) sl[i] = *gview(pd.bins + pd.n_ftiles + ft * kBinCap + i);
+  __syncthreads();
+  {
+    float KRt[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) KRt[k] = *gview(pd.state + kSlotKRt + k);
+    const int sx = stiles_x(fr.cols);
+    const int nsrc = all ? stiles(fr.rows, fr.cols) : cnt;
+    const int npts = nsrc * (kSTH * kSTW);
+    const int xlo = fx * kDTW, ylo = fy * kFH;
+    for (int base = 0; base < npts; base += 4 * kThreads) {
+      int idx[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = base + u * kThreads + threadIdx.x;
+        idx[u] = -1;
+        if (j < npts) {
+          const int w = j & (kSTH * kSTW - 1), si = j / (kSTH * kSTW);
+          const int st = all ? si : sl[si];
+          const int sty = st / sx, stx = st - sty * sx;
+          const int r = sty * kSTH + w / kSTW, c = stx * kSTW + (w & (kSTW - 1));
+          if (r < fr.rows && c < fr.cols) idx[u] = *gview(fr.index + r * fr.cols + c);
+        }
+      }
+      float4 p[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (idx[u] >= 0) p[u] = gload4(fr.pts + idx[u]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int x, y;
+        unsigned long long key;
+        if (idx[u] >= 0 && pinhole_project(pd, KRt, p[u], idx[u], x, y, key)) {
+          const int lx = x - xlo, ly = y - ylo;
+          if (lx >= 0 && lx < kDTW && ly >= 0 && ly < kFH) atomicMin(&zl[ly * kDTW + lx], key);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  float T[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) T[k] = *gview(pd.state + kSlotTinv + k);
+  const int nty = dtiles_y(pd.rows);
+#pragma unroll 1
+  for (int sub_t = 0; sub_t < kFH / kDTH; ++sub_t) {
+    const int ty = fy * (kFH / kDTH) + sub_t;
+    if (ty >= nty) break;
+    Acc2 acc[kVals];
+#pragma unroll
+    for (int v = 0; v < kVals; ++v) acc[v] = 0.0f;
+    const int t = ty * pd.dtiles_x + fx;
+    corr_lin_tile<true>(pd, fr, fc, T, ap, store_pairs, t, zl + sub_t * kDTH * kDTW, acc);
+    block_partial(acc, pd.partials + (size_t)t * kPartial);
+    __syncthreads();  // block_partial's LDS is reused by the next tile
+  }
 }
 
 // Linearizer::update over stored pairs at the transform in state slot `slot`.
@@ -1471,20 +1696,20 @@ namespace {
 // tiles part, part + kSolveLanes, ... in order with eight loads in flight, then a fixed tree
 // over the kSolveLanes sums. The order is fixed, so every call on the same partials gives
 // the same bits.
-__device__ void reduce_partials(const PairDev &pd, double *tot /* shared, kPartial */) {
+__device__ void reduce_partials(const PairDev &pd, int nparts, double *tot /* shared, kPartial */) {
   const int v = threadIdx.x / kSolveLanes, part = threadIdx.x - v * kSolveLanes;
   double s = 0.0;
   if (v < kVals) {
     const double *__restrict__ src = pd.partials + v;
     int i = part;
-    for (; i + 7 * kSolveLanes < pd.n_tiles; i += 8 * kSolveLanes) {
+    for (; i + 7 * kSolveLanes < nparts; i += 8 * kSolveLanes) {
       double x[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) x[u] = src[(size_t)(i + u * kSolveLanes) * kPartial];
 #pragma unroll
       for (int u = 0; u < 8; ++u) s += x[u];
     }
-    for (; i < pd.n_tiles; i += kSolveLanes) s += src[(size_t)i * kPartial];
+    for (; i < nparts; i += kSolveLanes) s += src[(size_t)i * kPartial];
   }
   // the slot's lanes are consecutive lanes of one wavefront
 #pragma unroll
@@ -1496,12 +1721,13 @@ __device__ void reduce_partials(const PairDev &pd, double *tot /* shared, kParti
 // One workgroup per pair: the FP64 totals by all its threads, then on thread 0 H + lambda I,
 // LDLT and the update of the SE(3) state.
 // mode 0: last inner iteration (updates T, Tinv, KRt); 1: inner iteration (updates invT_inner);
-// 2: statistics linearisation (stores the totals only).
+// 2: statistics linearisation (stores the totals only). dtiles: the partials come from
+// k_corr_lin's n_dtiles tiles (1) or from k_lin_pairs' n_tiles tiles (0).
 extern "C" __global__ __launch_bounds__(kSolveThreads) void k_solve(const PairDev *__restrict__ Pp, float lambda,
-                                                                   int mode, int from_inner) {
+                                                                   int mode, int from_inner, int dtiles) {
   const PairDev &pd = Pp[blockIdx.x];
   __shared__ double tot[kPartial];
-  reduce_partials(pd, tot);
+  reduce_partials(pd, dtiles ? pd.n_dtiles : pd.n_tiles, tot);
   __syncthreads();
   if (threadIdx.x != 0) return;
   double *res = pd.result + (mode == 2 ? kPartial : 0);
