@@ -303,8 +303,8 @@ int nicp_batch_set_autotune(nicp_batch *batch, int enable);
 /* forces strategy 0, 1 or 2 on every later align call of the batch (2 where it applies, else 0);
  * -1 returns to the autotuner's or the default choice */
 int nicp_batch_set_projection(nicp_batch *batch, int strategy);
-/* list capacity of the tile-binned projection, 0..128 source tiles per projector tile (default
- * 128); a projector tile reached by more scans every source tile of its reference frame (same
+/* list capacity of the tile-binned projection, 0..256 source tiles per projector tile (default
+ * 256); a projector tile reached by more scans every source tile of its reference frame (same
  * results, slower). */
 int nicp_batch_set_bin_capacity(nicp_batch *batch, int capacity);
 /* the tuned shapes: choice per shape, ms = 3 per shape (average ms per iteration of the projection

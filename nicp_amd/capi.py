@@ -533,7 +533,7 @@ class Batch:
         _check(lib().nicp_batch_set_projection(self.h, int(strategy)))
 
     def set_bin_capacity(self, capacity):
-        """Source tiles a projector tile of the tile-binned projection lists (0..128)."""
+        """Source tiles a projector tile of the tile-binned projection lists (0..256)."""
         _check(lib().nicp_batch_set_bin_capacity(self.h, int(capacity)))
 
     def autotune_state(self):

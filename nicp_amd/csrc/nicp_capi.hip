@@ -147,7 +147,7 @@ static const char *kKernelNames[KID_N] = {"k_count", "k_scan", "k_unproject", "k
                                           "k_zproject(reference)", "k_corr_lin", "k_solve", "k_lin_pairs",
                                           "k_solve(statistics)", "k_nn_query", "k_net_input", "initnet_forward",
                                           "start_probe", "k_integral_frame", "k_solution_stats", "k_src_tiles",
-                                          "k_bin", "k_proj_corr_lin"};
+                                          "k_sbox", "k_proj_corr_lin"};
 
 namespace {
 struct ProfEntry {
@@ -308,14 +308,14 @@ struct nicp_batch {
   int *d_blk_frame = nullptr, *d_xblk_frame = nullptr, *d_yblk_frame = nullptr;
   int total_tiles = 0, total_xblk = 0, total_yblk = 0;
   // align workspace
-  DevBuf pairs_tbl, maps, zbuf, cur_index, pairs_buf, state, partials, result, bins;
+  DevBuf pairs_tbl, maps, zbuf, cur_index, pairs_buf, state, partials, result, sbox;
   GraphCache graph;
   GraphCache conv_graph;
   DevBuf synth_buf;  // render job tables of nicp_batch_render_synthetic
   // projection strategy (nicp_batch_set_projection): -1 the autotuner's choice when it is on,
   // else the tile-binned projection where it applies; 0, 1, 2 forced
   int projection = -1;
-  int bin_cap = kBinCap;  // list capacity of the tile-binned projection (nicp_batch_set_bin_capacity)
+  int list_cap = kListCap;  // list capacity of the tile-binned projection (nicp_batch_set_bin_capacity)
   // autotuner of the reference projection (nicp_batch_set_autotune): per plan shape, the
   // strategy chosen from the timed iterations of the shape's first call
   int autotune = 0;
@@ -512,13 +512,14 @@ struct PairPlan {
   int total_pix_tiles = 0, total_rtiles = 0, total_ctiles = 0, total_dtiles = 0, total_sblk = 0, total_ftiles = 0;
   size_t total_pix = 0;
   size_t total_parts = 0;       // partial slots: max(n_tiles, n_dtiles) per pair
-  size_t total_bins = 0;        // ints of the fused tiles' lists: n_ftiles (1 + kBinCap) per pair
+  size_t total_sbox = 0;        // source-tile boxes (int4) of all pairs' reference frames
+  std::vector<int> n_stiles;    // source tiles of each pair's reference frame
   // device copies of blk_dpair, blk_spair, blk_fpair (set by prepare_align)
   int *d_dpair = nullptr, *d_spair = nullptr, *d_fpair = nullptr;
   // the tile-binned projection applies: pinhole pairs, every current index image the frame's own,
   // every reference frame with source-tile depth ranges, no subsampling (set by batch_align)
   bool fused_ok = false;
-  int bin_cap = kBinCap;        // source tiles a fused tile lists before it scans them all
+  int list_cap = kListCap;      // source tiles a fused tile lists before it scans them all
   // > 0: pairs q and q + twin share both frames (the two starts of the start probe), and
   // one k_zproject_twin pass over the first half's reference tiles projects both
   int twin = 0;
@@ -532,7 +533,6 @@ void plan_pixel_tiles(PairPlan &plan) {
   plan.blk_dpair.clear();
   plan.blk_fpair.clear();
   plan.total_parts = 0;
-  plan.total_bins = 0;
   for (size_t i = 0; i < plan.pairs.size(); ++i) {
     PairDev &pd = plan.pairs[i];
     const int GR = (pd.rows + pd.sub - 1) / pd.sub, GC = (pd.cols + pd.sub - 1) / pd.sub;
@@ -547,7 +547,6 @@ void plan_pixel_tiles(PairPlan &plan) {
     pd.n_ftiles = ftiles_y(GR) * pd.dtiles_x;
     pd.ftile_base = (int)plan.blk_fpair.size();
     plan.blk_fpair.insert(plan.blk_fpair.end(), pd.n_ftiles, (int)i);
-    plan.total_bins += (size_t)pd.n_ftiles * (1 + kBinCap);
   }
   plan.total_pix_tiles = (int)plan.blk_pair.size();
   plan.total_dtiles = (int)plan.blk_dpair.size();
@@ -562,6 +561,8 @@ void make_plan(const std::vector<FrameDev> &frames, int num_pairs, const int *re
   plan.blk_ref.clear();
   plan.blk_cur.clear();
   plan.blk_spair.clear();
+  plan.n_stiles.clear();
+  plan.total_sbox = 0;
   size_t pix = 0;
   for (int i = 0; i < num_pairs; ++i) {
     PairDev &pd = plan.pairs[i];
@@ -577,10 +578,14 @@ void make_plan(const std::vector<FrameDev> &frames, int num_pairs, const int *re
     pd.n_ctiles = (fc.rows * fc.cols + kTile - 1) / kTile;
     pd.ctile_base = (int)plan.blk_cur.size();
     for (int t = 0; t < pd.n_ctiles; ++t) plan.blk_cur.push_back(i);
-    pd.n_sblk = (stiles(fr.rows, fr.cols) + kThreads - 1) / kThreads;
+    const int ns = stiles(fr.rows, fr.cols);
+    pd.n_sblk = (ns + kThreads - 1) / kThreads;
     pd.sblk_base = (int)plan.blk_spair.size();
     plan.blk_spair.insert(plan.blk_spair.end(), pd.n_sblk, i);
-    pd.bins = nullptr;
+    // box offset of this pair, turned into a pointer by prepare_align
+    pd.sbox = (int4 *)(uintptr_t)plan.total_sbox;
+    plan.total_sbox += (size_t)ns;
+    plan.n_stiles.push_back(ns);
     for (int k = 0; k < 9; ++k) pd.K[k] = cams[i]->K[k];
     pd.min_d = cams[i]->min_distance;
     pd.max_d = cams[i]->max_distance;
@@ -624,7 +629,7 @@ void plan_batch_frames(const std::vector<FrameDev> &frames, PairPlan &plan) {
 
 struct AlignBuffers {
   DevBuf *pairs_tbl, *maps, *zbuf, *cur_index, *pairs_buf, *state, *partials, *result;
-  DevBuf *bins = nullptr;  // fused tiles' source lists (batches: the tile-binned projection)
+  DevBuf *sbox = nullptr;  // source-tile boxes of the tile-binned projection (batches)
 };
 
 // allocates/assigns the per-pair buffers and uploads tables and the initial state
@@ -637,8 +642,8 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
   CHK(B.state->reserve((size_t)np * kStateFloats * sizeof(float)));
   CHK(B.partials->reserve(std::max<size_t>(plan.total_parts, 1) * kPartial * sizeof(double)));
   CHK(B.result->reserve((size_t)np * kResultDoubles * sizeof(double)));
-  if (B.bins && plan.fused_ok) CHK(B.bins->reserve(std::max<size_t>(plan.total_bins, 1) * sizeof(int)));
-  size_t tile_part = 0, bin_off = 0;
+  if (B.sbox && plan.fused_ok) CHK(B.sbox->reserve(std::max<size_t>(plan.total_sbox, 1) * sizeof(int4)));
+  size_t tile_part = 0;
   std::vector<float> st((size_t)np * kStateFloats, 0.0f);
   for (int i = 0; i < np; ++i) {
     PairDev &pd = plan.pairs[i];
@@ -650,8 +655,15 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
     pd.partials = B.partials->as<double>() + tile_part * kPartial;
     tile_part += (size_t)std::max(pd.n_tiles, pd.n_dtiles);
     pd.result = B.result->as<double>() + (size_t)i * kResultDoubles;
-    pd.bins = B.bins && plan.fused_ok ? B.bins->as<int>() + bin_off : nullptr;
-    bin_off += (size_t)pd.n_ftiles * (1 + kBinCap);
+    if (B.sbox && plan.fused_ok) {
+      // the pair's boxes: one per source tile of its reference frame, inside the buffer
+      const size_t off = (size_t)(uintptr_t)pd.sbox;
+      if (off + (size_t)plan.n_stiles[i] > plan.total_sbox || plan.total_sbox * sizeof(int4) > B.sbox->bytes)
+        return set_err(NICP_ERR_STATE, "source-tile boxes of pair %d outside their buffer", i);
+      pd.sbox = B.sbox->as<int4>() + off;
+    } else {
+      pd.sbox = nullptr;
+    }
     // initial state: AlignerProjective::align, alignerprojective.cpp:36-54
     float *S = &st[(size_t)i * kStateFloats];
     float T[16];
@@ -733,7 +745,6 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
   const bool only_fused = fused_ok && !tune && zstrategy == 2;
   (void)hipGetLastError();
   if (!only_fused || plan.total_ctiles) LAUNCH(pf, KID_ZFILL, k_zfill, dim3(plan.total_pix_tiles), tb, s, d_pairs, d_blk_pair);
-  if (fused_ok) LAUNCH(pf, KID_BIN, k_bin_clear, dim3(1, np), tb, s, d_pairs, np);
   if (plan.total_ctiles) {
     if (sph) LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject_sph, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1, zstrategy);
     else LAUNCH(pf, KID_ZPROJECT_CUR, k_zproject, dim3(plan.total_ctiles), tb, s, d_pairs, d_frames, d_blk_cur, 1, zstrategy);
@@ -754,10 +765,9 @@ int launch_align(Prof *pf, hipStream_t s, const PairPlan &plan, const PairDev *d
       HIPCHK(hipEventRecord(e0, s));
     }
     if (zs == 2) {
-      if (plan.total_sblk)
-        LAUNCH(pf, KID_BIN, k_bin, dim3(plan.total_sblk), tb, s, d_pairs, d_frames, plan.d_spair, plan.bin_cap);
+      if (plan.total_sblk) LAUNCH(pf, KID_BIN, k_sbox, dim3(plan.total_sblk), tb, s, d_pairs, d_frames, plan.d_spair);
       LAUNCH(pf, KID_PROJ_CORR_LIN, k_proj_corr_lin, dim3(plan.total_ftiles), tb, s, d_pairs, d_frames, plan.d_fpair, a,
-             store, plan.bin_cap);
+             store, plan.list_cap);
     } else {
       if (plan.total_rtiles) {
         if (sph)
@@ -1640,7 +1650,7 @@ extern "C" int nicp_batch_destroy(nicp_batch *b) {
   b->graph.reset();
   b->conv_graph.reset();
   DevBuf *bufs[] = {&b->pairs_tbl, &b->maps,     &b->zbuf,   &b->cur_index, &b->pairs_buf,
-                    &b->state,     &b->partials, &b->result, &b->bins,      &b->synth_buf};
+                    &b->state,     &b->partials, &b->result, &b->sbox,      &b->synth_buf};
   for (DevBuf *x : bufs) x->release();
   (void)hipFree(b->arena);
   delete b;
@@ -1724,9 +1734,9 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
     plan.fused_ok = pd.model == NICP_PINHOLE && pd.cur_own && b->frames[pd.ref].stile_d && b->frames[pd.ref].index;
   }
   const bool fused = plan.fused_ok && ap->row_search_region == 0 && ap->col_search_region == 0;
-  plan.bin_cap = b->bin_cap;
+  plan.list_cap = b->list_cap;
   AlignBuffers B{&b->pairs_tbl, &b->maps, &b->zbuf, &b->cur_index, &b->pairs_buf, &b->state, &b->partials, &b->result,
-                 &b->bins};
+                 &b->sbox};
   int *d_bp, *d_br, *d_bc;
   PairDev *d_pairs;
   CHK(prepare_align(ctx->stream, plan, B, init_guesses16, &d_bp, &d_br, &d_bc, &d_pairs));
@@ -1798,7 +1808,7 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
     // uploaded before each launch, so only the argument addresses are baked into the graph)
     key.push_back(plan.total_sblk);
     key.push_back(plan.total_ftiles);
-    key.push_back(plan.bin_cap);
+    key.push_back(plan.list_cap);
     for (const void *ptr : {(const void *)d_pairs, (const void *)d_bp, (const void *)d_br, (const void *)d_bc,
                             (const void *)plan.d_dpair, (const void *)plan.d_spair, (const void *)plan.d_fpair,
                             (const void *)b->d_frames}) {
@@ -1896,8 +1906,8 @@ extern "C" int nicp_batch_autotune_read(nicp_batch *b, int capacity, int *num_sh
 
 extern "C" int nicp_batch_set_bin_capacity(nicp_batch *b, int capacity) {
   if (!b) return set_err(NICP_ERR_INVALID, "batch is null");
-  if (capacity < 0 || capacity > kBinCap) return set_err(NICP_ERR_INVALID, "capacity must lie in [0, %d]", kBinCap);
-  b->bin_cap = capacity;
+  if (capacity < 0 || capacity > kListCap) return set_err(NICP_ERR_INVALID, "capacity must lie in [0, %d]", kListCap);
+  b->list_cap = capacity;
   return NICP_OK;
 }
 

@@ -25,14 +25,17 @@ constexpr int kDTH = 16, kDTW = 64;
 __host__ __device__ inline int dtiles_x(int cols) { return (cols + kDTW - 1) / kDTW; }
 __host__ __device__ inline int dtiles_y(int rows) { return (rows + kDTH - 1) / kDTH; }
 
-// Tile-binned projection (k_bin, k_proj_corr_lin). Source tiles: kSTH x kSTW pixel blocks of a
+// Tile-binned projection (k_sbox, k_proj_corr_lin). Source tiles: kSTH x kSTW pixel blocks of a
 // reference frame, each with the depth range of its valid pixels. Fused tiles: kFH x kDTW pixels
 // of the projector image, kFH / kDTH correspondence tiles each, z-buffered in LDS; a fused tile
-// lists up to a capacity of at most kBinCap source tiles whose projection may reach it (more:
+// lists up to a capacity of at most kListCap source tiles whose projection may reach it (more:
 // every source tile of the reference frame is scanned).
 constexpr int kSTH = 16, kSTW = 32;
+constexpr int kBinMargin = 2;  // pixels added to each side of a projected source-tile box
+constexpr int kPhaseB = kSTH * kSTW / 64;  // source pixels per lane of a wavefront's source tile
 constexpr int kFH = 64;
-constexpr int kBinCap = 128;
+constexpr int kListCap = 256;
+static_assert(kSTH * kSTW % 64 == 0 && kListCap <= kThreads, "k_proj_corr_lin's source-tile loop");
 static_assert(kFH % kDTH == 0, "a fused tile holds whole correspondence tiles");
 __host__ __device__ inline int stiles_x(int cols) { return (cols + kSTW - 1) / kSTW; }
 __host__ __device__ inline int stiles(int rows, int cols) { return ((rows + kSTH - 1) / kSTH) * stiles_x(cols); }
@@ -83,8 +86,8 @@ struct PairDev {
   int tile_base, n_tiles;          // per-pixel blocks over the projector image
   int dtile_base, n_dtiles, dtiles_x;  // kDTH x kDTW tiles of the (sampled) projector image
   int ftile_base, n_ftiles;        // kFH x kDTW fused tiles of the projector image
-  int sblk_base, n_sblk;           // k_bin blocks over the reference frame's source tiles
-  int *bins;                       // n_ftiles counts, then kBinCap source tiles per fused tile
+  int sblk_base, n_sblk;           // k_sbox blocks over the reference frame's source tiles
+  int4 *sbox;                      // per source tile of the reference frame: its fused-tile box
   int rtile_base, n_rtiles;        // blocks over the reference cloud capacity
   int ctile_base, n_ctiles;        // blocks over the current cloud capacity
   int cur_own;                     // 1: cur_index is the current frame's own index image

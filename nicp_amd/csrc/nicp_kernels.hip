@@ -53,6 +53,7 @@ __device__ __forceinline__ void gstore4(float4 *p, float4 v) {
   *gmut((gvec4 *)p) = gvec4{v.x, v.y, v.z, v.w};
 }
 typedef float gvec2 __attribute__((ext_vector_type(2)));
+typedef int gint4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float depth_of(uint16_t raw, float scale) { return raw ? scale * (float)raw : 0.0f; }
 __device__ __forceinline__ bool depth_valid(float d, float mn, float mx) { return !(d == 0.0f || d < mn || d > mx); }
@@ -258,9 +259,9 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_unproject_sph(const Fra
 }
 
 // The depth range of the valid pixels of every kSTH x kSTW source tile of a pinhole frame (for
-// the tile-binned projection): one wavefront per tile, lane l covering 8 pixels of tile row
-// l / 4; an empty tile stores (+inf, -inf). Block t of a frame (its 1024-pixel tile map) takes
-// source tiles 4 t + wave, 4 t + wave + 4 n_tiles, ...
+// the tile-binned projection): one wavefront per tile, lane l covering its pixels l, l + 64, ...
+// in row-major order; an empty tile stores (+inf, -inf). Block t of a frame (its 1024-pixel tile
+// map) takes source tiles 4 t + wave, 4 t + wave + 4 n_tiles, ...
 extern "C" __global__ __launch_bounds__(kThreads) void k_src_tiles(const FrameDev *__restrict__ F,
                                                                   const int *__restrict__ blk_frame) {
   const int g = blockIdx.x;
@@ -270,18 +271,16 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_src_tiles(const FrameDe
   const int sx = stiles_x(fd.cols), ns = stiles(fd.rows, fd.cols);
   for (int s = 4 * (g - fd.tile_base) + wave; s < ns; s += 4 * fd.n_tiles) {
     const int ty = s / sx, tx = s - ty * sx;
-    const int r = ty * kSTH + (lane >> 2), c0 = tx * kSTW + (lane & 3) * 8;
     float lo = INFINITY, hi = -INFINITY;
-    if (r < fd.rows) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int c = c0 + u;
-        if (c < fd.cols) {
-          const float d = frame_depth(fd, r * fd.cols + c);
-          if (depth_valid(d, fd.min_d, fd.max_d)) {
-            lo = fminf(lo, d);
-            hi = fmaxf(hi, d);
-          }
+    for (int i = 0; i < (kSTH * kSTW + 63) / 64; ++i) {
+      const int w = i * 64 + lane;
+      const int r = ty * kSTH + w / kSTW, c = tx * kSTW + w % kSTW;
+      if (w < kSTH * kSTW && r < fd.rows && c < fd.cols) {
+        const float d = frame_depth(fd, r * fd.cols + c);
+        if (depth_valid(d, fd.min_d, fd.max_d)) {
+          lo = fminf(lo, d);
+          hi = fmaxf(hi, d);
         }
       }
     }
@@ -1485,87 +1484,76 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
 
 // ---- tile-binned projection fused with the correspondences and the linearisation ----------
 // The reference cloud's z-buffer of a pair (PinholePointProjector::project,
-// pinholepointprojector.cpp:33-66) is built per fused tile in LDS: k_bin lists for every fused
-// tile the source tiles of the reference frame whose projection may reach it, and one
-// k_proj_corr_lin block per fused tile projects exactly those points into its LDS z-buffer with
-// the 64-bit minimum on (depth bits << 32 | index), then runs k_corr_lin's correspondence and
-// linearisation code on it. Every point that lands in the tile comes from a listed source tile
-// and the minimum does not depend on the order, so the z-buffer, the correspondences and the
-// partial sums are those of k_zproject + k_corr_lin, bit for bit.
-
-// clears the fused tiles' list counts of all pairs (blk_fpair: one block per 256 fused tiles)
-extern "C" __global__ __launch_bounds__(kThreads) void k_bin_clear(const PairDev *__restrict__ Pp, int num_pairs) {
-  const int i = blockIdx.y;
-  if (i >= num_pairs) return;
-  const PairDev &pd = Pp[i];
-  for (int t = blockIdx.x * kThreads + threadIdx.x; t < pd.n_ftiles; t += gridDim.x * kThreads) pd.bins[t] = 0;
-}
+// pinholepointprojector.cpp:33-66) is built per fused tile in LDS: k_sbox bounds, for every source
+// tile of the reference frame, the fused tiles its projection may reach, and one k_proj_corr_lin
+// block per fused tile lists the source tiles whose bound holds it, projects exactly their points
+// into its LDS z-buffer with the 64-bit minimum on (depth bits << 32 | index), then runs
+// k_corr_lin's correspondence and linearisation code on it. Every point that lands in the tile
+// comes from a listed source tile and the minimum does not depend on the order, so the z-buffer,
+// the correspondences and the partial sums are those of k_zproject + k_corr_lin, bit for bit.
 
 // One thread per source tile of each pair's reference frame: the bounding box of the tile's
 // projection under the pair's KRt, from the eight corners of its frustum (the pixel corners of
 // the tile at its minimum and maximum depth: the tile's points lie in their convex hull, and a
 // perspective projection maps the hull into the hull of the projected corners when every corner
-// lies in front of the camera), widened by 2 pixels; the tile is appended to every fused tile
-// the box touches. A tile whose frustum may reach the camera plane goes to every fused tile.
-extern "C" __global__ __launch_bounds__(kThreads) void k_bin(const PairDev *__restrict__ Pp,
-                                                            const FrameDev *__restrict__ F,
-                                                            const int *__restrict__ blk_spair, int bin_cap) {
+// lies in front of the camera), widened by kBinMargin pixels, as the range of fused tiles
+// (fx0, fy0, fx1, fy1) it touches; empty (fx0 > fx1) when no point of the tile can project into
+// the image. A tile whose frustum may reach the camera plane covers every fused tile.
+extern "C" __global__ __launch_bounds__(kThreads) void k_sbox(const PairDev *__restrict__ Pp,
+                                                             const FrameDev *__restrict__ F,
+                                                             const int *__restrict__ blk_spair) {
   const int g = blockIdx.x;
   const PairDev &pd = Pp[blk_spair[g]];
   const FrameDev &fr = F[pd.ref];
   const int sx = stiles_x(fr.cols);
   const int s = (g - pd.sblk_base) * kThreads + threadIdx.x;
   if (s >= stiles(fr.rows, fr.cols)) return;
+  int4 box = make_int4(1, 1, 0, 0);
   const gvec2 dr = *gview((const gvec2 *)(fr.stile_d + s));
-  if (!(dr.x <= dr.y)) return;  // no valid pixel
-  float KRt[16];
+  if (dr.x <= dr.y) {  // the tile has valid pixels
+    float KRt[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) KRt[k] = *gview(pd.state + kSlotKRt + k);
-  const int ty = s / sx, tx = s - ty * sx;
-  const int c0 = tx * kSTW, c1 = min(c0 + kSTW, fr.cols) - 1, r0 = ty * kSTH, r1 = min(r0 + kSTH, fr.rows) - 1;
-  float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY, zmin = INFINITY, zmax = -INFINITY;
+    for (int k = 0; k < 16; ++k) KRt[k] = *gview(pd.state + kSlotKRt + k);
+    const int ty = s / sx, tx = s - ty * sx;
+    const int c0 = tx * kSTW, c1 = min(c0 + kSTW, fr.cols) - 1, r0 = ty * kSTH, r1 = min(r0 + kSTH, fr.rows) - 1;
+    float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY, zmin = INFINITY, zmax = -INFINITY;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const float4 p = unproject_point(fr.iKRt, (k & 1) ? c1 : c0, (k & 2) ? r1 : r0, (k & 4) ? dr.y : dr.x);
-    float ip[4];
-    mul4_packet(KRt, p.x, p.y, p.z, 1.0f, ip);
-    zmin = fminf(zmin, ip[2]);
-    zmax = fmaxf(zmax, ip[2]);
-    const float u = ip[0] / ip[2], v = ip[1] / ip[2];
-    umin = fminf(umin, u);
-    umax = fmaxf(umax, u);
-    vmin = fminf(vmin, v);
-    vmax = fmaxf(vmax, v);
-  }
-  // depth range with a relative margin for the rounding of the points' own depths
-  if (zmax < pd.min_d * (1.0f - 1e-4f) || zmin > pd.max_d * (1.0f + 1e-4f)) return;
-  int x0 = 0, x1 = pd.cols - 1, y0 = 0, y1 = pd.rows - 1;
-  if (zmin > 0.1f * pd.min_d && umin >= -1e6f && umax <= 1e6f && vmin >= -1e6f && vmax <= 1e6f) {
-    x0 = max(x0, (int)floorf(umin) - 2);
-    x1 = min(x1, (int)ceilf(umax) + 2);
-    y0 = max(y0, (int)floorf(vmin) - 2);
-    y1 = min(y1, (int)ceilf(vmax) + 2);
-    if (x0 > x1 || y0 > y1) return;
-  }
-  int *cnt = pd.bins;
-  __attribute__((address_space(1))) int *lst = gmut(pd.bins + pd.n_ftiles);
-  for (int fy = y0 / kFH; fy <= y1 / kFH; ++fy)
-    for (int fx = x0 / kDTW; fx <= x1 / kDTW; ++fx) {
-      const int ft = fy * pd.dtiles_x + fx;
-      const int slot = atomicAdd(cnt + ft, 1);
-      if (slot < bin_cap) lst[ft * kBinCap + slot] = s;
+    for (int k = 0; k < 8; ++k) {
+      const float4 p = unproject_point(fr.iKRt, (k & 1) ? c1 : c0, (k & 2) ? r1 : r0, (k & 4) ? dr.y : dr.x);
+      float ip[4];
+      mul4_packet(KRt, p.x, p.y, p.z, 1.0f, ip);
+      zmin = fminf(zmin, ip[2]);
+      zmax = fmaxf(zmax, ip[2]);
+      const float u = ip[0] / ip[2], v = ip[1] / ip[2];
+      umin = fminf(umin, u);
+      umax = fmaxf(umax, u);
+      vmin = fminf(vmin, v);
+      vmax = fmaxf(vmax, v);
     }
+    // depth range with a relative margin for the rounding of the points' own depths
+    if (!(zmax < pd.min_d * (1.0f - 1e-4f) || zmin > pd.max_d * (1.0f + 1e-4f))) {
+      int x0 = 0, x1 = pd.cols - 1, y0 = 0, y1 = pd.rows - 1;
+      if (zmin > 0.1f * pd.min_d && umin >= -1e6f && umax <= 1e6f && vmin >= -1e6f && vmax <= 1e6f) {
+        x0 = max(x0, (int)floorf(umin) - kBinMargin);
+        x1 = min(x1, (int)ceilf(umax) + kBinMargin);
+        y0 = max(y0, (int)floorf(vmin) - kBinMargin);
+        y1 = min(y1, (int)ceilf(vmax) + kBinMargin);
+      }
+      if (x0 <= x1 && y0 <= y1) box = make_int4(x0 / kDTW, y0 / kFH, x1 / kDTW, y1 / kFH);
+    }
+  }
+  *gmut((gint4 *)pd.sbox + s) = gint4{box.x, box.y, box.z, box.w};
 }
 
-// One block per fused tile (blk_fpair, in the XCD-aware order of k_corr_lin): the LDS z-buffer
-// of the tile's kFH x kDTW pixels from the listed source tiles (all of them when the list
-// overflowed), then the kFH / kDTH correspondence tiles' correspondences and linearisation terms,
-// one block partial each. Resets the tile's list count for the next iteration.
+// One block per fused tile (blk_fpair, in the XCD-aware order of k_corr_lin): the list of the
+// reference frame's source tiles whose box holds the tile (every source tile when more than
+// list_cap do), the LDS z-buffer of the tile's kFH x kDTW pixels from their points, then the
+// kFH / kDTH correspondence tiles' correspondences and linearisation terms, one block partial each.
 extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_proj_corr_lin(
     const PairDev *__restrict__ Pp, const FrameDev *__restrict__ F, const int *__restrict__ blk_fpair,
-    AlignParamsDev ap, int store_pairs, int bin_cap) {
+    AlignParamsDev ap, int store_pairs, int list_cap) {
   __shared__ unsigned long long zl[kFH * kDTW];
-  __shared__ int sl[kBinCap];
+  __shared__ int sl[kListCap];
   __shared__ int scount;
   const int g = xcd_tile(blockIdx.x, gridDim.x);
   const PairDev &pd = Pp[blk_fpair[g]];
@@ -1574,49 +1562,79 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
   const int ft = g - pd.ftile_base;
   const int fy = ft / pd.dtiles_x, fx = ft - fy * pd.dtiles_x;
   for (int i = threadIdx.x; i < kFH * kDTW; i += kThreads) zl[i] = kZInit;
-  if (threadIdx.x == 0) {
-    scount = *gview(pd.bins + ft);
-    *gmut(pd.bins + ft) = 0;
+  if (threadIdx.x == 0) scount = 0;
+  __syncthreads();
+  {
+    // the list, compacted per wavefront: one LDS atomic per wavefront and round
+    const int ns = stiles(fr.rows, fr.cols), lane = threadIdx.x & 63;
+    const __attribute__((address_space(1))) gint4 *box = gview((const gint4 *)pd.sbox);
+    for (int s0 = threadIdx.x - lane; s0 < ns; s0 += kThreads) {
+      const int s = s0 + lane;
+      bool hit = false;
+      if (s < ns) {
+        const gint4 b = box[s];
+        hit = fx >= b.x && fx <= b.z && fy >= b.y && fy <= b.w;
+      }
+      const unsigned long long m = __ballot(hit);
+      int base = 0;
+      if (lane == 0 && m) base = atomicAdd(&scount, __popcll(m));
+      base = __shfl(base, 0, 64);
+      const int slot = base + __popcll(m & (lane ? ~0ull >> (64 - lane) : 0ull));
+      if (hit && slot < list_cap) sl[slot] = s;
+    }
   }
   __syncthreads();
   const int cnt = scount;
-  const bool all = cnt > bin_cap;
-  for (int i = threadIdx.x; i < (all ? 0 : cnt); i += kThreads) sl[i] = *gview(pd.bins + pd.n_ftiles + ft * kBinCap + i);
-  __syncthreads();
+  const bool all = cnt > list_cap;
   {
+    // the frame and pair fields the loop needs, held in registers: a field read from the tables
+    // inside the loop is a scalar load, and waiting for it would also wait for the LDS atomics
     float KRt[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) KRt[k] = *gview(pd.state + kSlotKRt + k);
-    const int sx = stiles_x(fr.cols);
-    const int nsrc = all ? stiles(fr.rows, fr.cols) : cnt;
-    const int npts = nsrc * (kSTH * kSTW);
+    const int frows = fr.rows, fcols = fr.cols, prows = pd.rows, pcols = pd.cols;
+    const float mind = pd.min_d, maxd = pd.max_d;
+    const __attribute__((address_space(1))) int *findex = gview(fr.index);
+    const float4 *fpts = fr.pts;
+    const int sx = stiles_x(fcols);
+    const int nsrc = all ? stiles(frows, fcols) : cnt;
     const int xlo = fx * kDTW, ylo = fy * kFH;
-    for (int base = 0; base < npts; base += 4 * kThreads) {
-      int idx[4];
+    // wavefront w takes source tiles w, w + 4, ...: its list entries are read once (lane i holds
+    // entry w + 4 i) and taken with readlane, so no LDS wait inside the loop; lane l projects the
+    // tile's pixels l, l + 64, ... (kPhaseB of them), all loads in flight together
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = kThreads / 64;
+    const int mine = wave + nw * lane;
+    const int entry = !all && mine < nsrc ? sl[mine] : 0;
+    const int ntiles = nsrc > wave ? (nsrc - wave + nw - 1) / nw : 0;
+    for (int i = 0; i < ntiles; ++i) {
+      const int st = all ? wave + nw * i : __builtin_amdgcn_readlane(entry, i);
+      const int sty = st / sx, stx = st - sty * sx;
+      int idx[kPhaseB];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int j = base + u * kThreads + threadIdx.x;
-        idx[u] = -1;
-        if (j < npts) {
-          const int w = j & (kSTH * kSTW - 1), si = j / (kSTH * kSTW);
-          const int st = all ? si : sl[si];
-          const int sty = st / sx, stx = st - sty * sx;
-          const int r = sty * kSTH + w / kSTW, c = stx * kSTW + (w & (kSTW - 1));
-          if (r < fr.rows && c < fr.cols) idx[u] = *gview(fr.index + r * fr.cols + c);
-        }
+      for (int u = 0; u < kPhaseB; ++u) {
+        const int w = u * 64 + lane;
+        const int r = sty * kSTH + w / kSTW, c = stx * kSTW + w % kSTW;
+        idx[u] = r < frows && c < fcols ? findex[r * fcols + c] : -1;
       }
-      float4 p[4];
+      float4 p[kPhaseB];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (idx[u] >= 0) p[u] = gload4(fr.pts + idx[u]);
+      for (int u = 0; u < kPhaseB; ++u)
+        if (idx[u] >= 0) p[u] = gload4(fpts + idx[u]);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        int x, y;
-        unsigned long long key;
-        if (idx[u] >= 0 && pinhole_project(pd, KRt, p[u], idx[u], x, y, key)) {
-          const int lx = x - xlo, ly = y - ylo;
-          if (lx >= 0 && lx < kDTW && ly >= 0 && ly < kFH) atomicMin(&zl[ly * kDTW + lx], key);
-        }
+      for (int u = 0; u < kPhaseB; ++u) {
+        if (idx[u] < 0) continue;
+        // pinhole_project's arithmetic on the register copies
+        float ip[4];
+        mul4_packet(KRt, p[u].x, p[u].y, p[u].z, p[u].w, ip);
+        const float d = ip[2];
+        if (d < mind || d > maxd) continue;
+        const float inv = 1.0f / d;
+        const int x = (int)roundf(ip[0] * inv), y = (int)roundf(ip[1] * inv);
+        if (!(x >= 0 && x < pcols && y >= 0 && y < prows)) continue;
+        const int lx = x - xlo, ly = y - ylo;
+        if (lx >= 0 && lx < kDTW && ly >= 0 && ly < kFH)
+          atomicMin(&zl[ly * kDTW + lx], ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)idx[u]);
       }
     }
   }

@@ -78,7 +78,7 @@ def test_projection_strategies_identical(ctx, max_rot, max_trans, init):
     for cap in (0, 2):
         batch.set_bin_capacity(cap)
         _same(batch.align(ref, cur, init16, ap, use_graph=False), base)
-    batch.set_bin_capacity(128)
+    batch.set_bin_capacity(256)
     # inner iterations and the stored correspondences of the last iteration
     ap2 = capi.align_params(outer_iterations=4, inner_iterations=3)
     batch.set_projection(0)
@@ -88,7 +88,7 @@ def test_projection_strategies_identical(ctx, max_rot, max_trans, init):
     with pytest.raises(capi.NicpError):
         batch.set_projection(3)
     with pytest.raises(capi.NicpError):
-        batch.set_bin_capacity(129)
+        batch.set_bin_capacity(257)
     batch.close()
 
 
