@@ -61,6 +61,24 @@ __device__ __forceinline__ bool depth_valid(float d, float mn, float mx) { retur
 __device__ __forceinline__ float frame_depth(const FrameDev &fd, int p) {
   return fd.depthf ? *gview(fd.depthf + p) : depth_of(*gview(fd.depth + p), fd.depth_scale);
 }
+// the depths of N pixels (p[k] < 0: none, depth 0), every load issued before the first is used:
+// frame_depth's choice of the input type, taken per pixel, makes each load wait for its value
+template <int N>
+__device__ __forceinline__ void frame_depths(const FrameDev &fd, const int *p, float *d) {
+  const __attribute__((address_space(1))) float *f = gview(fd.depthf);
+  if (f) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) d[k] = p[k] >= 0 ? f[p[k]] : 0.0f;
+  } else {
+    const __attribute__((address_space(1))) uint16_t *u = gview(fd.depth);
+    const float sc = fd.depth_scale;
+    uint16_t raw[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) raw[k] = p[k] >= 0 ? u[p[k]] : (uint16_t)0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) d[k] = depth_of(raw[k], sc);
+  }
+}
 
 // PinholePointProjector::_unProject (pinholepointprojector.h:275-280): x = column, y = row
 __device__ __forceinline__ float4 unproject_point(const float *iKRt, int c, int r, float d) {
@@ -151,12 +169,17 @@ __device__ __forceinline__ void count_body(const FrameDev *__restrict__ F, const
   const FrameDev &fd = F[blk_frame[g]];
   const int P = fd.rows * fd.cols;
   const int base = (g - fd.tile_base) * kTile;
-  int cnt = 0;
+  int cnt = 0, px[4];
+  float dv[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    int p = base + k * kThreads + threadIdx.x;
-    if (p < P) cnt += pixel_valid<M>(fd, p, frame_depth(fd, p));
+    const int p = base + k * kThreads + threadIdx.x;
+    px[k] = p < P ? p : -1;
   }
+  frame_depths<4>(fd, px, dv);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (px[k] >= 0) cnt += pixel_valid<M>(fd, px[k], dv[k]);
   __shared__ int ws[kThreads / 64];
   cnt = wave_sum_i(cnt);
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = cnt;
@@ -213,12 +236,18 @@ __device__ __forceinline__ void unproject_body(const FrameDev *__restrict__ F, c
   // pixel base + k * kThreads + threadIdx.x: each load and store instruction of a wavefront
   // covers 64 consecutive pixels, and the valid ones take consecutive cloud indices
   float dv[4];
+  int px[4];
   unsigned long long bal[4];
   __shared__ int wsum[4][kThreads / 64];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int p = base + k * kThreads + threadIdx.x;
-    dv[k] = p < P ? frame_depth(fd, p) : 0.0f;
+    px[k] = p < P ? p : -1;
+  }
+  frame_depths<4>(fd, px, dv);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int p = base + k * kThreads + threadIdx.x;
     bal[k] = __ballot(p < P && pixel_valid<M>(fd, p, dv[k]));
     if (lane == 0) wsum[k][wave] = __popcll(bal[k]);
   }
@@ -268,22 +297,28 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_src_tiles(const FrameDe
   const FrameDev &fd = F[blk_frame[g]];
   if (!fd.stile_d) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int sx = stiles_x(fd.cols), ns = stiles(fd.rows, fd.cols);
+  const int rows = fd.rows, cols = fd.cols;
+  const float mind = fd.min_d, maxd = fd.max_d;
+  const int sx = stiles_x(cols), ns = stiles(rows, cols);
   for (int s = 4 * (g - fd.tile_base) + wave; s < ns; s += 4 * fd.n_tiles) {
     const int ty = s / sx, tx = s - ty * sx;
     float lo = INFINITY, hi = -INFINITY;
+    constexpr int kN = (kSTH * kSTW + 63) / 64;
+    int px[kN];
+    float d[kN];
 #pragma unroll
-    for (int i = 0; i < (kSTH * kSTW + 63) / 64; ++i) {
+    for (int i = 0; i < kN; ++i) {
       const int w = i * 64 + lane;
       const int r = ty * kSTH + w / kSTW, c = tx * kSTW + w % kSTW;
-      if (w < kSTH * kSTW && r < fd.rows && c < fd.cols) {
-        const float d = frame_depth(fd, r * fd.cols + c);
-        if (depth_valid(d, fd.min_d, fd.max_d)) {
-          lo = fminf(lo, d);
-          hi = fmaxf(hi, d);
-        }
-      }
+      px[i] = w < kSTH * kSTW && r < rows && c < cols ? r * cols + c : -1;
     }
+    frame_depths<kN>(fd, px, d);
+#pragma unroll
+    for (int i = 0; i < kN; ++i)
+      if (px[i] >= 0 && depth_valid(d[i], mind, maxd)) {
+        lo = fminf(lo, d[i]);
+        hi = fmaxf(hi, d[i]);
+      }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       lo = fminf(lo, __shfl_xor(lo, o, 64));
@@ -795,15 +830,16 @@ __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const
   const int base = (g - fd.tile_base) * kTile;
   // stage 1, all four pixels at once: index and depth loads, window radius and the row-major
   // offsets of the window's corners, so that the corner reads below wait on one load only
-  int idxs[4], offs[4][4];
+  int idxs[4], offs[4][4], px[4];
   float ds[4];
   bool oks[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int p = base + k * kThreads + threadIdx.x;
+    px[k] = p < P ? p : -1;
     idxs[k] = p < P ? *gview(fd.index + p) : -1;
-    ds[k] = p < P ? frame_depth(fd, p) : 0.0f;
   }
+  frame_depths<4>(fd, px, ds);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int p = base + k * kThreads + threadIdx.x;
