@@ -363,6 +363,17 @@ int nicp_batch_align_learned(nicp_batch *batch, nicp_initnet *net, int num_pairs
                              const int *cur_frame, const float *fallback16, const nicp_align_params *ap, int use_graph,
                              float *T_out16, nicp_align_result *results, float *start16, int *used_network);
 
+/* The start probe of nicp_batch_align_learned (no reference counterpart): one correspondence
+ * search and linearisation (Linearizer::update, nicp/nicp/linearizer.cpp:21-134) of each pair at
+ * its start (starts16: 16 floats per pair), over every sub-th row and column of the projector
+ * image (sub >= 1; the reference cloud is projected whole). results receive H, b, error, inliers
+ * and the correspondence count (no statistics). twin = 1 projects each reference point once for
+ * pairs i and i + num_pairs / 2, which must have the same frames (the learned aligner's two
+ * starts); the results are those of twin = 0. */
+int nicp_batch_probe(nicp_batch *batch, int num_pairs, const int *ref_frame, const int *cur_frame,
+                     const float *starts16, const nicp_align_params *ap, int sub, int twin,
+                     nicp_align_result *results);
+
 /* Per-kernel device time (no reference counterpart; the reference only has
  * Aligner::totalTime(), nicp/nicp/aligner.h:348). When enabled, every launch of the
  * context is bracketed by HIP events on its stream (event nodes inside captured graphs);

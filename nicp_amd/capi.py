@@ -79,7 +79,7 @@ EXPORTS = [
     "nicp_initnet_train_step_rows", "nicp_initnet_set_input_mode", "nicp_cloud_gaussians", "nicp_cloud_set_gaussians",
     "nicp_cloud_download_gaussians", "nicp_cloud_transform", "nicp_cloud_add", "nicp_merge",
     "nicp_batch_set_autotune", "nicp_batch_autotune_read", "nicp_align_nn_batch", "nicp_solution_statistics",
-    "nicp_align_noalloc", "nicp_batch_set_projection", "nicp_batch_set_bin_capacity",
+    "nicp_align_noalloc", "nicp_batch_set_projection", "nicp_batch_set_bin_capacity", "nicp_batch_probe",
 ]
 
 _lib = None
@@ -115,6 +115,7 @@ def lib():
             "nicp_solution_statistics": [vp, vp, vp, vp, vp],
             "nicp_align_noalloc": [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp],
             "nicp_batch_set_projection": [vp, C.c_int], "nicp_batch_set_bin_capacity": [vp, C.c_int],
+            "nicp_batch_probe": [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, vp],
             "nicp_batch_create": [vp, C.c_int, vp, vp, vp, vp, vp],
             "nicp_batch_destroy": [vp], "nicp_batch_upload_depth": [vp, C.c_int, vp, C.c_float],
             "nicp_batch_convert": [vp],
@@ -531,6 +532,17 @@ class Batch:
     def set_projection(self, strategy):
         """Forces projection strategy 0, 1 or 2 (tile-binned) on later align calls; -1: default."""
         _check(lib().nicp_batch_set_projection(self.h, int(strategy)))
+
+    def probe(self, ref, cur, starts16, ap, sub=4, twin=False):
+        """One linearisation per pair at its start over every sub-th row and column (the learned
+        aligner's start probe); results hold H, b, error, inliers and the correspondence count."""
+        ref = np.ascontiguousarray(ref, np.int32)
+        cur = np.ascontiguousarray(cur, np.int32)
+        n = len(ref)
+        st = np.ascontiguousarray(starts16, np.float32).reshape(n, 16)
+        res = (AlignResult * n)()
+        _check(lib().nicp_batch_probe(self.h, n, _p(ref), _p(cur), _p(st), C.byref(ap), int(sub), int(twin), res))
+        return res
 
     def set_bin_capacity(self, capacity):
         """Source tiles a projector tile of the tile-binned projection lists (0..256)."""

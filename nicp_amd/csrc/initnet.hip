@@ -732,6 +732,27 @@ extern "C" int nicp_batch_net_input(nicp_batch *b, nicp_initnet *n, int num_pair
   return launch_net_input(n, b, num_pairs, ref_frame, cur_frame, max_depth_cov, dev_out);
 }
 
+extern "C" int nicp_batch_probe(nicp_batch *b, int num_pairs, const int *ref_frame, const int *cur_frame,
+                                const float *starts16, const nicp_align_params *ap, int sub, int twin,
+                                nicp_align_result *results) {
+  if (!b || num_pairs <= 0 || !ref_frame || !cur_frame || !starts16 || !ap || !results || sub < 1)
+    return set_err(NICP_ERR_INVALID, "bad arguments");
+  if (twin && (num_pairs % 2 != 0)) return set_err(NICP_ERR_INVALID, "a twin probe needs an even number of pairs");
+  for (int i = 0; twin && i < num_pairs / 2; ++i)
+    if (ref_frame[i] != ref_frame[i + num_pairs / 2] || cur_frame[i] != cur_frame[i + num_pairs / 2])
+      return set_err(NICP_ERR_INVALID, "twin pairs %d and %d have different frames", i, i + num_pairs / 2);
+  nicp_align_params probe = *ap;
+  probe.outer_iterations = 1;
+  probe.inner_iterations = 1;
+  std::vector<double> raw;
+  CHK(batch_align(b, num_pairs, ref_frame, cur_frame, starts16, &probe, 0, nullptr, nullptr, false, &raw, sub,
+                  twin ? num_pairs / 2 : 0));
+  nicp_align_params none = probe;
+  none.outer_iterations = 0;  // no statistics were computed
+  for (int i = 0; i < num_pairs; ++i) fill_result(&raw[(size_t)i * kResultDoubles], &none, &results[i], 0.0f);
+  return NICP_OK;
+}
+
 extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_pairs, const int *ref_frame,
                                         const int *cur_frame, const float *fallback16, const nicp_align_params *ap,
                                         int use_graph, float *T_out16, nicp_align_result *results, float *start16,

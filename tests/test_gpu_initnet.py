@@ -310,3 +310,42 @@ def test_align_learned_policy(ctx):
     say("  large motion from identity: %.3f deg %.4f m" % ei)
     assert e0[0] < 2.0 and e0[1] < 0.02
     assert ei[0] > 2.0 * e0[0]
+
+
+@pytest.mark.parametrize("sub", [4, 2])
+def test_probe_twin_equals_separate(ctx, sub):
+    """The start probe's one-pass projection of both candidate starts (k_zproject_twin) gives the
+    linearisations of two separate projections, bit for bit: several seeds, ragged point counts
+    (different hole rates and a nearly empty frame), the predicted and the fallback start."""
+    say("case probe twin sub=%d" % sub)
+    rows, cols = 480, 640
+    _, sp = params()
+    _, ap = aparams()
+    cm = cams(rows, cols)[1]
+    frames, starts_net = [], []
+    for k, (seed, holes) in enumerate([(601, 0.1), (602, 0.3), (603, 0.0), (604, 0.97)]):
+        dr, dc, Tgt = synth.pair(seed, rows, cols, max_rot_deg=20.0, max_trans_m=0.3, holes=holes)
+        frames += [dr, dc]
+        starts_net.append(Tgt if k % 2 == 0 else O.v2t(np.array([0.02, -0.01, 0.03, 0.01, 0.02, -0.01], np.float32)))
+    b = capi.Batch(ctx, [(rows, cols)] * len(frames), [cm] * len(frames), sp)
+    for i, f in enumerate(frames):
+        b.upload(i, f)
+    b.convert()
+    n = len(frames) // 2
+    ref = np.arange(0, 2 * n, 2)
+    cur = ref + 1
+    I = np.eye(4, dtype=np.float32).flatten(order="F")
+    starts = np.concatenate([np.stack(starts_net), np.tile(I, (n, 1))])
+    ref2, cur2 = np.concatenate([ref, ref]), np.concatenate([cur, cur])
+    rt = b.probe(ref2, cur2, starts, ap, sub=sub, twin=True)
+    rs = b.probe(ref2, cur2, starts, ap, sub=sub, twin=False)
+    for x, y in zip(rt, rs):
+        assert x.inliers == y.inliers and x.num_correspondences == y.num_correspondences
+        np.testing.assert_array_equal(np.array(x.H).view(np.uint64), np.array(y.H).view(np.uint64))
+        np.testing.assert_array_equal(np.array(x.b).view(np.uint64), np.array(y.b).view(np.uint64))
+        assert np.float32(x.error).view(np.uint32) == np.float32(y.error).view(np.uint32)
+    say("  inliers %s" % [x.inliers for x in rt])
+    assert rt[0].inliers > 100
+    with pytest.raises(capi.NicpError):
+        b.probe(ref2, np.concatenate([cur, ref]), starts, ap, sub=sub, twin=True)
+    b.close()
