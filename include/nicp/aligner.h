@@ -132,10 +132,24 @@ class Aligner {
     _initialGuess.setBottomRow();
   }
   const Isometry3f &initialGuess() const { return _initialGuess; }
-  // only identity sensor offsets are supported by the device loops
+  // Aligner::setSensorOffset sets both offsets; each setter zeroes the bottom row
+  // (nicp/nicp/aligner.h:157-198). AlignerProjective applies them (alignerprojective.cpp:36,51);
+  // AlignerNN ignores them, as the reference's does.
   void setSensorOffset(const Isometry3f &o) {
-    if (!o.isIdentity()) throw std::runtime_error("nicp_amd: the aligners support identity sensor offsets only");
+    setReferenceSensorOffset(o);
+    setCurrentSensorOffset(o);
   }
+  const Isometry3f &sensorOffset() const { return _referenceSensorOffset; }
+  void setReferenceSensorOffset(const Isometry3f &o) {
+    _referenceSensorOffset = o;
+    _referenceSensorOffset.setBottomRow();
+  }
+  const Isometry3f &referenceSensorOffset() const { return _referenceSensorOffset; }
+  void setCurrentSensorOffset(const Isometry3f &o) {
+    _currentSensorOffset = o;
+    _currentSensorOffset.setBottomRow();
+  }
+  const Isometry3f &currentSensorOffset() const { return _currentSensorOffset; }
   const Isometry3f &T() const { return _T; }
   float error() const { return _error; }
   int inliers() const { return _inliers; }
@@ -208,7 +222,7 @@ class Aligner {
   int _outerIterations = 10, _innerIterations = 1, _minInliers = 100;
   float _lambda = 1e3f, _translationalMinEigenRatio = 50.0f, _rotationalMinEigenRatio = 50.0f;
   bool _debug = false, _solutionValid = false;
-  Isometry3f _initialGuess, _T;
+  Isometry3f _initialGuess, _T, _referenceSensorOffset, _currentSensorOffset;
   float _error = 0.0f, _totalTime = 0.0f;
   int _inliers = 0;
   Matrix6d _omega;

@@ -60,8 +60,9 @@ class AlignerProjective : public Aligner {
     nicp_camera cam = _projector->camera();
     nicp_align_params ap = params();
     nicp_align_result r{};
-    check(nicp_align(Context::get(), _referenceCloud->handle(), _currentCloud->handle(), &cam,
-                     _projector->imageRows(), _projector->imageCols(), _initialGuess.matrix(), &ap, _T.matrix(), &r));
+    check(nicp_align_offsets(Context::get(), _referenceCloud->handle(), _currentCloud->handle(), &cam,
+                             _projector->imageRows(), _projector->imageCols(), _initialGuess.matrix(),
+                             _referenceSensorOffset.matrix(), _currentSensorOffset.matrix(), &ap, _T.matrix(), &r));
     _adopt(r);
     CorrespondenceVector cv;
     _fetchCorrespondences(cv);

@@ -105,12 +105,10 @@ class DepthImageConverterIntegralImage {
   }
 
   // DepthImage_convert_16UC1_to_32FC1 + DepthImageConverterIntegralImage::compute
-  // (nicp/nicp/depthimageconverterintegralimage.cpp:18-64); only the identity sensor
-  // offset is supported, which is what the reference's apps pass.
+  // (nicp/nicp/depthimageconverterintegralimage.cpp:18-64); the cloud is moved by the
+  // sensor offset last (cloud.transformInPlace, line 63).
   void compute(Cloud &cloud, const RawDepthImage &depth, float depthScale,
                const Isometry3f &sensorOffset = Isometry3f::Identity()) {
-    if (!sensorOffset.isIdentity())
-      throw std::runtime_error("nicp_amd: DepthImageConverter supports the identity sensor offset only");
     _projector->setImageSize(depth.rows, depth.cols);
     nicp_camera cam = _projector->camera();
     nicp_stats_params sp = statsParams();
@@ -125,8 +123,6 @@ class DepthImageConverterIntegralImage {
   }
   // DepthImageConverterIntegralImage::compute(cloud, depthImage, sensorOffset) for a float image in metres
   void compute(Cloud &cloud, const DepthImage &depth, const Isometry3f &sensorOffset = Isometry3f::Identity()) {
-    if (!sensorOffset.isIdentity())
-      throw std::runtime_error("nicp_amd: DepthImageConverter supports the identity sensor offset only");
     _projector->setImageSize(depth.rows, depth.cols);
     nicp_camera cam = _projector->camera();
     nicp_stats_params sp = statsParams();

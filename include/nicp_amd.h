@@ -222,6 +222,15 @@ int nicp_linearize(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, 
 int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const nicp_camera *cam, int rows,
                int cols, const float init_guess[16], const nicp_align_params *ap, float T_out[16],
                nicp_align_result *result);
+/* AlignerProjective::align with the aligner's sensor offsets (Aligner::setReferenceSensorOffset /
+ * setCurrentSensorOffset, nicp/nicp/aligner.h:157-196): the current cloud is projected from
+ * current_offset (nicp/nicp/alignerprojective.cpp:36-39) and the reference cloud from
+ * T * reference_offset (nicp/nicp/alignerprojective.cpp:51). Column-major 4x4 each, the last row
+ * taken as 0 0 0 1; NULL is the identity. */
+int nicp_align_offsets(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const nicp_camera *cam, int rows,
+                       int cols, const float init_guess[16], const float reference_offset[16],
+                       const float current_offset[16], const nicp_align_params *ap, float T_out[16],
+                       nicp_align_result *result);
 /* Replaces Aligner::_computeStatistics (nicp/nicp/aligner.cpp:82-135, nicp/nicp/unscented.h:11-65)
  * for a given statistics linearisation H (column-major 6x6) at the final transform T: fills
  * result->mean, omega, translational_ratio, rotational_ratio and solution_valid (the other fields

@@ -141,6 +141,24 @@ inline void setInputParameters(Objects &o, std::map<std::string, float> &p) {
   o.aligner.setLinearizer(&o.linearizer);
 }
 
+// the initial pose of both apps (nicp_simple_aligner.cpp:57-70, nicp_aligner.cpp:55-67):
+// translation tx ty tz and quaternion qx qy qz qw (w defaults to 1)
+inline Isometry3f initialPose(std::map<std::string, float> &p) {
+  auto get = [&](const char *k, float d) { auto it = p.find(k); return it == p.end() ? d : it->second; };
+  const double qx = get("qx", 0.0f), qy = get("qy", 0.0f), qz = get("qz", 0.0f), qw = get("qw", 1.0f);
+  const double R[3][3] = {{1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)},
+                          {2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)},
+                          {2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)}};
+  Isometry3f T;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) T(r, c) = (float)R[r][c];
+  T(0, 3) = get("tx", 0.0f);
+  T(1, 3) = get("ty", 0.0f);
+  T(2, 3) = get("tz", 0.0f);
+  T.setBottomRow();
+  return T;
+}
+
 inline std::string odometryLine(const std::string &timestamp, const Isometry3f &T) {
   double q[4];
   toQuaternion(T, q);

@@ -33,19 +33,7 @@ int main(int argc, char **argv) {
   const float depthScale = get("depthScale", 0.001f);
   const int imageScale = (int)get("imageScale", 1.0f);
   const int chunkStep = (int)get("chunkStep", 10.0f);
-  // initial pose (nicp_aligner.cpp:55-67): translation and quaternion (w defaults to 1)
-  Isometry3f initialT;
-  {
-    const double qx = get("qx", 0.0f), qy = get("qy", 0.0f), qz = get("qz", 0.0f), qw = get("qw", 1.0f);
-    const double R[3][3] = {{1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)},
-                            {2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)},
-                            {2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)}};
-    for (int r = 0; r < 3; ++r)
-      for (int c = 0; c < 3; ++c) initialT(r, c) = (float)R[r][c];
-    initialT(0, 3) = get("tx", 0.0f);
-    initialT(1, 3) = get("ty", 0.0f);
-    initialT(2, 3) = get("tz", 0.0f);
-  }
+  const Isometry3f initialT = initialPose(p);
 
   Objects o;
   setInputParameters(o, p);

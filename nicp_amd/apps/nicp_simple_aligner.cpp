@@ -6,7 +6,8 @@
 // configuration: "key value" lines as in nicp/nicp_conf/nicp_aligner_1_1.conf
 // (unknown keys and "//" lines are ignored); depth list: "timestamp file.pgm"
 // lines naming 16-bit binary PGM depth images; output: one line per frame,
-// "timestamp tx ty tz qx qy qz qw" of the accumulated global transform.
+// "timestamp tx ty tz qx qy qz qw" of the accumulated global transform, and each
+// frame's cloud saved next to its depth image as <depth file>.nicp with that transform.
 #include <cstdio>
 #include <memory>
 
@@ -34,6 +35,7 @@ int main(int argc, char **argv) {
   PinholePointProjector &projector = o.projector;
   AlignerProjective &aligner = o.aligner;
   DepthImageConverterIntegralImage converter(&projector, &o.statsCalculator, &o.pointInfo, &o.normalInfo);
+  converter.setKeepStats(true);  // the saved clouds carry their Stats, as the reference's do
 
   std::ifstream is(argv[2]);
   if (!is) {
@@ -46,7 +48,9 @@ int main(int argc, char **argv) {
     return 1;
   }
   std::unique_ptr<Cloud> previous, cloud;
-  Isometry3f globalT;
+  // nicp_simple_aligner.cpp:125-128: the identity sensor offset, the initial pose
+  const Isometry3f sensorOffset = Isometry3f::Identity();
+  Isometry3f globalT = initialPose(p);
   std::string line;
   while (std::getline(is, line)) {
     std::istringstream iss(line);
@@ -66,14 +70,22 @@ int main(int argc, char **argv) {
       projector.scale(1.0f / imageScale);
     }
     cloud.reset(new Cloud());
-    converter.compute(*cloud, scaled);
+    converter.compute(*cloud, scaled, sensorOffset);
     if (previous) {
       aligner.setReferenceCloud(previous.get());
       aligner.setCurrentCloud(cloud.get());
       aligner.setInitialGuess(Isometry3f::Identity());
+      aligner.setSensorOffset(sensorOffset);
       aligner.align();
       globalT = globalT * aligner.T();
       globalT.setBottomRow();
+    }
+    // nicp_simple_aligner.cpp:175: Cloud::save(depth file + ".nicp", globalT, step 1, binary)
+    float v[6];
+    toVector6(globalT, v);
+    if (!cloud->save((file + ".nicp").c_str(), v, 1, true)) {
+      std::cerr << "cannot write " << file << ".nicp" << std::endl;
+      return 1;
     }
     os << odometryLine(timestamp, globalT) << std::endl;
     std::cout << "frame " << file << " points " << cloud->size() << " inliers " << aligner.inliers() << std::endl;

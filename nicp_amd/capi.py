@@ -68,7 +68,7 @@ EXPORTS = [
     "nicp_last_error", "nicp_ctx_create", "nicp_ctx_destroy", "nicp_ctx_synchronize", "nicp_ctx_stream",
     "nicp_cloud_from_depth_u16", "nicp_cloud_from_host", "nicp_cloud_destroy", "nicp_cloud_size",
     "nicp_cloud_download", "nicp_cloud_download_stats", "nicp_cloud_index_image", "nicp_project",
-    "nicp_correspondences", "nicp_linearize", "nicp_align", "nicp_last_correspondences", "nicp_batch_create",
+    "nicp_correspondences", "nicp_linearize", "nicp_align", "nicp_align_offsets", "nicp_last_correspondences", "nicp_batch_create",
     "nicp_batch_destroy", "nicp_batch_upload_depth", "nicp_batch_convert", "nicp_batch_align",
     "nicp_batch_frame_cloud", "nicp_kernel_count", "nicp_kernel_name", "nicp_ctx_profile",
     "nicp_ctx_profile_read", "nicp_nn_correspondences", "nicp_align_nn", "nicp_cloud_from_depth_f32",
@@ -116,6 +116,7 @@ def lib():
             "nicp_align_noalloc": [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp],
             "nicp_batch_set_projection": [vp, C.c_int], "nicp_batch_set_bin_capacity": [vp, C.c_int],
             "nicp_batch_probe": [vp, C.c_int, vp, vp, vp, vp, C.c_int, C.c_int, vp],
+            "nicp_align_offsets": [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp],
             "nicp_batch_create": [vp, C.c_int, vp, vp, vp, vp, vp],
             "nicp_batch_destroy": [vp], "nicp_batch_upload_depth": [vp, C.c_int, vp, C.c_float],
             "nicp_batch_convert": [vp],
@@ -344,6 +345,17 @@ class Context:
         res = AlignResult()
         _check(lib().nicp_align(self.h, ref.h, cur.h, C.byref(cam), rows, cols,
                                 _p(np.ascontiguousarray(init16, np.float32)), C.byref(ap), _p(T), C.byref(res)))
+        return T, res
+
+    def align_offsets(self, ref, cur, cam, rows, cols, init16, ref_offset16, cur_offset16, ap):
+        """nicp_align with the aligner's reference and current sensor offsets (None: identity)."""
+        T = np.empty(16, np.float32)
+        res = AlignResult()
+        ro = None if ref_offset16 is None else np.ascontiguousarray(ref_offset16, np.float32)
+        co = None if cur_offset16 is None else np.ascontiguousarray(cur_offset16, np.float32)
+        _check(lib().nicp_align_offsets(self.h, ref.h, cur.h, C.byref(cam), rows, cols,
+                                        _p(np.ascontiguousarray(init16, np.float32)), _p(ro), _p(co), C.byref(ap),
+                                        _p(T), C.byref(res)))
         return T, res
 
     def align_noalloc(self, ref, cur, cam, rows, cols, init16, ap):

@@ -520,6 +520,8 @@ struct PairPlan {
   // every reference frame with source-tile depth ranges, no subsampling (set by batch_align)
   bool fused_ok = false;
   int list_cap = kListCap;      // source tiles a fused tile lists before it scans them all
+  // the aligner's sensor offsets (null: identity), the same for every pair of the plan
+  const float *ref_offset = nullptr, *cur_offset = nullptr;
   // > 0: pairs q and q + twin share both frames (the two starts of the start probe), and
   // one k_zproject_twin pass over the first half's reference tiles projects both
   int twin = 0;
@@ -664,23 +666,29 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
     } else {
       pd.sbox = nullptr;
     }
-    // initial state: AlignerProjective::align, alignerprojective.cpp:36-54
+    // initial state: AlignerProjective::align, alignerprojective.cpp:36-54; the sensor offsets
+    // with their last rows set to 0 0 0 1 (aligner.h:177,194)
     float *S = &st[(size_t)i * kStateFloats];
-    float T[16];
+    float T[16], RO[16], CO[16];
     for (int k = 0; k < 16; ++k) T[k] = init16 ? init16[(size_t)i * 16 + k] : (k % 5 == 0 ? 1.0f : 0.0f);
     set_row3(T);
-    float Ti[16], I4[16], TR[16], KRt[16], iKRt[16];
+    set_identity4(RO);
+    set_identity4(CO);
+    if (plan.ref_offset) memcpy(RO, plan.ref_offset, sizeof(RO));
+    if (plan.cur_offset) memcpy(CO, plan.cur_offset, sizeof(CO));
+    set_row3(RO);
+    set_row3(CO);
+    float Ti[16], TR[16], KRt[16];
     iso_inverse(T, Ti);
-    set_identity4(I4);
-    iso_mul(T, I4, TR);
+    iso_mul(T, RO, TR);
     projector_state(pd.model, pd.K, TR, KRt);
     for (int k = 0; k < 16; ++k) {
       S[kSlotT + k] = T[k];
       S[kSlotTinv + k] = Ti[k];
       S[kSlotKRt + k] = KRt[k];
+      S[kSlotRefOffset + k] = RO[k];
     }
-    projector_state(pd.model, pd.K, I4, KRt);
-    (void)iKRt;
+    projector_state(pd.model, pd.K, CO, KRt);
     for (int k = 0; k < 16; ++k) S[kSlotKRtCur + k] = KRt[k];
   }
   const size_t nb = plan.blk_pair.size() + plan.blk_ref.size() + plan.blk_cur.size() + plan.blk_dpair.size() +
@@ -1407,7 +1415,8 @@ namespace {
 // buffers, or fixed views of the no-allocation workspace)
 int align_single(nicp_ctx *ctx, DevBuf &frames_tbl, AlignBuffers B, const nicp_cloud *ref, const nicp_cloud *cur,
                  const nicp_camera *cam, int rows, int cols, const float init_guess[16], const nicp_align_params *ap,
-                 float T_out[16], nicp_align_result *result) {
+                 float T_out[16], nicp_align_result *result, const float *ref_offset = nullptr,
+                 const float *cur_offset = nullptr) {
   if (!ctx || !ref || !cur) return set_err(NICP_ERR_INVALID, "null argument");
   if (rows <= 0 || cols <= 0) return set_err(NICP_ERR_INVALID, "bad image size");
   CHK(check_camera(cam));
@@ -1430,6 +1439,8 @@ int align_single(nicp_ctx *ctx, DevBuf &frames_tbl, AlignBuffers B, const nicp_c
   pd.sph = sph_params(cam, rows, cols);
   plan_pixel_tiles(plan);
   plan.total_pix = align_up((size_t)rows * cols, 64);
+  plan.ref_offset = ref_offset;
+  plan.cur_offset = cur_offset;
   int *d_bp, *d_br, *d_bc;
   PairDev *d_pairs;
   float I4[16];
@@ -1456,9 +1467,17 @@ int align_single(nicp_ctx *ctx, DevBuf &frames_tbl, AlignBuffers B, const nicp_c
 extern "C" int nicp_align(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const nicp_camera *cam,
                           int rows, int cols, const float init_guess[16], const nicp_align_params *ap, float T_out[16],
                           nicp_align_result *result) {
+  return nicp_align_offsets(ctx, ref, cur, cam, rows, cols, init_guess, nullptr, nullptr, ap, T_out, result);
+}
+
+extern "C" int nicp_align_offsets(nicp_ctx *ctx, const nicp_cloud *ref, const nicp_cloud *cur, const nicp_camera *cam,
+                                  int rows, int cols, const float init_guess[16], const float reference_offset[16],
+                                  const float current_offset[16], const nicp_align_params *ap, float T_out[16],
+                                  nicp_align_result *result) {
   if (!ctx) return set_err(NICP_ERR_INVALID, "null argument");
   ctx->last_valid = false;
-  CHK(align_single(ctx, ctx->frames_tbl, ctx_buffers(ctx), ref, cur, cam, rows, cols, init_guess, ap, T_out, result));
+  CHK(align_single(ctx, ctx->frames_tbl, ctx_buffers(ctx), ref, cur, cam, rows, cols, init_guess, ap, T_out, result,
+                   reference_offset, current_offset));
   ctx->last_rows = rows;
   ctx->last_cols = cols;
   ctx->last_valid = ap->outer_iterations > 0;

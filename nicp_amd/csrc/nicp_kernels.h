@@ -99,13 +99,14 @@ struct PairDev {
   unsigned long long *zbuf;        // rows*cols z-buffer keys (depth bits << 32 | index)
   int *cur_index;                  // rows*cols
   int2 *pairs;                     // rows*cols stored correspondences (or (-1,-1))
-  float *state;                    // kStateFloats: T, Tinv, KRt (reference), KRt (current), invT (inner)
+  float *state;                    // kStateFloats: T, Tinv, KRt (reference), KRt (current), invT (inner),
+                                   // reference sensor offset
   double *partials;                // max(n_tiles, n_dtiles) * kPartial
   double *result;                  // kResultDoubles: last linearisation, statistics linearisation, solution statistics
 };
 
-constexpr int kStateFloats = 80;
-constexpr int kSlotT = 0, kSlotTinv = 16, kSlotKRt = 32, kSlotKRtCur = 48, kSlotInvTInner = 64;
+constexpr int kStateFloats = 96;
+constexpr int kSlotT = 0, kSlotTinv = 16, kSlotKRt = 32, kSlotKRtCur = 48, kSlotInvTInner = 64, kSlotRefOffset = 80;
 
 struct StatsParamsDev {
   int min_r, max_r, min_points;

@@ -1820,12 +1820,12 @@ extern "C" __global__ __launch_bounds__(kSolveThreads) void k_solve(const PairDe
   t2v<float>(T, x);
   v2t<float>(x, T);
   set_row3(T);
-  float Ti[16], I4[16], TR[16], KRt[16], iKRt[16];
+  // the reference projector's transform T * reference sensor offset (alignerprojective.cpp:51)
+  float Ti[16], RO[16], TR[16], KRt[16];
   iso_inverse(T, Ti);
-  set_identity4(I4);
-  iso_mul(T, I4, TR);
+  for (int k = 0; k < 16; ++k) RO[k] = S[kSlotRefOffset + k];
+  iso_mul(T, RO, TR);
   projector_state(pd.model, pd.K, TR, KRt);
-  (void)iKRt;
   for (int k = 0; k < 16; ++k) {
     S[kSlotT + k] = T[k];
     S[kSlotTinv + k] = Ti[k];

@@ -242,6 +242,20 @@ def align(ref_cloud, cur_cloud, rows, cols, cam, init16, ap, num_threads=1, fp64
     return T, res
 
 
+def align_offsets(ref_cloud, cur_cloud, rows, cols, cam, init16, ref_offset16, cur_offset16, ap, num_threads=1,
+                  fp64_solve=False):
+    """AlignerProjective::align with the aligner's reference and current sensor offsets."""
+    T = np.empty(16, np.float32)
+    res = AlignResult()
+    ro = None if ref_offset16 is None else np.ascontiguousarray(ref_offset16, np.float32)
+    co = None if cur_offset16 is None else np.ascontiguousarray(cur_offset16, np.float32)
+    lib().orc_align_offsets(C.byref(ref_cloud), C.byref(cur_cloud), rows, cols, C.byref(cam),
+                            _p(np.ascontiguousarray(init16, np.float32)), _p(ro) if ro is not None else None,
+                            _p(co) if co is not None else None, C.byref(ap), num_threads, int(fp64_solve), _p(T),
+                            C.byref(res))
+    return T, res
+
+
 def stats_f64(raw, depth_scale, cam, sp, frame):
     n = frame.n
     nrm = np.empty((n, 3), np.float64)

@@ -1497,11 +1497,26 @@ void orc_align(const orc_cloud *ref, const orc_cloud *cur, int rows, int cols, c
 void orc_align_mode(const orc_cloud *ref, const orc_cloud *cur, int rows, int cols, const orc_camera *cam,
                     const float *init_guess, const orc_align_params *ap, int num_threads, int mode, float *T_out,
                     orc_align_result *res) {
+  orc_align_offsets(ref, cur, rows, cols, cam, init_guess, nullptr, nullptr, ap, num_threads, mode, T_out, res);
+}
+
+/* A13 with the aligner's sensor offsets (N/aligner.h:157-196): the current cloud is projected from
+ * the current offset (N/alignerprojective.cpp:36-39), the reference from T * reference offset
+ * (N/alignerprojective.cpp:51); each offset's last row is set to 0 0 0 1 (N/aligner.h:177,194);
+ * NULL is the identity. */
+void orc_align_offsets(const orc_cloud *ref, const orc_cloud *cur, int rows, int cols, const orc_camera *cam,
+                       const float *init_guess, const float *ref_offset, const float *cur_offset,
+                       const orc_align_params *ap, int num_threads, int mode, float *T_out, orc_align_result *res) {
   const size_t P = (size_t)rows * cols;
   std::vector<int> cur_index(P), ref_index(P), pairs(2 * P);
-  float I4[16];
-  set_identity4(I4);
-  project_cam(cam, I4, cur->points, cur->n, rows, cols, cur_index.data(), nullptr);
+  float RO[16], CO[16];
+  set_identity4(RO);
+  set_identity4(CO);
+  if (ref_offset) std::memcpy(RO, ref_offset, sizeof(RO));
+  if (cur_offset) std::memcpy(CO, cur_offset, sizeof(CO));
+  mr(RO, 3, 0) = 0.0f; mr(RO, 3, 1) = 0.0f; mr(RO, 3, 2) = 0.0f; mr(RO, 3, 3) = 1.0f;
+  mr(CO, 3, 0) = 0.0f; mr(CO, 3, 1) = 0.0f; mr(CO, 3, 2) = 0.0f; mr(CO, 3, 3) = 1.0f;
+  project_cam(cam, CO, cur->points, cur->n, rows, cols, cur_index.data(), nullptr);
   float T[16];
   std::memcpy(T, init_guess, sizeof(T));
   float H[36] = {0}, b[6] = {0}, err = 0.0f;
@@ -1509,7 +1524,7 @@ void orc_align_mode(const orc_cloud *ref, const orc_cloud *cur, int rows, int co
   for (int it = 0; it < ap->outer_iterations; ++it) {
     mr(T, 3, 0) = 0.0f; mr(T, 3, 1) = 0.0f; mr(T, 3, 2) = 0.0f; mr(T, 3, 3) = 1.0f;
     float TR[16];
-    iso_mul(T, I4, TR);
+    iso_mul(T, RO, TR);
     project_cam(cam, TR, ref->points, ref->n, rows, cols, ref_index.data(), nullptr);
     float Ti[16];
     iso_inverse(T, Ti);

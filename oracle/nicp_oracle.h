@@ -150,6 +150,9 @@ void orc_align(const orc_cloud *ref, const orc_cloud *cur, int rows, int cols,
                const orc_camera *cam, const float init_guess[16], const orc_align_params *ap,
                int num_threads, float T_out[16], orc_align_result *res);
 /* mode 0 = orc_align; mode 1 = the same loop with H, b and the LDLT in FP64 (updates in FP32) */
+void orc_align_offsets(const orc_cloud *ref, const orc_cloud *cur, int rows, int cols, const orc_camera *cam,
+                       const float init_guess[16], const float ref_offset[16], const float cur_offset[16],
+                       const orc_align_params *ap, int num_threads, int mode, float T_out[16], orc_align_result *res);
 void orc_align_mode(const orc_cloud *ref, const orc_cloud *cur, int rows, int cols,
                     const orc_camera *cam, const float init_guess[16], const orc_align_params *ap,
                     int num_threads, int mode, float T_out[16], orc_align_result *res);

@@ -66,3 +66,35 @@ def spherical_room(rows, cols, cam, pose=None, seed=0, noise=0.002, holes=0.05):
 def spherical_cams(rows, cols, hfov=np.pi, vfov=np.pi / 4, min_d=0.5, max_d=10.0):
     return (O.spherical_camera(rows, cols, hfov, vfov, min_d, max_d),
             capi.spherical_camera(rows, cols, hfov, vfov, min_d, max_d))
+
+
+def offset16(deg, axis, t):
+    """An isometry (column-major 16 floats) built with scipy: rotation by deg about axis, then t."""
+    from scipy.spatial.transform import Rotation
+    M = np.eye(4)
+    M[:3, :3] = Rotation.from_rotvec(np.radians(deg) * np.asarray(axis, np.float64) / np.linalg.norm(axis)).as_matrix()
+    M[:3, 3] = t
+    return M.astype(np.float32).flatten(order="F")
+
+
+def offset_frame_arrays(f, O16):
+    """Copies of an oracle frame's arrays moved by the sensor offset (Cloud::transformInPlace,
+    N/depthimageconverterintegralimage.cpp:63), with the FP64 transform written here:
+    points R p + t, normals R n, information matrices R W R^T."""
+    M = O16.reshape(4, 4, order="F").astype(np.float64)
+    R, t = M[:3, :3], M[:3, 3]
+    pts = f.points.astype(np.float64).copy()
+    pts[:, :3] = pts[:, :3] @ R.T + t
+    nrm = f.normals.astype(np.float64).copy()
+    nrm[:, :3] = nrm[:, :3] @ R.T
+    R4 = np.eye(4)
+    R4[:3, :3] = R
+    R4[3, 3] = 0.0
+
+    def om(W16):
+        W = W16.reshape(-1, 4, 4).transpose(0, 2, 1)  # column-major records
+        W = R4[None] @ W @ R4.T[None]
+        return np.ascontiguousarray(W.transpose(0, 2, 1).reshape(-1, 16), np.float32)
+
+    return (np.ascontiguousarray(pts, np.float32), np.ascontiguousarray(nrm, np.float32),
+            f.curvature.copy(), om(f.omega_p), om(f.omega_n))

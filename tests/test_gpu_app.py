@@ -87,6 +87,22 @@ def test_simple_aligner_trajectory(tmp_path):
         dist = np.linalg.norm(np.array(t[:3]) - P[:3, 3])
         print("frame %d: %.4f deg %.5f m" % (k, ang, dist), flush=True)
         assert ang < 0.1 * max(k, 1) + 1e-6 and dist < 0.003 * max(k, 1) + 1e-9
+    _check_saved_frames(tmp_path, traj)
+
+
+def _check_saved_frames(tmp_path, traj):
+    """Each frame's cloud saved as <depth file>.nicp (nicp_simple_aligner.cpp:175, Cloud::save
+    at cloud.cpp:99-150): header with the point count and step 1, the global transform as its
+    6-vector (t, qx qy qz with qw >= 0), one 128-byte binary record per point."""
+    for k, t in enumerate(traj):
+        raw = (tmp_path / ("d%d.pgm.nicp" % k)).read_bytes()
+        head, t2v, body = raw.split(b"\n", 2)
+        tag, n, step = head.split()
+        assert tag == b"NICPCLOUD" and int(step) == 1 and int(n) > 100000
+        assert len(body) == 128 * int(n)
+        q = np.array(t[3:7])
+        q = q if q[3] >= 0 else -q
+        np.testing.assert_allclose([float(v) for v in t2v.split()], list(t[:3]) + list(q[:3]), atol=2e-6)
 
 
 APP2 = os.path.join(ROOT, "nicp_amd", "nicp_aligner")

@@ -13,7 +13,7 @@ import pytest
 
 import oracle as O
 from nicp_amd import capi, synth
-from tests.helpers import aparams, cams, identity16, params
+from tests.helpers import aparams, cams, identity16, offset16, offset_frame_arrays, params
 
 pytestmark = pytest.mark.gpu
 
@@ -190,6 +190,45 @@ def test_align_vs_oracle(ctx, seed):
     # the device's last correspondences agree with the oracle's up to gate-boundary flips
     dp = ctx.last_correspondences()
     assert abs(len(dp) - r64.num_correspondences) <= max(5, r64.num_correspondences // 5000)
+
+
+def test_align_sensor_offsets_vs_oracle(ctx):
+    """AlignerProjective::align with reference and current sensor offsets (alignerprojective.cpp:
+    36, 51) on clouds moved by them, against the FP64-solve oracle. Equal offsets: within 1e-5 rad
+    / 1e-5 m. Offsets one degree and 2 cm apart make the projection and the error frames disagree:
+    residuals grow, ~0.02% of the inliers sit on a gate and rounding flips them (the oracle's own
+    FP32 and FP64 solves differ by 6e-6..4e-5 rad and 1e-5 m there), so the bar is 1e-4 rad / 1e-4 m,
+    still 300 times below what swapping the two offsets changes (2 deg, 4 cm). Identity offsets give
+    nicp_align bit for bit."""
+    rows, cols = 480, 640
+    say("case align_sensor_offsets 640x480")
+    fr, fc, _, ocam, dcam = _oracle_clouds(7, rows, cols, max_rot_deg=6.0, max_trans_m=0.06)
+    oap, dap = aparams()
+    Tp, rp = ctx.align(_upload(ctx, fr), _upload(ctx, fc), dcam, rows, cols, identity16(), dap)
+    Ti, ri = ctx.align_offsets(_upload(ctx, fr), _upload(ctx, fc), dcam, rows, cols, identity16(), identity16(),
+                               identity16(), dap)
+    np.testing.assert_array_equal(Ti.view(np.uint32), Tp.view(np.uint32))
+    assert ri.inliers == rp.inliers
+    S = offset16(25.0, (0.3, 1, 0.2), (0.15, -0.4, 0.9))
+    S2 = offset16(26.0, (0.3, 1, 0.2), (0.17, -0.4, 0.9))
+    got = {}
+    for ro, co, tag in ((S, S, "equal"), (S, S2, "S,S2"), (S2, S, "S2,S")):
+        ar, ac = offset_frame_arrays(fr, ro), offset_frame_arrays(fc, co)
+        oref, ocur = O.cloud_from_arrays(*ar), O.cloud_from_arrays(*ac)
+        To, r_o = O.align_offsets(oref, ocur, rows, cols, ocam, identity16(), ro, co, oap, 1, fp64_solve=True)
+        To32, _ = O.align_offsets(oref, ocur, rows, cols, ocam, identity16(), ro, co, oap, 1)
+        Td, rd = ctx.align_offsets(ctx.cloud_from_host(*ar), ctx.cloud_from_host(*ac), dcam, rows, cols,
+                                   identity16(), ro, co, dap)
+        rot, tr = synth.pose_error(Td, To)
+        rs, ts = synth.pose_error(To32, To)
+        bar_r = bar_t = 1e-5 if tag == "equal" else 1e-4
+        say("  offsets %s: device vs oracle %.2e rad %.2e m (bar %.0e), oracle FP32 vs FP64 %.2e rad %.2e m, "
+            "inliers %d/%d" % (tag, np.radians(rot), tr, bar_r, np.radians(rs), ts, rd.inliers, r_o.inliers))
+        assert np.radians(rot) < bar_r and tr < bar_t
+        assert abs(rd.inliers - r_o.inliers) <= max(5, r_o.inliers // 2000)
+        got[tag] = (Td, bar_r)
+    swap_r, _ = synth.pose_error(got["S,S2"][0], got["S2,S"][0])
+    assert np.radians(swap_r) > 100 * max(got["S,S2"][1], got["S2,S"][1])
 
 
 def test_solution_statistics_vs_oracle(ctx):

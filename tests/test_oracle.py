@@ -9,7 +9,7 @@ import pytest
 
 import oracle as O
 from nicp_amd import synth
-from tests.helpers import identity16
+from tests.helpers import identity16, offset16, offset_frame_arrays
 
 CAMS = [(131.25, 131.25, 79.5, 59.5), (262.5, 262.5, 159.5, 119.5), (525.0, 525.0, 319.5, 239.5),
         (1050.0, 1050.0, 639.5, 479.5)]
@@ -487,3 +487,36 @@ def test_depth_scale_closed_form():
         safe = np.abs(var - 0.01) > 1e-4
         assert np.allclose(out[keep & safe], mu[keep & safe], rtol=1e-6)
         assert np.all(out[~keep & safe] == 0)
+
+
+# ---- sensor offsets ---------------------------------------------------------------------
+def test_align_sensor_offsets_conjugate():
+    """AlignerProjective::align with sensor offsets (N/alignerprojective.cpp:36,51): clouds
+    converted with the offset S on both sides register to S T S^-1, where T registers the
+    clouds without offsets; identity or absent offsets give the plain align bit for bit."""
+    rows, cols = 240, 320
+    fr, fc, _, cam = _pair_frames(5, rows, cols, max_rot_deg=6.0, max_trans_m=0.06)
+    ap = O.align_params()
+    T0, r0 = O.align(fr.cloud(), fc.cloud(), rows, cols, cam, identity16(), ap, 1, fp64_solve=True)
+    for ro, co in ((None, None), (identity16(), identity16()), (None, identity16())):
+        T1, r1 = O.align_offsets(fr.cloud(), fc.cloud(), rows, cols, cam, identity16(), ro, co, ap, 1, fp64_solve=True)
+        np.testing.assert_array_equal(T1.view(np.uint32), T0.view(np.uint32))
+        assert r1.inliers == r0.inliers
+    for S16 in (offset16(12.0, (1, -2, 0.5), (0.0, 0.0, 0.0)), offset16(25.0, (0.3, 1, 0.2), (0.15, -0.4, 0.9))):
+        ar = offset_frame_arrays(fr, S16)
+        ac = offset_frame_arrays(fc, S16)
+        cr, cc = O.cloud_from_arrays(*ar), O.cloud_from_arrays(*ac)
+        Ts, rs = O.align_offsets(cr, cc, rows, cols, cam, identity16(), S16, S16, ap, 1, fp64_solve=True)
+        S = S16.reshape(4, 4, order="F").astype(np.float64)
+        expect = (S @ T0.reshape(4, 4, order="F").astype(np.float64) @ np.linalg.inv(S)).flatten(order="F")
+        rot, tr = synth.pose_error(Ts, expect)
+        # the same clouds aligned with the offsets ignored
+        Tw, _ = O.align(cr, cc, rows, cols, cam, identity16(), ap, 1, fp64_solve=True)
+        rw, tw = synth.pose_error(Tw, expect)
+        print("offset: %.2e deg %.2e m from S T S^-1 (inliers %d vs %d); without offsets %.3f deg %.4f m" % (
+            rot, tr, rs.inliers, r0.inliers, rw, tw))
+        # the bar: rotation within 2e-3 deg, translation within 1e-4 m (the rotation error times
+        # the offset's 1 m lever arm); ignoring the offsets misses it by an order of magnitude
+        assert rot < 2e-3 and tr < 1e-4
+        assert rw > 1e-2 or tw > 5e-4
+        assert abs(rs.inliers - r0.inliers) <= max(5, r0.inliers // 1000)
