@@ -346,6 +346,9 @@ def main():
     ap_.add_argument("--nn-steps", type=int, default=3)
     ap_.add_argument("--max-rot", type=float, default=MAX_ROT_DEG, help="motion range of the pairs (deg)")
     ap_.add_argument("--max-trans", type=float, default=MAX_TRANS_M, help="motion range of the pairs (m)")
+    ap_.add_argument("--projection", default="auto", choices=["auto", "0", "1", "2"],
+                     help="projection strategy of the batched refinement: autotuned, or forced (0: global "
+                          "z-buffer, 1: global z-buffer read first, 2: tile-binned and fused)")
     ap_.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                      help="only run a device-to-device copy of MIB MiB (to scale rocprof's FETCH_SIZE/WRITE_SIZE)")
     args = ap_.parse_args()
@@ -375,7 +378,10 @@ def main():
     net.set_input_mode(input_mode)
     ap = capi.align_params(outer_iterations=args.iters)
     batch = initnet.make_batch(ctx, B)
-    batch.set_autotune(True)  # the projection strategy is picked during the warm-up
+    if args.projection == "auto":
+        batch.set_autotune(True)  # the projection strategy is picked during the warm-up
+    else:
+        batch.set_projection(int(args.projection))
     ref = np.arange(0, 2 * B, 2, dtype=np.int32)
     cur = ref + 1
 
@@ -483,6 +489,9 @@ def main():
         c.close()
     # per pair: projector pixels, reference points, both-valid pixels, reference frame pixels
     pairs_info = [(ROWS * COLS, frames_info[2 * i][1], res[i].overlap_pixels, ROWS * COLS) for i in range(B)]
+    # the counter summaries hold the default workload only
+    default = (args.pairs, args.iters, args.max_rot, args.max_trans, args.projection) == (
+        128, 10, MAX_ROT_DEG, MAX_TRANS_M, "auto")
     kernels = {}
     total_ms = sum(v[0] for v in prof.values()) if prof else 0.0
     for name, (ms, cnt) in prof.items():
@@ -493,6 +502,10 @@ def main():
         kernels[name] = {"avg_ms": round(avg, 4), "launches": cnt,
                          "GB/s": round(nb / (avg * 1e-3) / 1e9, 1) if nb else None,
                          "share": round(ms / max(total_ms, 1e-9), 3)}
+        tr = measured_traffic(name) if default else None
+        if tr:
+            kernels[name]["traffic_GB"] = round(tr / 1e9, 3)
+            kernels[name]["traffic_over_algorithmic"] = round(tr / nb, 3) if nb else None
     hbm = [k for k in kernels if kernel_bytes(k, pairs_info, frames_info)]
     dom = max(hbm, key=lambda k: prof[k][0]) if hbm else None
     roofline = None
@@ -500,8 +513,6 @@ def main():
         nb = kernel_bytes(dom, pairs_info, frames_info)
         avg = kernels[dom]["avg_ms"]
         ach = nb / (avg * 1e-3) / 1e9
-        # the counter summary holds the default workload only
-        default = (args.pairs, args.iters, args.max_rot, args.max_trans) == (128, 10, MAX_ROT_DEG, MAX_TRANS_M)
         roofline = {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(ach / HBM_PEAK_GBS, 4),
                     "traffic": measured_traffic(dom) if default else None, "bytes_per_launch": nb,
