@@ -315,6 +315,9 @@ struct nicp_batch {
   // projection strategy (nicp_batch_set_projection): -1 the autotuner's choice when it is on,
   // else the tile-binned projection where it applies; 0, 1, 2 forced
   int projection = -1;
+  // set by the first nicp_batch_convert: before it the frames' index images, points and point
+  // counts are uninitialised memory, which align calls and frame clouds would index with
+  bool converted = false;
   int list_cap = kListCap;  // list capacity of the tile-binned projection (nicp_batch_set_bin_capacity)
   // autotuner of the reference projection (nicp_batch_set_autotune): per plan shape, the
   // strategy chosen from the timed iterations of the shape's first call
@@ -1696,6 +1699,7 @@ extern "C" int nicp_batch_convert(nicp_batch *b) {
   b->ctx->prof.sink = &b->ctx->prof.eager;
   CHK(launch_convert(&b->ctx->prof, b->ctx->stream, b->d_frames, b->num_frames, b->d_blk_frame, b->total_tiles, b->d_xblk_frame,
                      b->total_xblk, b->d_yblk_frame, b->total_yblk, b->sp, b->frames[0].model));
+  b->converted = true;
   return NICP_OK;
 }
 
@@ -1708,6 +1712,7 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
                 const nicp_align_params *ap, int use_graph, float *T_out16, nicp_align_result *results,
                 bool with_stats, std::vector<double> *raw, int sub = 1, int twin = 0) {
   if (!b || num_pairs <= 0 || !ref_frame || !cur_frame) return set_err(NICP_ERR_INVALID, "bad arguments");
+  if (!b->converted) return set_err(NICP_ERR_STATE, "the batch's frames have not been converted (nicp_batch_convert)");
   CHK(check_align_params(ap));
   for (int i = 0; i < num_pairs; ++i)
     if (ref_frame[i] < 0 || ref_frame[i] >= b->num_frames || cur_frame[i] < 0 || cur_frame[i] >= b->num_frames)
@@ -1939,6 +1944,7 @@ extern "C" int nicp_batch_set_projection(nicp_batch *b, int strategy) {
 
 extern "C" int nicp_batch_frame_cloud(nicp_batch *b, int frame, nicp_cloud **out) {
   if (!b || !out || frame < 0 || frame >= b->num_frames) return set_err(NICP_ERR_INVALID, "bad arguments");
+  if (!b->converted) return set_err(NICP_ERR_STATE, "the batch's frames have not been converted (nicp_batch_convert)");
   nicp_cloud *cl = new (std::nothrow) nicp_cloud();
   if (!cl) return set_err(NICP_ERR_OOM, "out of host memory");
   HIPCHK(hipSetDevice(b->ctx->device));

@@ -121,3 +121,24 @@ def test_ragged_batch_with_empty_and_tiny_frames(ctx):
         Ts, rs = ctx.align(cr, cc, dcam, rows, cols, identity16(), dap)
         assert np.array_equal(Tb[i].view(np.uint32), Ts.view(np.uint32)), i
         assert rb[i].inliers == rs.inliers
+
+
+def test_batch_refuses_work_before_conversion(ctx):
+    """A new batch's index images, points and point counts are uninitialised until the first
+    nicp_batch_convert: aligning its frames or taking a frame's cloud before it is an error."""
+    print("case edges batch before conversion", flush=True)
+    _, dsp = params()
+    _, dap = aparams()
+    _, dcam = cams(240, 320)
+    b = capi.Batch(ctx, [(240, 320)] * 2, [dcam, dcam], dsp)
+    dr, dc, _ = synth.pair(3, 240, 320)
+    b.upload(0, dr)
+    b.upload(1, dc)
+    with pytest.raises(capi.NicpError):
+        b.align([0], [1], None, dap, use_graph=False)
+    with pytest.raises(capi.NicpError):
+        b.frame_cloud(0, (240, 320))
+    b.convert()
+    T, r = b.align([0], [1], None, dap, use_graph=False)
+    assert np.all(np.isfinite(T)) and r[0].inliers > 0
+    b.close()
