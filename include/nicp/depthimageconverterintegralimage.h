@@ -113,6 +113,7 @@ class DepthImageConverterIntegralImage {
     nicp_camera cam = _projector->camera();
     nicp_stats_params sp = statsParams();
     nicp_cloud *h = nullptr;
+    check(nicp_ctx_set_fp64_eigen_gap(Context::get(), _fp64EigenGap));
     check(nicp_cloud_from_depth_u16(Context::get(), depth.data.data(), depth.rows, depth.cols, depthScale, &cam, &sp,
                                     _keepStats ? 1 : 0, &h));
     cloud.reset(h);
@@ -120,6 +121,7 @@ class DepthImageConverterIntegralImage {
     check(nicp_cloud_gaussians(h, _projector->baseline(), _projector->alpha()));
     _indexImage.create(depth.rows, depth.cols);
     check(nicp_cloud_index_image(h, _indexImage.data.data()));
+    applySensorOffset(h, sensorOffset);
   }
   // DepthImageConverterIntegralImage::compute(cloud, depthImage, sensorOffset) for a float image in metres
   void compute(Cloud &cloud, const DepthImage &depth, const Isometry3f &sensorOffset = Isometry3f::Identity()) {
@@ -127,6 +129,7 @@ class DepthImageConverterIntegralImage {
     nicp_camera cam = _projector->camera();
     nicp_stats_params sp = statsParams();
     nicp_cloud *h = nullptr;
+    check(nicp_ctx_set_fp64_eigen_gap(Context::get(), _fp64EigenGap));
     check(nicp_cloud_from_depth_f32(Context::get(), depth.data.data(), depth.rows, depth.cols, &cam, &sp,
                                     _keepStats ? 1 : 0, &h));
     cloud.reset(h);
@@ -134,8 +137,13 @@ class DepthImageConverterIntegralImage {
     check(nicp_cloud_gaussians(h, _projector->baseline(), _projector->alpha()));
     _indexImage.create(depth.rows, depth.cols);
     check(nicp_cloud_index_image(h, _indexImage.data.data()));
+    applySensorOffset(h, sensorOffset);
   }
   const IntImage &indexImage() const { return _indexImage; }
+  // FP64 eigensolve of near-degenerate windows (no reference counterpart; see
+  // nicp_ctx_set_fp64_eigen_gap): 0, the default, keeps the reference's FP32 solve
+  float fp64EigenGap() const { return _fp64EigenGap; }
+  void setFP64EigenGap(float gap) { _fp64EigenGap = gap; }
   PointProjector *projector() const { return _projector; }
   void setProjector(PointProjector *p) { _projector = p; }
   // The reference's clouds always hold their Stats (nicp/nicp/cloud.h:190); here they are
@@ -144,7 +152,15 @@ class DepthImageConverterIntegralImage {
   void setKeepStats(bool k) { _keepStats = k; }
 
  private:
+  // Cloud::transformInPlace(sensorOffset) (depthimageconverterintegralimage.cpp:63,
+  // cloud.cpp:191-204): bottom row zeroed, skipped for the identity
+  static void applySensorOffset(nicp_cloud *h, const Isometry3f &sensorOffset) {
+    Isometry3f o = sensorOffset;
+    o.setBottomRow();
+    if (!o.isIdentity()) check(nicp_cloud_transform(h, o.matrix()));
+  }
   bool _keepStats = false;
+  float _fp64EigenGap = 0.0f;
   PointProjector *_projector;
   StatsCalculatorIntegralImage *_statsCalculator;
   PointInformationMatrixCalculator *_pointCalc;

@@ -117,6 +117,15 @@ int nicp_ctx_destroy(nicp_ctx *ctx);
 int nicp_ctx_synchronize(nicp_ctx *ctx);
 /* the HIP stream the context launches on (hipStream_t) */
 void *nicp_ctx_stream(nicp_ctx *ctx);
+/* FP64 eigensolve of near-degenerate windows (no reference counterpart: the reference's
+ * StatsCalculatorIntegralImage::compute, nicp/nicp/statscalculatorintegralimage.cpp:60-61,
+ * always solves in FP32). gap > 0: every later conversion of this context solves a pixel's
+ * window covariance again in FP64 when its FP32 eigenvalues have a relative gap
+ * (l1 - l0) / l2 below gap, and takes those eigenvalues, eigenvectors, normal and curvature;
+ * the normals are then the eigenvectors of the same FP32 covariance to FP32 rounding, which
+ * near degeneracy departs from the reference's FP32 normals by up to ~1e-3. gap = 0 (the
+ * default) keeps the reference's FP32 solve everywhere. */
+int nicp_ctx_set_fp64_eigen_gap(nicp_ctx *ctx, float gap);
 /* plain device memory on the context's device (no reference counterpart; for callers that
  * keep datasets or network inputs resident). kind: 0 host->device, 1 device->host,
  * 2 device->device; copies complete before the call returns. */

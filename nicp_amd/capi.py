@@ -68,7 +68,7 @@ EXPORTS = [
     "nicp_last_error", "nicp_ctx_create", "nicp_ctx_destroy", "nicp_ctx_synchronize", "nicp_ctx_stream",
     "nicp_cloud_from_depth_u16", "nicp_cloud_from_host", "nicp_cloud_destroy", "nicp_cloud_size",
     "nicp_cloud_download", "nicp_cloud_download_stats", "nicp_cloud_index_image", "nicp_project",
-    "nicp_correspondences", "nicp_linearize", "nicp_align", "nicp_align_offsets", "nicp_last_correspondences", "nicp_batch_create",
+    "nicp_correspondences", "nicp_linearize", "nicp_align", "nicp_align_offsets", "nicp_ctx_set_fp64_eigen_gap", "nicp_last_correspondences", "nicp_batch_create",
     "nicp_batch_destroy", "nicp_batch_upload_depth", "nicp_batch_convert", "nicp_batch_align",
     "nicp_batch_frame_cloud", "nicp_kernel_count", "nicp_kernel_name", "nicp_ctx_profile",
     "nicp_ctx_profile_read", "nicp_nn_correspondences", "nicp_align_nn", "nicp_cloud_from_depth_f32",
@@ -100,7 +100,7 @@ def lib():
         L.nicp_kernel_count.restype = C.c_int
         sig = {
             "nicp_ctx_create": [C.c_int, vp], "nicp_ctx_destroy": [vp], "nicp_ctx_synchronize": [vp],
-            "nicp_ctx_stream": [vp],
+            "nicp_ctx_stream": [vp], "nicp_ctx_set_fp64_eigen_gap": [vp, C.c_float],
             "nicp_cloud_from_depth_u16": [vp, vp, C.c_int, C.c_int, C.c_float, vp, vp, C.c_int, vp],
             "nicp_cloud_from_host": [vp, C.c_int, vp, vp, vp, vp, vp, vp],
             "nicp_cloud_destroy": [vp], "nicp_cloud_size": [vp, vp],
@@ -255,6 +255,11 @@ class Context:
 
     def synchronize(self):
         _check(lib().nicp_ctx_synchronize(self.h))
+
+    def set_fp64_eigen_gap(self, gap):
+        """Later conversions solve windows whose FP32 eigenvalue gap (l1 - l0) / l2 is below
+        `gap` again in FP64 (0: the reference's FP32 solve everywhere)."""
+        _check(lib().nicp_ctx_set_fp64_eigen_gap(self.h, float(gap)))
 
     def profile(self, enable=True):
         """Starts (and resets) or stops per-kernel HIP-event timing on this context."""

@@ -251,6 +251,7 @@ struct Prof {
 // ---------------------------------------------------------------------------------------
 struct nicp_ctx {
   int device = 0;
+  float fp64_eigen_gap = 0.0f;  // nicp_ctx_set_fp64_eigen_gap: applied to every conversion
   void *noalloc_ws = nullptr;  // device address of the fixed workspace on this device
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -902,6 +903,13 @@ extern "C" int nicp_ctx_synchronize(nicp_ctx *c) {
 
 extern "C" void *nicp_ctx_stream(nicp_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
+extern "C" int nicp_ctx_set_fp64_eigen_gap(nicp_ctx *c, float gap) {
+  if (!c) return set_err(NICP_ERR_INVALID, "ctx is null");
+  if (!(gap >= 0.0f && gap <= 1.0f)) return set_err(NICP_ERR_INVALID, "the FP64 eigen gap must lie in [0, 1]");
+  c->fp64_eigen_gap = gap;
+  return NICP_OK;
+}
+
 // plain device buffers for callers that keep data resident (datasets, network inputs)
 extern "C" int nicp_device_alloc(nicp_ctx *c, size_t bytes, void **ptr) {
   if (!c || !ptr) return set_err(NICP_ERR_INVALID, "bad arguments");
@@ -1002,8 +1010,9 @@ int cloud_from_depth(nicp_ctx *ctx, const uint16_t *depth, const float *depthf, 
     if (e == hipSuccess) e = hipMemcpyAsync(d_maps, zeros.data(), zeros.size() * sizeof(int), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) { rc = set_err(NICP_ERR_HIP, "upload: %s", hipGetErrorString(e)); break; }
     ctx->prof.sink = &ctx->prof.eager;
-    rc = launch_convert(&ctx->prof, s, d_f, 1, d_maps, nt, d_maps + nt, nx, d_maps + nt + nx, ny, to_dev(sp, keep_stats != 0),
-                        cam->model);
+    StatsParamsDev spd = to_dev(sp, keep_stats != 0);
+    spd.fp64_gap = ctx->fp64_eigen_gap;
+    rc = launch_convert(&ctx->prof, s, d_f, 1, d_maps, nt, d_maps + nt, nx, d_maps + nt + nx, ny, spd, cam->model);
     if (rc != NICP_OK) break;
     e = hipMemcpyAsync(&cl->n, f.npoints, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1697,8 +1706,10 @@ extern "C" int nicp_batch_convert(nicp_batch *b) {
   if (!b) return set_err(NICP_ERR_INVALID, "batch is null");
   HIPCHK(hipSetDevice(b->ctx->device));
   b->ctx->prof.sink = &b->ctx->prof.eager;
+  StatsParamsDev spd = b->sp;
+  spd.fp64_gap = b->ctx->fp64_eigen_gap;
   CHK(launch_convert(&b->ctx->prof, b->ctx->stream, b->d_frames, b->num_frames, b->d_blk_frame, b->total_tiles, b->d_xblk_frame,
-                     b->total_xblk, b->d_yblk_frame, b->total_yblk, b->sp, b->frames[0].model));
+                     b->total_xblk, b->d_yblk_frame, b->total_yblk, spd, b->frames[0].model));
   b->converted = true;
   return NICP_OK;
 }

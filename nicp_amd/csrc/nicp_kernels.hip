@@ -805,6 +805,69 @@ __device__ __forceinline__ void eig3_direct(const float *cov, float *ev, float *
   ev[0] = ev[0] + shift; ev[1] = ev[1] + shift; ev[2] = ev[2] + shift;
 }
 
+// The symmetric 3x3 eigenproblem in FP64 by cyclic Jacobi rotations (a rotation per
+// off-diagonal element and sweep, t = sgn(th) / (|th| + sqrt(th^2 + 1)) with
+// th = (a_qq - a_pp) / (2 a_pq), until the off-diagonal part vanishes against the diagonal),
+// for the near-degenerate pixels of the FP64 option: when the two smallest eigenvalues nearly
+// coincide the FP32 eigenvector of the smallest one carries an error of about
+// FLT_EPSILON * l2 / (l1 - l0). cov column-major FP32 as the FP32 solve takes it; eigenvalues
+// ascending and eigenvectors (column 0 = smallest) rounded to FP32.
+__device__ __forceinline__ void eig3_jacobi_f64(const float *cov, float *ev, float *V) {
+  double a[3][3], v[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      // the symmetric part (the FP32 matrix is symmetric up to the order of its two products)
+      a[i][j] = 0.5 * ((double)cov[j * 3 + i] + (double)cov[i * 3 + j]);
+      v[i][j] = i == j ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    const double dia = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+    if (off <= 1e-34 * dia || off == 0.0) break;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int p = k == 2 ? 1 : 0, q = k == 0 ? 1 : 2;
+      const double apq = a[p][q];
+      if (apq == 0.0) continue;
+      const double th = (a[q][q] - a[p][p]) / (2.0 * apq);
+      const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {  // columns p, q of A and V
+        const double arp = a[r][p], arq = a[r][q];
+        a[r][p] = c * arp - sn * arq;
+        a[r][q] = sn * arp + c * arq;
+        const double vrp = v[r][p], vrq = v[r][q];
+        v[r][p] = c * vrp - sn * vrq;
+        v[r][q] = sn * vrp + c * vrq;
+      }
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {  // rows p, q of A
+        const double apr = a[p][r], aqr = a[q][r];
+        a[p][r] = c * apr - sn * aqr;
+        a[q][r] = sn * apr + c * aqr;
+      }
+      a[p][q] = a[q][p] = 0.0;
+    }
+  }
+  // ascending order
+  int o0 = 0, o1 = 1, o2 = 2, tmp;
+  if (a[o1][o1] < a[o0][o0]) { tmp = o0; o0 = o1; o1 = tmp; }
+  if (a[o2][o2] < a[o1][o1]) { tmp = o1; o1 = o2; o2 = tmp; }
+  if (a[o1][o1] < a[o0][o0]) { tmp = o0; o0 = o1; o1 = tmp; }
+  const int ord[3] = {o0, o1, o2};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int j = ord[k];
+    ev[k] = (float)a[j][j];
+    const double nrm = sqrt(v[0][j] * v[0][j] + v[1][j] * v[1][j] + v[2][j] * v[2][j]);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) V[3 * k + r] = (float)(v[r][j] / nrm);
+  }
+}
+
 // Omega = U diag(dg) U^T in the association order of Eigen's lazy products
 // (informationmatrixcalculator.cpp:26-28); returns the upper triangle 00 01 02 11 12 22
 __device__ __forceinline__ void omega6(const float *U, const float *dg, float *o) {
@@ -914,6 +977,7 @@ __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const
       cov[7] = acc[8] * dn - m1 * m2;
       cov[8] = acc[9] * dn - m2 * m2;
       eig3_direct(cov, ev, V);
+      if (sp.fp64_gap > 0.0f && !(ev[1] - ev[0] >= sp.fp64_gap * fabsf(ev[2]))) eig3_jacobi_f64(cov, ev, V);
       if (ev[0] < 0.0f) ev[0] = 0.0f;
       mean[0] = m0; mean[1] = m1; mean[2] = m2;
       // Stats::curvature (stats.h:93-98): the 1e-9 literal makes the last add and the divide double

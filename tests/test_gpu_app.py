@@ -155,3 +155,40 @@ def test_scene_aligner_trajectory(tmp_path):
     assert scenes == ["scene-001.nicp", "scene-004.nicp", "scene-005.nicp"], scenes
     for k in range(len(poses)):
         assert (tmp_path / ("d%d.pgm.nicp" % k)).stat().st_size > 128 * 100000
+
+
+@pytest.mark.gpu
+def test_front_end_sensor_offset(tmp_path):
+    """DepthImageConverterIntegralImage::compute(cloud, depth, offset) and
+    Aligner::setSensorOffset through the C++ front end give, bit for bit, the C ABI's clouds moved
+    by the offset (nicp_cloud_transform) aligned with nicp_align_offsets."""
+    from nicp_amd import capi
+    from tests.helpers import aparams, cams, identity16, offset16, params
+    print("case front end sensor offset", flush=True)
+    rows, cols = 480, 640
+    dr, dc, _ = synth.pair(11, rows, cols, max_rot_deg=6.0, max_trans_m=0.06)
+    write_pgm16(str(tmp_path / "a.pgm"), dr)
+    write_pgm16(str(tmp_path / "b.pgm"), dc)
+    S = offset16(20.0, (0.2, 1, -0.3), (0.1, -0.3, 0.8))
+    exe = str(tmp_path / "off")
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), "-o", exe,
+                           os.path.join(ROOT, "tests", "cpp", "offset_front.cpp"), "-L", os.path.join(ROOT, "nicp_amd"),
+                           "-lnicp_amd", "-Wl,-rpath," + os.path.join(ROOT, "nicp_amd")])
+    out = subprocess.run([exe, str(tmp_path / "a.pgm"), str(tmp_path / "b.pgm")] + ["%.9g" % v for v in S],
+                         capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0
+    lines = dict((l.split()[0], l.split()[1:]) for l in out.stdout.splitlines() if l.strip())
+    Tf = np.array([float(v) for v in lines["T"]], np.float32)
+    ctx = capi.Context(0)
+    _, dsp = params()
+    _, dap = aparams()
+    _, dcam = cams(rows, cols)
+    S32 = np.array(["%.9g" % v for v in S], np.float32)
+    cr, cc = ctx.cloud_from_depth(dr, dcam, dsp), ctx.cloud_from_depth(dc, dcam, dsp)
+    cr.transform(S32)
+    cc.transform(S32)
+    T, res = ctx.align_offsets(cr, cc, dcam, rows, cols, identity16(), S32, S32, dap)
+    np.testing.assert_array_equal(Tf.view(np.uint32), T.view(np.uint32))
+    assert int(lines["inliers"][0]) == res.inliers
+    ctx.close()
