@@ -195,3 +195,43 @@ def test_graph_replay_after_map_growth(ctx):
     np.testing.assert_array_equal(Tg1.view(np.uint32), Te.view(np.uint32))
     np.testing.assert_array_equal(Tg2.view(np.uint32), Te.view(np.uint32))
     b.close()
+
+
+def test_ground_truth_ceiling_640x480(ctx):
+    """The accuracy ceiling of the learned workload at its size: 1024 held-out 640x480 pairs
+    (<=30 deg / 0.5 m) aligned from the ground truth, from the identity and from the learned
+    start. No start can do better than the ground truth by more than chance, and at these motions
+    NICP's own fixed point leaves most pairs beyond 0.1 deg even from there."""
+    w = initnet.load_weights()
+    assert w is not None, "trained weights missing"
+    B, n_batches = 128, 8
+    say("case ground_truth_ceiling 640x480 pairs=%d" % (B * n_batches))
+    net = capi.InitNet(ctx, seed=1)
+    net.set_params(w)
+    _, ap = aparams()
+    b = initnet.make_batch(ctx, B)
+    ref = np.arange(0, 2 * B, 2, dtype=np.int32)
+    cur = ref + 1
+    a_gt = a_id = a_net = 0
+    drift = []
+    for k in range(n_batches):
+        seeds = np.arange(1000 + k * B, 1000 + (k + 1) * B, dtype=np.uint64)
+        Tgt = initnet.render_pairs(b, seeds, 30.0, 0.5)
+        b.convert()
+        Tg, _ = b.align(ref, cur, np.ascontiguousarray(Tgt, np.float32), ap, use_graph=True)
+        Ti, _ = b.align(ref, cur, None, ap, use_graph=True)
+        Tn, _, _, _ = b.align_learned(net, ref, cur, ap, use_graph=True)
+        eg = _errors(Tg, Tgt)
+        drift.append(eg[:, 0])
+        a_gt += int(_aligned(eg).sum())
+        a_id += int(_aligned(_errors(Ti, Tgt)).sum())
+        a_net += int(_aligned(_errors(Tn, Tgt)).sum())
+    drift = np.concatenate(drift)
+    n = B * n_batches
+    say("  aligned (<0.1 deg, <2 mm) of %d: from the ground truth %d, learned %d, identity %d; drift from the ground "
+        "truth: median %.3f deg, 90th percentile %.3f deg" % (n, a_gt, a_net, a_id, float(np.median(drift)),
+                                                             float(np.percentile(drift, 90))))
+    assert a_net <= a_gt + n // 100 and a_id <= a_gt + n // 100
+    assert a_gt < n // 4
+    b.close()
+    net.close()

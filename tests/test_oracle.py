@@ -2,7 +2,7 @@
 closed forms, brute force on small inputs, library routines (LAPACK, numpy),
 invariants and finite differences. The reference ships no tests, fixtures or
 golden vectors for this path, and it cannot be compiled here (Eigen3, OpenCV
-and FLANN are absent), so these checks and tests/golden/ are what pin it.
+and FLANN are absent), so these checks are what pin it.
 """
 import numpy as np
 import pytest
