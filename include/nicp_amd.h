@@ -372,7 +372,8 @@ int nicp_batch_net_input(nicp_batch *batch, nicp_initnet *net, int num_pairs, co
                          const int *cur_frame, float max_depth_cov, float *dev_out);
 /* AlignerProjective::align (nicp/nicp/alignerprojective.cpp:21-136) from a learned start:
  * the network predicts a start per pair; one linearisation at the prediction and at the
- * fallback guess (fallback16, or the identity when NULL) judges both, and the prediction is
+ * fallback guess (fallback16, or the identity when NULL), over every 8th row and column of the
+ * projector image with the whole reference cloud projected, judges both, and the prediction is
  * kept unless the fallback has a lower chi2 per correspondence with at least half the
  * prediction's correspondences (or the prediction has none). start16 (may be NULL) receives
  * the chosen starts, used_network (may be NULL) 1 where the prediction was kept.

@@ -30,7 +30,7 @@ namespace initnet {
 
 constexpr int kLayers = 6;
 constexpr int kFcSplits = 16;  // most reduction chunks of a forward GEMM (fwd_splits)
-constexpr int kProbeSub = 4;  // pixel stride of the start probe's linearisation
+constexpr int kProbeSub = 8;  // pixel stride of the start probe's linearisation
 constexpr int kInRows = 60, kInCols = 80, kInCh = 2;
 
 struct LayerShape {
@@ -687,14 +687,16 @@ int check_net_frames(const nicp_batch *b, int num_pairs, const int *ref, const i
 
 // the decision rule of nicp_batch_align_learned (chi2 and inlier count of one linearisation
 // at each start)
-bool fallback_wins(double chi2_net, double inl_net, double chi2_fb, double inl_fb) {
+NICP_HD bool fallback_wins(double chi2_net, double inl_net, double chi2_fb, double inl_fb) {
   if (!(inl_net > 0.0)) return inl_fb > 0.0;
   if (!(inl_fb >= 0.5 * inl_net)) return false;
   return chi2_fb / inl_fb < chi2_net / inl_net;
 }
 
-int launch_net_input(nicp_initnet *n, nicp_batch *b, int num_pairs, const int *ref, const int *cur, float max_cov,
-                     float *dst) {
+// The network input's job table, uploaded ahead of the kernels that read it: from the batch's
+// pinned staging when given (asynchronous), else from a pageable vector, waited for.
+int upload_net_jobs(nicp_initnet *n, nicp_batch *b, int num_pairs, const int *ref, const int *cur,
+                    PinnedStage *stage = nullptr) {
   hipStream_t s = b->ctx->stream;
   if (2 * num_pairs > n->jobs_cap) {
     HIPCHK(hipStreamSynchronize(s));
@@ -710,14 +712,91 @@ int launch_net_input(nicp_initnet *n, nicp_batch *b, int num_pairs, const int *r
       const FrameDev &fd = b->frames[ch ? cur[i] : ref[i]];
       jobs[2 * i + ch] = InputJob{fd.depth, fd.depth_scale, fd.rows, fd.cols, fd.cols / kInCols};
     }
-  HIPCHK(hipMemcpyAsync(n->jobs, jobs.data(), jobs.size() * sizeof(InputJob), hipMemcpyHostToDevice, s));
-  const size_t cnt = (size_t)num_pairs * kInRows * kInCols * kInCh;
-  hipLaunchKernelGGL(k_net_input, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, (const InputJob *)n->jobs,
-                     num_pairs, max_cov, dst);
-  HIPCHK(hipGetLastError());
-  // the host job table must outlive the copy, and the device table is rewritten by the next call
-  HIPCHK(hipStreamSynchronize(s));
+  const void *src = stage ? stage->put(jobs.data(), jobs.size() * sizeof(InputJob)) : jobs.data();
+  HIPCHK(hipMemcpyAsync(n->jobs, src, jobs.size() * sizeof(InputJob), hipMemcpyHostToDevice, s));
+  // a pageable vector must outlive the copy
+  if (!stage) HIPCHK(hipStreamSynchronize(s));
   return NICP_OK;
+}
+
+int enqueue_net_input(nicp_initnet *n, nicp_batch *b, int num_pairs, float max_cov, float *dst) {
+  const size_t cnt = (size_t)num_pairs * kInRows * kInCols * kInCh;
+  hipLaunchKernelGGL(k_net_input, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, b->ctx->stream,
+                     (const InputJob *)n->jobs, num_pairs, max_cov, dst);
+  HIPCHK(hipGetLastError());
+  return NICP_OK;
+}
+
+int launch_net_input(nicp_initnet *n, nicp_batch *b, int num_pairs, const int *ref, const int *cur, float max_cov,
+                     float *dst) {
+  CHK(upload_net_jobs(n, b, num_pairs, ref, cur));
+  CHK(enqueue_net_input(n, b, num_pairs, max_cov, dst));
+  // the device table is rewritten by the next call
+  HIPCHK(hipStreamSynchronize(b->ctx->stream));
+  return NICP_OK;
+}
+
+// A pair's loop state at start T (AlignerProjective::align, alignerprojective.cpp:44-51): T, its
+// inverse and the reference projector K (T * reference offset)^-1, with the offset already in
+// the state; the same arithmetic as the host's initial state.
+__device__ void set_start_state(const PairDev &pd, const float *Tin) {
+  float T[16], Ti[16], RO[16], TR[16], KRt[16];
+  for (int k = 0; k < 16; ++k) T[k] = Tin[k];
+  set_row3(T);
+  iso_inverse(T, Ti);
+  for (int k = 0; k < 16; ++k) RO[k] = pd.state[kSlotRefOffset + k];
+  iso_mul(T, RO, TR);
+  projector_state(pd.model, pd.K, TR, KRt);
+  for (int k = 0; k < 16; ++k) {
+    pd.state[kSlotT + k] = T[k];
+    pd.state[kSlotTinv + k] = Ti[k];
+    pd.state[kSlotKRt + k] = KRt[k];
+  }
+}
+
+// The two candidate starts of each pair: pair i of the probe starts at the network's prediction
+// v2t(x) (non-finite entries set to 0, the quaternion's vector part scaled into the ball of
+// radius sqrt(0.99)), pair n + i at the fallback guess its state already holds; starts16 keeps
+// both (2n x 16) for the choice, which runs after the probe has moved the probe's state.
+__global__ __launch_bounds__(64) void k_probe_starts(const float *__restrict__ pred, int n, const PairDev *__restrict__ Pp,
+                                                     float *__restrict__ starts16) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= n) return;
+  float x[6];
+  for (int k = 0; k < 6; ++k) {
+    const float v = pred[6 * i + k];
+    x[k] = isfinite(v) ? v : 0.0f;
+  }
+  const float qn = x[3] * x[3] + x[4] * x[4] + x[5] * x[5];
+  if (qn > 0.99f) {
+    const float sc = sqrtf(0.99f / qn);
+    x[3] *= sc;
+    x[4] *= sc;
+    x[5] *= sc;
+  }
+  float T[16];
+  v2t<float>(x, T);
+  set_start_state(Pp[i], T);
+  for (int k = 0; k < 16; ++k) {
+    starts16[16 * i + k] = Pp[i].state[kSlotT + k];
+    starts16[16 * (n + i) + k] = Pp[n + i].state[kSlotT + k];
+  }
+}
+
+// The choice: the prediction is kept unless the fallback has a lower chi2 per correspondence with
+// at least half the prediction's correspondences (result slots 27 = chi2, 28 = inliers of the
+// probe's linearisation), or the prediction has none; the refinement's pair i starts there.
+__global__ __launch_bounds__(64) void k_choose_starts(const PairDev *__restrict__ probe, const PairDev *__restrict__ refine,
+                                                      int n, const float *__restrict__ starts16,
+                                                      float *__restrict__ chosen16, int *__restrict__ used) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= n) return;
+  const double *a = probe[i].result, *f = probe[n + i].result;
+  const bool net = !fallback_wins(a[27], a[28], f[27], f[28]);
+  const float *T = starts16 + 16 * (size_t)(net ? i : n + i);
+  for (int k = 0; k < 16; ++k) chosen16[16 * i + k] = T[k];
+  used[i] = net ? 1 : 0;
+  set_start_state(refine[i], T);
 }
 
 }  // namespace
@@ -765,15 +844,52 @@ extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_
   nicp_ctx *ctx = b->ctx;
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  // 1. prediction
   Prof &pf = ctx->prof;
   pf.sink = &pf.eager;
+  // Everything the device needs is prepared first, so that the network, the start probe, the
+  // choice and the refinement then run back to back with no host round trip:
+  // - the probe: pairs [0, n) start at the prediction (set on the device), [n, 2n) at the
+  //   fallback guess; one linearisation over every kProbeSub-th row and column, the whole
+  //   reference cloud projected (pairs i and n + i share their frames, so one pass reads each
+  //   reference point for both starts);
+  // - the refinement of the n pairs, whose starts the choice sets on the device.
   CHK(net_reserve(n, num_pairs));
+  std::vector<float> init2((size_t)2 * num_pairs * 16);
+  std::vector<int> ref2(2 * (size_t)num_pairs), cur2(2 * (size_t)num_pairs);
+  for (int i = 0; i < num_pairs; ++i) {
+    for (int k = 0; k < 16; ++k) {
+      const float id = k % 5 == 0 ? 1.0f : 0.0f;
+      init2[16 * (size_t)i + k] = id;
+      init2[16 * ((size_t)num_pairs + i) + k] = fallback16 ? fallback16[16 * (size_t)i + k] : id;
+    }
+    ref2[i] = ref2[num_pairs + i] = ref_frame[i];
+    cur2[i] = cur2[num_pairs + i] = cur_frame[i];
+  }
+  nicp_align_params probe = *ap;
+  probe.outer_iterations = 1;
+  probe.inner_iterations = 1;
+  BatchCall pc, rc;
+  CHK(batch_plan(b, probe_buffers(b), 2 * num_pairs, ref2.data(), cur2.data(), &probe, false, kProbeSub, num_pairs,
+                 pc));
+  CHK(batch_plan(b, batch_buffers(b), num_pairs, ref_frame, cur_frame, ap, true, 1, 0, rc));
+  CHK(b->learned_aux.reserve(((size_t)3 * num_pairs * 16 + num_pairs) * sizeof(float)));
+  float *d_starts = b->learned_aux.as<float>();
+  float *d_chosen = d_starts + (size_t)2 * num_pairs * 16;
+  int *d_used = (int *)(d_chosen + (size_t)num_pairs * 16);
+  // all uploads through the batch's pinned staging: none of them waits for earlier work
+  CHK(b->stage.begin(prepare_stage_bytes(pc.plan) + prepare_stage_bytes(rc.plan) +
+                     PinnedStage::slot((size_t)2 * num_pairs * sizeof(InputJob))));
+  CHK(batch_prepare(b, pc, init2.data(), &b->stage));
+  CHK(batch_prepare(b, rc, nullptr, &b->stage));
+  CHK(upload_net_jobs(n, b, num_pairs, ref_frame, cur_frame, &b->stage));
+  CHK(b->stage.end(s));
+  HIPCHK(hipEventRecord(ctx->ev0, s));
+  // 1. prediction
   {
     ProfEntry en{};
     const bool p = pf.on;
     if (p) pf.begin(KID_NET_INPUT, s, en);
-    CHK(launch_net_input(n, b, num_pairs, ref_frame, cur_frame, 0.01f, n->act[0]));
+    CHK(enqueue_net_input(n, b, num_pairs, 0.01f, n->act[0]));
     if (p) pf.end(s, en);
   }
   {
@@ -783,65 +899,36 @@ extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_
     CHK(net_forward(n, num_pairs));
     if (p) pf.end(s, en);
   }
-  std::vector<float> pred((size_t)num_pairs * 6);
-  HIPCHK(hipMemcpyAsync(pred.data(), n->act[kLayers], pred.size() * sizeof(float), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  // 2. both starts judged by one linearisation over every kProbeSub-th row and column (the
-  //    whole reference cloud is projected, so the correspondences there are the exact ones):
-  //    pairs [0, n) start at the prediction, [n, 2n) at the fallback guess; pairs i and n + i
-  //    share their frames, so one pass reads each reference point for both projections
-  std::vector<float> starts((size_t)2 * num_pairs * 16);
-  std::vector<int> ref2(2 * (size_t)num_pairs), cur2(2 * (size_t)num_pairs);
-  for (int i = 0; i < num_pairs; ++i) {
-    float x[6];
-    for (int k = 0; k < 6; ++k) x[k] = std::isfinite(pred[6 * i + k]) ? pred[6 * i + k] : 0.0f;
-    // keep the quaternion's vector part inside the unit ball
-    const float qn = x[3] * x[3] + x[4] * x[4] + x[5] * x[5];
-    if (qn > 0.99f) {
-      const float sc = std::sqrt(0.99f / qn);
-      x[3] *= sc;
-      x[4] *= sc;
-      x[5] *= sc;
-    }
-    v2t<float>(x, &starts[16 * (size_t)i]);
-    float *fb = &starts[16 * ((size_t)num_pairs + i)];
-    for (int k = 0; k < 16; ++k) fb[k] = fallback16 ? fallback16[16 * (size_t)i + k] : (k % 5 == 0 ? 1.0f : 0.0f);
-    ref2[i] = ref2[num_pairs + i] = ref_frame[i];
-    cur2[i] = cur2[num_pairs + i] = cur_frame[i];
-  }
-  nicp_align_params probe = *ap;
-  probe.outer_iterations = 1;
-  probe.inner_iterations = 1;
-  std::vector<double> raw;
+  // 2. both starts judged by the probe, 3. the choice (timed as one stage: its kernels are not
+  //    profiled one by one)
   {
-    // timed as one stage: its kernels are not profiled one by one
     ProfEntry en{};
     const bool p = pf.on;
     if (p) pf.begin(KID_START_PROBE, s, en);
     pf.on = false;
-    const int rc = batch_align(b, 2 * num_pairs, ref2.data(), cur2.data(), starts.data(), &probe, 0, nullptr, nullptr,
-                               false, &raw, kProbeSub, num_pairs);
+    const unsigned g = (unsigned)((num_pairs + 63) / 64);
+    hipLaunchKernelGGL(k_probe_starts, dim3(g), dim3(64), 0, s, (const float *)n->act[kLayers], num_pairs,
+                       (const PairDev *)pc.d_pairs, d_starts);
+    int r = hipGetLastError() == hipSuccess ? NICP_OK : set_err(NICP_ERR_HIP, "k_probe_starts launch failed");
+    if (r == NICP_OK) r = batch_run(b, pc, 0, b->graph);
+    if (r == NICP_OK) {
+      hipLaunchKernelGGL(k_choose_starts, dim3(g), dim3(64), 0, s, (const PairDev *)pc.d_pairs,
+                         (const PairDev *)rc.d_pairs, num_pairs, (const float *)d_starts, d_chosen, d_used);
+      if (hipGetLastError() != hipSuccess) r = set_err(NICP_ERR_HIP, "k_choose_starts launch failed");
+    }
     pf.on = p;
-    CHK(rc);
+    CHK(r);
     if (p) {
       pf.sink = &pf.eager;
       pf.end(s, en);
     }
   }
-  // 3. judge the starts by chi2 per correspondence (result slots 27 = chi2, 28 = inliers of
-  //    the linearisation). With the robust kernel on, the Linearizer counts every gated
-  //    correspondence as an inlier (linearizer.cpp:82-90), and the 1 m distance gate passes
-  //    many wrong pairings at a wrong start, so the count alone cannot rank starts; the mean
-  //    chi2 can. The prediction is kept unless the fallback has a lower mean chi2 with at
-  //    least half the prediction's correspondences, or the prediction has none.
-  std::vector<float> chosen((size_t)num_pairs * 16);
-  for (int i = 0; i < num_pairs; ++i) {
-    const double *a = &raw[(size_t)i * kResultDoubles], *f = &raw[((size_t)num_pairs + i) * kResultDoubles];
-    const bool net = !fallback_wins(a[27], a[28], f[27], f[28]);
-    memcpy(&chosen[16 * (size_t)i], &starts[16 * (size_t)(net ? i : num_pairs + i)], 16 * sizeof(float));
-    if (used_network) used_network[i] = net ? 1 : 0;
-  }
-  if (start16) memcpy(start16, chosen.data(), chosen.size() * sizeof(float));
-  // 4. refinement from the chosen starts
-  return nicp_batch_align(b, num_pairs, ref_frame, cur_frame, chosen.data(), ap, use_graph, T_out16, results);
+  // 4. the refinement from the chosen starts
+  CHK(batch_run(b, rc, use_graph, b->graph));
+  HIPCHK(hipEventRecord(ctx->ev1, s));
+  if (start16) HIPCHK(hipMemcpyAsync(start16, d_chosen, (size_t)num_pairs * 16 * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (used_network) HIPCHK(hipMemcpyAsync(used_network, d_used, (size_t)num_pairs * sizeof(int), hipMemcpyDeviceToHost, s));
+  CHK(batch_collect(b, rc, T_out16, results, nullptr));
+  HIPCHK(hipStreamSynchronize(s));
+  return NICP_OK;
 }

@@ -282,6 +282,55 @@ struct nicp_cloud {
   void *gauss_alloc = nullptr;    // owned allocation of gauss when it is not inside alloc
 };
 
+// Pinned host staging of a batch's table uploads. Copies from pinned memory are asynchronous,
+// so a call can enqueue its uploads and kernels while earlier work (the conversion) still runs.
+// A call reserves all its bytes first (waiting for the previous call's copies, which read the
+// same memory), puts its tables, and records `done` after its copies.
+struct PinnedStage {
+  char *p = nullptr;
+  size_t cap = 0, used = 0;
+  hipEvent_t done = nullptr;
+  bool pending = false;
+  static size_t slot(size_t n) { return align_up(n, 256); }
+  int begin(size_t bytes) {
+    if (pending) {
+      HIPCHK(hipEventSynchronize(done));
+      pending = false;
+    }
+    used = 0;
+    if (bytes > cap) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      cap = 0;
+      const size_t want = align_up(std::max<size_t>(bytes, 1 << 16), 1 << 16);
+      HIPCHK(hipHostMalloc((void **)&p, want, hipHostMallocDefault));
+      cap = want;
+    }
+    if (!done) HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    return NICP_OK;
+  }
+  // a copy of n bytes of src in the staging memory (the caller reserved room with begin)
+  const void *put(const void *src, size_t n) {
+    char *dst = p + used;
+    if (n) memcpy(dst, src, n);
+    used += slot(n);
+    return dst;
+  }
+  int end(hipStream_t s) {
+    HIPCHK(hipEventRecord(done, s));
+    pending = true;
+    return NICP_OK;
+  }
+  void release() {
+    if (done) (void)hipEventDestroy(done);
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    done = nullptr;
+    cap = used = 0;
+    pending = false;
+  }
+};
+
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
   std::vector<int> key;
@@ -313,6 +362,10 @@ struct nicp_batch {
   GraphCache graph;
   GraphCache conv_graph;
   DevBuf synth_buf;  // render job tables of nicp_batch_render_synthetic
+  // the learned aligner's start probe, prepared next to the refinement (its own tables, state
+  // and results), and its starts, choices and flags (learned_aux)
+  DevBuf pr_pairs_tbl, pr_maps, pr_zbuf, pr_cur_index, pr_pairs_buf, pr_state, pr_partials, pr_result, learned_aux;
+  PinnedStage stage;  // table uploads of align calls
   // projection strategy (nicp_batch_set_projection): -1 the autotuner's choice when it is on,
   // else the tile-binned projection where it applies; 0, 1, 2 forced
   int projection = -1;
@@ -639,8 +692,17 @@ struct AlignBuffers {
 };
 
 // allocates/assigns the per-pair buffers and uploads tables and the initial state
+// The staged bytes prepare_align uploads for a plan (through a PinnedStage)
+size_t prepare_stage_bytes(const PairPlan &plan) {
+  const size_t np = plan.pairs.size();
+  const size_t nb = plan.blk_pair.size() + plan.blk_ref.size() + plan.blk_cur.size() + plan.blk_dpair.size() +
+                    plan.blk_spair.size() + plan.blk_fpair.size();
+  return PinnedStage::slot(np * sizeof(PairDev)) + PinnedStage::slot(nb * sizeof(int)) +
+         PinnedStage::slot(np * kStateFloats * sizeof(float));
+}
+
 int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *init16, int **d_blk_pair,
-                  int **d_blk_ref, int **d_blk_cur, PairDev **d_pairs) {
+                  int **d_blk_ref, int **d_blk_cur, PairDev **d_pairs, PinnedStage *stage = nullptr) {
   const int np = (int)plan.pairs.size();
   CHK(B.zbuf->reserve(plan.total_pix * sizeof(unsigned long long)));
   CHK(B.cur_index->reserve(plan.total_pix * sizeof(int)));
@@ -714,9 +776,15 @@ int prepare_align(hipStream_t s, PairPlan &plan, AlignBuffers &B, const float *i
   plan.d_spair = plan.d_dpair + plan.blk_dpair.size();
   plan.d_fpair = plan.d_spair + plan.blk_spair.size();
   *d_pairs = B.pairs_tbl->as<PairDev>();
-  HIPCHK(hipMemcpyAsync(*d_pairs, plan.pairs.data(), (size_t)np * sizeof(PairDev), hipMemcpyHostToDevice, s));
-  if (nb) HIPCHK(hipMemcpyAsync(*d_blk_pair, maps.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(B.state->p, st.data(), st.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  const void *src_pairs = plan.pairs.data(), *src_maps = maps.data(), *src_st = st.data();
+  if (stage) {
+    src_pairs = stage->put(plan.pairs.data(), (size_t)np * sizeof(PairDev));
+    src_maps = stage->put(maps.data(), nb * sizeof(int));
+    src_st = stage->put(st.data(), st.size() * sizeof(float));
+  }
+  HIPCHK(hipMemcpyAsync(*d_pairs, src_pairs, (size_t)np * sizeof(PairDev), hipMemcpyHostToDevice, s));
+  if (nb) HIPCHK(hipMemcpyAsync(*d_blk_pair, src_maps, nb * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(B.state->p, src_st, st.size() * sizeof(float), hipMemcpyHostToDevice, s));
   return NICP_OK;
 }
 
@@ -1680,9 +1748,12 @@ extern "C" int nicp_batch_destroy(nicp_batch *b) {
   if (b->ctx->pending_graph == &b->graph.events) b->ctx->pending_graph = nullptr;
   b->graph.reset();
   b->conv_graph.reset();
-  DevBuf *bufs[] = {&b->pairs_tbl, &b->maps,     &b->zbuf,   &b->cur_index, &b->pairs_buf,
-                    &b->state,     &b->partials, &b->result, &b->sbox,      &b->synth_buf};
+  DevBuf *bufs[] = {&b->pairs_tbl,    &b->maps,         &b->zbuf,     &b->cur_index,   &b->pairs_buf,
+                    &b->state,        &b->partials,     &b->result,   &b->sbox,        &b->synth_buf,
+                    &b->pr_pairs_tbl, &b->pr_maps,      &b->pr_zbuf,  &b->pr_cur_index, &b->pr_pairs_buf,
+                    &b->pr_state,     &b->pr_partials,  &b->pr_result, &b->learned_aux};
   for (DevBuf *x : bufs) x->release();
+  b->stage.release();
   (void)hipFree(b->arena);
   delete b;
   return NICP_OK;
@@ -1715,13 +1786,37 @@ extern "C" int nicp_batch_convert(nicp_batch *b) {
 }
 
 namespace {
-// AlignerProjective::align for a list of pairs of a batch. with_stats = false skips the
-// statistics linearisation (results then hold the in-loop linearisation only and no
-// solution statistics); raw, when not null, receives the kResultDoubles result slots per pair;
-// sub > 1 linearises over every sub-th row and column only (the start probe).
-int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *cur_frame, const float *init_guesses16,
-                const nicp_align_params *ap, int use_graph, float *T_out16, nicp_align_result *results,
-                bool with_stats, std::vector<double> *raw, int sub = 1, int twin = 0) {
+// One batched AlignerProjective::align call in three stages, so that several calls can be
+// prepared before any of them runs (the learned aligner prepares its start probe and its
+// refinement before the network): batch_setup validates, plans, uploads the tables and the
+// initial state into the buffers B and picks the projection strategy; batch_run launches
+// (eagerly, timing the strategies, or from the batch's graph); batch_collect copies the
+// transforms and results back. with_stats = false skips the statistics linearisation (results
+// then hold the in-loop linearisation only and no solution statistics); raw, when not null,
+// receives the kResultDoubles result slots per pair; sub > 1 linearises over every sub-th row and
+// column only (the start probe).
+struct BatchCall {
+  PairPlan plan;
+  AlignBuffers B{};
+  int *d_bp = nullptr, *d_br = nullptr, *d_bc = nullptr;
+  PairDev *d_pairs = nullptr;
+  nicp_align_params ap{};
+  bool with_stats = false, fused = false, tuning = false;
+  int num_pairs = 0, sub = 1, zstrategy = 0;
+};
+
+AlignBuffers batch_buffers(nicp_batch *b) {
+  return AlignBuffers{&b->pairs_tbl, &b->maps, &b->zbuf, &b->cur_index, &b->pairs_buf, &b->state, &b->partials,
+                      &b->result, &b->sbox};
+}
+// the start probe's (subsampled, so never tile-binned: no source-tile boxes)
+AlignBuffers probe_buffers(nicp_batch *b) {
+  return AlignBuffers{&b->pr_pairs_tbl, &b->pr_maps,     &b->pr_zbuf,   &b->pr_cur_index, &b->pr_pairs_buf,
+                      &b->pr_state,     &b->pr_partials, &b->pr_result, nullptr};
+}
+
+int batch_plan(nicp_batch *b, const AlignBuffers &B, int num_pairs, const int *ref_frame, const int *cur_frame,
+               const nicp_align_params *ap, bool with_stats, int sub, int twin, BatchCall &c) {
   if (!b || num_pairs <= 0 || !ref_frame || !cur_frame) return set_err(NICP_ERR_INVALID, "bad arguments");
   if (!b->converted) return set_err(NICP_ERR_STATE, "the batch's frames have not been converted (nicp_batch_convert)");
   CHK(check_align_params(ap));
@@ -1748,7 +1843,7 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
   }
   if (sub < 1 || (sub > 1 && (ap->inner_iterations > 1 || with_stats)))
     return set_err(NICP_ERR_INVALID, "a subsampled linearisation runs one inner iteration and no statistics");
-  PairPlan plan;
+  PairPlan &plan = c.plan;
   make_plan(b->frames, num_pairs, ref_frame, cur_frame, cams.data(), plan);
   plan_batch_frames(b->frames, plan);
   if (sub > 1) {
@@ -1770,11 +1865,7 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
   }
   const bool fused = plan.fused_ok && ap->row_search_region == 0 && ap->col_search_region == 0;
   plan.list_cap = b->list_cap;
-  AlignBuffers B{&b->pairs_tbl, &b->maps, &b->zbuf, &b->cur_index, &b->pairs_buf, &b->state, &b->partials, &b->result,
-                 &b->sbox};
-  int *d_bp, *d_br, *d_bc;
-  PairDev *d_pairs;
-  CHK(prepare_align(ctx->stream, plan, B, init_guesses16, &d_bp, &d_br, &d_bc, &d_pairs));
+  c.B = B;
   // strategy: forced, or the autotuner's (the first call of a shape times the applicable
   // strategies on alternate iterations, eagerly; later calls use the fastest), or by default
   // the tile-binned projection where it applies and its fused tiles fill the device (one
@@ -1800,8 +1891,30 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
       zstrategy = b->tuned_choice[shape];
     }
   }
-  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-  if (tuning) {
+  c.ap = *ap;
+  c.with_stats = with_stats;
+  c.fused = fused;
+  c.tuning = tuning;
+  c.num_pairs = num_pairs;
+  c.sub = sub;
+  c.zstrategy = zstrategy;
+  return NICP_OK;
+}
+
+// the buffers, tables and initial state of a planned call (staged: asynchronous copies)
+int batch_prepare(nicp_batch *b, BatchCall &c, const float *init_guesses16, PinnedStage *stage) {
+  return prepare_align(b->ctx->stream, c.plan, c.B, init_guesses16, &c.d_bp, &c.d_br, &c.d_bc, &c.d_pairs, stage);
+}
+
+int batch_run(nicp_batch *b, BatchCall &c, int use_graph, GraphCache &graph) {
+  nicp_ctx *ctx = b->ctx;
+  PairPlan &plan = c.plan;
+  const nicp_align_params *ap = &c.ap;
+  const bool with_stats = c.with_stats, fused = c.fused;
+  const int num_pairs = c.num_pairs, sub = c.sub, zstrategy = c.zstrategy;
+  int *d_bp = c.d_bp, *d_br = c.d_br, *d_bc = c.d_bc;
+  PairDev *d_pairs = c.d_pairs;
+  if (c.tuning) {
     ZTune tune;
     ctx->prof.sink = &ctx->prof.eager;
     int rc = launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats, 0,
@@ -1854,11 +1967,11 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
     key.push_back(with_stats ? 1 : 0);
     key.push_back(sub);
     key.push_back(zstrategy);
-    if (b->graph.exec == nullptr || b->graph.key != key) {
-      b->graph.reset();
+    if (graph.exec == nullptr || graph.key != key) {
+      graph.reset();
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      ctx->prof.sink = &b->graph.events;
+      ctx->prof.sink = &graph.events;
       int rc = launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats,
                             zstrategy);
       ctx->prof.sink = &ctx->prof.eager;
@@ -1868,30 +1981,37 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
         return rc;
       }
       if (e != hipSuccess) return set_err(NICP_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
-      e = hipGraphInstantiate(&b->graph.exec, g, nullptr, nullptr, 0);
+      e = hipGraphInstantiate(&graph.exec, g, nullptr, nullptr, 0);
       hipError_t de = hipGraphDestroy(g);
       if (e == hipSuccess && de != hipSuccess) e = de;
       if (e != hipSuccess) {
-        b->graph.exec = nullptr;
+        graph.exec = nullptr;
         return set_err(NICP_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
       }
-      b->graph.key = key;
+      graph.key = key;
     }
-    HIPCHK(hipGraphLaunch(b->graph.exec, ctx->stream));
-    ctx->pending_graph = ctx->prof.on ? &b->graph.events : nullptr;
+    HIPCHK(hipGraphLaunch(graph.exec, ctx->stream));
+    ctx->pending_graph = ctx->prof.on ? &graph.events : nullptr;
   } else {
     ctx->prof.sink = &ctx->prof.eager;
     CHK(launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats, zstrategy));
   }
-  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  return NICP_OK;
+}
+
+int batch_collect(nicp_batch *b, BatchCall &c, float *T_out16, nicp_align_result *results, std::vector<double> *raw) {
+  nicp_ctx *ctx = b->ctx;
+  const nicp_align_params *ap = &c.ap;
+  const int num_pairs = c.num_pairs;
+  const bool with_stats = c.with_stats;
   if (!T_out16 && !results && !raw) return NICP_OK;
   std::vector<float> st((size_t)num_pairs * kStateFloats);
   std::vector<double> local;
   std::vector<double> &rr = raw ? *raw : local;
   rr.assign((size_t)num_pairs * kResultDoubles, 0.0);
-  HIPCHK(hipMemcpyAsync(st.data(), b->state.p, st.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(st.data(), c.B.state->p, st.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   if ((results || raw) && ap->outer_iterations > 0)
-    HIPCHK(hipMemcpyAsync(rr.data(), b->result.p, rr.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(rr.data(), c.B.result->p, rr.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   float ms = 0.0f;
   HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -1910,6 +2030,21 @@ int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *c
   return NICP_OK;
 }
 }  // namespace
+
+
+int batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *cur_frame, const float *init_guesses16,
+                const nicp_align_params *ap, int use_graph, float *T_out16, nicp_align_result *results,
+                bool with_stats, std::vector<double> *raw, int sub = 1, int twin = 0) {
+  BatchCall c;
+  CHK(batch_plan(b, batch_buffers(b), num_pairs, ref_frame, cur_frame, ap, with_stats, sub, twin, c));
+  CHK(b->stage.begin(prepare_stage_bytes(c.plan)));
+  CHK(batch_prepare(b, c, init_guesses16, &b->stage));
+  CHK(b->stage.end(b->ctx->stream));
+  HIPCHK(hipEventRecord(b->ctx->ev0, b->ctx->stream));
+  CHK(batch_run(b, c, use_graph, b->graph));
+  HIPCHK(hipEventRecord(b->ctx->ev1, b->ctx->stream));
+  return batch_collect(b, c, T_out16, results, raw);
+}
 
 extern "C" int nicp_batch_align(nicp_batch *b, int num_pairs, const int *ref_frame, const int *cur_frame,
                                 const float *init_guesses16, const nicp_align_params *ap, int use_graph, float *T_out16,

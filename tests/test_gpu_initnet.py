@@ -312,7 +312,7 @@ def test_align_learned_policy(ctx):
     assert ei[0] > 2.0 * e0[0]
 
 
-@pytest.mark.parametrize("sub", [4, 2])
+@pytest.mark.parametrize("sub", [8, 4, 2])
 def test_probe_twin_equals_separate(ctx, sub):
     """The start probe's one-pass projection of both candidate starts (k_zproject_twin) gives the
     linearisations of two separate projections, bit for bit: several seeds, ragged point counts
