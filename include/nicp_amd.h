@@ -395,8 +395,9 @@ int nicp_batch_probe(nicp_batch *batch, int num_pairs, const int *ref_frame, con
 
 /* Per-kernel device time (no reference counterpart; the reference only has
  * Aligner::totalTime(), nicp/nicp/aligner.h:348). When enabled, every launch of the
- * context is bracketed by HIP events on its stream (event nodes inside captured graphs);
- * enabling resets the totals. total_ms / launches receive nicp_kernel_count() entries. */
+ * context is bracketed by HIP events on its stream, and calls that would replay a hipGraph
+ * launch eagerly instead (same results); enabling resets the totals. total_ms / launches
+ * receive nicp_kernel_count() entries. */
 int nicp_kernel_count(void);
 const char *nicp_kernel_name(int kernel_id);
 int nicp_ctx_profile(nicp_ctx *ctx, int enable);

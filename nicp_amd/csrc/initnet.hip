@@ -884,24 +884,23 @@ extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_
   CHK(upload_net_jobs(n, b, num_pairs, ref_frame, cur_frame, &b->stage));
   CHK(b->stage.end(s));
   HIPCHK(hipEventRecord(ctx->ev0, s));
-  // 1. prediction
-  {
-    ProfEntry en{};
-    const bool p = pf.on;
-    if (p) pf.begin(KID_NET_INPUT, s, en);
-    CHK(enqueue_net_input(n, b, num_pairs, 0.01f, n->act[0]));
-    if (p) pf.end(s, en);
-  }
-  {
-    ProfEntry en{};
-    const bool p = pf.on;
-    if (p) pf.begin(KID_NET_FORWARD, s, en);
-    CHK(net_forward(n, num_pairs));
-    if (p) pf.end(s, en);
-  }
-  // 2. both starts judged by the probe, 3. the choice (timed as one stage: its kernels are not
-  //    profiled one by one)
-  {
+  // 1. prediction, 2. both starts judged by the probe, 3. the choice (the probe and the choice
+  //    are timed as one stage: their kernels are not profiled one by one)
+  auto enqueue_front = [&]() -> int {
+    {
+      ProfEntry en{};
+      const bool p = pf.on;
+      if (p) pf.begin(KID_NET_INPUT, s, en);
+      CHK(enqueue_net_input(n, b, num_pairs, 0.01f, n->act[0]));
+      if (p) pf.end(s, en);
+    }
+    {
+      ProfEntry en{};
+      const bool p = pf.on;
+      if (p) pf.begin(KID_NET_FORWARD, s, en);
+      CHK(net_forward(n, num_pairs));
+      if (p) pf.end(s, en);
+    }
     ProfEntry en{};
     const bool p = pf.on;
     if (p) pf.begin(KID_START_PROBE, s, en);
@@ -910,7 +909,9 @@ extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_
     hipLaunchKernelGGL(k_probe_starts, dim3(g), dim3(64), 0, s, (const float *)n->act[kLayers], num_pairs,
                        (const PairDev *)pc.d_pairs, d_starts);
     int r = hipGetLastError() == hipSuccess ? NICP_OK : set_err(NICP_ERR_HIP, "k_probe_starts launch failed");
-    if (r == NICP_OK) r = batch_run(b, pc, 0, b->graph);
+    if (r == NICP_OK)
+      r = launch_align(&pf, s, pc.plan, pc.d_pairs, b->d_frames, pc.d_bp, pc.d_br, pc.d_bc, &pc.ap, false,
+                       pc.zstrategy);
     if (r == NICP_OK) {
       hipLaunchKernelGGL(k_choose_starts, dim3(g), dim3(64), 0, s, (const PairDev *)pc.d_pairs,
                          (const PairDev *)rc.d_pairs, num_pairs, (const float *)d_starts, d_chosen, d_used);
@@ -918,13 +919,31 @@ extern "C" int nicp_batch_align_learned(nicp_batch *b, nicp_initnet *n, int num_
     }
     pf.on = p;
     CHK(r);
-    if (p) {
-      pf.sink = &pf.eager;
-      pf.end(s, en);
-    }
+    if (p) pf.end(s, en);
+    return NICP_OK;
+  };
+  // 4. the refinement from the chosen starts. With use_graph the whole sequence is one graph,
+  //    unless this call's refinement shape is being autotuned (eagerly, with its own timing) or
+  //    the per-launch timing is on (then only the refinement is a graph, as nicp_batch_align's).
+  if (use_graph && !rc.tuning && !pf.on) {
+    std::vector<int> key = call_key(b, pc);
+    const std::vector<int> rkey = call_key(b, rc);
+    key.insert(key.end(), rkey.begin(), rkey.end());
+    key.push_back(num_pairs);
+    key.push_back(n->input_mode);
+    for (int l = 0; l <= kLayers; ++l) key_ptr(key, n->act[l]);
+    for (const void *ptr : {(const void *)n->jobs, (const void *)n->params, (const void *)n->chunks,
+                            (const void *)d_starts})
+      key_ptr(key, ptr);
+    CHK(graph_replay(ctx, b->learned_graph, key, [&]() -> int {
+      CHK(enqueue_front());
+      return launch_align(&pf, s, rc.plan, rc.d_pairs, b->d_frames, rc.d_bp, rc.d_br, rc.d_bc, &rc.ap, true,
+                          rc.zstrategy);
+    }));
+  } else {
+    CHK(enqueue_front());
+    CHK(batch_run(b, rc, use_graph, b->graph));
   }
-  // 4. the refinement from the chosen starts
-  CHK(batch_run(b, rc, use_graph, b->graph));
   HIPCHK(hipEventRecord(ctx->ev1, s));
   if (start16) HIPCHK(hipMemcpyAsync(start16, d_chosen, (size_t)num_pairs * 16 * sizeof(float), hipMemcpyDeviceToHost, s));
   if (used_network) HIPCHK(hipMemcpyAsync(used_network, d_used, (size_t)num_pairs * sizeof(int), hipMemcpyDeviceToHost, s));

@@ -261,7 +261,15 @@ struct nicp_ctx {
   int last_rows = 0, last_cols = 0;
   bool last_valid = false;
   Prof prof;
-  std::vector<ProfEntry> *pending_graph = nullptr;  // graph events not yet read
+  std::vector<std::vector<ProfEntry> *> pending_graphs;  // replayed graphs whose events are not yet read
+  void graph_pending(std::vector<ProfEntry> *ev) {
+    if (prof.on && std::find(pending_graphs.begin(), pending_graphs.end(), ev) == pending_graphs.end())
+      pending_graphs.push_back(ev);
+  }
+  void read_pending() {
+    for (std::vector<ProfEntry> *ev : pending_graphs) prof.read(*ev, false);
+    pending_graphs.clear();
+  }
   // kNN grid of the current cloud
   DevBuf grid_cells, grid_cursor, grid_sorted, grid_aux, nn_out, nn_order, grid_cells_c, grid_sorted_c;
   // allocations of destroyed clouds (size, pointer), reused for new clouds of the same size;
@@ -366,6 +374,7 @@ struct nicp_batch {
   // and results), and its starts, choices and flags (learned_aux)
   DevBuf pr_pairs_tbl, pr_maps, pr_zbuf, pr_cur_index, pr_pairs_buf, pr_state, pr_partials, pr_result, learned_aux;
   PinnedStage stage;  // table uploads of align calls
+  GraphCache learned_graph;  // nicp_batch_align_learned: network, start probe, choice and refinement
   // projection strategy (nicp_batch_set_projection): -1 the autotuner's choice when it is on,
   // else the tile-binned projection where it applies; 0, 1, 2 forced
   int projection = -1;
@@ -380,6 +389,41 @@ struct nicp_batch {
   std::vector<int> tuned_choice;
   std::vector<float> tuned_ms;  // kNumStrategies per shape: average ms per iteration (-1 unmeasured)
 };
+
+namespace {
+// Replays `graph`, captured from what enqueue() launches on the context's stream unless its key
+// differs from `key` (then it is captured again). Timing events recorded during the capture
+// become nodes of the graph and are read after the replay.
+template <typename F>
+int graph_replay(nicp_ctx *ctx, GraphCache &graph, const std::vector<int> &key, F enqueue) {
+  if (graph.exec == nullptr || graph.key != key) {
+    graph.reset();
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->prof.sink = &graph.events;
+    const int rc = enqueue();
+    ctx->prof.sink = &ctx->prof.eager;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc != NICP_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return set_err(NICP_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&graph.exec, g, nullptr, nullptr, 0);
+    hipError_t de = hipGraphDestroy(g);
+    if (e == hipSuccess && de != hipSuccess) e = de;
+    if (e != hipSuccess) {
+      graph.exec = nullptr;
+      return set_err(NICP_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    graph.key = key;
+  }
+  HIPCHK(hipGraphLaunch(graph.exec, ctx->stream));
+  ctx->graph_pending(&graph.events);
+  return NICP_OK;
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------------------------------
 // parameter conversion
@@ -1745,9 +1789,12 @@ extern "C" int nicp_batch_destroy(nicp_batch *b) {
   if (!b) return NICP_OK;
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
-  if (b->ctx->pending_graph == &b->graph.events) b->ctx->pending_graph = nullptr;
+  std::vector<std::vector<ProfEntry> *> &pg = b->ctx->pending_graphs;
+  for (std::vector<ProfEntry> *ev : {&b->graph.events, &b->learned_graph.events, &b->conv_graph.events})
+    pg.erase(std::remove(pg.begin(), pg.end(), ev), pg.end());
   b->graph.reset();
   b->conv_graph.reset();
+  b->learned_graph.reset();
   DevBuf *bufs[] = {&b->pairs_tbl,    &b->maps,         &b->zbuf,     &b->cur_index,   &b->pairs_buf,
                     &b->state,        &b->partials,     &b->result,   &b->sbox,        &b->synth_buf,
                     &b->pr_pairs_tbl, &b->pr_maps,      &b->pr_zbuf,  &b->pr_cur_index, &b->pr_pairs_buf,
@@ -1779,8 +1826,19 @@ extern "C" int nicp_batch_convert(nicp_batch *b) {
   b->ctx->prof.sink = &b->ctx->prof.eager;
   StatsParamsDev spd = b->sp;
   spd.fp64_gap = b->ctx->fp64_eigen_gap;
-  CHK(launch_convert(&b->ctx->prof, b->ctx->stream, b->d_frames, b->num_frames, b->d_blk_frame, b->total_tiles, b->d_xblk_frame,
-                     b->total_xblk, b->d_yblk_frame, b->total_yblk, spd, b->frames[0].model));
+  auto enqueue = [&]() {
+    return launch_convert(&b->ctx->prof, b->ctx->stream, b->d_frames, b->num_frames, b->d_blk_frame, b->total_tiles,
+                          b->d_xblk_frame, b->total_xblk, b->d_yblk_frame, b->total_yblk, spd, b->frames[0].model);
+  };
+  if (b->ctx->prof.on) {
+    // per-launch timing runs eagerly
+    CHK(enqueue());
+  } else {
+    // the conversion's launches depend only on the batch and the FP64 eigen setting: one graph
+    std::vector<int> key = {0};
+    memcpy(&key[0], &spd.fp64_gap, sizeof(float));
+    CHK(graph_replay(b->ctx, b->conv_graph, key, enqueue));
+  }
   b->converted = true;
   return NICP_OK;
 }
@@ -1906,6 +1964,34 @@ int batch_prepare(nicp_batch *b, BatchCall &c, const float *init_guesses16, Pinn
   return prepare_align(b->ctx->stream, c.plan, c.B, init_guesses16, &c.d_bp, &c.d_br, &c.d_bc, &c.d_pairs, stage);
 }
 
+void key_ptr(std::vector<int> &key, const void *p) {
+  key.push_back((int)(uint32_t)((uintptr_t)p));
+  key.push_back((int)(uint32_t)((uint64_t)(uintptr_t)p >> 32));
+}
+
+// The graph key of a planned call: the launch geometry, the parameters baked into the kernel
+// arguments, and every device address a captured launch takes as an argument (the tables'
+// contents are uploaded before each launch, so only their addresses are baked into a graph).
+std::vector<int> call_key(const nicp_batch *b, const BatchCall &c) {
+  const PairPlan &plan = c.plan;
+  std::vector<int> key = {c.num_pairs, plan.total_pix_tiles, plan.total_rtiles, plan.total_ctiles, plan.total_dtiles};
+  const int *apw = (const int *)&c.ap;
+  key.insert(key.end(), apw, apw + sizeof(nicp_align_params) / sizeof(int));
+  key.push_back(plan.total_sblk);
+  key.push_back(plan.total_ftiles);
+  key.push_back(plan.list_cap);
+  key.push_back(plan.twin);
+  for (const void *ptr : {(const void *)c.d_pairs, (const void *)c.d_bp, (const void *)c.d_br, (const void *)c.d_bc,
+                          (const void *)plan.d_dpair, (const void *)plan.d_spair, (const void *)plan.d_fpair,
+                          (const void *)b->d_frames})
+    key_ptr(key, ptr);
+  key.push_back(b->ctx->prof.on ? 1 : 0);
+  key.push_back(c.with_stats ? 1 : 0);
+  key.push_back(c.sub);
+  key.push_back(c.zstrategy);
+  return key;
+}
+
 int batch_run(nicp_batch *b, BatchCall &c, int use_graph, GraphCache &graph) {
   nicp_ctx *ctx = b->ctx;
   PairPlan &plan = c.plan;
@@ -1947,51 +2033,11 @@ int batch_run(nicp_batch *b, BatchCall &c, int use_graph, GraphCache &graph) {
       }
     }
     b->tuned_choice.push_back(best >= 0 ? best : zstrategy);
-  } else if (use_graph) {
-    // graph key: the launch geometry and the parameters baked into the kernel arguments
-    std::vector<int> key = {num_pairs, plan.total_pix_tiles, plan.total_rtiles, plan.total_ctiles, plan.total_dtiles};
-    const int *apw = (const int *)ap;
-    key.insert(key.end(), apw, apw + sizeof(nicp_align_params) / sizeof(int));
-    // every device address a captured launch takes as an argument (the tables' contents are
-    // uploaded before each launch, so only the argument addresses are baked into the graph)
-    key.push_back(plan.total_sblk);
-    key.push_back(plan.total_ftiles);
-    key.push_back(plan.list_cap);
-    for (const void *ptr : {(const void *)d_pairs, (const void *)d_bp, (const void *)d_br, (const void *)d_bc,
-                            (const void *)plan.d_dpair, (const void *)plan.d_spair, (const void *)plan.d_fpair,
-                            (const void *)b->d_frames}) {
-      key.push_back((int)(uint32_t)((uintptr_t)ptr));
-      key.push_back((int)(uint32_t)((uint64_t)(uintptr_t)ptr >> 32));
-    }
-    key.push_back(ctx->prof.on ? 1 : 0);
-    key.push_back(with_stats ? 1 : 0);
-    key.push_back(sub);
-    key.push_back(zstrategy);
-    if (graph.exec == nullptr || graph.key != key) {
-      graph.reset();
-      hipGraph_t g = nullptr;
-      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      ctx->prof.sink = &graph.events;
-      int rc = launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats,
-                            zstrategy);
-      ctx->prof.sink = &ctx->prof.eager;
-      hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-      if (rc != NICP_OK) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-      }
-      if (e != hipSuccess) return set_err(NICP_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
-      e = hipGraphInstantiate(&graph.exec, g, nullptr, nullptr, 0);
-      hipError_t de = hipGraphDestroy(g);
-      if (e == hipSuccess && de != hipSuccess) e = de;
-      if (e != hipSuccess) {
-        graph.exec = nullptr;
-        return set_err(NICP_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
-      }
-      graph.key = key;
-    }
-    HIPCHK(hipGraphLaunch(graph.exec, ctx->stream));
-    ctx->pending_graph = ctx->prof.on ? &graph.events : nullptr;
+  } else if (use_graph && !ctx->prof.on) {
+    CHK(graph_replay(ctx, graph, call_key(b, c), [&]() {
+      return launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats,
+                          zstrategy);
+    }));
   } else {
     ctx->prof.sink = &ctx->prof.eager;
     CHK(launch_align(&ctx->prof, ctx->stream, plan, d_pairs, b->d_frames, d_bp, d_br, d_bc, ap, with_stats, zstrategy));
@@ -2017,8 +2063,7 @@ int batch_collect(nicp_batch *b, BatchCall &c, float *T_out16, nicp_align_result
   HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
   if (ctx->prof.on) {
     ctx->prof.read(ctx->prof.eager, true);
-    if (ctx->pending_graph) ctx->prof.read(*ctx->pending_graph, false);
-    ctx->pending_graph = nullptr;
+    ctx->read_pending();
   }
   nicp_align_params ap_res = *ap;
   if (!with_stats) ap_res.outer_iterations = 0;  // no statistics were computed
@@ -2127,7 +2172,7 @@ extern "C" int nicp_ctx_profile(nicp_ctx *ctx, int enable) {
     ctx->prof.total_ms[i] = 0.0;
     ctx->prof.launches[i] = 0;
   }
-  ctx->pending_graph = nullptr;
+  ctx->pending_graphs.clear();
   return NICP_OK;
 }
 
@@ -2138,8 +2183,7 @@ extern "C" int nicp_ctx_profile_read(nicp_ctx *ctx, double *total_ms, long long 
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->prof.read(ctx->prof.eager, true);
-  if (ctx->pending_graph) ctx->prof.read(*ctx->pending_graph, false);
-  ctx->pending_graph = nullptr;
+  ctx->read_pending();
   for (int i = 0; i < KID_N; ++i) {
     if (total_ms) total_ms[i] = ctx->prof.total_ms[i];
     if (launches) launches[i] = ctx->prof.launches[i];

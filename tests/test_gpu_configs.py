@@ -235,3 +235,41 @@ def test_ground_truth_ceiling_640x480(ctx):
     assert a_gt < n // 4
     b.close()
     net.close()
+
+
+def test_profiling_with_graphs(ctx):
+    """Per-launch timing (nicp_ctx_profile) on a batch whose conversion, learned step and
+    refinement otherwise replay hipGraphs: the launches then run eagerly, every stage is timed,
+    and the transforms equal the graph replays' bit for bit."""
+    say("case profiling with graphs")
+    w = initnet.load_weights()
+    assert w is not None, "trained weights missing"
+    net = capi.InitNet(ctx, seed=1)
+    net.set_params(w)
+    _, ap = aparams()
+    B = 16
+    b = initnet.make_batch(ctx, B)
+    ref = np.arange(0, 2 * B, 2, dtype=np.int32)
+    cur = ref + 1
+    initnet.render_pairs(b, np.arange(4000, 4000 + B, dtype=np.uint64), 30.0, 0.5)
+    b.convert()
+    Tl, _, sl, ul = b.align_learned(net, ref, cur, ap, use_graph=True)
+    Tp, _ = b.align(ref, cur, None, ap, use_graph=True)
+    ctx.profile(True)
+    b.convert()
+    Tl2, _, sl2, ul2 = b.align_learned(net, ref, cur, ap, use_graph=True)
+    Tp2, _ = b.align(ref, cur, None, ap, use_graph=True)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    np.testing.assert_array_equal(Tl2.view(np.uint32), Tl.view(np.uint32))
+    np.testing.assert_array_equal(sl2.view(np.uint32), sl.view(np.uint32))
+    np.testing.assert_array_equal(np.asarray(ul2), np.asarray(ul))
+    np.testing.assert_array_equal(Tp2.view(np.uint32), Tp.view(np.uint32))
+    timed = {k: v for k, v in prof.items() if v[1] > 0}
+    say("  timed stages: " + ", ".join("%s %d x %.3f ms" % (k, v[1], v[0] / v[1]) for k, v in sorted(timed.items())))
+    for k in ("k_stats", "k_integral_frame", "initnet_forward", "start_probe", "k_solve", "k_solution_stats"):
+        assert timed.get(k, (0, 0))[1] > 0, k
+    # the refinement ran twice with 10 iterations: 20 projections whichever strategy
+    assert sum(timed.get(k, (0, 0))[1] for k in ("k_proj_corr_lin", "k_corr_lin")) == 20
+    b.close()
+    net.close()
