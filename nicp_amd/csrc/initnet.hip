@@ -12,7 +12,7 @@
 //   fc 1280 -> 256 ReLU, fc 256 -> 6
 //
 // Every layer is one GEMM on the matrix cores: bf16 operands, fp32 accumulation
-// (v_mfma_f32_16x16x16_bf16), with the im2col gather of the input and the bias
+// (gfx950's v_mfma_f32_16x16x32_bf16), with the im2col gather of the input and the bias
 // (an extra weight row multiplied by a constant 1) folded into the operand loads.
 // Backward: weight gradients are GEMMs over the batch (split over the reduction,
 // fp32 atomics), input gradients are GEMMs scattered back by col2im (fp32
@@ -58,7 +58,7 @@ __device__ __forceinline__ unsigned short to_bf16(float f) {
   return (unsigned short)(u >> 16);
 }
 
-typedef short __attribute__((ext_vector_type(4))) bf16x4;
+typedef __bf16 __attribute__((ext_vector_type(8))) bf16x8;
 typedef float __attribute__((ext_vector_type(4))) f32x4;
 
 // ---------------------------------------------------------------------------------
@@ -145,14 +145,16 @@ struct StoreCol2Im {  // input gradient: scatter dcol(m, k) into dX (col2im)
 
 // ---------------------------------------------------------------------------------
 // C[M][N] = sum_k A(m, k) B(k, n) over k in this block's chunk. A 64 x 64 output tile
-// per block of 4 wavefronts, each wavefront a 32 x 32 quadrant of 2 x 2 MFMA 16x16x16
-// tiles; the reduction advances 16 at a time through LDS in bf16. blockIdx.z selects
-// the chunk [z * kchunk, (z + 1) * kchunk) of the reduction.
+// per block of 4 wavefronts, each wavefront a 32 x 32 quadrant of 2 x 2 tiles of gfx950's
+// v_mfma_f32_16x16x32_bf16; the reduction advances 32 at a time through LDS in bf16.
+// blockIdx.z selects the chunk [z * kchunk, (z + 1) * kchunk) of the reduction.
 // ---------------------------------------------------------------------------------
+constexpr int kGemmK = 32;      // reduction depth of one MFMA step
+constexpr int kGemmLds = 40;    // LDS row pitch in bf16: 80 bytes, so a lane's 8 values are one 16-byte read
 template <class LA, class LB, class ST>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, LA la, LB lb, ST st) {
-  __shared__ unsigned short sA[64][16 + 4];  // [row][k]; the pad staggers the banks
-  __shared__ unsigned short sB[64][16 + 4];  // [col][k]
+  __shared__ __attribute__((aligned(16))) unsigned short sA[64][kGemmLds];  // [row][k]
+  __shared__ __attribute__((aligned(16))) unsigned short sB[64][kGemmLds];  // [col][k]
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -162,31 +164,40 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, L
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  // staging: thread t fills row/col (t >> 2) of both tiles at k offsets (t & 3) * 4 .. + 3
-  const int sr = tid >> 2, sk = (tid & 3) * 4;
-  const int fr = lane & 15, fk = (lane >> 4) * 4;
-  for (int k0 = kbeg; k0 < kend; k0 += 16) {
+  // staging: thread t fills row/col (t >> 2) of both tiles at k offsets (t & 3) * 8 .. + 7
+  const int sr = tid >> 2, sk = (tid & 3) * 8;
+  // operands: lane holds A[lane % 16][8 (lane / 16) + j] and B[8 (lane / 16) + j][lane % 16],
+  // j = 0..7. The matrix core pairs A's and B's values of one lane slot and register element,
+  // so this assignment of the 32 reduction indices gives the full dot product.
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  for (int k0 = kbeg; k0 < kend; k0 += kGemmK) {
+    unsigned short va[8], vb[8];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < 8; ++j) {
       const int k = k0 + sk + j, m = m0 + sr, n = n0 + sr;
       const bool kin = k < kend;
-      sA[sr][sk + j] = to_bf16(kin && m < M ? la(m, k) : 0.0f);
-      sB[sr][sk + j] = to_bf16(kin && n < N ? lb(k, n) : 0.0f);
+      va[j] = to_bf16(kin && m < M ? la(m, k) : 0.0f);
+      vb[j] = to_bf16(kin && n < N ? lb(k, n) : 0.0f);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      sA[sr][sk + j] = va[j];
+      sB[sr][sk + j] = vb[j];
     }
     __syncthreads();
-    // operand layout of v_mfma_f32_16x16x16_bf16: lane holds A[lane % 16][4 (lane / 16) + j]
-    // and B[4 (lane / 16) + j][lane % 16], j = 0..3
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      bf16x4 a;
+      bf16x8 a;
+      const unsigned short *ra = &sA[wm + i * 16 + fr][fk];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) a[j] = (short)sA[wm + i * 16 + fr][fk + j];
+      for (int j = 0; j < 8; ++j) a[j] = __builtin_bit_cast(__bf16, ra[j]);
 #pragma unroll
       for (int jj = 0; jj < 2; ++jj) {
-        bf16x4 b;
+        bf16x8 b;
+        const unsigned short *rb = &sB[wn + jj * 16 + fr][fk];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = (short)sB[wn + jj * 16 + fr][fk + j];
-        acc[i][jj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][jj], 0, 0, 0);
+        for (int j = 0; j < 8; ++j) b[j] = __builtin_bit_cast(__bf16, rb[j]);
+        acc[i][jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][jj], 0, 0, 0);
       }
     }
     __syncthreads();
@@ -204,8 +215,8 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, L
       }
 }
 
-// the reduction chunk of a GEMM split `splits` ways (a multiple of 16) and the chunk count
-inline int gemm_kchunk(int K, int splits) { return ((K + splits - 1) / splits + 15) / 16 * 16; }
+// the reduction chunk of a GEMM split `splits` ways (a multiple of kGemmK) and the chunk count
+inline int gemm_kchunk(int K, int splits) { return ((K + splits - 1) / splits + kGemmK - 1) / kGemmK * kGemmK; }
 inline int gemm_chunks(int K, int splits) { return (K + gemm_kchunk(K, splits) - 1) / gemm_kchunk(K, splits); }
 
 template <class LA, class LB, class ST>
