@@ -273,3 +273,44 @@ def test_profiling_with_graphs(ctx):
     assert sum(timed.get(k, (0, 0))[1] for k in ("k_proj_corr_lin", "k_corr_lin")) == 20
     b.close()
     net.close()
+
+
+def test_learned_iterations_to_align(ctx):
+    """The learned start's ablation in iterations: on 256 held-out 640x480 pairs (<=30 deg /
+    0.5 m), the pairs under 0.1 deg / 2 mm and the median rotation error after 2, 4, 6 and 10
+    outer iterations, from the learned start and from the identity."""
+    w = initnet.load_weights()
+    assert w is not None, "trained weights missing"
+    say("case learned iterations to align 640x480 pairs=256")
+    net = capi.InitNet(ctx, seed=1)
+    net.set_params(w)
+    B = 128
+    b = initnet.make_batch(ctx, B)
+    ref = np.arange(0, 2 * B, 2, dtype=np.int32)
+    cur = ref + 1
+    iters = (2, 4, 6, 10)
+    counts = {("net", k): 0 for k in iters}
+    counts.update({("id", k): 0 for k in iters})
+    med = {}
+    errs = {key: [] for key in counts}
+    for k in range(2):
+        Tgt = initnet.render_pairs(b, np.arange(1000 + k * B, 1000 + (k + 1) * B, dtype=np.uint64), 30.0, 0.5)
+        b.convert()
+        for it in iters:
+            _, ap = aparams(outer_iterations=it)
+            Tn, _, _, _ = b.align_learned(net, ref, cur, ap, use_graph=True)
+            Ti, _ = b.align(ref, cur, None, ap, use_graph=True)
+            for name, T in (("net", Tn), ("id", Ti)):
+                e = _errors(T, Tgt)
+                counts[(name, it)] += int(_aligned(e).sum())
+                errs[(name, it)].append(e[:, 0])
+    for key in errs:
+        med[key] = float(np.median(np.concatenate(errs[key])))
+    for it in iters:
+        say("  %2d iterations: aligned learned %3d identity %3d of 256; median rotation error learned %.3f deg "
+            "identity %.3f deg" % (it, counts[("net", it)], counts[("id", it)], med[("net", it)], med[("id", it)]))
+    for it in iters:
+        assert counts[("net", it)] >= counts[("id", it)] - 2
+        assert med[("net", it)] < med[("id", it)]
+    b.close()
+    net.close()

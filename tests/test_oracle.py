@@ -588,3 +588,46 @@ def test_accuracy_ceiling_from_ground_truth():
     assert rows[3][2] <= 4 and med[30] > 0.1
     assert a0 <= 6 and float(np.median(d0)) > 0.1
     assert abs(a64 - rows[3][2]) <= 1 and abs(float(np.median(d64)) - med[30]) < 0.1 * med[30]
+
+
+# ---- the learned aligner's start choice -------------------------------------------------
+def _keep_prediction(pred, fb):
+    """The choice of nicp_batch_align_learned (include/nicp_amd.h): the prediction is kept unless
+    the fallback has a lower chi2 per correspondence with at least half the prediction's
+    correspondences, or the prediction has none; pred / fb = (chi2, inliers)."""
+    if pred[1] <= 0:
+        return fb[1] <= 0
+    return not (fb[1] >= 0.5 * pred[1] and fb[0] / fb[1] < pred[0] / pred[1])
+
+
+def test_start_choice_on_the_reference_linearizer():
+    """One linearisation of the reference's aligner (the oracle, one outer iteration) at each
+    candidate start judges them. A guess 15 deg off the truth loses to the identity at small
+    motions (<= 5 deg / 5 cm); the ground truth wins over the identity at 20 deg although the identity
+    collects more gated correspondences there (the robust kernel counts every gated pair as an
+    inlier, linearizer.cpp:82-90), which is why the rule compares chi2 per correspondence."""
+    rows, cols = 240, 320
+    cam = O.camera(*synth.camera_for(rows, cols))
+    sp, ap = O.stats_params(), O.align_params(outer_iterations=1)
+    I = identity16()
+    off = offset16(15.0, (1, 0.5, -0.2), (0.1, 0.0, 0.05)).reshape(4, 4, order="F").astype(np.float64)
+
+    def probe(seed, rot, trans):
+        dr, dc, T = synth.pair(seed, rows, cols, max_rot_deg=rot, max_trans_m=trans)
+        fr, fc = O.Frame(dr, 0.001, cam, sp), O.Frame(dc, 0.001, cam, sp)
+        bad = (T.reshape(4, 4, order="F").astype(np.float64) @ off).astype(np.float32).flatten(order="F")
+        out = {}
+        for name, start in (("truth", T), ("identity", I), ("bad", bad)):
+            _, r = O.align(fr.cloud(), fc.cloud(), rows, cols, cam, start, ap, 1)
+            out[name] = (float(r.error), int(r.inliers))
+        return out
+
+    for seed, rot, trans in ((15, 5.0, 0.05), (16, 3.0, 0.03)):
+        p = probe(seed, rot, trans)
+        print(seed, p)
+        assert not _keep_prediction(p["bad"], p["identity"])      # the bad guess falls back
+        assert _keep_prediction(p["truth"], p["identity"])        # the truth is kept
+    p = probe(14, 20.0, 0.3)
+    print(14, p)
+    assert p["identity"][1] > p["truth"][1]                       # more inliers at the identity
+    assert _keep_prediction(p["truth"], p["identity"])            # and still the truth is kept
