@@ -1214,10 +1214,16 @@ namespace {
 // per-pair reduction of k_solve). FP32 slots take 31 registers instead of 62, which lets
 // the linearising kernels run three wavefronts per SIMD instead of two.
 typedef float Acc2;
-__device__ __forceinline__ void acc_add(Acc2 &a, float x) { a += x; }
 
 // The per-correspondence terms of Linearizer::update (linearizer.cpp:55-112) for
-// J_p = [I | S(p)], J_n = [0 | S(n)], S(v) = -2[v]x (bm_se3.h:55-66).
+// J_p = [I | S(p)], J_n = [0 | S(n)], S(v) = -2[v]x (bm_se3.h:55-66), p and n the transformed
+// reference point and normal. With C = [p]x and D = [n]x:
+//   H = [[Wp, -2 Wp C], [., 4 (C^T Wp C + D^T Wn D)]],
+//   b = [Wp e_p; -2 (C^T Wp e_p + D^T Wn e_n)].
+// The slots hold the terms without their constant factors -2 and 4 (slot_scale), which the
+// FP64 block reduction applies: powers of two, so the scaling is exact. The arithmetic is
+// contracted into FMAs: the sums only have to meet a tolerance against the FP64 twin, while the
+// gates that decide the correspondences keep the reference's uncontracted order.
 // Output order: H upper triangle row-major (21), b (6), chi2, inliers, count; slot 30 counts
 // the pixels where both index images hold a point.
 __device__ __forceinline__ void lin_terms(const float *rp, const float *rn, float4 cp, float4 cn, const float4 *om,
@@ -1229,57 +1235,68 @@ __device__ __forceinline__ void lin_terms(const float *rp, const float *rn, floa
   const float N00 = ns * o1.z, N01 = ns * o1.w, N02 = ns * o2.x, N11 = ns * o2.y, N12 = ns * o2.z, N22 = ns * o2.w;
   const float pe0 = rp[0] - cp.x, pe1 = rp[1] - cp.y, pe2 = rp[2] - cp.z;
   const float ne0 = rn[0] - cn.x, ne1 = rn[1] - cn.y, ne2 = rn[2] - cn.z;
-  const float ep0 = P00 * pe0 + P01 * pe1 + P02 * pe2;
-  const float ep1 = P01 * pe0 + P11 * pe1 + P12 * pe2;
-  const float ep2 = P02 * pe0 + P12 * pe1 + P22 * pe2;
-  const float en0 = N00 * ne0 + N01 * ne1 + N02 * ne2;
-  const float en1 = N01 * ne0 + N11 * ne1 + N12 * ne2;
-  const float en2 = N02 * ne0 + N12 * ne1 + N22 * ne2;
-  const float chi = (pe0 * ep0 + pe1 * ep1 + pe2 * ep2) + (ne0 * en0 + ne1 * en1 + ne2 * en2);
+  const float ep0 = fmaf(P02, pe2, fmaf(P01, pe1, P00 * pe0));
+  const float ep1 = fmaf(P12, pe2, fmaf(P11, pe1, P01 * pe0));
+  const float ep2 = fmaf(P22, pe2, fmaf(P12, pe1, P02 * pe0));
+  const float en0 = fmaf(N02, ne2, fmaf(N01, ne1, N00 * ne0));
+  const float en1 = fmaf(N12, ne2, fmaf(N11, ne1, N01 * ne0));
+  const float en2 = fmaf(N22, ne2, fmaf(N12, ne1, N02 * ne0));
+  const float chi = fmaf(ne2, en2, fmaf(ne1, en1, fmaf(ne0, en0, fmaf(pe2, ep2, fmaf(pe1, ep1, pe0 * ep0)))));
   float k = 1.0f;
-  acc_add(acc[29], 1.0f);
+  acc[29] += 1.0f;
   if (chi > ap.max_chi2) {
     if (ap.robust) k = ap.max_chi2 / chi;
     else return;
   }
-  // S(v) columns: col0 = (0, -2vz, 2vy), col1 = (2vz, 0, -2vx), col2 = (-2vy, 2vx, 0)
-  const float px = 2.0f * rp[0], py = 2.0f * rp[1], pz = 2.0f * rp[2];
-  const float nx = 2.0f * rn[0], ny = 2.0f * rn[1], nz = 2.0f * rn[2];
-  // A = Omega_p S(p): column j = Omega_p * S(p)_j
-  const float A00 = -P01 * pz + P02 * py, A10 = -P11 * pz + P12 * py, A20 = -P12 * pz + P22 * py;
-  const float A01 = P00 * pz - P02 * px, A11 = P01 * pz - P12 * px, A21 = P02 * pz - P22 * px;
-  const float A02 = -P00 * py + P01 * px, A12 = -P01 * py + P11 * px, A22 = -P02 * py + P12 * px;
-  const float B00 = -N01 * nz + N02 * ny, B10 = -N11 * nz + N12 * ny, B20 = -N12 * nz + N22 * ny;
-  const float B01 = N00 * nz - N02 * nx, B11 = N01 * nz - N12 * nx, B21 = N02 * nz - N22 * nx;
-  const float B02 = -N00 * ny + N01 * nx, B12 = -N01 * ny + N11 * nx, B22 = -N02 * ny + N12 * nx;
-  // S(v)^T w = (w1*(-2vz)... ) : S^T row i = S column i
-  // (S_p^T A)_{ij} = S_p col i . A col j
-#define STA(sx, sy, sz, a0, a1, a2, i) \
-  ((i) == 0 ? (-(sz)) * (a1) + (sy) * (a2) : (i) == 1 ? (sz) * (a0) - (sx) * (a2) : -(sy) * (a0) + (sx) * (a1))
-  const float R00 = STA(px, py, pz, A00, A10, A20, 0) + STA(nx, ny, nz, B00, B10, B20, 0);
-  const float R01 = STA(px, py, pz, A01, A11, A21, 0) + STA(nx, ny, nz, B01, B11, B21, 0);
-  const float R02 = STA(px, py, pz, A02, A12, A22, 0) + STA(nx, ny, nz, B02, B12, B22, 0);
-  const float R11 = STA(px, py, pz, A01, A11, A21, 1) + STA(nx, ny, nz, B01, B11, B21, 1);
-  const float R12 = STA(px, py, pz, A02, A12, A22, 1) + STA(nx, ny, nz, B02, B12, B22, 1);
-  const float R22 = STA(px, py, pz, A02, A12, A22, 2) + STA(nx, ny, nz, B02, B12, B22, 2);
-  const float br0 = STA(px, py, pz, ep0, ep1, ep2, 0) + STA(nx, ny, nz, en0, en1, en2, 0);
-  const float br1 = STA(px, py, pz, ep0, ep1, ep2, 1) + STA(nx, ny, nz, en0, en1, en2, 1);
-  const float br2 = STA(px, py, pz, ep0, ep1, ep2, 2) + STA(nx, ny, nz, en0, en1, en2, 2);
-#undef STA
+  const float px = rp[0], py = rp[1], pz = rp[2];
+  const float nx = rn[0], ny = rn[1], nz = rn[2];
+  // A = Wp C: column 0 = Wp (0, pz, -py), column 1 = Wp (-pz, 0, px), column 2 = Wp (py, -px, 0)
+  const float A00 = fmaf(P01, pz, -P02 * py), A10 = fmaf(P11, pz, -P12 * py), A20 = fmaf(P12, pz, -P22 * py);
+  const float A01 = fmaf(P02, px, -P00 * pz), A11 = fmaf(P12, px, -P01 * pz), A21 = fmaf(P22, px, -P02 * pz);
+  const float A02 = fmaf(P00, py, -P01 * px), A12 = fmaf(P01, py, -P11 * px), A22 = fmaf(P02, py, -P12 * px);
+  const float B00 = fmaf(N01, nz, -N02 * ny), B10 = fmaf(N11, nz, -N12 * ny), B20 = fmaf(N12, nz, -N22 * ny);
+  const float B01 = fmaf(N02, nx, -N00 * nz), B11 = fmaf(N12, nx, -N01 * nz), B21 = fmaf(N22, nx, -N02 * nz);
+  const float B02 = fmaf(N00, ny, -N01 * nx), B12 = fmaf(N01, ny, -N11 * nx), B22 = fmaf(N02, ny, -N12 * nx);
+  // C^T A + [n]x^T B: row i of C^T is column i of C
+#define CTA(sx, sy, sz, a0, a1, a2, i) \
+  ((i) == 0 ? fmaf(sz, a1, -(sy) * (a2)) : (i) == 1 ? fmaf(sx, a2, -(sz) * (a0)) : fmaf(sy, a0, -(sx) * (a1)))
+#define CTA_ADD(sx, sy, sz, a0, a1, a2, i, t)                                    \
+  ((i) == 0 ? fmaf(sz, a1, fmaf(-(sy), a2, t)) : (i) == 1 ? fmaf(sx, a2, fmaf(-(sz), a0, t)) \
+            : fmaf(sy, a0, fmaf(-(sx), a1, t)))
+  const float R00 = CTA_ADD(nx, ny, nz, B00, B10, B20, 0, CTA(px, py, pz, A00, A10, A20, 0));
+  const float R01 = CTA_ADD(nx, ny, nz, B01, B11, B21, 0, CTA(px, py, pz, A01, A11, A21, 0));
+  const float R02 = CTA_ADD(nx, ny, nz, B02, B12, B22, 0, CTA(px, py, pz, A02, A12, A22, 0));
+  const float R11 = CTA_ADD(nx, ny, nz, B01, B11, B21, 1, CTA(px, py, pz, A01, A11, A21, 1));
+  const float R12 = CTA_ADD(nx, ny, nz, B02, B12, B22, 1, CTA(px, py, pz, A02, A12, A22, 1));
+  const float R22 = CTA_ADD(nx, ny, nz, B02, B12, B22, 2, CTA(px, py, pz, A02, A12, A22, 2));
+  // C^T e_p + D^T e_n
+  const float br0 = CTA_ADD(nx, ny, nz, en0, en1, en2, 0, CTA(px, py, pz, ep0, ep1, ep2, 0));
+  const float br1 = CTA_ADD(nx, ny, nz, en0, en1, en2, 1, CTA(px, py, pz, ep0, ep1, ep2, 1));
+  const float br2 = CTA_ADD(nx, ny, nz, en0, en1, en2, 2, CTA(px, py, pz, ep0, ep1, ep2, 2));
+#undef CTA
+#undef CTA_ADD
   // H upper triangle, row-major over the 6x6: rows 0-2 = [Htt | Htr], rows 3-5 = [. | Hrr]
-  acc_add(acc[0], k * P00); acc_add(acc[1], k * P01); acc_add(acc[2], k * P02);
-  acc_add(acc[3], k * A00); acc_add(acc[4], k * A01); acc_add(acc[5], k * A02);
-  acc_add(acc[6], k * P11); acc_add(acc[7], k * P12);
-  acc_add(acc[8], k * A10); acc_add(acc[9], k * A11); acc_add(acc[10], k * A12);
-  acc_add(acc[11], k * P22);
-  acc_add(acc[12], k * A20); acc_add(acc[13], k * A21); acc_add(acc[14], k * A22);
-  acc_add(acc[15], k * R00); acc_add(acc[16], k * R01); acc_add(acc[17], k * R02);
-  acc_add(acc[18], k * R11); acc_add(acc[19], k * R12);
-  acc_add(acc[20], k * R22);
-  acc_add(acc[21], k * ep0); acc_add(acc[22], k * ep1); acc_add(acc[23], k * ep2);
-  acc_add(acc[24], k * br0); acc_add(acc[25], k * br1); acc_add(acc[26], k * br2);
-  acc_add(acc[27], k * chi);
-  acc_add(acc[28], 1.0f);
+  acc[0] = fmaf(k, P00, acc[0]); acc[1] = fmaf(k, P01, acc[1]); acc[2] = fmaf(k, P02, acc[2]);
+  acc[3] = fmaf(k, A00, acc[3]); acc[4] = fmaf(k, A01, acc[4]); acc[5] = fmaf(k, A02, acc[5]);
+  acc[6] = fmaf(k, P11, acc[6]); acc[7] = fmaf(k, P12, acc[7]);
+  acc[8] = fmaf(k, A10, acc[8]); acc[9] = fmaf(k, A11, acc[9]); acc[10] = fmaf(k, A12, acc[10]);
+  acc[11] = fmaf(k, P22, acc[11]);
+  acc[12] = fmaf(k, A20, acc[12]); acc[13] = fmaf(k, A21, acc[13]); acc[14] = fmaf(k, A22, acc[14]);
+  acc[15] = fmaf(k, R00, acc[15]); acc[16] = fmaf(k, R01, acc[16]); acc[17] = fmaf(k, R02, acc[17]);
+  acc[18] = fmaf(k, R11, acc[18]); acc[19] = fmaf(k, R12, acc[19]);
+  acc[20] = fmaf(k, R22, acc[20]);
+  acc[21] = fmaf(k, ep0, acc[21]); acc[22] = fmaf(k, ep1, acc[22]); acc[23] = fmaf(k, ep2, acc[23]);
+  acc[24] = fmaf(k, br0, acc[24]); acc[25] = fmaf(k, br1, acc[25]); acc[26] = fmaf(k, br2, acc[26]);
+  acc[27] = fmaf(k, chi, acc[27]);
+  acc[28] += 1.0f;
+}
+
+// the constant factor of each slot's terms (lin_terms): -2 for H_tr and b_r, 4 for H_rr
+__device__ __forceinline__ double slot_scale(int i) {
+  return (i == 3 || i == 4 || i == 5 || i == 8 || i == 9 || i == 10 || i == 12 || i == 13 || i == 14 ||
+          (i >= 24 && i <= 26))
+             ? -2.0
+             : (i >= 15 && i <= 20) ? 4.0 : 1.0;
 }
 
 // the value of lane ^ O (O = 1, 2, 4, 8: within a DPP row of 16 lanes)
@@ -1300,28 +1317,6 @@ __device__ __forceinline__ double dpp_xor_f64(double x) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// one halving step of the reduce-scatter for partner offsets 32 and 16: a permlane swap of
-// (v[i], v[h + i]) leaves each lane its kept slot and its partner's copy of it, one in each
-// register, whichever half the lane keeps
-template <int O>
-__device__ __forceinline__ double swap_add(double x, double y) {
-  const unsigned long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
-  const unsigned int xl = (unsigned int)bx, xh = (unsigned int)(bx >> 32);
-  const unsigned int yl = (unsigned int)by, yh = (unsigned int)(by >> 32);
-  unsigned int nxl, nxh, nyl, nyh;
-  if (O == 32) {
-    const auto l = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
-    const auto h = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
-    nxl = l[0]; nyl = l[1]; nxh = h[0]; nyh = h[1];
-  } else {
-    const auto l = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
-    const auto h = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
-    nxl = l[0]; nyl = l[1]; nxh = h[0]; nyh = h[1];
-  }
-  return __longlong_as_double((long long)(((unsigned long long)nxh << 32) | nxl)) +
-         __longlong_as_double((long long)(((unsigned long long)nyh << 32) | nyl));
-}
-
 // halving step for partner offsets 8, 4, 2 (DPP within a row)
 template <int O, int H>
 __device__ __forceinline__ void dpp_halve(double *v) {
@@ -1333,24 +1328,38 @@ __device__ __forceinline__ void dpp_halve(double *v) {
   }
 }
 
-// block-wide FP64 reduction of the kVals accumulators into one partial. Within a wavefront a
+// one halving step of the reduce-scatter on FP32 slots: the permlane swap of (x, y) leaves the
+// lane its kept slot and its partner's copy, one in each register
+template <int O>
+__device__ __forceinline__ float swap_add_f32(float x, float y) {
+  const auto r = O == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false)
+                         : __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// block-wide reduction of the kVals accumulators into one FP64 partial. Within a wavefront a
 // reduce-scatter: the kPartial (32) slots are halved five times; at partner offset o a lane
 // keeps the half of its slots selected by its lane bit o and adds its partner's copy of it
-// (32 FP64 exchanges instead of 6 per slot). Lane l then holds slot l >> 1 summed over
-// the half of the wavefront with its lane bit 0, and one more exchange with lane l ^ 1
-// completes it. The order of the additions is fixed. The exchanges run on the vector ALU:
-// permlane swaps for offsets 32 and 16, DPP moves below.
+// (32 exchanges instead of 6 per slot). Lane l then holds slot l >> 1 summed over the half of
+// the wavefront with its lane bit 0, and one more exchange with lane l ^ 1 completes it. The
+// first two halvings (offsets 32 and 16, permlane swaps) add in FP32, so an FP32 sum covers at
+// most the sixteen correspondences of four lanes; the rest (DPP moves, offsets 8 to 1), the sum
+// over the wavefronts and each slot's constant factor (slot_scale, a power of two) are FP64. The
+// order of the additions is fixed.
 __device__ void block_partial(Acc2 *acc, double *out) {
   static_assert(kPartial == 32, "the reduce-scatter halves 32 slots five times");
   __shared__ double red[kThreads / 64][kPartial];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double v[kPartial];
+  float f[kPartial];
 #pragma unroll
-  for (int i = 0; i < kPartial; ++i) v[i] = i < kVals ? (double)acc[i] : 0.0;
+  for (int i = 0; i < kPartial; ++i) f[i] = i < kVals ? acc[i] : 0.0f;
 #pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = swap_add<32>(v[i], v[16 + i]);
+  for (int i = 0; i < 16; ++i) f[i] = swap_add_f32<32>(f[i], f[16 + i]);
 #pragma unroll
-  for (int i = 0; i < 8; ++i) v[i] = swap_add<16>(v[i], v[8 + i]);
+  for (int i = 0; i < 8; ++i) f[i] = swap_add_f32<16>(f[i], f[8 + i]);
+  double v[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = (double)f[i];
   dpp_halve<8, 4>(v);
   dpp_halve<4, 2>(v);
   dpp_halve<2, 1>(v);
@@ -1361,7 +1370,7 @@ __device__ void block_partial(Acc2 *acc, double *out) {
     double s = 0.0;
     if (threadIdx.x < kVals)
       for (int w = 0; w < kThreads / 64; ++w) s += red[w][threadIdx.x];
-    *gmut(out + threadIdx.x) = s;
+    *gmut(out + threadIdx.x) = s * slot_scale(threadIdx.x);
   }
 }
 
@@ -1381,7 +1390,7 @@ __device__ __forceinline__ bool gates_loaded(const float4 &cn, const float4 &rn0
                    dot4(rn0.x, rn0.y, rn0.z, 0.0f, rn0.x, rn0.y, rn0.z, 0.0f) == 0.0f))
     return false;
   iso_mul3(T, rp0.x, rp0.y, rp0.z, rp0.w, rp);
-  iso_mul3(T, rn0.x, rn0.y, rn0.z, 0.0f, rn);
+  iso_rot3(T, rn0.x, rn0.y, rn0.z, rn);
   const float dv0 = cp.x - rp[0], dv1 = cp.y - rp[1], dv2 = cp.z - rp[2], dv3 = cp.w - rp0.w;
   if (dot4(dv0, dv1, dv2, dv3, dv0, dv1, dv2, dv3) > ap.sq_thr) return false;
   if (!ap.gicp) {
@@ -1406,7 +1415,7 @@ __device__ __forceinline__ bool gates(const FrameDev &fr, const FrameDev &fc, in
   cp = fc.pts[ci];
   const float4 rp0 = fr.pts[ri];
   iso_mul3(T, rp0.x, rp0.y, rp0.z, rp0.w, rp);
-  iso_mul3(T, rn0.x, rn0.y, rn0.z, 0.0f, rn);
+  iso_rot3(T, rn0.x, rn0.y, rn0.z, rn);
   const float dv0 = cp.x - rp[0], dv1 = cp.y - rp[1], dv2 = cp.z - rp[2], dv3 = cp.w - rp0.w;
   if (dot4(dv0, dv1, dv2, dv3, dv0, dv1, dv2, dv3) > ap.sq_thr) return false;
   if (!ap.gicp) {
@@ -1427,7 +1436,7 @@ __device__ int window_search(const FrameDev &fr, const FrameDev &fc, const int *
   if (dot4(rn0.x, rn0.y, rn0.z, 0.0f, rn0.x, rn0.y, rn0.z, 0.0f) == 0.0f) return -1;
   const float4 rp0 = fr.pts[ri];
   iso_mul3(T, rp0.x, rp0.y, rp0.z, rp0.w, rp);
-  iso_mul3(T, rn0.x, rn0.y, rn0.z, 0.0f, rn);
+  iso_rot3(T, rn0.x, rn0.y, rn0.z, rn);
   float rc = rn0.w;
   if (rc < ap.flat) rc = ap.flat;
   const int r0 = max(r - ap.row_region, 0), r1 = min(r + ap.row_region + 1, rows);
@@ -1507,7 +1516,7 @@ __device__ __forceinline__ void corr_lin_tile(const PairDev &pd, const FrameDev 
     for (int k = 0; k < 4; ++k) {
       ok[k] = false;
       if (ri[k] >= 0 && ci[k] >= 0) {
-        acc_add(acc[30], 1.0f);
+        acc[30] += 1.0f;
         ok[k] = gates_loaded(cn[k], rn0[k], cp[k], rp0[k], T, ap, rp[k], rn[k]);
       }
     }

@@ -40,11 +40,20 @@ NICP_HD void mul4_packet(const float *A, float v0, float v1, float v2, float v3,
   }
 }
 
-// P3 (only the first three components; w is the input's)
+// Isometry3f * Vector4f under Eigen's packet order (a0 v0 + a1 v1) + (a2 v2 + a3 v3); only the
+// first three components (w is the input's)
 NICP_HD void iso_mul3(const float *A, float v0, float v1, float v2, float v3, float *o) {
 #pragma unroll
   for (int i = 0; i < 3; ++i)
     o[i] = (mc(A, i, 0) * v0 + mc(A, i, 1) * v1) + (mc(A, i, 2) * v2 + mc(A, i, 3) * v3);
+}
+
+// iso_mul3 of a direction (w = 0): with a finite translation a3 * 0 is a signed zero, and adding
+// a signed zero changes no value (at most the sign of a zero result), so the term is dropped;
+// every comparison and sum downstream sees the same values
+NICP_HD void iso_rot3(const float *A, float v0, float v1, float v2, float *o) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) o[i] = (mc(A, i, 0) * v0 + mc(A, i, 1) * v1) + mc(A, i, 2) * v2;
 }
 
 // P4

@@ -60,6 +60,42 @@ def _oracle_align_on_device_clouds(b, f_ref, f_cur, shape, init16, ap_oracle, ca
     return T, res
 
 
+def _reference_spread(b, f_ref, f_cur, shape, init16, ap_oracle, cam):
+    """How far the reference's own result moves with the rounding of its sums on these clouds: the
+    largest distance between its FP32 loop at 1, 2, 4 and 8 OpenMP threads (the thread count
+    regroups the partial sums) and the FP64-solve design: (rad, m)."""
+    rows, cols = shape
+    cr = b.frame_cloud(f_ref, shape)
+    cc = b.frame_cloud(f_cur, shape)
+    ocr, occ = _oracle_cloud(cr), _oracle_cloud(cc)
+    Ts = [O.align(ocr, occ, rows, cols, cam, init16, ap_oracle, n)[0] for n in (1, 2, 4, 8)]
+    Ts.append(O.align(ocr, occ, rows, cols, cam, init16, ap_oracle, 1, fp64_solve=True)[0])
+    cr.close()
+    cc.close()
+    sr = st = 0.0
+    for i in range(len(Ts)):
+        for j in range(i):
+            rot, tr = synth.pose_error(Ts[i], Ts[j])
+            sr, st = max(sr, np.radians(rot)), max(st, tr)
+    return sr, st
+
+
+def _one_iteration_vs_oracle(ctx, b, f_ref, f_cur, shape, cam_dev, cam_oracle):
+    """One outer iteration from the identity on the batch's clouds, device against the FP64-solve
+    oracle: the correspondences are those of the same start, so only the arithmetic differs."""
+    rows, cols = shape
+    oap1, dap1 = aparams(outer_iterations=1)
+    cr = b.frame_cloud(f_ref, shape)
+    cc = b.frame_cloud(f_cur, shape)
+    Td, _ = ctx.align(cr, cc, cam_dev, rows, cols, identity16(), dap1)
+    To, _ = O.align(_oracle_cloud(cr), _oracle_cloud(cc), rows, cols, cam_oracle, identity16(), oap1, 1,
+                    fp64_solve=True)
+    cr.close()
+    cc.close()
+    rot, tr = synth.pose_error(Td, To)
+    return np.radians(rot), tr
+
+
 def test_learned_gain_640x480(ctx):
     """What the trained network gains over the identity start on held-out pairs (seeds from 1000;
     training used seeds from 10,000,000)."""
@@ -112,7 +148,12 @@ def test_learned_gain_640x480(ctx):
 
 
 def test_uneven_batch_512(ctx):
-    """512 pairs of the four resolutions in one batch, 15 iterations, autotuned, graph replay."""
+    """512 pairs of the four resolutions in one batch, 15 iterations, autotuned, graph replay.
+    Against the FP64-solve oracle after 15 iterations the bar is 1e-5 rad / 1e-5 m, or twice the
+    reference's own spread (its FP32 loop across OpenMP thread counts and its FP64 design) where
+    that is larger: a different rounding of H flips a few gate-boundary correspondences, and over
+    15 iterations that moves a 1280x960 pair's fixed point by up to ~2e-5 m. After one iteration
+    from the same start the correspondences are the same, and the bar is 1e-6."""
     n_pairs = 512
     say("case uneven_batch pairs=%d" % n_pairs)
     rng = np.random.default_rng(7)
@@ -165,9 +206,13 @@ def test_uneven_batch_512(ctx):
         i = int(al[0]) if len(al) else int(idx[0])
         To, ro = _oracle_align_on_device_clouds(b, int(ref[i]), int(cur[i]), (r, c), identity16(), oap, ocam)
         rot, tr = synth.pose_error(Tg[i], To)
-        say("  %dx%d pair %d vs oracle: %.2e rad %.2e m (aligned %s)" % (c, r, i, np.radians(rot), tr, bool(ok[i])))
+        sr, st = _reference_spread(b, int(ref[i]), int(cur[i]), (r, c), identity16(), oap, ocam)
+        r1, t1 = _one_iteration_vs_oracle(ctx, b, int(ref[i]), int(cur[i]), (r, c), dcam, ocam)
+        say("  %dx%d pair %d vs oracle: %.2e rad %.2e m (aligned %s; reference spread %.2e rad %.2e m; "
+            "one iteration %.2e rad %.2e m)" % (c, r, i, np.radians(rot), tr, bool(ok[i]), sr, st, r1, t1))
+        assert r1 < 1e-6 and t1 < 1e-6
         if ok[i]:
-            assert np.radians(rot) < 1e-5 and tr < 1e-5
+            assert np.radians(rot) < max(1e-5, 2 * sr) and tr < max(1e-5, 2 * st)
     for k in (1, 2, 3):
         assert ok[pick == k].sum() >= 1
     b.close()
