@@ -364,6 +364,9 @@ struct nicp_batch {
   void *arena = nullptr;
   FrameDev *d_frames = nullptr;
   int *d_blk_frame = nullptr, *d_xblk_frame = nullptr, *d_yblk_frame = nullptr;
+  // frames by pixel count, largest first: the order in which k_integral_frame takes them (one
+  // workgroup per frame), so the largest frames of an uneven batch start first
+  int *d_frame_order = nullptr;
   int total_tiles = 0, total_xblk = 0, total_yblk = 0;
   // align workspace
   DevBuf pairs_tbl, maps, zbuf, cur_index, pairs_buf, state, partials, result, sbox;
@@ -580,7 +583,8 @@ constexpr int kFusedIntegralFrames = 16;
 // Launch sequence of the conversion for a frame table already on the device.
 // All frames of one launch share the projector model.
 int launch_convert(Prof *pf, hipStream_t s, const FrameDev *d_frames, int num_frames, const int *d_blk, int total_tiles,
-                   const int *d_x, int total_x, const int *d_y, int total_y, const StatsParamsDev &sp, int model) {
+                   const int *d_x, int total_x, const int *d_y, int total_y, const StatsParamsDev &sp, int model,
+                   const int *d_order = nullptr) {
   if (total_tiles == 0) return NICP_OK;
   (void)hipGetLastError();
   const bool sph = model == NICP_SPHERICAL;
@@ -591,8 +595,9 @@ int launch_convert(Prof *pf, hipStream_t s, const FrameDev *d_frames, int num_fr
   else LAUNCH(pf, KID_UNPROJECT, k_unproject, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
   if (!sph) LAUNCH(pf, KID_SRC_TILES, k_src_tiles, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
   if (num_frames >= kFusedIntegralFrames) {
-    if (sph) LAUNCH(pf, KID_INTEGRAL_FRAME, k_integral_frame_sph, dim3(num_frames), dim3(kIFrameThreads), s, d_frames);
-    else LAUNCH(pf, KID_INTEGRAL_FRAME, k_integral_frame, dim3(num_frames), dim3(kIFrameThreads), s, d_frames);
+    if (sph)
+      LAUNCH(pf, KID_INTEGRAL_FRAME, k_integral_frame_sph, dim3(num_frames), dim3(kIFrameThreads), s, d_frames, d_order);
+    else LAUNCH(pf, KID_INTEGRAL_FRAME, k_integral_frame, dim3(num_frames), dim3(kIFrameThreads), s, d_frames, d_order);
   } else {
     if (sph) LAUNCH(pf, KID_INTEGRAL_X, k_integral_x_sph, dim3(total_x), dim3(kThreads), s, d_frames, d_x);
     else LAUNCH(pf, KID_INTEGRAL_X, k_integral_x, dim3(total_x), dim3(kThreads), s, d_frames, d_x);
@@ -1756,6 +1761,12 @@ extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows,
   const size_t o_blk = cv.take<int>(blk.size());
   const size_t o_x = cv.take<int>(xblk.size());
   const size_t o_y = cv.take<int>(yblk.size());
+  const size_t o_ord = cv.take<int>(num_frames);
+  std::vector<int> order(num_frames);
+  for (int i = 0; i < num_frames; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return (long long)rows[x] * cols[x] > (long long)rows[y] * cols[y];
+  });
   hipError_t e = hipMalloc(&b->arena, cv.off);
   if (e != hipSuccess) {
     delete b;
@@ -1767,6 +1778,7 @@ extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows,
   b->d_blk_frame = (int *)(base + o_blk);
   b->d_xblk_frame = (int *)(base + o_x);
   b->d_yblk_frame = (int *)(base + o_y);
+  b->d_frame_order = (int *)(base + o_ord);
   b->total_tiles = tiles;
   b->total_xblk = xb;
   b->total_yblk = yb;
@@ -1775,6 +1787,7 @@ extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows,
   if (e == hipSuccess) e = hipMemcpyAsync(b->d_blk_frame, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(b->d_xblk_frame, xblk.data(), xblk.size() * sizeof(int), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(b->d_yblk_frame, yblk.data(), yblk.size() * sizeof(int), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->d_frame_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     (void)hipFree(b->arena);
@@ -1828,7 +1841,8 @@ extern "C" int nicp_batch_convert(nicp_batch *b) {
   spd.fp64_gap = b->ctx->fp64_eigen_gap;
   auto enqueue = [&]() {
     return launch_convert(&b->ctx->prof, b->ctx->stream, b->d_frames, b->num_frames, b->d_blk_frame, b->total_tiles,
-                          b->d_xblk_frame, b->total_xblk, b->d_yblk_frame, b->total_yblk, spd, b->frames[0].model);
+                          b->d_xblk_frame, b->total_xblk, b->d_yblk_frame, b->total_yblk, spd, b->frames[0].model,
+                          b->d_frame_order);
   };
   if (b->ctx->prof.on) {
     // per-launch timing runs eagerly

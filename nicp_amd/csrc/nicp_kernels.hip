@@ -542,9 +542,9 @@ __device__ __forceinline__ float x_channel1(const FrameDev &fd, const XRows &x, 
 }
 
 template <int M>
-__device__ __forceinline__ void integral_frame_body(const FrameDev *__restrict__ F) {
+__device__ __forceinline__ void integral_frame_body(const FrameDev *__restrict__ F, const int *__restrict__ order) {
   __shared__ float tile[2][kIBand * kIStride];
-  const FrameDev &fd = F[blockIdx.x];
+  const FrameDev &fd = F[order ? order[blockIdx.x] : blockIdx.x];
   const int rows = fd.rows, cols = fd.cols;
   const int nchunks = (cols + kIChunk - 1) / kIChunk, nbands = (rows + kIBand - 1) / kIBand;
   const int steps = nbands * nchunks;
@@ -664,11 +664,14 @@ __device__ __forceinline__ void integral_frame_body(const FrameDev *__restrict__
     __syncthreads();
   }
 }
-extern "C" __global__ __launch_bounds__(kIFrameThreads) void k_integral_frame(const FrameDev *__restrict__ F) {
-  integral_frame_body<kPinhole>(F);
+// one workgroup per frame; workgroup i takes frame order[i] (order null: frame i)
+extern "C" __global__ __launch_bounds__(kIFrameThreads) void k_integral_frame(const FrameDev *__restrict__ F,
+                                                                              const int *__restrict__ order) {
+  integral_frame_body<kPinhole>(F, order);
 }
-extern "C" __global__ __launch_bounds__(kIFrameThreads) void k_integral_frame_sph(const FrameDev *__restrict__ F) {
-  integral_frame_body<kSpherical>(F);
+extern "C" __global__ __launch_bounds__(kIFrameThreads) void k_integral_frame_sph(const FrameDev *__restrict__ F,
+                                                                                  const int *__restrict__ order) {
+  integral_frame_body<kSpherical>(F, order);
 }
 
 // ---- Eigen SelfAdjointEigenSolver<Matrix3f>::computeDirect, restated (3x3 FP32) ----

@@ -195,6 +195,9 @@ def main(argv=None):
     ap.add_argument("--init-seed", type=int, default=1)
     ap.add_argument("--input-mode", type=int, default=1, help="0: depth, 1: inverse depth")
     ap.add_argument("--no-augment", action="store_true")
+    ap.add_argument("--lr-min", type=float, default=1e-5)
+    ap.add_argument("--init-weights", default=None,
+                    help="start from these weights (.npy of bf16 bits, with their .json record) instead of the seed")
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     a = ap.parse_args(argv)
 
@@ -205,6 +208,18 @@ def main(argv=None):
     net = capi.InitNet(ctx, seed=a.init_seed)
     net.set_loss_weights(LOSS_WEIGHTS)
     net.set_input_mode(a.input_mode)
+    previous = None
+    if a.init_weights:
+        w = load_weights(a.init_weights)
+        if w is None:
+            raise SystemExit("no weights at %s" % a.init_weights)
+        net.set_params(w)
+        rec = os.path.splitext(a.init_weights)[0] + ".json"
+        if os.path.exists(rec):
+            with open(rec) as f:
+                previous = json.load(f)
+        if previous and previous.get("input_mode", a.input_mode) != a.input_mode:
+            raise SystemExit("the initial weights were trained with input mode %d" % previous["input_mode"])
     log("generating %d training pairs" % a.pairs)
     X, Y, gen = generate(ctx, net, a.pairs, a.seed_base, log=log)
     log("generating %d validation pairs" % a.val_pairs)
@@ -214,12 +229,16 @@ def main(argv=None):
         if ep % 10 == 0 and ep < a.epochs:
             log("  validation after %d epochs: %s" % (ep, evaluate(ctx, net, Xv, Yv)))
 
-    tr = train(ctx, net, X, Y, a.epochs, a.batch, a.lr, augment=not a.no_augment, log=log, on_epoch=on_epoch)
+    tr = train(ctx, net, X, Y, a.epochs, a.batch, a.lr, a.lr_min, augment=not a.no_augment, log=log,
+               on_epoch=on_epoch)
     val = evaluate(ctx, net, Xv, Yv)
     log("validation: %s" % val)
     record = {"generation": gen, "training": tr, "validation": val, "train_seeds": [a.seed_base, a.seed_base + a.pairs],
               "loss_weights": LOSS_WEIGHTS, "motion": {"max_rot_deg": 30.0, "max_trans_m": 0.5},
               "resolution": [COLS, ROWS], "init_seed": a.init_seed, "input_mode": a.input_mode}
+    if previous is not None:
+        # the run this one started from (its own predecessors stay in its record's "previous")
+        record["previous"] = previous
     save_weights(a.out, net.get_params(), record)
     log("saved %s" % a.out)
     return 0
