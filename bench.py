@@ -3,22 +3,28 @@ from a learned initial guess against plain single-pair NICP.
 
 Workload: held-out seeded synthetic pairs (20-plane/box scenes, second camera moved
 by up to 30 deg / 0.5 m, depth noise sigma = 1.5e-3 z^2, 10 % holes), rendered on
-the device into a batch of frames before each step (input generation, not timed).
-One step = the registration path over one batch of pairs resident in HBM: both
+the device before the timed region (input generation, not timed) into one packed device
+store that holds every step's frames.
+One step = the registration path over one batch of pairs resident in HBM: the copy of
+its depth images from the store into a batch, both
 frames of every pair converted to clouds (unprojection, integral-image normals,
 information matrices), the network input and forward pass of the learned initial
 guess, one linearisation at the predicted and at the identity start to choose the
 start, 10 Gauss-Newton iterations of projective NICP from the chosen start, and
 Aligner::_computeStatistics (the statistics linearisation and the FP64 unscented
 solution statistics, on the device) -- one launch per stage for the whole batch, with
-every pair's transform and result record copied back. A pair counts as aligned when its
+every pair's transform and result record copied back. The timed region keeps --streams
+batches in flight (default 2: one context, HIP stream and host thread each, step k on lane
+k mod 2), so one batch's conversion overlaps another's refinement; ms_per_step is the wall
+time of all timed steps divided by their number. A pair counts as aligned when its
 rotation error is below 0.1 deg and its translation error below 2 mm against the
 sampled ground truth.
 
 Prints one JSON line. Besides the contract keys: the plain baselines (single pair
 per call from the identity, and batched from the identity = the same pairs without
 the network), per-kernel device times from HIP events recorded on the library's
-stream inside the timed region, the HBM roofline of the dominant kernel, the
+stream in a separate single-stream eager pass over the first timed batches (its
+transforms checked equal to the timed ones), the HBM roofline of the dominant kernel, the
 matrix-core rate of the network's forward pass, and the CPU oracle on a bounded
 sample on the host cores.
 """
@@ -27,6 +33,7 @@ import json
 import os
 import re
 import sys
+import threading
 import time
 
 import numpy as np
@@ -349,6 +356,9 @@ def main():
     ap_.add_argument("--projection", default="auto", choices=["auto", "0", "1", "2"],
                      help="projection strategy of the batched refinement: autotuned, or forced (0: global "
                           "z-buffer, 1: global z-buffer read first, 2: tile-binned and fused)")
+    ap_.add_argument("--streams", type=int, default=2,
+                     help="batches in flight in the timed region: each on its own context (HIP stream), driven "
+                          "by its own host thread, so one batch's conversion overlaps another's refinement")
     ap_.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                      help="only run a device-to-device copy of MIB MiB (to scale rocprof's FETCH_SIZE/WRITE_SIZE)")
     args = ap_.parse_args()
@@ -372,18 +382,24 @@ def main():
     input_mode = json.load(open(record_path)).get("input_mode", 0) if os.path.exists(record_path) else 0
 
     B = args.pairs
-    ctx = capi.Context(0)
-    net = capi.InitNet(ctx, seed=1)
-    net.set_params(weights)
-    net.set_input_mode(input_mode)
     ap = capi.align_params(outer_iterations=args.iters)
-    batch = initnet.make_batch(ctx, B)
-    if args.projection == "auto":
-        batch.set_autotune(True)  # the projection strategy is picked during the warm-up
-    else:
-        batch.set_projection(int(args.projection))
     ref = np.arange(0, 2 * B, 2, dtype=np.int32)
     cur = ref + 1
+    # lanes: a context (its own HIP stream), network and batch each; lane 0 also runs the eager
+    # per-kernel pass and the byte-model counts
+    lanes = []
+    for _ in range(max(1, args.streams)):
+        c = capi.Context(0)
+        n_ = capi.InitNet(c, seed=1)
+        n_.set_params(weights)
+        n_.set_input_mode(input_mode)
+        b_ = initnet.make_batch(c, B)
+        if args.projection == "auto":
+            b_.set_autotune(True)  # the projection strategy is picked during the warm-up
+        else:
+            b_.set_projection(int(args.projection))
+        lanes.append((c, n_, b_))
+    ctx, net, batch = lanes[0]
 
     def seeds_of(base, k):
         return np.arange(base + k * B, base + (k + 1) * B, dtype=np.uint64)
@@ -391,46 +407,79 @@ def main():
     # a step is the whole AlignerProjective::align of every pair, results included: the
     # statistics linearisation and Aligner::_computeStatistics run on the device, and every
     # pair's transform, H, b, error, inliers, Omega and validity are copied back
-    def learned_step(use_graph):
-        batch.convert()
-        T, res, _, used = batch.align_learned(net, ref, cur, ap, use_graph=use_graph, want_results=True)
+    def learned_step(lane, use_graph):
+        _, n_, b_ = lanes[lane]
+        b_.convert()
+        T, res, _, used = b_.align_learned(n_, ref, cur, ap, use_graph=use_graph, want_results=True)
         return T, used
 
-    def plain_step(use_graph):
-        batch.convert()
-        T, _ = batch.align(ref, cur, None, ap, use_graph=use_graph, want_results=True)
+    def plain_step(lane, use_graph):
+        _, _, b_ = lanes[lane]
+        b_.convert()
+        T, _ = b_.align(ref, cur, None, ap, use_graph=use_graph, want_results=True)
         return T
 
-    # warm-up (separate seeds)
-    for k in range(args.warmup):
-        initnet.render_pairs(batch, seeds_of(WARMUP_SEED_BASE, k), max_rot, max_trans)
-        learned_step(True)
-        plain_step(True)
-    # set-up of the production path on a warm-up batch: the first graph-mode call tunes the
-    # projection strategy (eagerly), the second captures the refinement's hipGraph
-    initnet.render_pairs(batch, seeds_of(WARMUP_SEED_BASE, 0), max_rot, max_trans)
-    learned_step(True)
-    learned_step(True)
+    # warm-up (separate seeds) and the set-up of the production path on every lane: the first
+    # graph-mode call tunes the projection strategy (eagerly), the second captures the graphs
+    for lane, (c, _, b_) in enumerate(lanes):
+        for k in range(args.warmup):
+            initnet.render_pairs(b_, seeds_of(WARMUP_SEED_BASE, k), max_rot, max_trans)
+            learned_step(lane, True)
+            plain_step(lane, True)
+        initnet.render_pairs(b_, seeds_of(WARMUP_SEED_BASE, 0), max_rot, max_trans)
+        learned_step(lane, True)
+        learned_step(lane, True)
+        c.synchronize()
+
+    # inputs of the timed steps, resident in HBM before the timed region: every step's frames
+    # rendered once (a new batch of pairs per step) into one packed device store
+    n_pix = batch.num_pixels()
+    store = capi.DeviceBuffer(ctx, (args.steps, n_pix), np.uint16)
+    Tgts = []
+    for k in range(args.steps):
+        Tgts.append(initnet.render_pairs(batch, seeds_of(EVAL_SEED_BASE, k), max_rot, max_trans))
+        batch.download_depth_device(store.row_ptr(k))
     ctx.synchronize()
 
-    # timed region: every step a new batch, rendered beforehand; the refinement is replayed
-    # from its hipGraph, with no per-launch events
-    t_total = 0.0
+    def run_lanes(step):
+        """Every timed step once, step k on lane k % lanes, one host thread per lane; returns the
+        wall time from the first upload to the last result and the transforms by step."""
+        out = {}
+        errors = []
+
+        def work(lane):
+            try:
+                c, _, b_ = lanes[lane]
+                for k in range(lane, args.steps, len(lanes)):
+                    b_.upload_depth_device(store.row_ptr(k))
+                    out[k] = step(lane, True)
+                c.synchronize()
+            except Exception as e:  # surfaced after the join
+                errors.append(e)
+
+        threads = [threading.Thread(target=work, args=(i,)) for i in range(len(lanes))]
+        t0 = time.perf_counter()
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        dt = time.perf_counter() - t0
+        if errors:
+            raise errors[0]
+        return dt, out
+
+    # timed region: the steps' conversions, learned starts and refinements (replayed from their
+    # hipGraphs, no per-launch events), lanes overlapping
+    t_total, out = run_lanes(learned_step)
     n_ok = 0
     used_total = 0
-    Ts, Tgts = [], []
+    Ts = []
     for k in range(args.steps):
-        Tgt = initnet.render_pairs(batch, seeds_of(EVAL_SEED_BASE, k), max_rot, max_trans)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        T, used = learned_step(True)
-        ctx.synchronize()
-        t_total += time.perf_counter() - t0
-        ok, _ = aligned(T, Tgt)
+        T, used = out[k]
+        ok, _ = aligned(T, Tgts[k])
         n_ok += int(ok.sum())
         used_total += int(used.sum())
         Ts.append(T)
-        Tgts.append(Tgt)
     tuned_main = batch.autotune_state()
 
     # per-kernel times: the same step run eagerly with HIP events around every launch, over the
@@ -441,10 +490,10 @@ def main():
     t_eager = 0.0
     eager_same = True
     for k in range(psteps):
-        initnet.render_pairs(batch, seeds_of(EVAL_SEED_BASE, k), max_rot, max_trans)
+        batch.upload_depth_device(store.row_ptr(k))
         ctx.synchronize()
         t0 = time.perf_counter()
-        T, _ = learned_step(False)
+        T, _ = learned_step(0, False)
         ctx.synchronize()
         t_eager += time.perf_counter() - t0
         eager_same = eager_same and np.array_equal(T, Ts[k], equal_nan=True)
@@ -465,22 +514,14 @@ def main():
 
     # the same pairs without the network (batched, identity start, graph replay); bit-identical
     # to the single-pair path, so its accuracy is the plain single-pair accuracy
-    t_plain_b = 0.0
-    n_ok_plain = 0
-    for k in range(args.steps):
-        Tgt = initnet.render_pairs(batch, seeds_of(EVAL_SEED_BASE, k), max_rot, max_trans)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        T = plain_step(True)
-        ctx.synchronize()
-        t_plain_b += time.perf_counter() - t0
-        n_ok_plain += int(aligned(T, Tgt)[0].sum())
+    t_plain_b, out_plain = run_lanes(plain_step)
+    n_ok_plain = sum(int(aligned(out_plain[k], Tgts[k])[0].sum()) for k in range(args.steps))
     frac_plain = n_ok_plain / n_pairs
     log("batched from identity: %.2f ms/step, %d/%d aligned" % (1000 * t_plain_b / args.steps, n_ok_plain, n_pairs))
 
     # per-frame / per-pair counts of the last timed batch for the byte model
     frames_info, pairs_info = [], []
-    initnet.render_pairs(batch, seeds_of(EVAL_SEED_BASE, args.steps - 1), max_rot, max_trans)
+    batch.upload_depth_device(store.row_ptr(args.steps - 1))
     batch.convert()
     _, res, _, _ = batch.align_learned(net, ref, cur, ap, use_graph=False, want_results=True)
     for f in range(2 * B):
@@ -584,14 +625,16 @@ def main():
                                "both conversions + learned initial guess (bf16 CNN, chi2 fallback to identity) + "
                                "%d-iteration projective NICP per pair" % (B, max_rot, max_trans, args.iters),
                    "pairs_per_step": B, "eval_pairs": n_pairs, "resolution": "640x480", "iterations": args.iters,
-                   "init": "learned", "weights": os.path.relpath(weights_path, ROOT)},
+                   "init": "learned", "weights": os.path.relpath(weights_path, ROOT), "streams": len(lanes)},
         "aligned_fraction": round(n_ok / n_pairs, 4),
         "pairs_per_s": round(n_pairs / t_total, 2),
         "network_start_kept": round(used_total / n_pairs, 4),
         "rot_err_median_deg": round(float(np.median(err_learned[:, 0])), 4),
         "diverged_pairs": int(np.sum(~np.isfinite(err_learned[:, 0]))),
-        "timing": "refinement replayed from a hipGraph; per-kernel times from a separate eager pass "
-                  "with HIP events around every launch",
+        "timing": "wall time of all timed steps, %d batches in flight (one context/stream and host thread "
+                  "each; step k on lane k %% %d), inputs resident in HBM; conversion and learned step replayed "
+                  "from hipGraphs; per-kernel times from a separate single-stream eager pass with HIP events "
+                  "around every launch" % (len(lanes), len(lanes)),
         "eager_ms_per_step": round(1000.0 * t_eager / psteps, 3),
         "eager_equals_graph": bool(eager_same),
         "plain_single_pair_pairs_per_s": round(plain, 2) if plain else None,
@@ -612,7 +655,9 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    ctx.close()
+    store.close()
+    for c, _, _ in lanes:
+        c.close()
 
 
 if __name__ == "__main__":

@@ -294,6 +294,15 @@ int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows, const int 
                       const nicp_stats_params *sp, nicp_batch **out);
 int nicp_batch_destroy(nicp_batch *batch);
 int nicp_batch_upload_depth(nicp_batch *batch, int frame, const uint16_t *depth, float depth_scale);
+/* Depth already in device memory (no reference counterpart; it serves callers whose frames
+ * arrive in HBM, and pipelines that keep several batches in flight): every frame's uint16 image
+ * back to back in frame order (frame i starts after the rows*cols pixels of the frames before
+ * it; nicp_batch_num_pixels gives the total). The copy is enqueued on the batch's stream and
+ * the call returns without waiting: the source must stay valid until the stream passes it. */
+int nicp_batch_upload_depth_device(nicp_batch *batch, const uint16_t *depth_device, float depth_scale);
+/* the frames' depth images into device memory, in the same packed layout (enqueued, no wait) */
+int nicp_batch_download_depth_device(nicp_batch *batch, uint16_t *depth_device);
+int nicp_batch_num_pixels(const nicp_batch *batch, long long *num_pixels);
 /* DepthImageConverterIntegralImage::compute for every frame of the batch */
 int nicp_batch_convert(nicp_batch *batch);
 /* AlignerProjective::align for pairs (ref_frame[i], cur_frame[i]); T_out and results may be NULL.

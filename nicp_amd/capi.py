@@ -80,6 +80,7 @@ EXPORTS = [
     "nicp_cloud_download_gaussians", "nicp_cloud_transform", "nicp_cloud_add", "nicp_merge",
     "nicp_batch_set_autotune", "nicp_batch_autotune_read", "nicp_align_nn_batch", "nicp_solution_statistics",
     "nicp_align_noalloc", "nicp_batch_set_projection", "nicp_batch_set_bin_capacity", "nicp_batch_probe",
+    "nicp_batch_upload_depth_device", "nicp_batch_download_depth_device", "nicp_batch_num_pixels",
 ]
 
 _lib = None
@@ -119,6 +120,8 @@ def lib():
             "nicp_align_offsets": [vp, vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp],
             "nicp_batch_create": [vp, C.c_int, vp, vp, vp, vp, vp],
             "nicp_batch_destroy": [vp], "nicp_batch_upload_depth": [vp, C.c_int, vp, C.c_float],
+            "nicp_batch_upload_depth_device": [vp, vp, C.c_float], "nicp_batch_download_depth_device": [vp, vp],
+            "nicp_batch_num_pixels": [vp, vp],
             "nicp_batch_convert": [vp],
             "nicp_batch_align": [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp],
             "nicp_batch_frame_cloud": [vp, C.c_int, vp],
@@ -527,6 +530,20 @@ class Batch:
     def upload(self, frame, depth_u16, depth_scale=0.001):
         d = np.ascontiguousarray(depth_u16, np.uint16)
         _check(lib().nicp_batch_upload_depth(self.h, frame, _p(d), depth_scale))
+
+    def num_pixels(self):
+        n = C.c_longlong()
+        _check(lib().nicp_batch_num_pixels(self.h, C.byref(n)))
+        return n.value
+
+    def upload_depth_device(self, src_ptr, depth_scale=0.001):
+        """Every frame's depth from device memory at src_ptr (uint16, frames back to back); enqueued
+        on the context's stream, no wait."""
+        _check(lib().nicp_batch_upload_depth_device(self.h, C.c_void_p(src_ptr), depth_scale))
+
+    def download_depth_device(self, dst_ptr):
+        """Every frame's depth into device memory at dst_ptr (the layout of upload_depth_device)."""
+        _check(lib().nicp_batch_download_depth_device(self.h, C.c_void_p(dst_ptr)))
 
     def convert(self):
         _check(lib().nicp_batch_convert(self.h))

@@ -367,6 +367,9 @@ struct nicp_batch {
   // frames by pixel count, largest first: the order in which k_integral_frame takes them (one
   // workgroup per frame), so the largest frames of an uneven batch start first
   int *d_frame_order = nullptr;
+  // first pixel of each frame in the packed depth arrays of nicp_batch_upload/download_depth_device
+  long long *d_pix_base = nullptr;
+  long long total_pix = 0;
   int total_tiles = 0, total_xblk = 0, total_yblk = 0;
   // align workspace
   DevBuf pairs_tbl, maps, zbuf, cur_index, pairs_buf, state, partials, result, sbox;
@@ -1762,6 +1765,13 @@ extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows,
   const size_t o_x = cv.take<int>(xblk.size());
   const size_t o_y = cv.take<int>(yblk.size());
   const size_t o_ord = cv.take<int>(num_frames);
+  const size_t o_pb = cv.take<long long>(num_frames);
+  std::vector<long long> pix_base(num_frames);
+  long long total_pix = 0;
+  for (int i = 0; i < num_frames; ++i) {
+    pix_base[i] = total_pix;
+    total_pix += (long long)rows[i] * cols[i];
+  }
   std::vector<int> order(num_frames);
   for (int i = 0; i < num_frames; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -1779,6 +1789,8 @@ extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows,
   b->d_xblk_frame = (int *)(base + o_x);
   b->d_yblk_frame = (int *)(base + o_y);
   b->d_frame_order = (int *)(base + o_ord);
+  b->d_pix_base = (long long *)(base + o_pb);
+  b->total_pix = total_pix;
   b->total_tiles = tiles;
   b->total_xblk = xb;
   b->total_yblk = yb;
@@ -1788,6 +1800,8 @@ extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows,
   if (e == hipSuccess) e = hipMemcpyAsync(b->d_xblk_frame, xblk.data(), xblk.size() * sizeof(int), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(b->d_yblk_frame, yblk.data(), yblk.size() * sizeof(int), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(b->d_frame_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_pix_base, pix_base.data(), pix_base.size() * sizeof(long long), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     (void)hipFree(b->arena);
@@ -1830,6 +1844,38 @@ extern "C" int nicp_batch_upload_depth(nicp_batch *b, int frame, const uint16_t 
   HIPCHK(hipMemcpyAsync((void *)f.depth, depth, (size_t)f.rows * f.cols * sizeof(uint16_t), hipMemcpyHostToDevice,
                         b->ctx->stream));
   HIPCHK(hipStreamSynchronize(b->ctx->stream));
+  return NICP_OK;
+}
+
+extern "C" int nicp_batch_upload_depth_device(nicp_batch *b, const uint16_t *depth, float depth_scale) {
+  if (!b || !depth) return set_err(NICP_ERR_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  bool scale_changed = false;
+  for (FrameDev &f : b->frames) {
+    scale_changed = scale_changed || f.depth_scale != depth_scale;
+    f.depth_scale = depth_scale;
+  }
+  if (scale_changed)
+    HIPCHK(hipMemcpyAsync(b->d_frames, b->frames.data(), b->frames.size() * sizeof(FrameDev), hipMemcpyHostToDevice,
+                          b->ctx->stream));
+  LAUNCH((Prof *)nullptr, 0, k_batch_depth_copy, dim3(b->total_tiles), dim3(kThreads), b->ctx->stream, b->d_frames,
+         b->d_blk_frame, b->d_pix_base, depth, (uint16_t *)nullptr, 1);
+  // the host copy of the frame table is pageable: wait for it before the caller may change it
+  if (scale_changed) HIPCHK(hipStreamSynchronize(b->ctx->stream));
+  return NICP_OK;
+}
+
+extern "C" int nicp_batch_download_depth_device(nicp_batch *b, uint16_t *depth) {
+  if (!b || !depth) return set_err(NICP_ERR_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  LAUNCH((Prof *)nullptr, 0, k_batch_depth_copy, dim3(b->total_tiles), dim3(kThreads), b->ctx->stream, b->d_frames,
+         b->d_blk_frame, b->d_pix_base, (const uint16_t *)nullptr, depth, 0);
+  return NICP_OK;
+}
+
+extern "C" int nicp_batch_num_pixels(const nicp_batch *b, long long *num_pixels) {
+  if (!b || !num_pixels) return set_err(NICP_ERR_INVALID, "bad arguments");
+  *num_pixels = b->total_pix;
   return NICP_OK;
 }
 

@@ -664,6 +664,31 @@ __device__ __forceinline__ void integral_frame_body(const FrameDev *__restrict__
     __syncthreads();
   }
 }
+// Depth images of a batch against one packed device array (frame f at pix_base[f], rows*cols
+// entries), per kTile-pixel tile as the conversion's per-pixel launches: to_frames copies the
+// packed array into the frames' depth, otherwise the frames' depth into the packed array
+extern "C" __global__ __launch_bounds__(kThreads) void k_batch_depth_copy(const FrameDev *__restrict__ F,
+                                                                         const int *__restrict__ blk_frame,
+                                                                         const long long *__restrict__ pix_base,
+                                                                         const uint16_t *__restrict__ src,
+                                                                         uint16_t *__restrict__ dst, int to_frames) {
+  const int g = blockIdx.x;
+  const int f = blk_frame[g];
+  const FrameDev &fd = F[f];
+  const int P = fd.rows * fd.cols;
+  const int base = (g - fd.tile_base) * kTile;
+  const long long o = pix_base[f];
+  uint16_t *depth = const_cast<uint16_t *>(fd.depth);
+#pragma unroll
+  for (int k = 0; k < kTile / kThreads; ++k) {
+    const int p = base + k * kThreads + threadIdx.x;
+    if (p < P) {
+      if (to_frames) depth[p] = src[o + p];
+      else dst[o + p] = depth[p];
+    }
+  }
+}
+
 // one workgroup per frame; workgroup i takes frame order[i] (order null: frame i)
 extern "C" __global__ __launch_bounds__(kIFrameThreads) void k_integral_frame(const FrameDev *__restrict__ F,
                                                                               const int *__restrict__ order) {
