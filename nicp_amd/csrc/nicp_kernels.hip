@@ -679,13 +679,19 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_batch_depth_copy(const 
   const int base = (g - fd.tile_base) * kTile;
   const long long o = pix_base[f];
   uint16_t *depth = const_cast<uint16_t *>(fd.depth);
+  const uint16_t *from = to_frames ? src + o : depth;
+  uint16_t *to = to_frames ? depth : dst + o;
+  static_assert(kTile == 4 * kThreads, "four pixels per thread");
+  if ((P & 3) == 0 && (((uintptr_t)from | (uintptr_t)to) & 7) == 0) {
+    // four pixels of a thread as one 8-byte access
+    const int p = base + 4 * threadIdx.x;
+    if (p < P) *(uint2 *)(to + p) = *(const uint2 *)(from + p);
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < kTile / kThreads; ++k) {
     const int p = base + k * kThreads + threadIdx.x;
-    if (p < P) {
-      if (to_frames) depth[p] = src[o + p];
-      else dst[o + p] = depth[p];
-    }
+    if (p < P) to[p] = from[p];
   }
 }
 
