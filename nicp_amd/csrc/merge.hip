@@ -436,6 +436,7 @@ int alloc_points(size_t cap, bool with_stats, bool with_gauss, PointStore &ps) {
 // makes `ps` the cloud's storage holding n points: frees what the cloud owned and drops the
 // index / depth images of a converted cloud
 int adopt_store(nicp_cloud *cl, PointStore &ps, int n, int cap) {
+  cl->fd.omega_closed = 0;  // the new arrays' information matrices are added or merged ones
   hipStream_t s = cl->ctx->stream;
   HIPCHK(hipMemcpyAsync(ps.npoints, &n, sizeof(int), hipMemcpyHostToDevice, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -489,6 +490,7 @@ int transform_cloud(nicp_cloud *cl, const float T[16]) {
   memcpy(X.m, T, sizeof(X.m));
   set_row3(X.m);
   if (is_identity(X.m) || cl->n == 0) return NICP_OK;  // cloud.cpp:196
+  cl->fd.omega_closed = 0;  // the rotated information matrices are used as stored
   memcpy(X.mz, X.m, sizeof(X.mz));
   for (int k = 0; k < 4; ++k) {
     X.mz[k * 4 + 3] = 0.0f;
@@ -610,6 +612,7 @@ extern "C" int nicp_merge(nicp_cloud *cl, const float transform[16], const nicp_
     if (num_suppressed) *num_suppressed = 0;
     return NICP_OK;
   }
+  cl->fd.omega_closed = 0;  // merging adds information matrices to the kept points
   // z-buffer projection of the cloud from `transform` (merger.cpp:24-27)
   CHK(project_device(ctx, cl, transform, cam, rows, cols));
   // per-point work arrays: collapsed, slot, count, offsets, bucket; tile counts / offsets; total

@@ -507,6 +507,22 @@ int check_align_params(const nicp_align_params *a) {
 }
 
 // fills the geometric members of a frame descriptor
+// The closed form of the information matrices a conversion with these parameters writes
+// (FrameDev::omega_closed): PointInformationMatrixCalculator and NormalInformationMatrixCalculator
+// (informationmatrixcalculator.cpp:9-57) give U diag(d) U^T with U the window's eigenvectors,
+// the normal first; with d1 == d2 that is d1 I + (d0 - d1) n n^T.
+void set_omega_model(FrameDev &f, const StatsParamsDev &sp) {
+  const float *d[4] = {sp.pflat, sp.pnonflat, sp.nflat, sp.nnonflat};
+  f.omega_closed = 1;
+  for (int i = 0; i < 4; ++i) {
+    f.omega_closed = f.omega_closed && d[i][1] == d[i][2];
+    f.om_a[i] = d[i][0];
+    f.om_b[i] = d[i][1];
+  }
+  f.om_thr[0] = sp.pthr;
+  f.om_thr[1] = sp.nthr;
+}
+
 void frame_geometry(FrameDev &f, int rows, int cols, const nicp_camera *cam, float world_radius, float depth_scale) {
   f.rows = rows;
   f.cols = cols;
@@ -1110,6 +1126,7 @@ int cloud_from_depth(nicp_ctx *ctx, const uint16_t *depth, const float *depthf, 
   FrameDev &f = cl->fd;
   frame_geometry(f, rows, cols, cam, sp->world_radius, depth_scale);
   frame_pointers(f, (char *)mem, L, keep_stats != 0, true, depthf != nullptr);
+  set_omega_model(f, to_dev(sp, keep_stats != 0));
   f.tile_base = 0;
   f.xblk_base = 0;
   f.yblk_base = 0;
@@ -1748,6 +1765,7 @@ extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows,
     off = align_up(lay[i].total, 256);
     FrameDev &f = b->frames[i];
     frame_geometry(f, rows[i], cols[i], &cams[i], sp->world_radius, 0.001f);
+    set_omega_model(f, b->sp);
     f.tile_base = tiles;
     f.xblk_base = xb;
     f.yblk_base = yb;

@@ -76,6 +76,14 @@ struct FrameDev {
   int *npoints;         // device scalar
   int *tile_count;      // per tile
   int *tile_off;        // per tile
+  // Set by the depth conversion when both information-matrix calculators' diagonals have equal
+  // second and third entries: a point's information matrices are then b I + (a - b) n n^T in
+  // the eigenbasis's closed form, from its normal n and curvature alone (the flat pair where the
+  // curvature is below om_thr, else the non-flat pair; zero where the normal is zero), and the
+  // linearisers compute them instead of loading omg. Cleared by anything that changes omg.
+  int omega_closed;
+  float om_thr[2];      // curvature thresholds: points, normals
+  float om_a[4], om_b[4];  // (a, b) of: point flat, point non-flat, normal flat, normal non-flat
 };
 
 // One registration pair: reference and current frame ids, the projector

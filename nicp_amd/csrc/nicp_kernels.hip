@@ -1260,9 +1260,8 @@ typedef float Acc2;
 // gates that decide the correspondences keep the reference's uncontracted order.
 // Output order: H upper triangle row-major (21), b (6), chi2, inliers, count; slot 30 counts
 // the pixels where both index images hold a point.
-__device__ __forceinline__ void lin_terms(const float *rp, const float *rn, float4 cp, float4 cn, const float4 *om,
-                                          const AlignParamsDev &ap, Acc2 *acc) {
-  const float4 o0 = om[0], o1 = om[1], o2 = om[2];
+__device__ __forceinline__ void lin_terms(const float *rp, const float *rn, float4 cp, float4 cn, float4 o0, float4 o1,
+                                          float4 o2, const AlignParamsDev &ap, Acc2 *acc) {
   // Omega_p / Omega_n (symmetric)
   const float P00 = o0.x, P01 = o0.y, P02 = o0.z, P11 = o0.w, P12 = o1.x, P22 = o1.y;
   const float ns = ap.gicp ? 0.0f : ap.nscale;
@@ -1323,6 +1322,43 @@ __device__ __forceinline__ void lin_terms(const float *rp, const float *rn, floa
   acc[24] = fmaf(k, br0, acc[24]); acc[25] = fmaf(k, br1, acc[25]); acc[26] = fmaf(k, br2, acc[26]);
   acc[27] = fmaf(k, chi, acc[27]);
   acc[28] += 1.0f;
+}
+
+// A frame's closed-form information matrices (FrameDev::omega_closed), held in registers
+struct OmegaModel {
+  float thr[2], a[4], b[4];
+};
+__device__ __forceinline__ OmegaModel omega_model(const FrameDev &f) {
+  OmegaModel m;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) m.thr[i] = f.om_thr[i];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    m.a[i] = f.om_a[i];
+    m.b[i] = f.om_b[i];
+  }
+  return m;
+}
+
+// The information matrices of a point with normal n (curvature in w) in closed form: b I + (a - b)
+// n n^T with the flat pair where the curvature is below the threshold, else the non-flat pair
+// (PointInformationMatrixCalculator / NormalInformationMatrixCalculator::compute,
+// informationmatrixcalculator.cpp:9-57, with equal second and third diagonal entries); zero where
+// the normal is zero, as the conversion stores them. Same layout as the stored omg records.
+__device__ __forceinline__ void closed_omega(const OmegaModel &m, float4 n, float4 &o0, float4 &o1, float4 &o2) {
+  if (n.x == 0.0f && n.y == 0.0f && n.z == 0.0f) {
+    o0 = o1 = o2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  const bool pf = n.w < m.thr[0], nf = n.w < m.thr[1];
+  const float pa = pf ? m.a[0] : m.a[1], pb = pf ? m.b[0] : m.b[1];
+  const float na = nf ? m.a[2] : m.a[3], nb = nf ? m.b[2] : m.b[3];
+  const float kp = pa - pb, kn = na - nb;
+  const float px = kp * n.x, py = kp * n.y, pz = kp * n.z;
+  const float qx = kn * n.x, qy = kn * n.y, qz = kn * n.z;
+  o0 = make_float4(fmaf(px, n.x, pb), px * n.y, px * n.z, fmaf(py, n.y, pb));
+  o1 = make_float4(py * n.z, fmaf(pz, n.z, pb), fmaf(qx, n.x, nb), qx * n.y);
+  o2 = make_float4(qx * n.z, fmaf(qy, n.y, nb), qy * n.z, fmaf(qz, n.z, nb));
 }
 
 // the constant factor of each slot's terms (lin_terms): -2 for H_tr and b_r, 4 for H_rr
@@ -1506,10 +1542,10 @@ __device__ int window_search(const FrameDev &fr, const FrameDev &fc, const int *
 // reference index of a pixel comes from the z-buffer key: in global memory (Lds = false; the cell
 // is reset for the next projection) or in the block's LDS tile (Lds = true, cell = tile offset).
 // Both variants run the same arithmetic in the same order, so they give the same partial sums.
-template <bool Lds>
+template <bool Lds, bool Closed>
 __device__ __forceinline__ void corr_lin_tile(const PairDev &pd, const FrameDev &fr, const FrameDev &fc, const float *T,
                                               const AlignParamsDev &ap, int store_pairs, int t,
-                                              const unsigned long long *zl, Acc2 *acc) {
+                                              const unsigned long long *zl, const OmegaModel &om_model, Acc2 *acc) {
   const int sub = pd.sub;
   const int GR = (pd.rows + sub - 1) / sub, GC = (pd.cols + sub - 1) / sub;
   const int ty = t / pd.dtiles_x, tx = t - ty * pd.dtiles_x;
@@ -1554,21 +1590,27 @@ __device__ __forceinline__ void corr_lin_tile(const PairDev &pd, const FrameDev 
         ok[k] = gates_loaded(cn[k], rn0[k], cp[k], rp0[k], T, ap, rp[k], rn[k]);
       }
     }
+    // the current points' information matrices: loaded, or in closed form from their normals
     float4 om[4][3];
+    if (!Closed) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (ok[k]) {
-        const float4 *o = fc.omg + 3 * (size_t)ci[k];
-        om[k][0] = gload4(o);
-        om[k][1] = gload4(o + 1);
-        om[k][2] = gload4(o + 2);
-      }
+      for (int k = 0; k < 4; ++k)
+        if (ok[k]) {
+          const float4 *o = fc.omg + 3 * (size_t)ci[k];
+          om[k][0] = gload4(o);
+          om[k][1] = gload4(o + 1);
+          om[k][2] = gload4(o + 2);
+        }
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (store_pairs && pix[k] >= 0)
         *gmut((unsigned long long *)pd.pairs + pix[k]) =
             ok[k] ? ((unsigned long long)(unsigned int)ci[k] << 32) | (unsigned int)ri[k] : ~0ull;
-      if (ok[k]) lin_terms(rp[k], rn[k], cp[k], cn[k], om[k], ap, acc);
+      if (ok[k]) {
+        if (Closed) closed_omega(om_model, cn[k], om[k][0], om[k][1], om[k][2]);
+        lin_terms(rp[k], rn[k], cp[k], cn[k], om[k][0], om[k][1], om[k][2], ap, acc);
+      }
     }
   }
 }
@@ -1592,7 +1634,8 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
 #pragma unroll
   for (int v = 0; v < kVals; ++v) acc[v] = 0.0f;
   if (!multi) {
-    corr_lin_tile<false>(pd, fr, fc, T, ap, store_pairs, t, nullptr, acc);
+    if (fc.omega_closed) corr_lin_tile<false, true>(pd, fr, fc, T, ap, store_pairs, t, nullptr, omega_model(fc), acc);
+    else corr_lin_tile<false, false>(pd, fr, fc, T, ap, store_pairs, t, nullptr, OmegaModel{}, acc);
   } else {
     const int sub = pd.sub;
     const int GR = (pd.rows + sub - 1) / sub, GC = (pd.cols + sub - 1) / sub;
@@ -1619,7 +1662,18 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
         }
       }
       if (store_pairs) pd.pairs[p] = ok ? make_int2(ri, ci) : make_int2(-1, -1);
-      if (ok) lin_terms(rp, rn, cp, cn, fc.omg + 3 * (size_t)ci, ap, acc);
+      if (ok) {
+        float4 o0, o1, o2;
+        if (fc.omega_closed) {
+          closed_omega(omega_model(fc), cn, o0, o1, o2);
+        } else {
+          const float4 *o = fc.omg + 3 * (size_t)ci;
+          o0 = o[0];
+          o1 = o[1];
+          o2 = o[2];
+        }
+        lin_terms(rp, rn, cp, cn, o0, o1, o2, ap, acc);
+      }
     }
   }
   block_partial(acc, pd.partials + (size_t)t * kPartial);
@@ -1786,6 +1840,8 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
 #pragma unroll
   for (int k = 0; k < 16; ++k) T[k] = *gview(pd.state + kSlotTinv + k);
   const int nty = dtiles_y(pd.rows);
+  const bool closed = fc.omega_closed != 0;
+  const OmegaModel om_model = closed ? omega_model(fc) : OmegaModel{};
 #pragma unroll 1
   for (int sub_t = 0; sub_t < kFH / kDTH; ++sub_t) {
     const int ty = fy * (kFH / kDTH) + sub_t;
@@ -1794,7 +1850,8 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
 #pragma unroll
     for (int v = 0; v < kVals; ++v) acc[v] = 0.0f;
     const int t = ty * pd.dtiles_x + fx;
-    corr_lin_tile<true>(pd, fr, fc, T, ap, store_pairs, t, zl + sub_t * kDTH * kDTW, acc);
+    if (closed) corr_lin_tile<true, true>(pd, fr, fc, T, ap, store_pairs, t, zl + sub_t * kDTH * kDTW, om_model, acc);
+    else corr_lin_tile<true, false>(pd, fr, fc, T, ap, store_pairs, t, zl + sub_t * kDTH * kDTW, om_model, acc);
     block_partial(acc, pd.partials + (size_t)t * kPartial);
     __syncthreads();  // block_partial's LDS is reused by the next tile
   }
@@ -1828,6 +1885,9 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
       pr[k] = make_int2(-1, -1);
     }
   }
+  // the current points' information matrices: loaded, or in closed form from their normals
+  const bool closed = fc.omega_closed != 0;
+  const OmegaModel om_model = closed ? omega_model(fc) : OmegaModel{};
   float4 rp0[4], rn0[4], cp[4], cn[4], om[4][3];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
@@ -1836,18 +1896,21 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
       rn0[k] = gload4(fr.nrm + pr[k].x);
       cp[k] = gload4(fc.pts + pr[k].y);
       cn[k] = gload4(fc.nrm + pr[k].y);
-      const float4 *o = fc.omg + 3 * (size_t)pr[k].y;
-      om[k][0] = gload4(o);
-      om[k][1] = gload4(o + 1);
-      om[k][2] = gload4(o + 2);
+      if (!closed) {
+        const float4 *o = fc.omg + 3 * (size_t)pr[k].y;
+        om[k][0] = gload4(o);
+        om[k][1] = gload4(o + 1);
+        om[k][2] = gload4(o + 2);
+      }
     }
 #pragma unroll
   for (int k = 0; k < 4; ++k)
     if (pr[k].x >= 0) {
       float rp[3], rn[3];
       iso_mul3(T, rp0[k].x, rp0[k].y, rp0[k].z, rp0[k].w, rp);
-      iso_mul3(T, rn0[k].x, rn0[k].y, rn0[k].z, 0.0f, rn);
-      lin_terms(rp, rn, cp[k], cn[k], om[k], ap, acc);
+      iso_rot3(T, rn0[k].x, rn0[k].y, rn0[k].z, rn);
+      if (closed) closed_omega(om_model, cn[k], om[k][0], om[k][1], om[k][2]);
+      lin_terms(rp, rn, cp[k], cn[k], om[k][0], om[k][1], om[k][2], ap, acc);
     }
   block_partial(acc, pd.partials + (size_t)t * kPartial);
 }

@@ -149,6 +149,74 @@ def test_linearize_vs_fp64(ctx):
     assert abs(ed - e64) / max(e64, 1e-30) < 1e-5
 
 
+def test_linearize_closed_omega_vs_fp64(ctx):
+    """Clouds converted on the device carry their information matrices in closed form (both
+    calculators' diagonals have equal second and third entries: b I + (a - b) n n^T), and the
+    linearisers compute them from the normals instead of loading them. Pinned two ways: the
+    closed form against the stored matrices (the reference's U diag(d) U^T, FP32) of every point
+    within 1e-4 of the largest entry, and the device's H and b on such clouds against the FP64
+    twin run on the stored matrices."""
+    rows, cols = 480, 640
+    say("case linearize_closed_omega")
+    dr, dc, Tgt = _frames(7, rows, cols)
+    ocam, dcam = cams(rows, cols)
+    oap, dap = aparams()
+    _, dsp = params()
+    cr = ctx.cloud_from_depth(dr, dcam, dsp)
+    cc = ctx.cloud_from_depth(dc, dcam, dsp)
+    pr_, nr_, kr_, opr, onr = cr.download()
+    pc_, nc_, kc_, opc, onc = cc.download()
+    # the closed form of every current point's matrices against the stored ones
+    n = np.asarray(nc_, np.float64)[:, :3]
+    curv = np.asarray(kc_, np.float64)
+    sp = dsp
+    pa = np.where(curv < sp.point_curvature_threshold, sp.point_flat[0], sp.point_nonflat[0])
+    pb = np.where(curv < sp.point_curvature_threshold, sp.point_flat[1], sp.point_nonflat[1])
+    na = np.where(curv < sp.normal_curvature_threshold, sp.normal_flat[0], sp.normal_nonflat[0])
+    nb = np.where(curv < sp.normal_curvature_threshold, sp.normal_flat[1], sp.normal_nonflat[1])
+    iu = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    has = np.any(n != 0.0, axis=1)
+    for a, b, stored in ((pa, pb, opc), (na, nb, onc)):
+        closed = np.stack([(a - b) * n[:, i] * n[:, j] + (b if i == j else 0.0) for i, j in iu], axis=1)
+        closed[~has] = 0.0
+        m = np.asarray(stored, np.float64).reshape(-1, 4, 4)  # 4x4 per point, the 3x3 block used
+        st = np.stack([m[:, i, j] for i, j in iu], axis=1)
+        err = np.abs(closed - st).max(axis=1) / np.maximum(np.abs(st).max(axis=1), 1e-30)
+        say("  closed form vs stored: max rel %.2e (99.9%% %.2e) over %d points" % (
+            err[has].max(), np.quantile(err[has], 0.999), int(has.sum())))
+        # the stored U diag(d) U^T carries the FP32 eigenvectors' departure from orthonormality
+        # (up to ~3e-5 at near-degenerate windows), which b I + (a - b) n n^T does not; the bar is
+        # that of the information matrices against the oracle (test_frame_conversion)
+        assert err[has].max() < 1e-4
+        assert np.all(st[~has] == 0.0)
+    # linearisation on the device's clouds against the FP64 twin on the stored matrices
+    fr = O.cloud_from_arrays(pr_, _no_w(nr_), kr_, opr, onr)
+    fc = O.cloud_from_arrays(pc_, _no_w(nc_), kc_, opc, onc)
+    KRt, _ = O.projector_matrices(np.array(ocam.K, np.float32), Tgt)
+    ri, _ = O.project(np.asarray(pr_, np.float32), KRt, rows, cols, 0.5, 4.5)
+    KRt0, _ = O.projector_matrices(np.array(ocam.K, np.float32), identity16())
+    ci, _ = O.project(np.asarray(pc_, np.float32), KRt0, rows, cols, 0.5, 4.5)
+    Tinv = np.linalg.inv(Tgt.reshape(4, 4, order="F").astype(np.float64)).astype(np.float32).flatten(order="F")
+    pairs = O.correspondences(fr, fc, ri, ci, Tinv, oap, 1)
+    H64, b64, e64, i64 = O.linearize_f64(fr, fc, pairs, Tinv, oap)
+    Hd, bd, ed, idd = ctx.linearize(cr, cc, pairs, Tinv, dap)
+    relH = np.abs(Hd - H64).max() / np.abs(H64).max()
+    relb = np.abs(bd - b64).max() / np.abs(b64).max()
+    say("  pairs %d  device (closed form) vs fp64 (stored): H %.2e b %.2e" % (len(pairs), relH, relb))
+    assert len(pairs) > 500
+    assert idd == i64
+    assert relH < 1e-5 and relb < 1e-4
+    assert abs(ed - e64) / max(e64, 1e-30) < 1e-5
+    cr.close()
+    cc.close()
+
+
+def _no_w(nrm):
+    n = np.array(nrm, np.float32, copy=True)
+    n[:, 3] = 0.0
+    return n
+
+
 @pytest.mark.parametrize("seed", [1, 4])
 def test_align_vs_oracle(ctx, seed):
     """AlignerProjective::align on identical clouds. The device computes H and b in

@@ -32,7 +32,9 @@ def _batch(ctx, shapes):
 
 
 def test_depth_device_roundtrip(ctx):
-    shapes = [(120, 160), (480, 640), (240, 320), (960, 1280), (120, 160)]
+    # the odd-sized frame leaves the frames after it at odd offsets of the packed array, so
+    # both the 8-byte and the 2-byte copy paths run
+    shapes = [(120, 160), (480, 640), (121, 161), (240, 320), (960, 1280), (120, 160)]
     say("case depth_device_roundtrip frames=%d" % len(shapes))
     imgs = [synth.pair(40 + i, r, c)[0] for i, (r, c) in enumerate(shapes)]
     a = _batch(ctx, shapes)
