@@ -78,11 +78,18 @@ def kernel_bytes(name, pairs_info, frames_info):
         # and both information matrices (48 B)
         return sum(20 * p + 104 * c2 for p, _, c2, _ in pairs_info)
     if name == "k_proj_corr_lin":
-        # the projection's inputs: the reference index image (4 B per pixel) and points (16 B per
-        # point); the current index (4 B per pixel); per overlap pixel the reference point +
-        # normal/curvature (28 B), the current point + normal/curvature (28 B) and both
-        # information matrices (48 B); the z-buffer stays in LDS
-        return sum(4 * pr + 16 * n + 4 * p + 104 * c2 for p, n, c2, pr in pairs_info)
+        # one outer iteration's projection, correspondences and linearisation: the per-iteration
+        # figure B_iter = 24 P + 12 N + 104 C2 of the algorithm (per projector pixel the z-buffer's
+        # initialisation, atomic and read, 8 B each, with both index images; 12 B per reference
+        # point; per overlap pixel both points and normals with curvature, 2 x 28 B, and the
+        # current point's two information matrices, 48 B)
+        return sum(24 * p + 12 * n + 104 * c2 for p, n, c2, _ in pairs_info)
+    if name == "k_proj_corr_lin(reads)":
+        # what the fused kernel itself reads for the same work: the reference index image (4 B
+        # per pixel) and points (16 B per point) for the projection, the current index (4 B per
+        # pixel) and per overlap pixel both points and normals (2 x 32 B); its z-buffer stays in
+        # LDS and the information matrices of converted frames are formed from the normals
+        return sum(4 * pr + 16 * n + 4 * p + 64 * c2 for p, n, c2, pr in pairs_info)
     if name == "k_stats":
         # depth (2 B) + index (4 B) + one pass over the 40 B integral image per pixel;
         # normal + curvature (16 B) and two 6-float information matrices (48 B) per point
@@ -558,6 +565,12 @@ def main():
                     "frac": round(ach / HBM_PEAK_GBS, 4),
                     "traffic": measured_traffic(dom) if default else None, "bytes_per_launch": nb,
                     "avg_launch_ms": avg}
+        own = kernel_bytes(dom + "(reads)", pairs_info, frames_info)
+        if own:
+            # the same launch against the bytes the kernel itself has to read
+            roofline["bytes_model"] = "B_iter = 24 P + 12 N + 104 C2 per pair (one outer iteration)"
+            roofline["kernel_reads_per_launch"] = own
+            roofline["kernel_reads_frac"] = round(own / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     net_roofline = None
     if "initnet_forward" in kernels:
         avg = kernels["initnet_forward"]["avg_ms"]
