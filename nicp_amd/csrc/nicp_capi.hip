@@ -367,6 +367,10 @@ struct nicp_batch {
   // frames by pixel count, largest first: the order in which k_integral_frame takes them (one
   // workgroup per frame), so the largest frames of an uneven batch start first
   int *d_frame_order = nullptr;
+  // the conversion skipped storing the information matrices (the frames' closed form stands for
+  // them); nicp_batch_frame_cloud fills a frame's arrays from its normals before handing it out
+  bool omega_deferred = false;
+  std::vector<char> omega_filled;
   // first pixel of each frame in the packed depth arrays of nicp_batch_upload/download_depth_device
   long long *d_pix_base = nullptr;
   long long total_pix = 0;
@@ -1903,6 +1907,9 @@ extern "C" int nicp_batch_convert(nicp_batch *b) {
   b->ctx->prof.sink = &b->ctx->prof.eager;
   StatsParamsDev spd = b->sp;
   spd.fp64_gap = b->ctx->fp64_eigen_gap;
+  // the linearisers form closed-form information matrices from the normals, so the conversion
+  // does not store them (frames share the batch's calculators, so one frame's model is all's)
+  spd.skip_omega = b->frames[0].omega_closed ? 1 : 0;
   auto enqueue = [&]() {
     return launch_convert(&b->ctx->prof, b->ctx->stream, b->d_frames, b->num_frames, b->d_blk_frame, b->total_tiles,
                           b->d_xblk_frame, b->total_xblk, b->d_yblk_frame, b->total_yblk, spd, b->frames[0].model,
@@ -1918,6 +1925,8 @@ extern "C" int nicp_batch_convert(nicp_batch *b) {
     CHK(graph_replay(b->ctx, b->conv_graph, key, enqueue));
   }
   b->converted = true;
+  b->omega_deferred = spd.skip_omega != 0;
+  b->omega_filled.assign(b->num_frames, 0);
   return NICP_OK;
 }
 
@@ -2217,6 +2226,14 @@ extern "C" int nicp_batch_frame_cloud(nicp_batch *b, int frame, nicp_cloud **out
   nicp_cloud *cl = new (std::nothrow) nicp_cloud();
   if (!cl) return set_err(NICP_ERR_OOM, "out of host memory");
   HIPCHK(hipSetDevice(b->ctx->device));
+  if (b->omega_deferred && !b->omega_filled[frame]) {
+    // the frame's stored information matrices, from its normals (the closed form the batch's
+    // linearisers use)
+    const int cap = b->frames[frame].rows * b->frames[frame].cols;
+    LAUNCH((Prof *)nullptr, 0, k_omega_fill, dim3((cap + kThreads - 1) / kThreads), dim3(kThreads), b->ctx->stream,
+           b->d_frames + frame);
+    b->omega_filled[frame] = 1;
+  }
   int n = 0;
   HIPCHK(hipMemcpyAsync(&n, b->frames[frame].npoints, sizeof(int), hipMemcpyDeviceToHost, b->ctx->stream));
   HIPCHK(hipStreamSynchronize(b->ctx->stream));

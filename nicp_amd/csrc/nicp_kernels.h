@@ -122,6 +122,9 @@ struct StatsParamsDev {
   float pflat[3], pnonflat[3], nflat[3], nnonflat[3];
   float pthr, nthr;
   int keep_stats;
+  // set: the information matrices are not stored (the frames' closed form stands for them until
+  // something asks for the stored arrays, which are then filled from the normals)
+  int skip_omega;
   // > 0: pixels whose FP32 eigenvalues have a relative gap (l1 - l0) / l2 below this value are
   // solved again in FP64 (nicp_ctx_set_fp64_eigen_gap); 0 keeps the reference's FP32 solve
   float fp64_gap;

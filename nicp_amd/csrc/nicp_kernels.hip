@@ -1027,17 +1027,21 @@ __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const
           dp[i] = curv < sp.pthr ? sp.pflat[i] : sp.pnonflat[i];
           dq[i] = curv < sp.nthr ? sp.nflat[i] : sp.nnonflat[i];
         }
-        omega6(V, dp, op);
-        omega6(V, dq, on);
+        if (!sp.skip_omega) {
+          omega6(V, dp, op);
+          omega6(V, dq, on);
+        }
       } else {
         nrm = make_float4(0.0f, 0.0f, 0.0f, curv);
       }
     }
     gstore4(fd.nrm + idx, nrm);
-    float4 *o = fd.omg + 3 * (size_t)idx;
-    gstore4(o, make_float4(op[0], op[1], op[2], op[3]));
-    gstore4(o + 1, make_float4(op[4], op[5], on[0], on[1]));
-    gstore4(o + 2, make_float4(on[2], on[3], on[4], on[5]));
+    if (!sp.skip_omega) {
+      float4 *o = fd.omg + 3 * (size_t)idx;
+      gstore4(o, make_float4(op[0], op[1], op[2], op[3]));
+      gstore4(o + 1, make_float4(op[4], op[5], on[0], on[1]));
+      gstore4(o + 2, make_float4(on[2], on[3], on[4], on[5]));
+    }
     if (sp.keep_stats) {
       float4 *s = fd.stats + 4 * (size_t)idx;
       gstore4(s, make_float4(V[0], V[1], V[2], V[3]));
@@ -1535,6 +1539,21 @@ __device__ int window_search(const FrameDev &fr, const FrameDev &fc, const int *
 }
 
 }  // namespace
+
+// The stored information matrices of frame F[0]'s points from their normals in closed form
+// (FrameDev::omega_closed), for a frame converted without storing them; one thread per point
+extern "C" __global__ __launch_bounds__(kThreads) void k_omega_fill(const FrameDev *__restrict__ F) {
+  const FrameDev &fd = F[0];
+  const int n = *gview(fd.npoints);
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  float4 o0, o1, o2;
+  closed_omega(omega_model(fd), gload4(fd.nrm + i), o0, o1, o2);
+  float4 *o = fd.omg + 3 * (size_t)i;
+  gstore4(o, o0);
+  gstore4(o + 1, o1);
+  gstore4(o + 2, o2);
+}
 
 // The correspondences and linearisation terms of one kDTH x kDTW tile t of a pair's (sampled)
 // projector image, accumulated into the thread's slots. Thread x of the block takes tile column

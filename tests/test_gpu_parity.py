@@ -416,8 +416,11 @@ def test_depth_pyramid_and_float_input(ctx):
 def test_batch_conversion_equals_single(ctx):
     """A batch of many frames (>= 16: both prefix passes then run as one workgroup per frame)
     converts every frame to the clouds of single-frame conversion (the per-row and per-column
-    prefix kernels), bit for bit: index images, points, normals, curvature and information
-    matrices. Odd sizes exercise partial bands and chunks."""
+    prefix kernels), bit for bit: index images, points, normals and curvature. A batch does not
+    store the information matrices (its linearisers form them from the normals in closed form,
+    b I + (a - b) n n^T) and fills a frame's arrays that way when the frame is handed out; they
+    are within 1e-4 of the largest entry of the single conversion's U diag(d) U^T, the bar of
+    the information matrices against the oracle. Odd sizes exercise partial bands and chunks."""
     say("case batch_conversion_equals_single")
     shapes = [(120, 160), (480, 640), (37, 53), (1, 1), (240, 320), (41, 49), (81, 96), (960, 1280), (130, 20)] * 2
     _, dsp = params(min_image_radius=1, min_points=3)
@@ -441,5 +444,12 @@ def test_batch_conversion_equals_single(ctx):
         bc = b.frame_cloud(k, (rows, cols))
         assert bc.n == s.n, k
         np.testing.assert_array_equal(bc.index_image(), s.index_image())
-        for x, y in zip(bc.download(), s.download()):
+        xb, xs = bc.download(), s.download()
+        for x, y in zip(xb[:3], xs[:3]):
             np.testing.assert_array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))
+        for x, y in zip(xb[3:], xs[3:]):
+            x = np.asarray(x, np.float64).reshape(-1, 16)
+            y = np.asarray(y, np.float64).reshape(-1, 16)
+            scale = np.maximum(np.abs(y).max(axis=1), 1e-30)
+            assert (np.abs(x - y).max(axis=1) <= 1e-4 * scale).all(), k
+            np.testing.assert_array_equal(x[scale <= 1e-30], 0.0)
