@@ -91,9 +91,10 @@ def kernel_bytes(name, pairs_info, frames_info):
         # LDS and the information matrices of converted frames are formed from the normals
         return sum(4 * pr + 16 * n + 4 * p + 64 * c2 for p, n, c2, pr in pairs_info)
     if name == "k_stats":
-        # depth (2 B) + index (4 B) + one pass over the 40 B integral image per pixel;
-        # normal + curvature (16 B) and two 6-float information matrices (48 B) per point
-        return 46 * P + 64 * N
+        # depth (2 B) + index (4 B) + one pass over the 40 B integral image per pixel; normal +
+        # curvature (16 B) per point (a batch conversion leaves the information matrices to the
+        # closed form its linearisers use)
+        return 46 * P + 16 * N
     if name == "k_integral_x":
         return 42 * P
     if name == "k_integral_y":
