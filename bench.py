@@ -85,11 +85,12 @@ def kernel_bytes(name, pairs_info, frames_info):
         # current point's two information matrices, 48 B)
         return sum(24 * p + 12 * n + 104 * c2 for p, n, c2, _ in pairs_info)
     if name == "k_proj_corr_lin(reads)":
-        # what the fused kernel itself reads for the same work: the reference index image (4 B
-        # per pixel) and points (16 B per point) for the projection, the current index (4 B per
-        # pixel) and per overlap pixel both points and normals (2 x 32 B); its z-buffer stays in
-        # LDS and the information matrices of converted frames are formed from the normals
-        return sum(4 * pr + 16 * n + 4 * p + 64 * c2 for p, n, c2, pr in pairs_info)
+        # what the fused kernel itself reads for the same work: the reference frame's packed
+        # source pixels (depth and index, 8 B per point) for the projection, the current index
+        # (4 B per pixel) and per overlap pixel both points and normals (2 x 32 B); its z-buffer
+        # stays in LDS and the information matrices of converted frames are formed from the
+        # normals
+        return sum(8 * n + 4 * p + 64 * c2 for p, n, c2, _ in pairs_info)
     if name == "k_stats":
         # depth (2 B) + index (4 B) + one pass over the 40 B integral image per pixel; normal +
         # curvature (16 B) per point (a batch conversion leaves the information matrices to the
@@ -106,6 +107,9 @@ def kernel_bytes(name, pairs_info, frames_info):
         return 6 * P + 16 * N
     if name == "k_count":
         return 2 * P
+    if name == "k_src_tiles":
+        # depth (2 B) and index (4 B) per pixel in, the packed source pixels (8 B per point) out
+        return 6 * P + 8 * N
     if name == "k_zproject(reference)":
         # reference points (16 B) and one 8 B atomic per point
         return sum(24 * n for _, n, _, _ in pairs_info)

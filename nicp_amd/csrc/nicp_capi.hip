@@ -557,9 +557,11 @@ int yblocks(int cols) { return (cols * 10 + kThreads - 1) / kThreads; }
 
 // Layout of the per-frame device arrays inside one allocation.
 struct FrameLayout {
-  size_t depth, depthf, index, ii, pts, nrm, omg, stats, stile, npoints, tcount, toff, total;
+  size_t depth, depthf, index, ii, pts, nrm, omg, stats, stile, spix, stn, npoints, tcount, toff, total;
 };
-FrameLayout frame_layout(int rows, int cols, bool with_stats, bool with_ii, size_t base, bool with_f32 = false) {
+// with_spack: the packed source-tile pixels of the tile-binned projection (batch frames)
+FrameLayout frame_layout(int rows, int cols, bool with_stats, bool with_ii, size_t base, bool with_f32 = false,
+                         bool with_spack = false) {
   Carver cv;
   cv.off = base;
   const size_t P = (size_t)rows * cols;
@@ -574,6 +576,8 @@ FrameLayout frame_layout(int rows, int cols, bool with_stats, bool with_ii, size
   L.omg = cv.take<float4>(P * 3);
   L.stats = with_stats ? cv.take<float4>(P * 4) : 0;
   L.stile = with_ii ? cv.take<float2>((size_t)stiles(rows, cols)) : 0;
+  L.spix = with_ii && with_spack ? cv.take<uint2>((size_t)stiles(rows, cols) * kSTH * kSTW) : 0;
+  L.stn = with_ii && with_spack ? cv.take<int>((size_t)stiles(rows, cols)) : 0;
   L.npoints = cv.take<int>(1);
   L.tcount = cv.take<int>(T);
   L.toff = cv.take<int>(T);
@@ -582,7 +586,7 @@ FrameLayout frame_layout(int rows, int cols, bool with_stats, bool with_ii, size
 }
 
 void frame_pointers(FrameDev &f, char *base, const FrameLayout &L, bool with_stats, bool with_ii,
-                    bool with_f32 = false) {
+                    bool with_f32 = false, bool with_spack = false) {
   f.depth = (const uint16_t *)(base + L.depth);
   f.depthf = with_f32 ? (const float *)(base + L.depthf) : nullptr;
   f.index = (int *)(base + L.index);
@@ -593,6 +597,9 @@ void frame_pointers(FrameDev &f, char *base, const FrameLayout &L, bool with_sta
   f.stats = with_stats ? (float4 *)(base + L.stats) : nullptr;
   // source-tile depth ranges of the tile-binned projection (pinhole frames)
   f.stile_d = with_ii && f.model == NICP_PINHOLE ? (float2 *)(base + L.stile) : nullptr;
+  const bool spack = with_ii && with_spack && f.model == NICP_PINHOLE && (long long)f.rows * f.cols <= (1ll << kSPosShift);
+  f.spix = spack ? (uint2 *)(base + L.spix) : nullptr;
+  f.stile_n = spack ? (int *)(base + L.stn) : nullptr;
   f.npoints = (int *)(base + L.npoints);
   f.tile_count = (int *)(base + L.tcount);
   f.tile_off = (int *)(base + L.toff);
@@ -1765,7 +1772,7 @@ extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows,
   int tiles = 0, xb = 0, yb = 0;
   std::vector<int> blk, xblk, yblk;
   for (int i = 0; i < num_frames; ++i) {
-    lay[i] = frame_layout(rows[i], cols[i], false, true, off);
+    lay[i] = frame_layout(rows[i], cols[i], false, true, off, false, true);
     off = align_up(lay[i].total, 256);
     FrameDev &f = b->frames[i];
     frame_geometry(f, rows[i], cols[i], &cams[i], sp->world_radius, 0.001f);
@@ -1805,7 +1812,7 @@ extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows,
     return set_err(NICP_ERR_OOM, "hipMalloc(%zu): %s", cv.off, hipGetErrorString(e));
   }
   char *base = (char *)b->arena;
-  for (int i = 0; i < num_frames; ++i) frame_pointers(b->frames[i], base, lay[i], false, true);
+  for (int i = 0; i < num_frames; ++i) frame_pointers(b->frames[i], base, lay[i], false, true, false, true);
   b->d_frames = (FrameDev *)(base + o_tbl);
   b->d_blk_frame = (int *)(base + o_blk);
   b->d_xblk_frame = (int *)(base + o_x);
@@ -2006,7 +2013,8 @@ int batch_plan(nicp_batch *b, const AlignBuffers &B, int num_pairs, const int *r
   plan.fused_ok = sub == 1;
   for (int i = 0; plan.fused_ok && i < num_pairs; ++i) {
     const PairDev &pd = plan.pairs[i];
-    plan.fused_ok = pd.model == NICP_PINHOLE && pd.cur_own && b->frames[pd.ref].stile_d && b->frames[pd.ref].index;
+    plan.fused_ok = pd.model == NICP_PINHOLE && pd.cur_own && b->frames[pd.ref].stile_d && b->frames[pd.ref].spix &&
+                    b->frames[pd.ref].index;
   }
   const bool fused = plan.fused_ok && ap->row_search_region == 0 && ap->col_search_region == 0;
   plan.list_cap = b->list_cap;

@@ -35,6 +35,8 @@ constexpr int kBinMargin = 2;  // pixels added to each side of a projected sourc
 constexpr int kPhaseB = kSTH * kSTW / 64;  // source pixels per lane of a wavefront's source tile
 constexpr int kFH = 64;
 constexpr int kListCap = 256;
+constexpr int kSPosShift = 23;  // bit of a packed source pixel's offset in the tile (FrameDev::spix)
+static_assert(kSTH * kSTW <= (1 << (32 - kSPosShift)), "a packed source pixel's offset fits above its index");
 static_assert(kSTH * kSTW % 64 == 0 && kListCap <= kThreads, "k_proj_corr_lin's source-tile loop");
 static_assert(kFH % kDTH == 0, "a fused tile holds whole correspondence tiles");
 __host__ __device__ inline int stiles_x(int cols) { return (cols + kSTW - 1) / kSTW; }
@@ -73,6 +75,11 @@ struct FrameDev {
   float4 *omg;          // 3 float4 per point: Omega_p (00 01 02 11 12 22), Omega_n (same)
   float4 *stats;        // 4 float4 per point or null: eigvecs (9), eigvals (3), mean (3), n
   float2 *stile_d;      // per source tile: (min, max) depth of its valid pixels, or null
+  // per source tile, kSTH * kSTW slots: its valid pixels compacted in the tile's row-major order,
+  // each as (depth as float bits, cloud index | pixel offset in the tile << kSPosShift), and
+  // their count; or null (frames of more than 2^kSPosShift pixels have none)
+  uint2 *spix;
+  int *stile_n;
   int *npoints;         // device scalar
   int *tile_count;      // per tile
   int *tile_off;        // per tile

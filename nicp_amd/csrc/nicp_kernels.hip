@@ -53,6 +53,7 @@ __device__ __forceinline__ void gstore4(float4 *p, float4 v) {
   *gmut((gvec4 *)p) = gvec4{v.x, v.y, v.z, v.w};
 }
 typedef float gvec2 __attribute__((ext_vector_type(2)));
+typedef unsigned int gvec2u __attribute__((ext_vector_type(2)));
 typedef int gint4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float depth_of(uint16_t raw, float scale) { return raw ? scale * (float)raw : 0.0f; }
@@ -289,8 +290,11 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_unproject_sph(const Fra
 
 // The depth range of the valid pixels of every kSTH x kSTW source tile of a pinhole frame (for
 // the tile-binned projection): one wavefront per tile, lane l covering its pixels l, l + 64, ...
-// in row-major order; an empty tile stores (+inf, -inf). Block t of a frame (its 1024-pixel tile
-// map) takes source tiles 4 t + wave, 4 t + wave + 4 n_tiles, ...
+// in row-major order; an empty tile stores (+inf, -inf). Where the frame has packed source-tile
+// pixels (FrameDev::spix), the same wavefront writes its tile's valid pixels (cloud index >= 0 in
+// the index image), compacted in row-major order: depth, cloud index and offset in the tile.
+// Block t of a frame (its 1024-pixel tile map) takes source tiles 4 t + wave, 4 t + wave +
+// 4 n_tiles, ...
 extern "C" __global__ __launch_bounds__(kThreads) void k_src_tiles(const FrameDev *__restrict__ F,
                                                                   const int *__restrict__ blk_frame) {
   const int g = blockIdx.x;
@@ -312,6 +316,11 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_src_tiles(const FrameDe
       const int r = ty * kSTH + w / kSTW, c = tx * kSTW + w % kSTW;
       px[i] = w < kSTH * kSTW && r < rows && c < cols ? r * cols + c : -1;
     }
+    int idx[kN];
+    if (fd.spix) {
+#pragma unroll
+      for (int i = 0; i < kN; ++i) idx[i] = px[i] >= 0 ? *gview(fd.index + px[i]) : -1;
+    }
     frame_depths<kN>(fd, px, d);
 #pragma unroll
     for (int i = 0; i < kN; ++i)
@@ -319,12 +328,26 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_src_tiles(const FrameDe
         lo = fminf(lo, d[i]);
         hi = fmaxf(hi, d[i]);
       }
+    if (fd.spix) {
+      const unsigned long long below = lane ? ~0ull >> (64 - lane) : 0ull;
+      uint2 *out = fd.spix + (size_t)s * (kSTH * kSTW);
+      int at = 0;
+#pragma unroll
+      for (int i = 0; i < kN; ++i) {
+        const unsigned long long m = __ballot(idx[i] >= 0);
+        if (idx[i] >= 0)
+          *gmut((gvec2u *)(out + at + __popcll(m & below))) =
+              gvec2u{__float_as_uint(d[i]), (unsigned int)idx[i] | ((unsigned int)(i * 64 + lane) << kSPosShift)};
+        at += __popcll(m);
+      }
+      if (lane == 0) *gmut(fd.stile_n + s) = at;
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       lo = fminf(lo, __shfl_xor(lo, o, 64));
       hi = fmaxf(hi, __shfl_xor(hi, o, 64));
     }
-    if (lane == 0) fd.stile_d[s] = make_float2(lo, hi);
+    if (lane == 0) *gmut((gvec2 *)(fd.stile_d + s)) = gvec2{lo, hi};
   }
 }
 
@@ -1809,40 +1832,44 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
 #pragma unroll
     for (int k = 0; k < 16; ++k) KRt[k] = *gview(pd.state + kSlotKRt + k);
     const int frows = fr.rows, fcols = fr.cols, prows = pd.rows, pcols = pd.cols;
+    float iKRt[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) iKRt[k] = fr.iKRt[k];
     const float mind = pd.min_d, maxd = pd.max_d;
-    const __attribute__((address_space(1))) int *findex = gview(fr.index);
-    const float4 *fpts = fr.pts;
+    const __attribute__((address_space(1))) gvec2u *spix = gview((const gvec2u *)fr.spix);
+    const __attribute__((address_space(1))) int *stn = gview(fr.stile_n);
     const int sx = stiles_x(fcols);
     const int nsrc = all ? stiles(frows, fcols) : cnt;
     const int xlo = fx * kDTW, ylo = fy * kFH;
-    // wavefront w takes source tiles w, w + 4, ...: its list entries are read once (lane i holds
-    // entry w + 4 i) and taken with readlane, so no LDS wait inside the loop; lane l projects the
-    // tile's pixels l, l + 64, ... (kPhaseB of them), all loads in flight together
+    // wavefront w takes source tiles w, w + 4, ...: its list entries and their point counts are
+    // read once (lane i holds entry w + 4 i) and taken with readlane, so no LDS wait inside the
+    // loop; lane l projects the tile's packed pixels l, l + 64, ... (at most kPhaseB of them),
+    // all loads in flight together. A pixel's point is unproject_point of its column, row and
+    // depth, as the conversion computed it: the cloud point's bits without loading the point.
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nw = kThreads / 64;
     const int mine = wave + nw * lane;
     const int entry = !all && mine < nsrc ? sl[mine] : 0;
+    const int ecount = !all && mine < nsrc ? stn[entry] : 0;
     const int ntiles = nsrc > wave ? (nsrc - wave + nw - 1) / nw : 0;
     for (int i = 0; i < ntiles; ++i) {
       const int st = all ? wave + nw * i : __builtin_amdgcn_readlane(entry, i);
+      const int n = all ? stn[st] : __builtin_amdgcn_readlane(ecount, i);
       const int sty = st / sx, stx = st - sty * sx;
-      int idx[kPhaseB];
-#pragma unroll
-      for (int u = 0; u < kPhaseB; ++u) {
-        const int w = u * 64 + lane;
-        const int r = sty * kSTH + w / kSTW, c = stx * kSTW + w % kSTW;
-        idx[u] = r < frows && c < fcols ? findex[r * fcols + c] : -1;
-      }
-      float4 p[kPhaseB];
+      const __attribute__((address_space(1))) gvec2u *tp = spix + (size_t)st * (kSTH * kSTW);
+      gvec2u q[kPhaseB];
 #pragma unroll
       for (int u = 0; u < kPhaseB; ++u)
-        if (idx[u] >= 0) p[u] = gload4(fpts + idx[u]);
+        if (u * 64 + lane < n) q[u] = tp[u * 64 + lane];
 #pragma unroll
       for (int u = 0; u < kPhaseB; ++u) {
-        if (idx[u] < 0) continue;
+        if (u * 64 + lane >= n) continue;
+        const unsigned int at = q[u].y >> kSPosShift, idx = q[u].y & ((1u << kSPosShift) - 1u);
+        const float4 p = unproject_point(iKRt, stx * kSTW + (int)(at % kSTW), sty * kSTH + (int)(at / kSTW),
+                                         __uint_as_float(q[u].x));
         // pinhole_project's arithmetic on the register copies
         float ip[4];
-        mul4_packet(KRt, p[u].x, p[u].y, p[u].z, p[u].w, ip);
+        mul4_packet(KRt, p.x, p.y, p.z, p.w, ip);
         const float d = ip[2];
         if (d < mind || d > maxd) continue;
         const float inv = 1.0f / d;
@@ -1850,7 +1877,7 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
         if (!(x >= 0 && x < pcols && y >= 0 && y < prows)) continue;
         const int lx = x - xlo, ly = y - ylo;
         if (lx >= 0 && lx < kDTW && ly >= 0 && ly < kFH)
-          atomicMin(&zl[ly * kDTW + lx], ((unsigned long long)__float_as_uint(d) << 32) | (unsigned int)idx[u]);
+          atomicMin(&zl[ly * kDTW + lx], ((unsigned long long)__float_as_uint(d) << 32) | idx);
       }
     }
   }
