@@ -775,7 +775,11 @@ __device__ __forceinline__ void eig_extract_kernel(const float *mat, float *res,
   }
 }
 
-// cov: symmetric 3x3 column-major; evals ascending, evecs column-major (col 0 = smallest)
+// cov: symmetric 3x3 column-major; evals ascending, evecs column-major (col 0 = smallest).
+// All = false computes column 0 only (V[0..2], the rest undefined) where it does not depend on
+// the others: it is extract_kernel of (m - ev[0] I) in both branches of the reference's choice,
+// unless the smallest eigenvalue is the one orthonormalised against the largest one's vector.
+template <bool All = true>
 __device__ __forceinline__ void eig3_direct(const float *cov, float *ev, float *V) {
   float m[9];
 #pragma unroll
@@ -822,6 +826,15 @@ __device__ __forceinline__ void eig3_direct(const float *cov, float *ev, float *
     float d1 = ev[1] - ev[0];
     const bool swapped = d0 > d1;  // k = 2, l = 0 when true (else k = 0, l = 2)
     if (swapped) d0 = d1;
+    if (!All && !(swapped && d0 <= 2.0f * eps * d1)) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) tmp[i] = m[i];
+      tmp[0] -= ev[0]; tmp[4] -= ev[0]; tmp[8] -= ev[0];
+      eig_extract_kernel(tmp, V, nullptr);
+      ev[0] = ev[0] * scale; ev[1] = ev[1] * scale; ev[2] = ev[2] * scale;
+      ev[0] = ev[0] + shift; ev[1] = ev[1] + shift; ev[2] = ev[2] + shift;
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < 9; ++i) tmp[i] = m[i];
     const float ek = swapped ? ev[2] : ev[0];
@@ -909,19 +922,34 @@ __device__ __forceinline__ void eig3_jacobi_f64(const float *cov, float *ev, flo
       a[p][q] = a[q][p] = 0.0;
     }
   }
-  // ascending order
-  int o0 = 0, o1 = 1, o2 = 2, tmp;
-  if (a[o1][o1] < a[o0][o0]) { tmp = o0; o0 = o1; o1 = tmp; }
-  if (a[o2][o2] < a[o1][o1]) { tmp = o1; o1 = o2; o2 = tmp; }
-  if (a[o1][o1] < a[o0][o0]) { tmp = o0; o0 = o1; o1 = tmp; }
-  const int ord[3] = {o0, o1, o2};
+  // ascending order: compare-exchanges of positions (0, 1), (1, 2), (0, 1) on the eigenvalues
+  // and their vectors, as selects (an index chosen at run time would put the arrays in memory)
+  double e[3] = {a[0][0], a[1][1], a[2][2]}, w[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) w[k][r] = v[r][k];
+  auto exchange = [&](int i, int j) {
+    const bool sw = e[j] < e[i];
+    const double ei = e[i], ej = e[j];
+    e[i] = sw ? ej : ei;
+    e[j] = sw ? ei : ej;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const double wi = w[i][r], wj = w[j][r];
+      w[i][r] = sw ? wj : wi;
+      w[j][r] = sw ? wi : wj;
+    }
+  };
+  exchange(0, 1);
+  exchange(1, 2);
+  exchange(0, 1);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const int j = ord[k];
-    ev[k] = (float)a[j][j];
-    const double nrm = sqrt(v[0][j] * v[0][j] + v[1][j] * v[1][j] + v[2][j] * v[2][j]);
+    ev[k] = (float)e[k];
+    const double nrm = sqrt(w[k][0] * w[k][0] + w[k][1] * w[k][1] + w[k][2] * w[k][2]);
 #pragma unroll
-    for (int r = 0; r < 3; ++r) V[3 * k + r] = (float)(v[r][j] / nrm);
+    for (int r = 0; r < 3; ++r) V[3 * k + r] = (float)(w[k][r] / nrm);
   }
 }
 
@@ -941,7 +969,9 @@ __device__ __forceinline__ void omega6(const float *U, const float *dg, float *o
 
 }  // namespace
 
-template <int M>
+// Full = false when only the normals and curvatures are stored (no information matrices, no
+// Stats): the eigensolve then computes the smallest eigenvalue's vector alone
+template <int M, bool Full>
 __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const int *__restrict__ blk_frame,
                                            const StatsParamsDev &sp) {
   const int g = blockIdx.x;
@@ -1033,7 +1063,7 @@ __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const
       cov[6] = acc[6] * dn - m0 * m2;
       cov[7] = acc[8] * dn - m1 * m2;
       cov[8] = acc[9] * dn - m2 * m2;
-      eig3_direct(cov, ev, V);
+      eig3_direct<Full>(cov, ev, V);
       if (sp.fp64_gap > 0.0f && !(ev[1] - ev[0] >= sp.fp64_gap * fabsf(ev[2]))) eig3_jacobi_f64(cov, ev, V);
       if (ev[0] < 0.0f) ev[0] = 0.0f;
       mean[0] = m0; mean[1] = m1; mean[2] = m2;
@@ -1050,7 +1080,7 @@ __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const
           dp[i] = curv < sp.pthr ? sp.pflat[i] : sp.pnonflat[i];
           dq[i] = curv < sp.nthr ? sp.nflat[i] : sp.nnonflat[i];
         }
-        if (!sp.skip_omega) {
+        if (Full && !sp.skip_omega) {
           omega6(V, dp, op);
           omega6(V, dq, on);
         }
@@ -1059,13 +1089,13 @@ __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const
       }
     }
     gstore4(fd.nrm + idx, nrm);
-    if (!sp.skip_omega) {
+    if (Full && !sp.skip_omega) {
       float4 *o = fd.omg + 3 * (size_t)idx;
       gstore4(o, make_float4(op[0], op[1], op[2], op[3]));
       gstore4(o + 1, make_float4(op[4], op[5], on[0], on[1]));
       gstore4(o + 2, make_float4(on[2], on[3], on[4], on[5]));
     }
-    if (sp.keep_stats) {
+    if (Full && sp.keep_stats) {
       float4 *s = fd.stats + 4 * (size_t)idx;
       gstore4(s, make_float4(V[0], V[1], V[2], V[3]));
       gstore4(s + 1, make_float4(V[4], V[5], V[6], V[7]));
@@ -1076,11 +1106,13 @@ __device__ __forceinline__ void stats_body(const FrameDev *__restrict__ F, const
 }
 extern "C" __global__ __launch_bounds__(kThreads) void k_stats(const FrameDev *__restrict__ F,
                                                               const int *__restrict__ blk_frame, StatsParamsDev sp) {
-  stats_body<kPinhole>(F, blk_frame, sp);
+  if (sp.keep_stats || !sp.skip_omega) stats_body<kPinhole, true>(F, blk_frame, sp);
+  else stats_body<kPinhole, false>(F, blk_frame, sp);
 }
 extern "C" __global__ __launch_bounds__(kThreads) void k_stats_sph(const FrameDev *__restrict__ F,
                                                                   const int *__restrict__ blk_frame, StatsParamsDev sp) {
-  stats_body<kSpherical>(F, blk_frame, sp);
+  if (sp.keep_stats || !sp.skip_omega) stats_body<kSpherical, true>(F, blk_frame, sp);
+  else stats_body<kSpherical, false>(F, blk_frame, sp);
 }
 
 // ---------------------------------------------------------------------------------------
