@@ -1873,6 +1873,7 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
     const int sx = stiles_x(fcols);
     const int nsrc = all ? stiles(frows, fcols) : cnt;
     const int xlo = fx * kDTW, ylo = fy * kFH;
+    const int xspan = min(kDTW, pcols - xlo), yspan = min(kFH, prows - ylo);  // the tile's part of the image
     // wavefront w takes source tiles w, w + 4, ...: its list entries and their point counts are
     // read once (lane i holds entry w + 4 i) and taken with readlane, so no LDS wait inside the
     // loop; lane l projects the tile's packed pixels l, l + 64, ... (at most kPhaseB of them),
@@ -1897,18 +1898,19 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
       for (int u = 0; u < kPhaseB; ++u) {
         if (u * 64 + lane >= n) continue;
         const unsigned int at = q[u].y >> kSPosShift, idx = q[u].y & ((1u << kSPosShift) - 1u);
+        // the cloud point's w is 1 exactly: the last row of iKRt is (0, 0, 0, 1) and the column,
+        // row and depth are finite and non-negative
         const float4 p = unproject_point(iKRt, stx * kSTW + (int)(at % kSTW), sty * kSTH + (int)(at / kSTW),
                                          __uint_as_float(q[u].x));
-        // pinhole_project's arithmetic on the register copies
+        // pinhole_project's arithmetic on the register copies; the image bounds and the tile's
+        // bounds together are one range per axis
         float ip[4];
-        mul4_packet(KRt, p.x, p.y, p.z, p.w, ip);
+        mul4_packet(KRt, p.x, p.y, p.z, 1.0f, ip);
         const float d = ip[2];
         if (d < mind || d > maxd) continue;
         const float inv = 1.0f / d;
-        const int x = (int)roundf(ip[0] * inv), y = (int)roundf(ip[1] * inv);
-        if (!(x >= 0 && x < pcols && y >= 0 && y < prows)) continue;
-        const int lx = x - xlo, ly = y - ylo;
-        if (lx >= 0 && lx < kDTW && ly >= 0 && ly < kFH)
+        const int lx = (int)roundf(ip[0] * inv) - xlo, ly = (int)roundf(ip[1] * inv) - ylo;
+        if ((unsigned int)lx < (unsigned int)xspan && (unsigned int)ly < (unsigned int)yspan)
           atomicMin(&zl[ly * kDTW + lx], ((unsigned long long)__float_as_uint(d) << 32) | idx);
       }
     }
