@@ -1474,10 +1474,13 @@ __device__ __forceinline__ float swap_add_f32(float x, float y) {
 // first two halvings (offsets 32 and 16, permlane swaps) add in FP32, so an FP32 sum covers at
 // most the sixteen correspondences of four lanes; the rest (DPP moves, offsets 8 to 1), the sum
 // over the wavefronts and each slot's constant factor (slot_scale, a power of two) are FP64. The
-// order of the additions is fixed.
-__device__ void block_partial(Acc2 *acc, double *out) {
+// order of the additions is fixed. The wavefronts' sums go through one of two LDS buffers
+// (buf): a block that reduces several tiles in turn alternates them, so the next tile's writes
+// never meet the previous tile's reads and need no barrier of their own (the barrier inside the
+// next call orders a buffer's reads before its reuse two calls later).
+__device__ void block_partial(Acc2 *acc, double *out, int buf = 0) {
   static_assert(kPartial == 32, "the reduce-scatter halves 32 slots five times");
-  __shared__ double red[kThreads / 64][kPartial];
+  __shared__ double red[2][kThreads / 64][kPartial];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float f[kPartial];
 #pragma unroll
@@ -1493,12 +1496,12 @@ __device__ void block_partial(Acc2 *acc, double *out) {
   dpp_halve<4, 2>(v);
   dpp_halve<2, 1>(v);
   v[0] += dpp_xor_f64<1>(v[0]);
-  if ((lane & 1) == 0) red[wave][lane >> 1] = v[0];
+  if ((lane & 1) == 0) red[buf][wave][lane >> 1] = v[0];
   __syncthreads();
   if (threadIdx.x < kPartial) {
     double s = 0.0;
     if (threadIdx.x < kVals)
-      for (int w = 0; w < kThreads / 64; ++w) s += red[w][threadIdx.x];
+      for (int w = 0; w < kThreads / 64; ++w) s += red[buf][w][threadIdx.x];
     *gmut(out + threadIdx.x) = s * slot_scale(threadIdx.x);
   }
 }
@@ -1932,8 +1935,7 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
     const int t = ty * pd.dtiles_x + fx;
     if (closed) corr_lin_tile<true, true>(pd, fr, fc, T, ap, store_pairs, t, zl + sub_t * kDTH * kDTW, om_model, acc);
     else corr_lin_tile<true, false>(pd, fr, fc, T, ap, store_pairs, t, zl + sub_t * kDTH * kDTW, om_model, acc);
-    block_partial(acc, pd.partials + (size_t)t * kPartial);
-    __syncthreads();  // block_partial's LDS is reused by the next tile
+    block_partial(acc, pd.partials + (size_t)t * kPartial, sub_t & 1);
   }
 }
 
