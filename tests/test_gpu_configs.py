@@ -9,6 +9,7 @@
   from a hipGraph; pairs of every resolution against single-pair calls (bit for bit) and
   against the oracle (1e-5 rad / 1e-5 m).
 - A captured graph replayed after the batch's block maps were reallocated by a larger call.
+- A batch with a frame beyond the packed source pixels' size limit (2^23 pixels).
 
 Reference: AlignerProjective::align, nicp/nicp/alignerprojective.cpp:21-136, and the
 initial guess it starts from, nicp/nicp/aligner.h:123-141.
@@ -240,6 +241,46 @@ def test_graph_replay_after_map_growth(ctx):
     np.testing.assert_array_equal(Tg1.view(np.uint32), Te.view(np.uint32))
     np.testing.assert_array_equal(Tg2.view(np.uint32), Te.view(np.uint32))
     b.close()
+
+
+def test_frame_beyond_packed_source_limit(ctx):
+    """A frame of more than 2^23 pixels has no packed source pixels (their cloud index shares a
+    32-bit word with the pixel's offset in its source tile), so a batch call with it takes the
+    global z-buffer even when the tile-binned projection is forced: its pair equals the
+    single-pair call bit for bit, and a small pair of the same call equals the same pair aligned
+    with the tile-binned projection in a batch of small frames."""
+    shapes = [(2112, 4096), (2112, 4096), (240, 320), (240, 320)]
+    say("case frame_beyond_packed_source_limit pixels=%d" % (2112 * 4096))
+    assert 2112 * 4096 > (1 << 23)
+    _, dsp = params()
+    _, dap = aparams()
+    cam_list = [cams(r, c)[1] for r, c in shapes]
+    b = capi.Batch(ctx, shapes, cam_list, dsp)
+    big = synth.pair(51, 2112, 4096)
+    small = synth.pair(52, 240, 320)
+    for f, d in enumerate((big[0], big[1], small[0], small[1])):
+        b.upload(f, d)
+    b.convert()
+    b.set_projection(2)
+    T, _ = b.align([0, 2], [1, 3], None, dap, use_graph=False)
+    cr, cc = b.frame_cloud(0, shapes[0]), b.frame_cloud(1, shapes[1])
+    Ts, _ = ctx.align(cr, cc, cam_list[0], 2112, 4096, identity16(), dap)
+    cr.close()
+    cc.close()
+    np.testing.assert_array_equal(T[0].view(np.uint32), Ts.view(np.uint32))
+    rot, tr = synth.pose_error(T[0], big[2])
+    say("  big pair: rot err %.4f deg, trans err %.5f m" % (rot, tr))
+    assert rot < 1.0 and tr < 0.02
+    b.close()
+    # the small pair in a batch of small frames, where the tile-binned projection runs
+    b2 = capi.Batch(ctx, shapes[2:], cam_list[2:], dsp)
+    b2.upload(0, small[0])
+    b2.upload(1, small[1])
+    b2.convert()
+    b2.set_projection(2)
+    T2, _ = b2.align([0], [1], None, dap, use_graph=False)
+    np.testing.assert_array_equal(T[1].view(np.uint32), T2[0].view(np.uint32))
+    b2.close()
 
 
 def test_ground_truth_ceiling_640x480(ctx):
