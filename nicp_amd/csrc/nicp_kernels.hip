@@ -1515,12 +1515,19 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return x * q + min(x, r) + (b >> 3);
 }
 
-// CorrespondenceFinderProjective gates on already loaded values (correspondencefinderprojective.cpp:62-95)
+// CorrespondenceFinderProjective gates on already loaded values (correspondencefinderprojective.cpp:62-95).
+// Converted: both clouds come from depth conversions, whose normals are unit vectors or exactly
+// zero, so a normal's squared norm is zero exactly when its three components are.
+template <bool Converted>
 __device__ __forceinline__ bool gates_loaded(const float4 &cn, const float4 &rn0, const float4 &cp, const float4 &rp0,
                                              const float *T, const AlignParamsDev &ap, float *rp, float *rn) {
-  if (!ap.gicp && (dot4(cn.x, cn.y, cn.z, 0.0f, cn.x, cn.y, cn.z, 0.0f) == 0.0f ||
-                   dot4(rn0.x, rn0.y, rn0.z, 0.0f, rn0.x, rn0.y, rn0.z, 0.0f) == 0.0f))
+  if (Converted) {
+    if (!ap.gicp && ((cn.x == 0.0f && cn.y == 0.0f && cn.z == 0.0f) || (rn0.x == 0.0f && rn0.y == 0.0f && rn0.z == 0.0f)))
+      return false;
+  } else if (!ap.gicp && (dot4(cn.x, cn.y, cn.z, 0.0f, cn.x, cn.y, cn.z, 0.0f) == 0.0f ||
+                          dot4(rn0.x, rn0.y, rn0.z, 0.0f, rn0.x, rn0.y, rn0.z, 0.0f) == 0.0f)) {
     return false;
+  }
   iso_mul3(T, rp0.x, rp0.y, rp0.z, rp0.w, rp);
   iso_rot3(T, rn0.x, rn0.y, rn0.z, rn);
   const float dv0 = cp.x - rp[0], dv1 = cp.y - rp[1], dv2 = cp.z - rp[2], dv3 = cp.w - rp0.w;
@@ -1619,6 +1626,8 @@ extern "C" __global__ __launch_bounds__(kThreads) void k_omega_fill(const FrameD
 // reference index of a pixel comes from the z-buffer key: in global memory (Lds = false; the cell
 // is reset for the next projection) or in the block's LDS tile (Lds = true, cell = tile offset).
 // Both variants run the same arithmetic in the same order, so they give the same partial sums.
+// Closed: both clouds are depth conversions (FrameDev::omega_closed), so the current points'
+// information matrices are formed from their normals and the zero-normal gate tests components.
 template <bool Lds, bool Closed>
 __device__ __forceinline__ void corr_lin_tile(const PairDev &pd, const FrameDev &fr, const FrameDev &fc, const float *T,
                                               const AlignParamsDev &ap, int store_pairs, int t,
@@ -1664,7 +1673,7 @@ __device__ __forceinline__ void corr_lin_tile(const PairDev &pd, const FrameDev 
       ok[k] = false;
       if (ri[k] >= 0 && ci[k] >= 0) {
         acc[30] += 1.0f;
-        ok[k] = gates_loaded(cn[k], rn0[k], cp[k], rp0[k], T, ap, rp[k], rn[k]);
+        ok[k] = gates_loaded<Closed>(cn[k], rn0[k], cp[k], rp0[k], T, ap, rp[k], rn[k]);
       }
     }
     // the current points' information matrices: loaded, or in closed form from their normals
@@ -1711,7 +1720,8 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
 #pragma unroll
   for (int v = 0; v < kVals; ++v) acc[v] = 0.0f;
   if (!multi) {
-    if (fc.omega_closed) corr_lin_tile<false, true>(pd, fr, fc, T, ap, store_pairs, t, nullptr, omega_model(fc), acc);
+    if (fc.omega_closed && fr.omega_closed)
+      corr_lin_tile<false, true>(pd, fr, fc, T, ap, store_pairs, t, nullptr, omega_model(fc), acc);
     else corr_lin_tile<false, false>(pd, fr, fc, T, ap, store_pairs, t, nullptr, OmegaModel{}, acc);
   } else {
     const int sub = pd.sub;
@@ -1923,7 +1933,7 @@ extern "C" __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_pe
 #pragma unroll
   for (int k = 0; k < 16; ++k) T[k] = *gview(pd.state + kSlotTinv + k);
   const int nty = dtiles_y(pd.rows);
-  const bool closed = fc.omega_closed != 0;
+  const bool closed = fc.omega_closed != 0 && fr.omega_closed != 0;
   const OmegaModel om_model = closed ? omega_model(fc) : OmegaModel{};
 #pragma unroll 1
   for (int sub_t = 0; sub_t < kFH / kDTH; ++sub_t) {

@@ -50,9 +50,12 @@ def _act_shape(l, B):
     return (B, oh, ow, oc) if conv else (B, oc)
 
 
-def _read_acts(d, B, what):
-    return [d.read(what, l, int(np.prod(_act_shape(l, B)))).reshape(_act_shape(l, B)).astype(np.float64)
-            for l in range(len(net.LAYERS) + 1)]
+def _read_acts(d, B, what, first=0):
+    """Layers first ... L of an activation (what = 0) or gradient (what = 1) buffer; layers below
+    first are None (the backward pass writes no gradient for the network's input)."""
+    return [None] * first + [
+        d.read(what, l, int(np.prod(_act_shape(l, B)))).reshape(_act_shape(l, B)).astype(np.float64)
+        for l in range(first, len(net.LAYERS) + 1)]
 
 
 def test_forward_layers(ctx):
@@ -88,7 +91,7 @@ def test_backward_layers(ctx):
     t = np.random.default_rng(5).normal(size=(B, 6)).astype(np.float32)
     d.train_step(x, t, lr=0.0)
     acts = _read_acts(d, B, 0)
-    dacts = _read_acts(d, B, 1)
+    dacts = _read_acts(d, B, 1, first=1)
     grads = d.read(2, 0, net.num_params()).astype(np.float64)
     # loss gradient: 2 w (y - t) / B in fp32, bit-exact
     y32 = acts[-1].astype(np.float32)
