@@ -555,11 +555,18 @@ void frame_geometry(FrameDev &f, int rows, int cols, const nicp_camera *cam, flo
 int xblocks(int rows) { return (rows * 10 + kThreads - 1) / kThreads; }
 int yblocks(int cols) { return (cols * 10 + kThreads - 1) / kThreads; }
 
+// Whether a batch frame carries packed source-tile pixels: pinhole frames whose cloud indices
+// fit below the pixel offset of a packed entry (kSPosShift bits)
+bool spack_frame(int model, int rows, int cols) {
+  return model == NICP_PINHOLE && (long long)rows * cols <= (1ll << kSPosShift);
+}
+
 // Layout of the per-frame device arrays inside one allocation.
 struct FrameLayout {
   size_t depth, depthf, index, ii, pts, nrm, omg, stats, stile, spix, stn, npoints, tcount, toff, total;
 };
-// with_spack: the packed source-tile pixels of the tile-binned projection (batch frames)
+// with_spack: the packed source-tile pixels of the tile-binned projection (batch frames for which
+// spack_frame holds)
 FrameLayout frame_layout(int rows, int cols, bool with_stats, bool with_ii, size_t base, bool with_f32 = false,
                          bool with_spack = false) {
   Carver cv;
@@ -597,7 +604,7 @@ void frame_pointers(FrameDev &f, char *base, const FrameLayout &L, bool with_sta
   f.stats = with_stats ? (float4 *)(base + L.stats) : nullptr;
   // source-tile depth ranges of the tile-binned projection (pinhole frames)
   f.stile_d = with_ii && f.model == NICP_PINHOLE ? (float2 *)(base + L.stile) : nullptr;
-  const bool spack = with_ii && with_spack && f.model == NICP_PINHOLE && (long long)f.rows * f.cols <= (1ll << kSPosShift);
+  const bool spack = with_ii && with_spack && spack_frame(f.model, f.rows, f.cols);
   f.spix = spack ? (uint2 *)(base + L.spix) : nullptr;
   f.stile_n = spack ? (int *)(base + L.stn) : nullptr;
   f.npoints = (int *)(base + L.npoints);
@@ -1772,7 +1779,7 @@ extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows,
   int tiles = 0, xb = 0, yb = 0;
   std::vector<int> blk, xblk, yblk;
   for (int i = 0; i < num_frames; ++i) {
-    lay[i] = frame_layout(rows[i], cols[i], false, true, off, false, true);
+    lay[i] = frame_layout(rows[i], cols[i], false, true, off, false, spack_frame(cams[i].model, rows[i], cols[i]));
     off = align_up(lay[i].total, 256);
     FrameDev &f = b->frames[i];
     frame_geometry(f, rows[i], cols[i], &cams[i], sp->world_radius, 0.001f);
@@ -1812,7 +1819,8 @@ extern "C" int nicp_batch_create(nicp_ctx *ctx, int num_frames, const int *rows,
     return set_err(NICP_ERR_OOM, "hipMalloc(%zu): %s", cv.off, hipGetErrorString(e));
   }
   char *base = (char *)b->arena;
-  for (int i = 0; i < num_frames; ++i) frame_pointers(b->frames[i], base, lay[i], false, true, false, true);
+  for (int i = 0; i < num_frames; ++i)
+    frame_pointers(b->frames[i], base, lay[i], false, true, false, spack_frame(cams[i].model, rows[i], cols[i]));
   b->d_frames = (FrameDev *)(base + o_tbl);
   b->d_blk_frame = (int *)(base + o_blk);
   b->d_xblk_frame = (int *)(base + o_x);
