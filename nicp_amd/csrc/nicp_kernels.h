@@ -35,6 +35,7 @@ constexpr int kBinMargin = 2;  // pixels added to each side of a projected sourc
 constexpr int kPhaseB = kSTH * kSTW / 64;  // source pixels per lane of a wavefront's source tile
 constexpr int kFH = 64;
 constexpr int kListCap = 256;
+constexpr int kDepthCopyBlocks = 32;  // blocks per frame of k_batch_depth_copy
 constexpr int kSPosShift = 23;  // bit of a packed source pixel's offset in the tile (FrameDev::spix)
 static_assert(kSTH * kSTW <= (1 << (32 - kSPosShift)), "a packed source pixel's offset fits above its index");
 static_assert(kSTH * kSTW % 64 == 0 && kListCap <= kThreads, "k_proj_corr_lin's source-tile loop");

@@ -688,37 +688,33 @@ __device__ __forceinline__ void integral_frame_body(const FrameDev *__restrict__
   }
 }
 // Depth images of a batch against one packed device array (frame f at pix_base[f], rows*cols
-// entries), per kTile-pixel tile as the conversion's per-pixel launches: to_frames copies the
-// packed array into the frames' depth, otherwise the frames' depth into the packed array
+// entries): blockIdx.y is the frame, whose kDepthCopyBlocks blocks stride over its pixels;
+// to_frames copies the packed array into the frames' depth, otherwise the frames' depth into the
+// packed array. 16-byte accesses where both sides of the frame are 16-byte aligned and its pixel
+// count is a multiple of eight, else 2-byte ones.
 extern "C" __global__ __launch_bounds__(kThreads) void k_batch_depth_copy(const FrameDev *__restrict__ F,
-                                                                         const int *__restrict__ blk_frame,
                                                                          const long long *__restrict__ pix_base,
                                                                          const uint16_t *__restrict__ src,
                                                                          uint16_t *__restrict__ dst, int to_frames) {
-  const int g = blockIdx.x;
-  const int f = blk_frame[g];
+  const int f = blockIdx.y;
   const FrameDev &fd = F[f];
-  const int P = fd.rows * fd.cols;
-  const int base = (g - fd.tile_base) * kTile;
+  const long long P = (long long)fd.rows * fd.cols;
   const long long o = pix_base[f];
   uint16_t *depth = const_cast<uint16_t *>(fd.depth);
   const uint16_t *from = to_frames ? src + o : depth;
   uint16_t *to = to_frames ? depth : dst + o;
-  static_assert(kTile == 4 * kThreads, "four pixels per thread");
-  if ((P & 3) == 0 && (((uintptr_t)from | (uintptr_t)to) & 7) == 0) {
-    // four pixels of a thread as one 8-byte access
-    const int p = base + 4 * threadIdx.x;
-    if (p < P) *(uint2 *)(to + p) = *(const uint2 *)(from + p);
+  const long long stride = (long long)gridDim.x * kThreads;
+  const long long t0 = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if ((P & 7) == 0 && (((uintptr_t)from | (uintptr_t)to) & 15) == 0) {
+    const __attribute__((address_space(1))) gint4 *s4 = gview((const gint4 *)from);
+    __attribute__((address_space(1))) gint4 *d4 = gmut((gint4 *)to);
+    for (long long i = t0; i < (P >> 3); i += stride) d4[i] = s4[i];
     return;
   }
-#pragma unroll
-  for (int k = 0; k < kTile / kThreads; ++k) {
-    const int p = base + k * kThreads + threadIdx.x;
-    if (p < P) to[p] = from[p];
-  }
+  const __attribute__((address_space(1))) uint16_t *s2 = gview(from);
+  __attribute__((address_space(1))) uint16_t *d2 = gmut(to);
+  for (long long i = t0; i < P; i += stride) d2[i] = s2[i];
 }
-
-// one workgroup per frame; workgroup i takes frame order[i] (order null: frame i)
 extern "C" __global__ __launch_bounds__(kIFrameThreads) void k_integral_frame(const FrameDev *__restrict__ F,
                                                                               const int *__restrict__ order) {
   integral_frame_body<kPinhole>(F, order);
