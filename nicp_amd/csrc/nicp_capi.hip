@@ -625,8 +625,9 @@ int launch_convert(Prof *pf, hipStream_t s, const FrameDev *d_frames, int num_fr
   if (total_tiles == 0) return NICP_OK;
   (void)hipGetLastError();
   const bool sph = model == NICP_SPHERICAL;
-  if (sph) LAUNCH(pf, KID_COUNT, k_count_sph, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
-  else LAUNCH(pf, KID_COUNT, k_count, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
+  const dim3 count_grid((total_tiles + kCountTiles - 1) / kCountTiles);
+  if (sph) LAUNCH(pf, KID_COUNT, k_count_sph, count_grid, dim3(kThreads), s, d_frames, d_blk, total_tiles);
+  else LAUNCH(pf, KID_COUNT, k_count, count_grid, dim3(kThreads), s, d_frames, d_blk, total_tiles);
   LAUNCH(pf, KID_SCAN, k_scan, dim3(num_frames), dim3(1024), s, d_frames);
   if (sph) LAUNCH(pf, KID_UNPROJECT, k_unproject_sph, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);
   else LAUNCH(pf, KID_UNPROJECT, k_unproject, dim3(total_tiles), dim3(kThreads), s, d_frames, d_blk);

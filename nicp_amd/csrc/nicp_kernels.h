@@ -36,6 +36,7 @@ constexpr int kPhaseB = kSTH * kSTW / 64;  // source pixels per lane of a wavefr
 constexpr int kFH = 64;
 constexpr int kListCap = 256;
 constexpr int kDepthCopyBlocks = 32;  // blocks per frame of k_batch_depth_copy
+constexpr int kCountTiles = 4;        // 1024-pixel tiles per block of k_count
 constexpr int kSPosShift = 23;  // bit of a packed source pixel's offset in the tile (FrameDev::spix)
 static_assert(kSTH * kSTW <= (1 << (32 - kSPosShift)), "a packed source pixel's offset fits above its index");
 static_assert(kSTH * kSTW % 64 == 0 && kListCap <= kThreads, "k_proj_corr_lin's source-tile loop");

@@ -164,36 +164,78 @@ __device__ __forceinline__ bool pixel_valid(const FrameDev &fd, int p, float d) 
   return depth_valid(d, fd.min_d, fd.max_d);
 }
 
+// Block g counts the valid pixels of the kCountTiles tiles kCountTiles g + j (each tile's frame
+// from the tile map), all their depth loads in flight before the counts.
 template <int M>
-__device__ __forceinline__ void count_body(const FrameDev *__restrict__ F, const int *__restrict__ blk_frame) {
-  const int g = blockIdx.x;
-  const FrameDev &fd = F[blk_frame[g]];
-  const int P = fd.rows * fd.cols;
-  const int base = (g - fd.tile_base) * kTile;
-  int cnt = 0, px[4];
-  float dv[4];
+__device__ __forceinline__ void count_body(const FrameDev *__restrict__ F, const int *__restrict__ blk_frame,
+                                           int total_tiles) {
+  // frame_depths' loads for all tiles first (uint16 bits, or float bits for float depth), then the
+  // conversions, so that no tile's loads wait for another's
+  int px[kCountTiles][4];
+  unsigned int raw[kCountTiles][4];
+  float dv[kCountTiles][4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int p = base + k * kThreads + threadIdx.x;
-    px[k] = p < P ? p : -1;
+  for (int j = 0; j < kCountTiles; ++j) {
+    const int g = kCountTiles * blockIdx.x + j;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      px[j][k] = -1;
+      raw[j][k] = 0u;
+    }
+    if (g >= total_tiles) continue;
+    const FrameDev &fd = F[blk_frame[g]];
+    const int P = fd.rows * fd.cols;
+    const int base = (g - fd.tile_base) * kTile;
+    const __attribute__((address_space(1))) float *df = gview(fd.depthf);
+    const __attribute__((address_space(1))) uint16_t *du = gview(fd.depth);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int p = base + k * kThreads + threadIdx.x;
+      px[j][k] = p < P ? p : -1;
+      if (p < P) raw[j][k] = df ? __float_as_uint(df[p]) : (unsigned int)du[p];
+    }
   }
-  frame_depths<4>(fd, px, dv);
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if (px[k] >= 0) cnt += pixel_valid<M>(fd, px[k], dv[k]);
-  __shared__ int ws[kThreads / 64];
-  cnt = wave_sum_i(cnt);
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = cnt;
+  for (int j = 0; j < kCountTiles; ++j) {
+    const int g = kCountTiles * blockIdx.x + j;
+    if (g >= total_tiles) continue;
+    const FrameDev &fd = F[blk_frame[g]];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      dv[j][k] = fd.depthf ? __uint_as_float(raw[j][k]) : depth_of((uint16_t)raw[j][k], fd.depth_scale);
+  }
+  __shared__ int ws[kCountTiles][kThreads / 64];
+#pragma unroll
+  for (int j = 0; j < kCountTiles; ++j) {
+    const int g = kCountTiles * blockIdx.x + j;
+    int cnt = 0;
+    if (g < total_tiles) {
+      const FrameDev &fd = F[blk_frame[g]];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (px[j][k] >= 0) cnt += pixel_valid<M>(fd, px[j][k], dv[j][k]);
+    }
+    cnt = wave_sum_i(cnt);
+    if ((threadIdx.x & 63) == 0) ws[j][threadIdx.x >> 6] = cnt;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) fd.tile_count[g - fd.tile_base] = ws[0] + ws[1] + ws[2] + ws[3];
+  if (threadIdx.x < kCountTiles) {
+    const int g = kCountTiles * blockIdx.x + threadIdx.x;
+    if (g < total_tiles) {
+      const FrameDev &fd = F[blk_frame[g]];
+      *gmut(fd.tile_count + (g - fd.tile_base)) =
+          ws[threadIdx.x][0] + ws[threadIdx.x][1] + ws[threadIdx.x][2] + ws[threadIdx.x][3];
+    }
+  }
 }
 extern "C" __global__ __launch_bounds__(kThreads) void k_count(const FrameDev *__restrict__ F,
-                                                              const int *__restrict__ blk_frame) {
-  count_body<kPinhole>(F, blk_frame);
+                                                              const int *__restrict__ blk_frame, int total_tiles) {
+  count_body<kPinhole>(F, blk_frame, total_tiles);
 }
 extern "C" __global__ __launch_bounds__(kThreads) void k_count_sph(const FrameDev *__restrict__ F,
-                                                                  const int *__restrict__ blk_frame) {
-  count_body<kSpherical>(F, blk_frame);
+                                                                  const int *__restrict__ blk_frame,
+                                                                  int total_tiles) {
+  count_body<kSpherical>(F, blk_frame, total_tiles);
 }
 
 // one block of 1024 threads per frame
