@@ -1896,8 +1896,8 @@ extern "C" int nicp_batch_upload_depth_device(nicp_batch *b, const uint16_t *dep
   if (scale_changed)
     HIPCHK(hipMemcpyAsync(b->d_frames, b->frames.data(), b->frames.size() * sizeof(FrameDev), hipMemcpyHostToDevice,
                           b->ctx->stream));
-  LAUNCH((Prof *)nullptr, 0, k_batch_depth_copy, dim3(kDepthCopyBlocks, b->num_frames), dim3(kThreads), b->ctx->stream,
-         b->d_frames, b->d_pix_base, depth, (uint16_t *)nullptr, 1);
+  LAUNCH((Prof *)nullptr, 0, k_batch_depth_copy, dim3(kDepthCopyBlocks, std::min(b->num_frames, 65535)), dim3(kThreads),
+         b->ctx->stream, b->d_frames, b->d_pix_base, b->num_frames, depth, (uint16_t *)nullptr, 1);
   // the host copy of the frame table is pageable: wait for it before the caller may change it
   if (scale_changed) HIPCHK(hipStreamSynchronize(b->ctx->stream));
   return NICP_OK;
@@ -1906,8 +1906,8 @@ extern "C" int nicp_batch_upload_depth_device(nicp_batch *b, const uint16_t *dep
 extern "C" int nicp_batch_download_depth_device(nicp_batch *b, uint16_t *depth) {
   if (!b || !depth) return set_err(NICP_ERR_INVALID, "bad arguments");
   HIPCHK(hipSetDevice(b->ctx->device));
-  LAUNCH((Prof *)nullptr, 0, k_batch_depth_copy, dim3(kDepthCopyBlocks, b->num_frames), dim3(kThreads), b->ctx->stream,
-         b->d_frames, b->d_pix_base, (const uint16_t *)nullptr, depth, 0);
+  LAUNCH((Prof *)nullptr, 0, k_batch_depth_copy, dim3(kDepthCopyBlocks, std::min(b->num_frames, 65535)), dim3(kThreads),
+         b->ctx->stream, b->d_frames, b->d_pix_base, b->num_frames, (const uint16_t *)nullptr, depth, 0);
   return NICP_OK;
 }
 

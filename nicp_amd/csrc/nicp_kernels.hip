@@ -730,32 +730,34 @@ __device__ __forceinline__ void integral_frame_body(const FrameDev *__restrict__
   }
 }
 // Depth images of a batch against one packed device array (frame f at pix_base[f], rows*cols
-// entries): blockIdx.y is the frame, whose kDepthCopyBlocks blocks stride over its pixels;
-// to_frames copies the packed array into the frames' depth, otherwise the frames' depth into the
-// packed array. 16-byte accesses where both sides of the frame are 16-byte aligned and its pixel
-// count is a multiple of eight, else 2-byte ones.
+// entries): blockIdx.y strides over the frames (gridDim.y is capped at 65535), and a frame's
+// kDepthCopyBlocks blocks stride over its pixels; to_frames copies the packed array into the
+// frames' depth, otherwise the frames' depth into the packed array. 16-byte accesses where both
+// sides of the frame are 16-byte aligned and its pixel count is a multiple of eight, else 2-byte
+// ones.
 extern "C" __global__ __launch_bounds__(kThreads) void k_batch_depth_copy(const FrameDev *__restrict__ F,
                                                                          const long long *__restrict__ pix_base,
-                                                                         const uint16_t *__restrict__ src,
+                                                                         int num_frames, const uint16_t *__restrict__ src,
                                                                          uint16_t *__restrict__ dst, int to_frames) {
-  const int f = blockIdx.y;
-  const FrameDev &fd = F[f];
-  const long long P = (long long)fd.rows * fd.cols;
-  const long long o = pix_base[f];
-  uint16_t *depth = const_cast<uint16_t *>(fd.depth);
-  const uint16_t *from = to_frames ? src + o : depth;
-  uint16_t *to = to_frames ? depth : dst + o;
   const long long stride = (long long)gridDim.x * kThreads;
   const long long t0 = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if ((P & 7) == 0 && (((uintptr_t)from | (uintptr_t)to) & 15) == 0) {
-    const __attribute__((address_space(1))) gint4 *s4 = gview((const gint4 *)from);
-    __attribute__((address_space(1))) gint4 *d4 = gmut((gint4 *)to);
-    for (long long i = t0; i < (P >> 3); i += stride) d4[i] = s4[i];
-    return;
+  for (int f = blockIdx.y; f < num_frames; f += gridDim.y) {
+    const FrameDev &fd = F[f];
+    const long long P = (long long)fd.rows * fd.cols;
+    const long long o = pix_base[f];
+    uint16_t *depth = const_cast<uint16_t *>(fd.depth);
+    const uint16_t *from = to_frames ? src + o : depth;
+    uint16_t *to = to_frames ? depth : dst + o;
+    if ((P & 7) == 0 && (((uintptr_t)from | (uintptr_t)to) & 15) == 0) {
+      const __attribute__((address_space(1))) gint4 *s4 = gview((const gint4 *)from);
+      __attribute__((address_space(1))) gint4 *d4 = gmut((gint4 *)to);
+      for (long long i = t0; i < (P >> 3); i += stride) d4[i] = s4[i];
+    } else {
+      const __attribute__((address_space(1))) uint16_t *s2 = gview(from);
+      __attribute__((address_space(1))) uint16_t *d2 = gmut(to);
+      for (long long i = t0; i < P; i += stride) d2[i] = s2[i];
+    }
   }
-  const __attribute__((address_space(1))) uint16_t *s2 = gview(from);
-  __attribute__((address_space(1))) uint16_t *d2 = gmut(to);
-  for (long long i = t0; i < P; i += stride) d2[i] = s2[i];
 }
 extern "C" __global__ __launch_bounds__(kIFrameThreads) void k_integral_frame(const FrameDev *__restrict__ F,
                                                                               const int *__restrict__ order) {
