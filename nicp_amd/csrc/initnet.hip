@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int kchunk, L
   __shared__ __attribute__((aligned(16))) unsigned short sA[64][kGemmLds];  // [row][k]
   __shared__ __attribute__((aligned(16))) unsigned short sB[64][kGemmLds];  // [col][k]
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;  // output tiles: rows on x (unbounded), columns on y
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int kbeg = blockIdx.z * kchunk, kend = min(K, kbeg + kchunk);
   f32x4 acc[2][2];
@@ -223,7 +223,7 @@ template <class LA, class LB, class ST>
 hipError_t gemm(hipStream_t s, int M, int N, int K, int splits, LA la, LB lb, ST st) {
   const int kchunk = gemm_kchunk(K, splits);
   const int z = gemm_chunks(K, splits);
-  dim3 grid((N + 63) / 64, (M + 63) / 64, z);
+  dim3 grid((M + 63) / 64, (N + 63) / 64, z);
   hipLaunchKernelGGL((k_gemm<LA, LB, ST>), grid, dim3(256), 0, s, M, N, K, kchunk, la, lb, st);
   return hipGetLastError();
 }
@@ -383,12 +383,15 @@ int net_reserve(nicp_initnet *n, int B) {
   HIPCHK(hipMalloc(&n->target, (size_t)B * 6 * sizeof(float)));
   if (n->chunks) (void)hipFree(n->chunks);
   n->chunks = nullptr;
+  // the split reductions' chunks for every batch size up to B: a later call with fewer rows
+  // (which reuses these buffers) splits its reductions further, and can need more room than B
   size_t most = 1;
-  for (int l = 0; l < kLayers; ++l) {
-    const LayerShape sh = layer_shape(l);
-    const int M = B * sh.oh * sh.ow, sp = fwd_splits(M, sh.oc);
-    if (sp > 1) most = std::max(most, (size_t)gemm_chunks(layer_k(l) + 1, sp) * M * sh.oc);
-  }
+  for (int b = 1; b <= B; ++b)
+    for (int l = 0; l < kLayers; ++l) {
+      const LayerShape sh = layer_shape(l);
+      const int M = b * sh.oh * sh.ow, sp = fwd_splits(M, sh.oc);
+      if (sp > 1) most = std::max(most, (size_t)gemm_chunks(layer_k(l) + 1, sp) * M * sh.oc);
+    }
   HIPCHK(hipMalloc(&n->chunks, most * sizeof(float)));
   n->cap = B;
   return NICP_OK;

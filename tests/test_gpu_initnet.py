@@ -79,6 +79,31 @@ def test_forward_layers(ctx):
         assert np.all(err <= 2e-5 * bound + 1e-30), l
 
 
+def test_forward_large_batch(ctx):
+    """A forward pass of 4096 inputs: the first layer's GEMM has 76800 row tiles, more than a grid's
+    y dimension holds (65535), so the row tiles run on x. Then forwards of 256 inputs on the same
+    network, whose buffers were reserved for 4096: with fewer rows the fully connected layers split
+    their reductions further and need more chunk room than the 4096-row call (the reservation
+    covers every smaller batch). Every output row agrees with the same input's row of the
+    256-input forwards to bf16 precision: the two batch sizes split the fully connected layers'
+    reductions differently, and a different FP32 order can flip the bf16 rounding of an
+    activation that the next layer reads (2^-8 relative). Rows beyond the first 3495 inputs (row
+    tiles past 65535) would be garbage or missing if the grid were wrong."""
+    B = 4096
+    say("case initnet forward large batch B=%d" % B)
+    d = capi.InitNet(ctx, seed=1)
+    d.set_params(_params(1))
+    x = _input(3, B)
+    y = d.forward(x)
+    ys = np.concatenate([d.forward(x[i:i + 256]) for i in range(0, B, 256)])
+    assert np.all(np.isfinite(y))
+    err = np.abs(y.astype(np.float64) - ys.astype(np.float64))
+    scale = max(1.0, float(np.abs(ys).max()))
+    say("  max |y - y_256| = %.3g, mean %.3g (max |y| %.3g)" % (err.max(), err.mean(), np.abs(ys).max()))
+    assert err.max() <= 2e-2 * scale
+    assert err.mean() <= 1e-3 * scale
+
+
 def test_backward_layers(ctx):
     say("case initnet backward layers B=3")
     B = 3
