@@ -6,6 +6,9 @@
   from the host.
 - Two contexts driven from two host threads (the pipelined benchmark) give every pair the
   transform, bit for bit, that the same calls give one after the other on one context.
+- A learned-start call with many pairs, then one with fewer on the same network (whose buffers,
+  reserved for the larger call, are reused): the smaller call equals the same call on a fresh
+  network, bit for bit, replayed from its graphs.
 
 Reference: DepthImageConverterIntegralImage::compute (nicp/nicp/depthimageconverterintegralimage.cpp:18-64)
 and AlignerProjective::align (nicp/nicp/alignerprojective.cpp:21-136), which both paths run.
@@ -111,3 +114,37 @@ def test_two_contexts_equal_one(ctx):
         batch.close()
         net.close()
     ctx2.close()
+
+
+def test_learned_fewer_pairs_after_more(ctx):
+    """Learned starts for 64 pairs, then for the first 8 of them on the same network and batch,
+    equal the same 8-pair call on a fresh network and batch, bit for bit (graph replay). The
+    network's buffers stay reserved for 64 pairs; an 8-pair forward splits its fully connected
+    layers' reductions further than a 64-pair one, so this covers the reservation of every
+    smaller batch."""
+    w = initnet.load_weights()
+    assert w is not None, "trained weights missing"
+    say("case learned_fewer_pairs_after_more pairs=64,8")
+    ap = capi.align_params()
+    seeds = np.arange(7000, 7064, dtype=np.uint64)
+
+    def setup(B):
+        net = capi.InitNet(ctx, seed=1)
+        net.set_params(w)
+        net.set_input_mode(0)
+        batch = initnet.make_batch(ctx, B)
+        initnet.render_pairs(batch, seeds[:B], 30.0, 0.5)
+        batch.convert()
+        return net, batch
+
+    net_a, batch_a = setup(64)
+    ref = np.arange(0, 128, 2, dtype=np.int32)
+    batch_a.align_learned(net_a, ref, ref + 1, ap, use_graph=True, want_results=True)
+    Ta = [batch_a.align_learned(net_a, ref[:8], ref[:8] + 1, ap, use_graph=True, want_results=True)[0]
+          for _ in range(2)]
+    net_b, batch_b = setup(8)
+    Tb = batch_b.align_learned(net_b, ref[:8], ref[:8] + 1, ap, use_graph=True, want_results=True)[0]
+    for T in Ta:
+        np.testing.assert_array_equal(T.view(np.uint32), Tb.view(np.uint32))
+    for o in (batch_a, net_a, batch_b, net_b):
+        o.close()
